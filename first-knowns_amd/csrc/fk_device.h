@@ -1,0 +1,73 @@
+// fk_device.h -- the structures the host planner hands to the kernels.
+//
+// Vocabulary.  Every family is a sieve over one integer line (Euler: p,
+// Dickson: j with k = W j, AP: p).  A WHEEL of the first few kill primes of
+// the family is enumerated rather than sieved: its admissible residues
+// ("lines") off_i < D, with D the wheel period, leave candidates
+// value = base + off_i + D * v, and line position v is what a tile's bit
+// stands for.  For a sieve prime q the killed values F_q (family-specific)
+// become the killed line positions K_q = F_q * D^-1 (mod q), shifted per
+// line and window by t = (base + off_i) * D^-1 (mod q).  Small primes are
+// OR-ed in as precomputed 64-bit pattern words, medium primes are
+// stride-marked in LDS, and primes above the LDS cutoff are tested per
+// surviving candidate in compaction rounds.
+#pragma once
+
+#include <stdint.h>
+
+#include "fk_arith.h"
+
+namespace fk {
+
+enum Family : int32_t { kEuler = 1, kDickson = 2, kAP = 3 };
+
+// Per-engine constants and table pointers (device memory), read by every
+// kernel through one pointer per window.
+struct DevEngine {
+    int32_t family;
+    int32_t n;
+    uint32_t nlines;          // wheel residues
+    uint32_t nlds;            // LDS-stage primes (pattern + marking)
+    uint32_t npat;            // the first npat of them use pattern words
+    uint32_t npairs;          // (prime, residue) marking pairs
+    uint32_t pat_words;       // total pattern words (sum of q)
+    uint32_t round_first;     // global prime-table index of the first round prime
+    uint32_t round_end;       // one past the last prime below q2
+    uint32_t xxmax;           // Euler: (n-1)^2 + (n-1)
+    uint64_t D;               // wheel period of the line
+    uint32_t W;               // Dickson: k = W j (product of primes <= n+1)
+    uint32_t d[8];            // AP: P(n), the common difference, in limbs
+    const uint64_t* offs;     // [nlines] line residues off_i < D
+    const uint32_t* lds_q;    // [nlds] primes
+    const uint32_t* lds_m32;  // [nlds] floor(2^32 / q)
+    const uint64_t* lds_m64;  // [nlds] floor(2^64 / q)
+    const uint32_t* lds_dinv; // [nlds] D^-1 mod q
+    const uint32_t* pat_off;  // [npat] first word of each prime's table
+    const uint64_t* pat;      // [pat_words] pattern words
+    const uint32_t* pairs;    // [npairs] (prime index << 16) | residue
+    const uint32_t* xxbits;   // Euler: bitset of x^2+x, x < n
+};
+
+// One piece of one window: a contiguous range of line positions of every
+// line, small enough that positions fit 32 bits.
+struct DevPiece {
+    uint32_t base[4];         // value of (line 0 residue 0, position 0): B
+    uint64_t lo_rel;          // window start minus B (values below are out)
+    uint64_t hi_rel;          // window end minus B (values at/above are out)
+    uint32_t npos;            // positions per line in this piece
+    uint32_t nseg;            // tiles per line
+    uint32_t tile0;           // first global tile index
+    uint32_t engine;          // index into the engine array
+    uint32_t window;          // caller's window index
+    uint32_t bq_off;          // offset of this piece's (B mod q) block
+};
+
+// A candidate in flight: the value the kill tests and the strong test need
+// (Euler p, Dickson k = W j, AP p) and its window.
+struct DevRec {
+    uint32_t t[4];
+    uint32_t window;
+    int32_t run;
+};
+
+}  // namespace fk

@@ -1,0 +1,1023 @@
+// fk_engine.hip -- host runtime and C ABI (include/fk_engine.h).
+//
+// An engine owns one family's resident tables on one device: the wheel
+// residues, the LDS-stage prime tables (pattern words and marking pairs)
+// and its place in the device-wide prime table that the per-candidate
+// rounds read.  A batch owns one HIP stream and the grow-only queues a
+// launch sequence needs; a sweep is a batch of one window.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fk_engine.h"
+#include "fk_device.h"
+#include "fk_launch.h"
+#include "fk_plan.h"
+
+using namespace fk;
+
+// ------------------------------------------------------------- errors
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                       \
+    do {                                                                   \
+        hipError_t e_ = (call);                                            \
+        if (e_ != hipSuccess)                                              \
+            return fail(e_ == hipErrorOutOfMemory ? FK_E_OOM : FK_E_DEVICE, \
+                        "%s failed: %s", #call, hipGetErrorString(e_));    \
+    } while (0)
+
+// ----------------------------------------------- device-wide prime table
+
+namespace {
+
+struct DeviceCtx {
+    int device = -1;
+    int cus = 0;
+    uint32_t limit = 0;             // primes below this are in the table
+    std::vector<uint32_t> primes;   // host copy
+    PrimeK* d_primes = nullptr;
+};
+
+std::mutex g_ctx_mu;
+std::vector<std::unique_ptr<DeviceCtx>> g_ctx;
+
+int ctx_for(int device, uint32_t need_limit, DeviceCtx** out)
+{
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    DeviceCtx* c = nullptr;
+    for (auto& p : g_ctx)
+        if (p->device == device) c = p.get();
+    if (!c) {
+        g_ctx.emplace_back(new DeviceCtx());
+        c = g_ctx.back().get();
+        c->device = device;
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, device));
+        c->cus = prop.multiProcessorCount;
+    }
+    if (need_limit > c->limit) {
+        uint32_t lim = std::max(need_limit, 1u << 16);
+        std::vector<uint32_t> ps = primes_below(lim);
+        std::vector<PrimeK> pk(ps.size());
+        for (size_t i = 0; i < ps.size(); i++) pk[i] = make_primek(ps[i]);
+        PrimeK* d = nullptr;
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipMalloc(&d, pk.size() * sizeof(PrimeK)));
+        HIPCHK(hipMemcpy(d, pk.data(), pk.size() * sizeof(PrimeK),
+                         hipMemcpyHostToDevice));
+        // engines still hold indices only, which stay valid: the table is
+        // the same ascending prime list, extended
+        if (c->d_primes) (void)hipFree(c->d_primes);
+        c->d_primes = d;
+        c->primes.swap(ps);
+        c->limit = lim;
+    }
+    *out = c;
+    return FK_OK;
+}
+
+uint32_t prime_index(const std::vector<uint32_t>& ps, uint32_t q)
+{
+    return (uint32_t)(std::lower_bound(ps.begin(), ps.end(), q) - ps.begin());
+}
+
+template <class T>
+int dev_upload(const std::vector<T>& v, T** out)
+{
+    *out = nullptr;
+    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+    HIPCHK(hipMalloc(out, bytes));
+    if (!v.empty())
+        HIPCHK(hipMemcpy(*out, v.data(), v.size() * sizeof(T),
+                         hipMemcpyHostToDevice));
+    return FK_OK;
+}
+
+inline u128 mk128(const uint64_t v[2]) { return ((u128)v[1] << 64) | v[0]; }
+
+}  // namespace
+
+// --------------------------------------------------------------- engine
+
+struct fk_batch;
+
+struct fk_engine {
+    fk_engine_desc desc;
+    int device = 0;
+    int family = 0, n = 0;
+    uint32_t q2 = 0;
+    std::vector<uint32_t> wheel;
+    uint64_t D = 1;
+    std::vector<uint64_t> offs;
+    uint64_t W = 1;
+    u128 dval = 0;
+    uint32_t q_pat = 0, q_med = 0, threads = 256, wpt = 8;
+    uint32_t round_chunk = 64, growth = 2;
+    uint32_t npat = 0;
+    std::vector<uint32_t> lds_q;
+    std::vector<uint32_t> kill_w;    // |K_q| per LDS prime
+    uint32_t pat_words = 0;
+    uint32_t round_first = 0, round_end = 0;
+    uint32_t nrounds = 0;
+    double surv_lds = 1.0;
+    double line_density = 1.0;       // nlines / D
+    DevEngine host{};                // with device pointers
+    DevEngine* d_engine = nullptr;   // device copy of `host`
+    std::vector<void*> allocs;
+    uint64_t device_bytes = 0;
+    fk_batch* own = nullptr;
+};
+
+static void engine_free(fk_engine* e)
+{
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    for (void* p : e->allocs) (void)hipFree(p);
+    e->allocs.clear();
+}
+
+template <class T>
+static int eng_upload(fk_engine* e, const std::vector<T>& v, const T** out)
+{
+    T* d = nullptr;
+    int rc = dev_upload(v, &d);
+    if (rc) return rc;
+    e->allocs.push_back(d);
+    e->device_bytes += std::max<size_t>(v.size(), 1) * sizeof(T);
+    *out = d;
+    return FK_OK;
+}
+
+static int build_engine(fk_engine* e)
+{
+    const fk_engine_desc& d = e->desc;
+    e->family = d.family;
+    e->n = d.n;
+    e->q2 = d.q2;
+    e->device = d.device;
+    if (d.family < 1 || d.family > 3)
+        return fail(FK_E_ARG, "unknown family %d", d.family);
+    if (d.n < 1 || d.n > 64)
+        return fail(FK_E_ARG, "n = %d outside [1, 64]", d.n);
+    if (d.family == FK_FAMILY_AP && d.n > 26)
+        return fail(FK_E_ARG, "AP n = %d: P(n) must fit 128 bits (n <= 26)",
+                    d.n);
+    if (d.q2 < 3 || d.q2 > (1u << 30))
+        return fail(FK_E_ARG, "q2 = %u outside [3, 2^30]", d.q2);
+
+    const uint32_t first = first_kill_prime(d.family, d.n);
+    std::vector<uint32_t> kp;
+    for (uint32_t q : primes_below(d.q2))
+        if (q >= first) kp.push_back(q);
+
+    // ---- the wheel
+    std::vector<uint32_t> wheel;
+    if (d.wheel_primes[0]) {
+        for (int i = 0; i < 12 && d.wheel_primes[i]; i++)
+            wheel.push_back(d.wheel_primes[i]);
+        for (size_t i = 0; i < wheel.size(); i++)
+            if (i >= kp.size() || wheel[i] != kp[i])
+                return fail(FK_E_ARG, "wheel primes must be the first kill "
+                            "primes of the family, in order");
+    } else {
+        int nw = d.nwheel;
+        if (nw <= 0) {              // widest wheel with period <= 2^17
+            uint64_t m = 1;
+            nw = 0;
+            while (nw < (int)kp.size() && m * kp[nw] <= (1u << 17)) {
+                m *= kp[nw];
+                nw++;
+            }
+            if (nw == 0 && !kp.empty()) nw = 1;
+        }
+        for (int i = 0; i < nw && i < (int)kp.size(); i++)
+            wheel.push_back(kp[i]);
+    }
+    if (wheel.empty()) return fail(FK_E_ARG, "no kill prime below q2");
+    bool ok = false;
+    e->offs = wheel_offsets(d.family, d.n, wheel, &e->D, &ok);
+    if (!ok) return fail(FK_E_ARG, "wheel period exceeds 2^63");
+    if (e->offs.empty())
+        return fail(FK_E_ARG, "the wheel admits no residue at n = %d", d.n);
+    if (e->offs.size() > (1u << 30))
+        return fail(FK_E_ARG, "wheel has %zu residues (> 2^30)",
+                    e->offs.size());
+    e->wheel = wheel;
+    e->line_density = (double)e->offs.size() / (double)e->D;
+
+    // ---- tuning
+    e->threads = d.threads ? d.threads : 256;
+    e->wpt = d.words_per_thread ? d.words_per_thread : 8;
+    if (!sieve_shape_ok((int)e->threads, (int)e->wpt))
+        return fail(FK_E_ARG, "unsupported sieve shape threads=%u wpt=%u",
+                    e->threads, e->wpt);
+    e->q_pat = d.q_pat ? d.q_pat : 128;
+    e->q_med = d.q_med ? d.q_med : 4096;
+    if (e->q_med > 65536) e->q_med = 65536;     // pairs pack q in 16 bits
+    if (e->q_med > d.q2) e->q_med = d.q2;
+    if (e->q_pat > e->q_med) e->q_pat = e->q_med;
+    e->round_chunk = d.round_chunk ? d.round_chunk : 64;
+    e->growth = d.round_growth >= 2 ? d.round_growth : 2;
+
+    // ---- LDS-stage primes: past the wheel, below q_med
+    std::vector<uint32_t> lq, lm32, ldinv, pat_off, pairs;
+    std::vector<uint64_t> lm64, pat;
+    double surv = 1.0;
+    for (size_t i = wheel.size(); i < kp.size() && kp[i] < e->q_med; i++) {
+        const uint32_t q = kp[i];
+        const uint32_t dinv = inv_mod((uint32_t)(e->D % q), q);
+        std::vector<uint32_t> ks = kill_set(d.family, d.n, q);
+        std::vector<uint32_t> kl;                 // line-domain residues
+        for (uint32_t f : ks) kl.push_back((uint32_t)((uint64_t)f * dinv % q));
+        std::sort(kl.begin(), kl.end());
+        const uint32_t idx = (uint32_t)lq.size();
+        lq.push_back(q);
+        lm32.push_back((uint32_t)(((uint64_t)1 << 32) / q));
+        lm64.push_back(make_primek(q).m64);
+        ldinv.push_back(dinv);
+        e->kill_w.push_back((uint32_t)kl.size());
+        surv *= 1.0 - (double)kl.size() / q;
+        if (q < e->q_pat) {
+            pat_off.push_back((uint32_t)pat.size());
+            std::vector<uint8_t> dead(q, 0);
+            for (uint32_t r : kl) dead[r] = 1;
+            for (uint32_t s = 0; s < q; s++) {
+                uint64_t w = 0;
+                for (int b = 0; b < 64; b++)
+                    if (dead[(s + b) % q]) w |= 1ull << b;
+                pat.push_back(w);
+            }
+            e->npat = idx + 1;
+        } else {
+            for (uint32_t r : kl) pairs.push_back((idx << 16) | r);
+        }
+    }
+    // patterns must be a prefix of the LDS primes
+    e->lds_q = lq;
+    e->pat_words = (uint32_t)pat.size();
+    e->surv_lds = surv;
+
+    // ---- round primes: [q_med, q2) in the device-wide table
+    DeviceCtx* ctx = nullptr;
+    int rc = ctx_for(d.device, d.q2, &ctx);
+    if (rc) return rc;
+    e->round_first = prime_index(ctx->primes, std::max(e->q_med, first));
+    e->round_first = std::max(e->round_first,
+                              prime_index(ctx->primes, wheel.back() + 1));
+    e->round_end = prime_index(ctx->primes, d.q2);
+    if (e->round_end < e->round_first) e->round_end = e->round_first;
+    {
+        uint32_t r = 0;
+        for (uint64_t b = e->q_med; b < d.q2; b *= e->growth) r++;
+        e->nrounds = r;
+    }
+
+    // ---- family constants
+    e->W = (d.family == FK_FAMILY_DICKSON) ? dickson_wheel(d.n) : 1;
+    e->dval = (d.family == FK_FAMILY_AP) ? ap_difference(d.n) : 0;
+    std::vector<uint32_t> xx;
+    uint32_t xxmax = 0;
+    if (d.family == FK_FAMILY_EULER) {
+        xxmax = (uint32_t)((d.n - 1) * (d.n - 1) + (d.n - 1));
+        xx.assign(xxmax / 32 + 1, 0);
+        for (int x = 0; x < d.n; x++) {
+            const uint32_t v = (uint32_t)(x * x + x);
+            xx[v >> 5] |= 1u << (v & 31);
+        }
+    }
+    if (e->W > 0xffffffffull) return fail(FK_E_ARG, "W(n) exceeds 32 bits");
+
+    HIPCHK(hipSetDevice(d.device));
+    DevEngine& h = e->host;
+    h.family = d.family;
+    h.n = d.n;
+    h.nlines = (uint32_t)e->offs.size();
+    h.nlds = (uint32_t)lq.size();
+    h.npat = e->npat;
+    h.npairs = (uint32_t)pairs.size();
+    h.pat_words = e->pat_words;
+    h.round_first = e->round_first;
+    h.round_end = e->round_end;
+    h.xxmax = xxmax;
+    h.D = e->D;
+    h.W = (uint32_t)e->W;
+    for (int i = 0; i < 4; i++) {
+        h.d[i] = (uint32_t)(e->dval >> (32 * i));
+        h.d[4 + i] = 0;
+    }
+    if ((rc = eng_upload(e, e->offs, &h.offs))) return rc;
+    if ((rc = eng_upload(e, lq, &h.lds_q))) return rc;
+    if ((rc = eng_upload(e, lm32, &h.lds_m32))) return rc;
+    if ((rc = eng_upload(e, lm64, &h.lds_m64))) return rc;
+    if ((rc = eng_upload(e, ldinv, &h.lds_dinv))) return rc;
+    if ((rc = eng_upload(e, pat_off, &h.pat_off))) return rc;
+    if ((rc = eng_upload(e, pat, &h.pat))) return rc;
+    if ((rc = eng_upload(e, pairs, &h.pairs))) return rc;
+    if (xx.empty()) xx.push_back(0);
+    if ((rc = eng_upload(e, xx, &h.xxbits))) return rc;
+    std::vector<DevEngine> one(1, h);
+    const DevEngine* de = nullptr;
+    if ((rc = eng_upload(e, one, &de))) return rc;
+    e->d_engine = const_cast<DevEngine*>(de);
+    return FK_OK;
+}
+
+// ---------------------------------------------------------------- batch
+
+struct fk_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    bool pending = false;
+    bool have_results = false;
+    // device buffers (grow-only)
+    DevEngine* d_engines = nullptr;   size_t cap_engines = 0;
+    DevPiece* d_pieces = nullptr;     size_t cap_pieces = 0;
+    uint32_t* d_win_engine = nullptr; size_t cap_win = 0;
+    DevWinInfo* d_wins = nullptr;
+    uint32_t* d_bq = nullptr;         size_t cap_bq = 0;
+    DevRec* d_q[2] = {nullptr, nullptr}; size_t cap_q = 0;
+    uint32_t* d_flags = nullptr;
+    uint32_t* d_counts = nullptr;     size_t cap_counts = 0;
+    // pinned host mirrors
+    uint32_t* h_counts = nullptr;
+    DevRec* h_recs = nullptr;         size_t cap_hrecs = 0;
+    // the submitted job, kept for results() and re-runs
+    std::vector<fk_engine*> engines;
+    std::vector<uint32_t> engine_of;
+    std::vector<fk_window> windows;
+    std::vector<fk_classify_spec> cls;
+    uint32_t final_slot = 0;          // which counter holds the final count
+    uint32_t final_q = 0;             // which queue holds the final records
+    uint32_t queue_cap = 0;
+    uint32_t copied = 0;
+    uint64_t tiles = 0, candidates = 0;
+    std::vector<fk_record> out;       // sorted results
+    std::vector<uint64_t> counts;
+    float ms[5] = {0, 0, 0, 0, 0};
+};
+
+static int batch_free(fk_batch* b)
+{
+    (void)hipSetDevice(b->device);
+    void* ds[] = {b->d_engines, b->d_pieces, b->d_win_engine, b->d_wins,
+                  b->d_bq, b->d_q[0], b->d_q[1], b->d_flags, b->d_counts};
+    for (void* p : ds) if (p) (void)hipFree(p);
+    if (b->h_counts) (void)hipHostFree(b->h_counts);
+    if (b->h_recs) (void)hipHostFree(b->h_recs);
+    for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    return FK_OK;
+}
+
+template <class T>
+static int grow(T** p, size_t* cap, size_t need)
+{
+    if (need <= *cap && *p) return FK_OK;
+    size_t nc = std::max<size_t>(need, *cap * 3 / 2);
+    nc = std::max<size_t>(nc, 16);
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    HIPCHK(hipMalloc(p, nc * sizeof(T)));
+    *cap = nc;
+    return FK_OK;
+}
+
+static int create_batch(int device, fk_batch** out)
+{
+    fk_batch* b = new fk_batch();
+    b->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream,
+                                                      hipStreamNonBlocking);
+    for (int i = 0; i < 6 && e == hipSuccess; i++)
+        e = hipEventCreate(&b->ev[i]);
+    if (e != hipSuccess) {
+        batch_free(b);
+        delete b;
+        return fail(FK_E_DEVICE, "batch setup failed: %s",
+                    hipGetErrorString(e));
+    }
+    *out = b;
+    return FK_OK;
+}
+
+// Expected survivors of the LDS stage for a window: wheel density times
+// the product of (1 - |K_q|/q) over the LDS primes.
+static double expected_lds(const fk_engine* e, double span)
+{
+    return span * e->line_density * e->surv_lds;
+}
+
+// Round boundaries by prime value: q_med * growth^r.
+static double round_survival(const fk_engine* e, double lo, double hi)
+{
+    if (hi <= lo) return 1.0;
+    // sum over primes in [lo, hi) of w/q ~ w ln(ln hi / ln lo)
+    const double w = (e->family == FK_FAMILY_DICKSON) ? e->n : e->n;
+    return std::pow(std::log(lo) / std::log(hi), w);
+}
+
+static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
+{
+    const size_t nwin = b->windows.size();
+    fk_engine* e0 = b->engines[0];
+    DeviceCtx* ctx = nullptr;
+    uint32_t need_limit = 0;
+    for (fk_engine* e : b->engines) need_limit = std::max(need_limit, e->q2);
+    int rc = ctx_for(b->device, need_limit, &ctx);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(b->device));
+
+    // ---- pieces
+    const uint32_t T = e0->threads;
+    uint64_t max_npos = 1;
+    struct WinPlan { u128 B; uint64_t H; };
+    std::vector<WinPlan> wp(nwin);
+    for (size_t w = 0; w < nwin; w++) {
+        const fk_engine* e = b->engines[b->engine_of[w]];
+        const u128 lo = mk128(b->windows[w].lo), hi = mk128(b->windows[w].hi);
+        const u128 B = lo - lo % e->D;
+        wp[w].B = B;
+        wp[w].H = (uint64_t)(hi - B);
+        const uint64_t npos = wp[w].H / e->D + 1;
+        max_npos = std::max(max_npos, npos);
+    }
+    // tile length: no longer than the longest line needs
+    uint32_t wpt = e0->wpt;
+    while (wpt > 1 && (uint64_t)64 * T * (wpt / 2) >= max_npos) wpt /= 2;
+    const uint64_t S = (uint64_t)64 * T * wpt;
+    const uint64_t PIECE = std::max<uint64_t>(S, ((uint64_t)1 << 31) / S * S);
+
+    std::vector<DevPiece> pieces;
+    std::vector<DevWinInfo> wins(nwin);
+    std::vector<uint32_t> win_engine(nwin);
+    uint64_t tiles = 0, bq_total = 0;
+    uint32_t nlds_max = 0, pat_max = 0;
+    double expect = 0.0, cand = 0.0;
+    for (fk_engine* e : b->engines) {
+        nlds_max = std::max(nlds_max, e->host.nlds);
+        pat_max = std::max(pat_max, e->pat_words);
+    }
+    for (size_t w = 0; w < nwin; w++) {
+        const fk_engine* e = b->engines[b->engine_of[w]];
+        win_engine[w] = b->engine_of[w];
+        wins[w].engine = b->engine_of[w];
+        wins[w].mode = b->cls[w].mode;
+        wins[w].cap = b->cls[w].cap;
+        wins[w].exact_from = b->cls[w].exact_from;
+        const u128 lo = mk128(b->windows[w].lo), hi = mk128(b->windows[w].hi);
+        if (hi <= lo) continue;
+        const uint64_t npos_all = wp[w].H / e->D + 1;
+        for (uint64_t p0 = 0; p0 < npos_all; p0 += PIECE) {
+            DevPiece P{};
+            const u128 base = wp[w].B + (u128)e->D * p0;
+            for (int i = 0; i < 4; i++) P.base[i] = (uint32_t)(base >> (32 * i));
+            P.lo_rel = lo > base ? (uint64_t)(lo - base) : 0;
+            P.hi_rel = (uint64_t)(hi - base);
+            P.npos = (uint32_t)std::min<uint64_t>(PIECE, npos_all - p0);
+            P.nseg = (uint32_t)((P.npos + S - 1) / S);
+            P.tile0 = (uint32_t)tiles;
+            P.engine = b->engine_of[w];
+            P.window = (uint32_t)w;
+            P.bq_off = (uint32_t)bq_total;
+            const uint64_t t = (uint64_t)P.nseg * e->offs.size();
+            if (tiles + t >= (1ull << 32))
+                return fail(FK_E_ARG, "batch too large: more than 2^32 tiles");
+            tiles += t;
+            bq_total += e->host.nlds;
+            pieces.push_back(P);
+        }
+        const double span = (double)(hi - lo);
+        expect += expected_lds(e, span);
+        cand += span * e->line_density;
+    }
+    b->tiles = tiles;
+    b->candidates = (uint64_t)cand;
+    if (queue_cap == 0) {
+        const double c = 1.5 * expect + 8.0 * std::sqrt(expect) + 4096.0;
+        queue_cap = (uint32_t)std::min(c, 1.0e9);
+    }
+    b->queue_cap = queue_cap;
+
+    // ---- rounds (shared boundaries by prime value)
+    uint32_t qmin = 0xffffffffu, qmax = 0;
+    for (fk_engine* e : b->engines) {
+        qmin = std::min(qmin, e->q_med);
+        qmax = std::max(qmax, e->q2);
+    }
+    const uint32_t growth = e0->growth;
+    std::vector<std::pair<uint32_t, uint32_t>> rounds;   // prime values
+    for (uint64_t lo = qmin; lo < qmax; lo *= growth)
+        rounds.push_back({(uint32_t)lo,
+                          (uint32_t)std::min<uint64_t>(lo * growth, qmax)});
+    const size_t ncounts = 2 + rounds.size();
+
+    // ---- buffers
+    if ((rc = grow(&b->d_engines, &b->cap_engines, b->engines.size()))) return rc;
+    if ((rc = grow(&b->d_pieces, &b->cap_pieces, std::max<size_t>(pieces.size(), 1)))) return rc;
+    if (nwin > b->cap_win || !b->d_win_engine) {
+        size_t c = 0;
+        if ((rc = grow(&b->d_win_engine, &c, nwin))) return rc;
+        size_t c2 = 0;
+        if ((rc = grow(&b->d_wins, &c2, c))) return rc;
+        b->cap_win = c;
+    }
+    if ((rc = grow(&b->d_bq, &b->cap_bq, std::max<uint64_t>(bq_total, 1)))) return rc;
+    if (queue_cap > b->cap_q || !b->d_q[0]) {
+        size_t c = 0;
+        for (int i = 0; i < 2; i++) {
+            c = 0;
+            if (b->d_q[i]) (void)hipFree(b->d_q[i]);
+            b->d_q[i] = nullptr;
+            if ((rc = grow(&b->d_q[i], &c, queue_cap))) return rc;
+        }
+        size_t cf = 0;
+        if (b->d_flags) (void)hipFree(b->d_flags);
+        b->d_flags = nullptr;
+        if ((rc = grow(&b->d_flags, &cf, c))) return rc;
+        b->cap_q = c;
+    }
+    if (ncounts > b->cap_counts || !b->d_counts) {
+        size_t c = 0;
+        if ((rc = grow(&b->d_counts, &c, ncounts))) return rc;
+        if (b->h_counts) (void)hipHostFree(b->h_counts);
+        HIPCHK(hipHostMalloc(&b->h_counts, c * sizeof(uint32_t)));
+        b->cap_counts = c;
+    }
+
+    // ---- uploads (small)
+    std::vector<DevEngine> des;
+    for (fk_engine* e : b->engines) des.push_back(e->host);
+    hipStream_t s = b->stream;
+    HIPCHK(hipMemcpyAsync(b->d_engines, des.data(), des.size() * sizeof(DevEngine),
+                          hipMemcpyHostToDevice, s));
+    if (!pieces.empty())
+        HIPCHK(hipMemcpyAsync(b->d_pieces, pieces.data(),
+                              pieces.size() * sizeof(DevPiece),
+                              hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->d_win_engine, win_engine.data(),
+                          nwin * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->d_wins, wins.data(), nwin * sizeof(DevWinInfo),
+                          hipMemcpyHostToDevice, s));
+    // the host vectors must outlive the async copies: wait for them here
+    // (they are small; the device work below is what stays asynchronous)
+    HIPCHK(hipStreamSynchronize(s));
+
+    HIPCHK(hipEventRecord(b->ev[0], s));
+    HIPCHK(launch_zero(b->d_counts, (uint32_t)ncounts, s));
+    HIPCHK(launch_zero(b->d_flags, queue_cap, s));
+    HIPCHK(launch_prep(b->d_engines, b->d_pieces, (uint32_t)pieces.size(),
+                       nlds_max, b->d_bq, s));
+    HIPCHK(hipEventRecord(b->ev[1], s));
+
+    // ---- sieve
+    if (tiles) {
+        SieveArgs a{};
+        a.engines = b->d_engines;
+        a.pieces = b->d_pieces;
+        a.npieces = (uint32_t)pieces.size();
+        a.ntiles = (uint32_t)tiles;
+        a.nlds_max = nlds_max;
+        a.bq = b->d_bq;
+        a.out = b->d_q[0];
+        a.out_count = b->d_counts + 0;
+        a.out_cap = queue_cap;
+        const size_t lds = 8 * (S / 64) + 8 * ((nlds_max + 1) / 2) + 8 * (size_t)pat_max;
+        if (lds > 160 * 1024)
+            return fail(FK_E_ARG, "LDS need %zu bytes > 160 KiB (lower q_med"
+                        " or q_pat)", lds);
+        int occ = sieve_occupancy((int)T, (int)wpt, lds);
+        if (occ < 1) occ = 1;
+        uint64_t grid = (uint64_t)occ * ctx->cus;
+        grid = std::min<uint64_t>(grid, tiles);
+        HIPCHK(launch_sieve(a, (int)T, (int)wpt, (uint32_t)grid, lds, s));
+    }
+    HIPCHK(hipEventRecord(b->ev[2], s));
+
+    // ---- rounds
+    uint32_t cur = 0, cslot = 0;
+    double alive = expect;
+    for (size_t r = 0; r < rounds.size(); r++) {
+        RoundArgs ra{};
+        ra.engines = b->d_engines;
+        ra.win_engine = b->d_win_engine;
+        ra.primes = ctx->d_primes;
+        ra.recs = b->d_q[cur];
+        ra.count = b->d_counts + cslot;
+        ra.cap = queue_cap;
+        ra.flags = b->d_flags;
+        ra.p_begin = prime_index(ctx->primes, rounds[r].first);
+        ra.p_end = prime_index(ctx->primes, rounds[r].second);
+        ra.chunk = e0->round_chunk;
+        const uint32_t np = ra.p_end > ra.p_begin ? ra.p_end - ra.p_begin : 0;
+        ra.nchunks = (np + ra.chunk - 1) / ra.chunk;
+        if (ra.nchunks == 0) continue;
+        uint32_t gy = (uint32_t)std::min(256.0, std::ceil(alive / 256.0));
+        HIPCHK(launch_round_test(ra, gy, s));
+        CompactArgs ca{};
+        ca.in = b->d_q[cur];
+        ca.in_count = b->d_counts + cslot;
+        ca.cap = queue_cap;
+        ca.flags = b->d_flags;
+        ca.out = b->d_q[cur ^ 1];
+        ca.out_count = b->d_counts + 2 + r;
+        uint32_t cg = (uint32_t)std::min(1024.0, std::ceil(alive / 256.0) + 1);
+        HIPCHK(launch_round_compact(ca, cg, s));
+        cur ^= 1;
+        cslot = 2 + (uint32_t)r;
+        double sv = 1.0;
+        for (fk_engine* e : b->engines)
+            sv = std::min(sv, round_survival(e, rounds[r].first, rounds[r].second));
+        alive *= sv;
+    }
+    HIPCHK(hipEventRecord(b->ev[3], s));
+
+    // ---- classification
+    bool any_cls = false;
+    for (size_t w = 0; w < nwin; w++) any_cls = any_cls || b->cls[w].mode != 0;
+    if (any_cls) {
+        ClassifyArgs ca{};
+        ca.engines = b->d_engines;
+        ca.wins = b->d_wins;
+        ca.recs = b->d_q[cur];
+        ca.count = b->d_counts + cslot;
+        ca.cap = queue_cap;
+        uint32_t g = (uint32_t)std::min(8192.0, std::ceil(alive / 128.0) + 1);
+        HIPCHK(launch_classify(ca, g, s));
+    }
+    HIPCHK(hipEventRecord(b->ev[4], s));
+
+    // ---- readback: counters and the first `copy` final records
+    HIPCHK(hipMemcpyAsync(b->h_counts, b->d_counts, ncounts * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, s));
+    uint32_t copy = (uint32_t)std::min<double>(queue_cap,
+                                               4.0 * alive + 4096.0);
+    if (copy > b->cap_hrecs || !b->h_recs) {
+        if (b->h_recs) (void)hipHostFree(b->h_recs);
+        b->h_recs = nullptr;
+        size_t c = std::max<size_t>(copy, b->cap_hrecs * 3 / 2);
+        HIPCHK(hipHostMalloc(&b->h_recs, c * sizeof(DevRec)));
+        b->cap_hrecs = c;
+    }
+    HIPCHK(hipMemcpyAsync(b->h_recs, b->d_q[cur], (size_t)copy * sizeof(DevRec),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(b->ev[5], s));
+    b->final_slot = cslot;
+    b->final_q = cur;
+    b->copied = copy;
+    b->pending = true;
+    b->have_results = false;
+    return FK_OK;
+}
+
+static int batch_collect(fk_batch* b)
+{
+    if (!b->pending) {
+        if (b->have_results) return FK_OK;
+        return fail(FK_E_ARG, "nothing submitted");
+    }
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipEventSynchronize(b->ev[5]));
+    b->pending = false;
+    const uint32_t tile_count = b->h_counts[0];
+    if (tile_count > b->queue_cap) {
+        // the LDS stage found more survivors than the queue holds: size the
+        // queue from the true count and run the whole batch again
+        uint32_t cap = (uint32_t)std::min<double>(1.25 * tile_count + 65536.0,
+                                                  4.0e9);
+        if (tile_count >= 0xf0000000u)
+            return fail(FK_E_OVERFLOW, "survivor queue overflow: %u",
+                        tile_count);
+        int rc = batch_enqueue(b, cap);
+        if (rc) return rc;
+        return batch_collect(b);
+    }
+    const uint32_t n = b->h_counts[b->final_slot];
+    HIPCHK(hipEventElapsedTime(&b->ms[0], b->ev[0], b->ev[1]));
+    HIPCHK(hipEventElapsedTime(&b->ms[1], b->ev[1], b->ev[2]));
+    HIPCHK(hipEventElapsedTime(&b->ms[2], b->ev[2], b->ev[3]));
+    HIPCHK(hipEventElapsedTime(&b->ms[3], b->ev[3], b->ev[4]));
+    HIPCHK(hipEventElapsedTime(&b->ms[4], b->ev[0], b->ev[5]));
+    std::vector<DevRec> extra;
+    const DevRec* recs = b->h_recs;
+    if (n > b->copied) {
+        extra.resize(n);
+        HIPCHK(hipMemcpy(extra.data(), b->d_q[b->final_q], (size_t)n * sizeof(DevRec),
+                         hipMemcpyDeviceToHost));
+        recs = extra.data();
+    }
+    const size_t nwin = b->windows.size();
+    b->out.resize(n);
+    b->counts.assign(nwin, 0);
+    for (uint32_t i = 0; i < n; i++) {
+        const DevRec& r = recs[i];
+        u128 t = ((u128)r.t[3] << 96) | ((u128)r.t[2] << 64) |
+                 ((u128)r.t[1] << 32) | r.t[0];
+        const fk_engine* e = b->engines[b->engine_of[r.window]];
+        if (e->family == FK_FAMILY_DICKSON) t /= e->W;
+        fk_record& o = b->out[i];
+        o.value[0] = (uint64_t)t;
+        o.value[1] = (uint64_t)(t >> 64);
+        o.run = r.run;
+        o.window = r.window;
+        b->counts[r.window]++;
+    }
+    std::sort(b->out.begin(), b->out.end(),
+              [](const fk_record& x, const fk_record& y) {
+                  if (x.window != y.window) return x.window < y.window;
+                  if (x.value[1] != y.value[1]) return x.value[1] < y.value[1];
+                  return x.value[0] < y.value[0];
+              });
+    b->have_results = true;
+    return FK_OK;
+}
+
+static int batch_submit(fk_batch* b, fk_engine* const* engines,
+                        size_t nengines, const uint32_t* engine_of,
+                        const fk_window* windows, const fk_classify_spec* cls,
+                        size_t nwin)
+{
+    if (b->pending) return fail(FK_E_BUSY, "batch has work in flight");
+    if (nengines == 0 || nwin == 0)
+        return fail(FK_E_ARG, "empty batch");
+    for (size_t i = 0; i < nengines; i++) {
+        if (!engines[i]) return fail(FK_E_ARG, "null engine %zu", i);
+        if (engines[i]->device != b->device)
+            return fail(FK_E_ARG, "engine %zu lives on another device", i);
+    }
+    b->engines.assign(engines, engines + nengines);
+    b->engine_of.assign(engine_of, engine_of + nwin);
+    b->windows.assign(windows, windows + nwin);
+    b->cls.resize(nwin);
+    for (size_t w = 0; w < nwin; w++) {
+        if (b->engine_of[w] >= nengines)
+            return fail(FK_E_ARG, "window %zu names engine %u", w,
+                        b->engine_of[w]);
+        const fk_engine* e = engines[b->engine_of[w]];
+        const u128 lo = mk128(windows[w].lo), hi = mk128(windows[w].hi);
+        if (hi < lo) return fail(FK_E_RANGE, "window %zu has hi < lo", w);
+        if ((hi >> 126) != 0)
+            return fail(FK_E_RANGE, "window %zu reaches past 2^126", w);
+        const u128 B = lo - lo % e->D;
+        if (((hi - B) >> 63) != 0)
+            return fail(FK_E_RANGE, "window %zu is wider than 2^63", w);
+        u128 top = hi;
+        if (e->family == FK_FAMILY_DICKSON) top = hi * e->W;
+        if ((top >> 127) != 0)
+            return fail(FK_E_RANGE, "window %zu values exceed 127 bits", w);
+        fk_classify_spec c{0, 0, 0};
+        if (cls) c = cls[w];
+        if (c.mode < 0 || c.mode > 2)
+            return fail(FK_E_ARG, "classification mode %d", c.mode);
+        if (c.mode && (c.cap < 0 || c.cap > 64))
+            return fail(FK_E_ARG, "classification cap %d outside [0, 64]",
+                        c.cap);
+        b->cls[w] = c;
+    }
+    return batch_enqueue(b, 0);
+}
+
+// ------------------------------------------------------------------ ABI
+
+extern "C" {
+
+int fk_abi_version(void) { return FK_ABI_VERSION; }
+
+int fk_last_error(char* buf, size_t len)
+{
+    if (!buf || len == 0) return FK_E_ARG;
+    snprintf(buf, len, "%s", g_err.c_str());
+    return FK_OK;
+}
+
+int fk_device_count(int* count)
+{
+    if (!count) return fail(FK_E_ARG, "null count");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return FK_OK;
+}
+
+int fk_engine_create(const fk_engine_desc* desc, fk_engine** out)
+{
+    if (!desc || !out) return fail(FK_E_ARG, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(FK_E_NODEV, "no HIP device visible");
+    if (desc->device < 0 || desc->device >= ndev)
+        return fail(FK_E_NODEV, "device %d not present (%d visible)",
+                    desc->device, ndev);
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, desc->device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(FK_E_NODEV, "device %d is %s, this library is built for "
+                    "gfx950", desc->device, prop.gcnArchName);
+    fk_engine* e = new fk_engine();
+    e->desc = *desc;
+    int rc = build_engine(e);
+    if (rc == FK_OK) rc = create_batch(e->device, &e->own);
+    if (rc) {
+        std::string keep = g_err;
+        engine_free(e);
+        delete e;
+        g_err = keep;
+        return rc;
+    }
+    *out = e;
+    return FK_OK;
+}
+
+int fk_engine_destroy(fk_engine* eng)
+{
+    if (!eng) return FK_OK;
+    if (eng->own) {
+        batch_free(eng->own);
+        delete eng->own;
+    }
+    engine_free(eng);
+    delete eng;
+    return FK_OK;
+}
+
+int fk_engine_get_info(const fk_engine* e, fk_engine_info* info)
+{
+    if (!e || !info) return fail(FK_E_ARG, "null argument");
+    std::memset(info, 0, sizeof *info);
+    info->family = e->family;
+    info->n = e->n;
+    info->q2 = e->q2;
+    info->nwheel = (uint32_t)e->wheel.size();
+    for (size_t i = 0; i < e->wheel.size() && i < 16; i++)
+        info->wheel_primes[i] = e->wheel[i];
+    info->line_step = e->D;
+    info->nlines = e->offs.size();
+    info->W = e->W;
+    info->d[0] = (uint64_t)e->dval;
+    info->d[1] = (uint64_t)(e->dval >> 64);
+    info->q_pat = e->q_pat;
+    info->q_med = e->q_med;
+    info->threads = e->threads;
+    info->words_per_thread = e->wpt;
+    info->n_lds_primes = e->host.nlds;
+    info->n_pattern_primes = e->npat;
+    info->n_round_primes = e->round_end - e->round_first;
+    info->n_rounds = e->nrounds;
+    info->device_bytes = e->device_bytes;
+    info->survival_lds = e->surv_lds;
+    return FK_OK;
+}
+
+int fk_engine_sweep(fk_engine* eng, const fk_window* win,
+                    const fk_classify_spec* cls, fk_record* out,
+                    size_t out_cap, size_t* n_out)
+{
+    if (!eng || !win || !n_out) return fail(FK_E_ARG, "null argument");
+    fk_batch* b = eng->own;
+    uint32_t zero = 0;
+    fk_engine* es[1] = {eng};
+    int rc = batch_submit(b, es, 1, &zero, win, cls, 1);
+    if (rc) return rc;
+    rc = batch_collect(b);
+    if (rc) return rc;
+    *n_out = b->out.size();
+    if (b->out.size() > out_cap)
+        return fail(FK_E_OVERFLOW, "%zu survivors, room for %zu",
+                    b->out.size(), out_cap);
+    if (!b->out.empty())
+        std::memcpy(out, b->out.data(), b->out.size() * sizeof(fk_record));
+    return FK_OK;
+}
+
+int fk_engine_classify(fk_engine* e, const uint64_t* values, size_t n,
+                       const fk_classify_spec* cls, int32_t* runs)
+{
+    if (!e || (!values && n) || !cls || (!runs && n))
+        return fail(FK_E_ARG, "null argument");
+    if (n == 0) return FK_OK;
+    if (cls->mode < 1 || cls->mode > 2 || cls->cap < 0 || cls->cap > 64)
+        return fail(FK_E_ARG, "bad classification spec");
+    std::vector<uint32_t> limbs(4 * n);
+    for (size_t i = 0; i < n; i++) {
+        const u128 v = ((u128)values[2 * i + 1] << 64) | values[2 * i];
+        for (int k = 0; k < 4; k++) limbs[4 * i + k] = (uint32_t)(v >> (32 * k));
+    }
+    HIPCHK(hipSetDevice(e->device));
+    uint32_t* d_vals = nullptr;
+    int32_t* d_out = nullptr;
+    HIPCHK(hipMalloc(&d_vals, limbs.size() * sizeof(uint32_t)));
+    hipError_t err = hipMalloc(&d_out, n * sizeof(int32_t));
+    if (err == hipSuccess)
+        err = hipMemcpy(d_vals, limbs.data(), limbs.size() * sizeof(uint32_t),
+                        hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = launch_classify_values(e->d_engine, d_vals, (uint32_t)n,
+                                     cls->mode, cls->cap, cls->exact_from,
+                                     d_out, e->own->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->own->stream);
+    if (err == hipSuccess)
+        err = hipMemcpy(runs, d_out, n * sizeof(int32_t), hipMemcpyDeviceToHost);
+    (void)hipFree(d_vals);
+    if (d_out) (void)hipFree(d_out);
+    if (err != hipSuccess)
+        return fail(FK_E_DEVICE, "classification failed: %s",
+                    hipGetErrorString(err));
+    return FK_OK;
+}
+
+int fk_batch_create(int device, fk_batch** out)
+{
+    if (!out) return fail(FK_E_ARG, "null argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return fail(FK_E_NODEV, "device %d not present", device);
+    return create_batch(device, out);
+}
+
+int fk_batch_destroy(fk_batch* b)
+{
+    if (!b) return FK_OK;
+    if (b->pending) (void)hipEventSynchronize(b->ev[5]);
+    batch_free(b);
+    delete b;
+    return FK_OK;
+}
+
+int fk_batch_submit(fk_batch* b, fk_engine* const* engines, size_t nengines,
+                    const uint32_t* engine_of, const fk_window* windows,
+                    const fk_classify_spec* cls, size_t nwin)
+{
+    if (!b || !engines || !engine_of || !windows)
+        return fail(FK_E_ARG, "null argument");
+    return batch_submit(b, engines, nengines, engine_of, windows, cls, nwin);
+}
+
+int fk_batch_query(fk_batch* b, int* done)
+{
+    if (!b || !done) return fail(FK_E_ARG, "null argument");
+    if (!b->pending) { *done = 1; return FK_OK; }
+    hipError_t e = hipEventQuery(b->ev[5]);
+    if (e == hipSuccess) *done = 1;
+    else if (e == hipErrorNotReady) *done = 0;
+    else return fail(FK_E_DEVICE, "event query: %s", hipGetErrorString(e));
+    return FK_OK;
+}
+
+int fk_batch_results(fk_batch* b, fk_record* out, size_t out_cap,
+                     size_t* n_out, uint64_t* counts)
+{
+    if (!b || !n_out) return fail(FK_E_ARG, "null argument");
+    int rc = batch_collect(b);
+    if (rc) return rc;
+    *n_out = b->out.size();
+    if (counts)
+        std::memcpy(counts, b->counts.data(), b->counts.size() * sizeof(uint64_t));
+    if (b->out.size() > out_cap)
+        return fail(FK_E_OVERFLOW, "%zu records, room for %zu", b->out.size(),
+                    out_cap);
+    if (!b->out.empty())
+        std::memcpy(out, b->out.data(), b->out.size() * sizeof(fk_record));
+    return FK_OK;
+}
+
+int fk_batch_timing(fk_batch* b, float* ms5, uint64_t* tiles,
+                    uint64_t* candidates)
+{
+    if (!b) return fail(FK_E_ARG, "null argument");
+    if (ms5) std::memcpy(ms5, b->ms, sizeof b->ms);
+    if (tiles) *tiles = b->tiles;
+    if (candidates) *candidates = b->candidates;
+    return FK_OK;
+}
+
+int fk_engine_batch(fk_engine* eng, fk_batch** out)
+{
+    if (!eng || !out) return fail(FK_E_ARG, "null argument");
+    *out = eng->own;
+    return FK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// fk_launch.h -- kernel argument blocks and the host-side launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fk_device.h"
+
+namespace fk {
+
+// Classification request of one window (mode 0: none, 1: all seven bases,
+// 2: base-2 chain first, redone with all bases from exact_from up).
+struct DevWinInfo {
+    uint32_t engine;
+    int32_t mode;
+    int32_t cap;
+    int32_t exact_from;
+};
+
+struct SieveArgs {
+    const DevEngine* engines;
+    const DevPiece* pieces;
+    uint32_t npieces;
+    uint32_t ntiles;
+    uint32_t nlds_max;        // LDS room reserved for s0[] (max over engines)
+    const uint32_t* bq;       // per piece, per LDS prime: base mod q
+    DevRec* out;
+    uint32_t* out_count;
+    uint32_t out_cap;
+};
+
+struct RoundArgs {
+    const DevEngine* engines;
+    const uint32_t* win_engine;
+    const PrimeK* primes;     // the device-wide prime table
+    const DevRec* recs;
+    const uint32_t* count;
+    uint32_t cap;
+    uint32_t* flags;
+    uint32_t p_begin, p_end;  // global prime indices of this round
+    uint32_t chunk;           // primes per work item
+    uint32_t nchunks;
+};
+
+struct CompactArgs {
+    const DevRec* in;
+    const uint32_t* in_count;
+    uint32_t cap;
+    uint32_t* flags;
+    DevRec* out;
+    uint32_t* out_count;
+};
+
+struct ClassifyArgs {
+    const DevEngine* engines;
+    const DevWinInfo* wins;
+    DevRec* recs;
+    const uint32_t* count;
+    uint32_t cap;
+};
+
+hipError_t launch_prep(const DevEngine* engines, const DevPiece* pieces,
+                       uint32_t npieces, uint32_t max_nlds, uint32_t* bq,
+                       hipStream_t stream);
+hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,
+                        uint32_t grid, size_t lds_bytes, hipStream_t stream);
+int sieve_occupancy(int threads, int wpt, size_t lds_bytes);
+bool sieve_shape_ok(int threads, int wpt);
+hipError_t launch_round_test(const RoundArgs& a, uint32_t gy,
+                             hipStream_t stream);
+hipError_t launch_round_compact(const CompactArgs& a, uint32_t grid,
+                                hipStream_t stream);
+hipError_t launch_classify(const ClassifyArgs& a, uint32_t grid,
+                           hipStream_t stream);
+hipError_t launch_classify_values(const DevEngine* E, const uint32_t* vals,
+                                  uint32_t n, int mode, int cap,
+                                  int exact_from, int32_t* out,
+                                  hipStream_t stream);
+hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t stream);
+
+}  // namespace fk

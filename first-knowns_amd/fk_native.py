@@ -1,0 +1,303 @@
+"""ctypes binding of libfk_amd.so (include/fk_engine.h).
+
+This module is the only place Python touches the C ABI.  It refuses to run
+without the built library: there is no CPU fallback anywhere in the
+product path, and an engine on a machine without a gfx950 device raises
+RuntimeError at construction (FK_E_NODEV), as the reference's GPU engines
+raise when CuPy is missing (dickson-ladders/ladder_gpu.py:431-432,
+primorial-ap/ap_gpu.py:402-403).
+"""
+
+import ctypes
+import os
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libfk_amd.so"
+
+FAMILY_EULER, FAMILY_DICKSON, FAMILY_AP = 1, 2, 3
+CLASSIFY_NONE, CLASSIFY_FULL, CLASSIFY_TWO_PASS = 0, 1, 2
+
+FK_OK, FK_E_ARG, FK_E_RANGE, FK_E_OVERFLOW = 0, 1, 2, 3
+FK_E_DEVICE, FK_E_OOM, FK_E_NODEV, FK_E_BUSY = 4, 5, 6, 7
+ABI_VERSION = 1
+
+# The exported entry points, in the order include/fk_engine.h declares them.
+EXPORTS = ("fk_abi_version", "fk_last_error", "fk_device_count",
+           "fk_engine_create", "fk_engine_destroy", "fk_engine_get_info",
+           "fk_engine_sweep", "fk_engine_classify", "fk_batch_create",
+           "fk_batch_destroy", "fk_batch_submit", "fk_batch_query",
+           "fk_batch_results", "fk_batch_timing", "fk_engine_batch")
+
+u32, i32, u64 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
+
+
+class EngineDesc(ctypes.Structure):
+    _fields_ = [("family", i32), ("n", i32), ("q2", u32), ("device", i32),
+                ("nwheel", i32), ("wheel_primes", u32 * 12),
+                ("q_pat", u32), ("q_med", u32), ("threads", u32),
+                ("words_per_thread", u32), ("round_chunk", u32),
+                ("round_growth", u32)]
+
+
+class EngineInfo(ctypes.Structure):
+    _fields_ = [("family", i32), ("n", i32), ("q2", u32), ("nwheel", u32),
+                ("wheel_primes", u32 * 16), ("line_step", u64),
+                ("nlines", u64), ("W", u64), ("d", u64 * 4),
+                ("q_pat", u32), ("q_med", u32), ("threads", u32),
+                ("words_per_thread", u32), ("n_lds_primes", u32),
+                ("n_pattern_primes", u32), ("n_round_primes", u32),
+                ("n_rounds", u32), ("device_bytes", u64),
+                ("survival_lds", ctypes.c_double)]
+
+
+class Window(ctypes.Structure):
+    _fields_ = [("lo", u64 * 2), ("hi", u64 * 2)]
+
+
+class ClassifySpec(ctypes.Structure):
+    _fields_ = [("mode", i32), ("cap", i32), ("exact_from", i32)]
+
+
+RECORD_DTYPE = np.dtype([("v0", "<u8"), ("v1", "<u8"), ("run", "<i4"),
+                         ("window", "<u4")])
+
+_lib = None
+
+
+class FkError(RuntimeError):
+    """A RuntimeError from the library (device, memory, overflow)."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def lib():
+    """The loaded library; raises if it was never built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__"
+            f" as g; g.build()'` (hipcc --offload-arch=gfx950) first")
+    L = ctypes.CDLL(str(LIB_PATH))
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    P = ctypes.POINTER
+    L.fk_abi_version.restype = ctypes.c_int
+    L.fk_last_error.argtypes = [ctypes.c_char_p, sz]
+    L.fk_device_count.argtypes = [P(ctypes.c_int)]
+    L.fk_engine_create.argtypes = [P(EngineDesc), P(vp)]
+    L.fk_engine_destroy.argtypes = [vp]
+    L.fk_engine_get_info.argtypes = [vp, P(EngineInfo)]
+    L.fk_engine_sweep.argtypes = [vp, P(Window), P(ClassifySpec), vp, sz,
+                                  P(sz)]
+    L.fk_engine_classify.argtypes = [vp, vp, sz, P(ClassifySpec), vp]
+    L.fk_batch_create.argtypes = [ctypes.c_int, P(vp)]
+    L.fk_batch_destroy.argtypes = [vp]
+    L.fk_batch_submit.argtypes = [vp, vp, sz, vp, vp, vp, sz]
+    L.fk_batch_query.argtypes = [vp, P(ctypes.c_int)]
+    L.fk_batch_results.argtypes = [vp, vp, sz, P(sz), vp]
+    L.fk_batch_timing.argtypes = [vp, vp, P(u64), P(u64)]
+    L.fk_engine_batch.argtypes = [vp, P(vp)]
+    for name in EXPORTS:
+        getattr(L, name).restype = ctypes.c_int
+    if L.fk_abi_version() != ABI_VERSION:
+        raise RuntimeError("libfk_amd.so ABI version mismatch")
+    _lib = L
+    return L
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    lib().fk_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(rc):
+    """Map a status code to the reference's exception taxonomy."""
+    if rc == FK_OK:
+        return
+    msg = last_error()
+    if rc in (FK_E_ARG, FK_E_RANGE):
+        raise ValueError(msg)
+    raise FkError(rc, msg)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(lib().fk_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def split128(v):
+    v = int(v)
+    if v < 0 or v >> 128:
+        raise ValueError(f"value {v} outside [0, 2^128)")
+    return v & 0xFFFFFFFFFFFFFFFF, v >> 64
+
+
+def make_window(lo, hi):
+    w = Window()
+    w.lo[0], w.lo[1] = split128(lo)
+    w.hi[0], w.hi[1] = split128(hi)
+    return w
+
+
+def record_values(recs):
+    """Record values as Python ints (exact past 2^64)."""
+    if recs.size == 0:
+        return []
+    if not recs["v1"].any():
+        return [int(x) for x in recs["v0"].tolist()]
+    return [int(a) | (int(b) << 64)
+            for a, b in zip(recs["v0"].tolist(), recs["v1"].tolist())]
+
+
+class Engine:
+    """One family's resident device engine (fk_engine)."""
+
+    def __init__(self, family, n, q2, device=0, nwheel=0, wheel_primes=None,
+                 q_pat=0, q_med=0, threads=0, words_per_thread=0,
+                 round_chunk=0, round_growth=0):
+        d = EngineDesc()
+        d.family, d.n, d.q2, d.device = int(family), int(n), int(q2), device
+        d.nwheel = int(nwheel)
+        for i, q in enumerate(wheel_primes or []):
+            d.wheel_primes[i] = int(q)
+        d.q_pat, d.q_med = int(q_pat), int(q_med)
+        d.threads, d.words_per_thread = int(threads), int(words_per_thread)
+        d.round_chunk, d.round_growth = int(round_chunk), int(round_growth)
+        h = ctypes.c_void_p()
+        check(lib().fk_engine_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self.desc = d
+        info = EngineInfo()
+        check(lib().fk_engine_get_info(h, ctypes.byref(info)))
+        self.info = info
+        self.family, self.n, self.q2 = int(family), int(n), int(q2)
+        self.line_step = int(info.line_step)
+        self.nlines = int(info.nlines)
+        self.wheel = [int(info.wheel_primes[i]) for i in range(info.nwheel)]
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fk_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sweep(self, lo, hi, mode=CLASSIFY_NONE, cap=0, exact_from=0):
+        """Sorted records (numpy RECORD_DTYPE) of the survivors in [lo, hi)."""
+        w = make_window(lo, hi)
+        c = ClassifySpec(int(mode), int(cap), int(exact_from))
+        n = ctypes.c_size_t(0)
+        cap_out = 1 << 12
+        out = np.empty(cap_out, dtype=RECORD_DTYPE)
+        rc = lib().fk_engine_sweep(self._h, ctypes.byref(w), ctypes.byref(c),
+                                   out.ctypes.data, cap_out, ctypes.byref(n))
+        if rc == FK_E_OVERFLOW:
+            out = np.empty(n.value, dtype=RECORD_DTYPE)
+            b = ctypes.c_void_p()
+            check(lib().fk_engine_batch(self._h, ctypes.byref(b)))
+            rc = lib().fk_batch_results(b, out.ctypes.data, n.value,
+                                        ctypes.byref(n), None)
+        check(rc)
+        return out[:n.value].copy()
+
+    def classify(self, values, mode=CLASSIFY_FULL, cap=64, exact_from=0):
+        """Chain lengths of line values (Python ints), computed on device."""
+        vals = list(values)
+        if not vals:
+            return []
+        arr = np.empty(2 * len(vals), dtype=np.uint64)
+        for i, v in enumerate(vals):
+            arr[2 * i], arr[2 * i + 1] = split128(v)
+        runs = np.empty(len(vals), dtype=np.int32)
+        c = ClassifySpec(int(mode), int(cap), int(exact_from))
+        check(lib().fk_engine_classify(self._h, arr.ctypes.data, len(vals),
+                                       ctypes.byref(c), runs.ctypes.data))
+        return [int(r) for r in runs.tolist()]
+
+    def last_timing(self):
+        """(ms[prep, sieve, rounds, classify, whole], tiles, candidates) of
+        the engine's last sweep."""
+        b = ctypes.c_void_p()
+        check(lib().fk_engine_batch(self._h, ctypes.byref(b)))
+        return batch_timing(b)
+
+
+def batch_timing(handle):
+    ms = (ctypes.c_float * 5)()
+    tiles, cand = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    check(lib().fk_batch_timing(handle, ms, ctypes.byref(tiles),
+                                ctypes.byref(cand)))
+    return list(ms), int(tiles.value), int(cand.value)
+
+
+class Batch:
+    """An fk_batch: many windows of many engines, submitted asynchronously."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().fk_batch_create(int(device), ctypes.byref(h)))
+        self._h = h
+        self._nwin = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fk_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, engines, engine_of, windows, specs):
+        """engines: list[Engine]; engine_of: engine index per window;
+        windows: list of (lo, hi); specs: list of (mode, cap, exact_from)."""
+        ne = len(engines)
+        eh = (ctypes.c_void_p * ne)(*[e.handle.value for e in engines])
+        nw = len(windows)
+        eo = (u32 * nw)(*[int(i) for i in engine_of])
+        ws = (Window * nw)(*[make_window(lo, hi) for lo, hi in windows])
+        cs = (ClassifySpec * nw)(*[ClassifySpec(int(m), int(c), int(x))
+                                   for m, c, x in specs])
+        self._keep = (eh, eo, ws, cs)
+        check(lib().fk_batch_submit(self._h, eh, ne, eo, ws, cs, nw))
+        self._nwin = nw
+
+    def done(self):
+        d = ctypes.c_int(0)
+        check(lib().fk_batch_query(self._h, ctypes.byref(d)))
+        return bool(d.value)
+
+    def results(self):
+        """(records sorted by (window, value), per-window counts)."""
+        n = ctypes.c_size_t(0)
+        counts = np.zeros(max(self._nwin, 1), dtype=np.uint64)
+        out = np.empty(1 << 12, dtype=RECORD_DTYPE)
+        rc = lib().fk_batch_results(self._h, out.ctypes.data, out.size,
+                                    ctypes.byref(n), counts.ctypes.data)
+        if rc == FK_E_OVERFLOW:
+            out = np.empty(n.value, dtype=RECORD_DTYPE)
+            rc = lib().fk_batch_results(self._h, out.ctypes.data, out.size,
+                                        ctypes.byref(n), counts.ctypes.data)
+        check(rc)
+        return out[:n.value].copy(), counts[:self._nwin]
+
+    def timing(self):
+        return batch_timing(self._h)

@@ -1,0 +1,212 @@
+/*
+ * fk_engine.h -- C ABI of the MI355X (gfx950) sieve-and-test engine.
+ *
+ * One shared library (libfk_amd.so) behind plain pointers and sizes.  The
+ * Python drop-ins (euler_gpu.py, ladder_gpu.py, ap_gpu.py in
+ * first-knowns_amd/) bind it with ctypes and keep the reference's engine
+ * API on top of it.
+ *
+ * Conventions
+ *   - Every function returns an fk_status; on failure fk_last_error()
+ *     returns a message for the calling thread.  FK_E_RANGE and FK_E_ARG
+ *     are the reference's ValueError cases, FK_E_OVERFLOW / FK_E_DEVICE /
+ *     FK_E_OOM / FK_E_NODEV its RuntimeError cases.  Overflow is always an
+ *     error, never silent loss.
+ *   - Values wider than 64 bits travel as little-endian uint64_t pairs.
+ *   - "Line value" means the family's search line: Euler p, Dickson j
+ *     (k = W j), AP p.
+ *   - An fk_engine handle is not thread-safe; distinct handles may be used
+ *     from distinct threads.  An fk_batch owns one HIP stream.
+ *   - The caller owns every output buffer; the library owns device state.
+ */
+#ifndef FK_ENGINE_H
+#define FK_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FK_ABI_VERSION 1
+
+enum fk_family {
+    FK_FAMILY_EULER = 1,    /* x^2 + x + p prime for x = 0..n-1 */
+    FK_FAMILY_DICKSON = 2,  /* m k^2 + 1 prime for m = 1..n, k = W(n) j */
+    FK_FAMILY_AP = 3        /* p + i P(n) prime for i = 0..n-1 */
+};
+
+enum fk_status {
+    FK_OK = 0,
+    FK_E_ARG = 1,       /* bad parameter */
+    FK_E_RANGE = 2,     /* window outside the engine's floor/ceiling */
+    FK_E_OVERFLOW = 3,  /* an output buffer is too small */
+    FK_E_DEVICE = 4,    /* a HIP call failed */
+    FK_E_OOM = 5,       /* device or host allocation failed */
+    FK_E_NODEV = 6,     /* no gfx950 device visible */
+    FK_E_BUSY = 7       /* the batch still has work in flight */
+};
+
+/* Classification of each survivor (the record's `run`). */
+enum fk_classify {
+    FK_CLASSIFY_NONE = 0,     /* run = -1 */
+    FK_CLASSIFY_FULL = 1,     /* chain length with all seven bases */
+    FK_CLASSIFY_TWO_PASS = 2  /* base-2 chain; redone with all bases when it
+                                 reaches exact_from */
+};
+
+typedef struct fk_engine fk_engine;
+typedef struct fk_batch fk_batch;
+
+typedef struct fk_engine_desc {
+    int32_t family;          /* fk_family */
+    int32_t n;               /* depth: values per candidate */
+    uint32_t q2;             /* sieve every prime below q2 (<= 2^30) */
+    int32_t device;          /* HIP device ordinal */
+    int32_t nwheel;          /* kill primes enumerated by the wheel (0 = default) */
+    uint32_t wheel_primes[12];  /* explicit wheel (Euler), else zeros */
+    uint32_t q_pat;          /* primes below: 64-bit pattern words (0 = default) */
+    uint32_t q_med;          /* primes below: LDS stride marking; at and above:
+                                per-candidate rounds (0 = default) */
+    uint32_t threads;        /* sieve workgroup size: 256, 512, 1024 (0 = default) */
+    uint32_t words_per_thread; /* tile = 64 * threads * this bits (0 = default) */
+    uint32_t round_chunk;    /* primes per round work item (0 = default) */
+    uint32_t round_growth;   /* prime bound grows by this factor per round
+                                (0 = default 2) */
+} fk_engine_desc;
+
+typedef struct fk_engine_info {
+    int32_t family, n;
+    uint32_t q2;
+    uint32_t nwheel;
+    uint32_t wheel_primes[16];
+    uint64_t line_step;      /* D: period of the wheel on the search line */
+    uint64_t nlines;         /* admissible residues of the wheel */
+    uint64_t W;              /* Dickson k-wheel W(n); 1 otherwise */
+    uint64_t d[4];           /* AP difference P(n) (256-bit LE); 0 otherwise */
+    uint32_t q_pat, q_med, threads, words_per_thread;
+    uint32_t n_lds_primes, n_pattern_primes, n_round_primes, n_rounds;
+    uint64_t device_bytes;   /* device memory the engine holds */
+    double survival_lds;     /* expected fraction of wheel candidates that
+                                survive the LDS stage */
+} fk_engine_info;
+
+typedef struct fk_window {   /* [lo, hi) on the family's search line */
+    uint64_t lo[2];
+    uint64_t hi[2];
+} fk_window;
+
+typedef struct fk_record {
+    uint64_t value[2];       /* line value of the survivor */
+    int32_t run;             /* classification (-1 when not classified) */
+    uint32_t window;         /* index of its window in the call */
+} fk_record;
+
+typedef struct fk_classify_spec {
+    int32_t mode;            /* fk_classify */
+    int32_t cap;             /* longest chain looked at */
+    int32_t exact_from;      /* TWO_PASS threshold */
+} fk_classify_spec;
+
+/* ABI version compiled into the library (FK_ABI_VERSION). */
+int fk_abi_version(void);
+
+/* Copies the calling thread's last error message (NUL-terminated). */
+int fk_last_error(char* buf, size_t len);
+
+/* Number of HIP devices visible (0 on a machine without one). */
+int fk_device_count(int* count);
+
+/*
+ * Build an engine: tables, wheel, device buffers.  Replaces the
+ * constructors euler-prime-runs/euler_gpu.py:1010 (GpuEngine(n,
+ * wheel_primes)), dickson-ladders/ladder_gpu.py:427-430 (GpuEngine(n, q2,
+ * ...)) and primorial-ap/ap_gpu.py:401 (GpuEngine(n, q2, launch_u)).
+ */
+int fk_engine_create(const fk_engine_desc* desc, fk_engine** out);
+
+int fk_engine_destroy(fk_engine* eng);
+
+/*
+ * Engine facts the callers' cursors count in (period, wheel, W, P(n)).
+ * Replaces the attributes eng.M / eng.wheel_primes
+ * (euler-prime-runs/launch.py:191-218), eng.W / eng.q2
+ * (dickson-ladders/launch.py:491-497) and eng.floor
+ * (primorial-ap/launch.py:182-187, :271-273).
+ */
+int fk_engine_get_info(const fk_engine* eng, fk_engine_info* info);
+
+/*
+ * Sweep one window: every survivor of the sieve in [lo, hi), ascending,
+ * optionally classified.  *n_out is the survivor count even when it
+ * exceeds out_cap (then FK_E_OVERFLOW is returned and nothing is lost:
+ * call again with room).  Replaces the range-sweep calls
+ * euler-prime-runs/euler_gpu.py:1335 (survivors_pre_mr),
+ * dickson-ladders/ladder_gpu.py:757-810 (survivors_j, survivors_j_deep,
+ * survivors_pre_mr) and primorial-ap/ap_gpu.py:555-579 (survivors).
+ */
+int fk_engine_sweep(fk_engine* eng, const fk_window* win,
+                    const fk_classify_spec* cls, fk_record* out,
+                    size_t out_cap, size_t* n_out);
+
+/*
+ * Chain classification of caller-given values on the device: Euler p
+ * (x^2 + x + p, x = 0..), Dickson k (m k^2 + 1, m = 1..), AP p (p + i P(n),
+ * i = 0..).  Replaces euler_search.py:258-262 (mr_run_length),
+ * dickson-ladders/ladder_gpu.py:813-817 (run_length),
+ * dickson-ladders/launch.py:310-352 (sprp_run, _classify_chunk) and
+ * primorial-ap/ap_search.py:160-194 (chain_depth).  values: 2 uint64_t per
+ * value; runs: one int32 per value.
+ */
+int fk_engine_classify(fk_engine* eng, const uint64_t* values, size_t n,
+                       const fk_classify_spec* cls, int32_t* runs);
+
+/*
+ * Batched engine: one stream, device queues sized for many windows of
+ * many engines at once.  Every window's tiles go into one persistent
+ * sieve launch, the rounds and the classification run once over the
+ * merged queue, and results come back in one copy.  No reference
+ * counterpart: the reference runs one window per engine call.
+ */
+int fk_batch_create(int device, fk_batch** out);
+int fk_batch_destroy(fk_batch* b);
+
+/*
+ * Enqueue windows (window i runs on engines[engine_of[i]] with cls[i]) and
+ * return without waiting.  Tuning (threads, words_per_thread) comes from
+ * engines[0]; FK_E_BUSY if the batch has unfinished work.
+ */
+int fk_batch_submit(fk_batch* b, fk_engine* const* engines, size_t nengines,
+                    const uint32_t* engine_of, const fk_window* windows,
+                    const fk_classify_spec* cls, size_t nwin);
+
+/* *done = 1 when the submitted work has finished on the device. */
+int fk_batch_query(fk_batch* b, int* done);
+
+/*
+ * Wait for the submitted work and copy the records out, sorted by
+ * (window, value).  counts (optional, nwin entries) receives per-window
+ * survivor counts.  *n_out = total records; FK_E_OVERFLOW if out_cap is
+ * too small (the results stay available for another call).
+ */
+int fk_batch_results(fk_batch* b, fk_record* out, size_t out_cap,
+                     size_t* n_out, uint64_t* counts);
+
+/*
+ * Milliseconds the device spent on each stage of the last completed batch
+ * or sweep (HIP events on the batch's stream): [0] prep, [1] sieve,
+ * [2] rounds, [3] classify, [4] whole.  Also reports the tiles and
+ * candidates of the sieve stage for roofline accounting.
+ */
+int fk_batch_timing(fk_batch* b, float* ms5, uint64_t* tiles,
+                    uint64_t* candidates);
+
+/* The engine's own internal batch, for timing its last sweep. */
+int fk_engine_batch(fk_engine* eng, fk_batch** out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FK_ENGINE_H */

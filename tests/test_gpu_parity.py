@@ -1,0 +1,214 @@
+"""GPU parity: the HIP engine's survivor streams and classifications equal
+the oracle's, bit for bit, on populated windows of every family, across
+heights, launch cuts and tile shapes.  Every comparison asserts a minimum
+population (an empty-vs-empty comparison proves nothing)."""
+import random
+import sys
+
+import numpy as np
+import pytest
+
+import fk_oracle as O
+import fk_native as N
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+# ------------------------------------------------------------- Dickson
+
+DICKSON_WINDOWS = [
+    # (n, j_lo, span, q2, min population): populations measured with the
+    # oracle; the deep-sieve windows keep rounds busy, the shallow ones
+    # keep the LDS stage populated
+    (10, 10**6, 4 * 10**7, 4096, 150),
+    (10, 4 * 10**9, 4 * 10**7, 8192, 60),
+    (10, 10**12, 1 << 27, 65536, 30),
+    (12, 6 * 10**11, 2 * 10**7, 2048, 90),
+    (13, 4 * 10**18 - 4 * 10**7, 4 * 10**7, 2048, 40),
+    (10, 10**12, 1 << 24, 4096, 60),
+    (6, 10**5, 300_000, 300, 1500),
+]
+
+
+@pytest.mark.parametrize("case", DICKSON_WINDOWS)
+def test_dickson_stream_matches_oracle(case):
+    import ladder_gpu
+    n, j_lo, span, q2, need = case
+    say("dickson", case)
+    eng = ladder_gpu.GpuEngine(n, q2=q2)
+    got = eng.survivors_j(j_lo, j_lo + span)
+    want = O.dickson_survivors(n, j_lo, j_lo + span, q2=q2)
+    assert len(want) >= need
+    assert got.dtype == np.uint64
+    assert got.tolist() == want
+
+
+def test_dickson_deep_zone_past_u64():
+    import ladder_gpu
+    n, q2 = 10, 2048
+    eng = ladder_gpu.GpuEngine(n, q2=q2)
+    j_lo = eng.J_REACH - 3 * 10**6
+    say("dickson deep", j_lo)
+    got = eng.survivors_j_deep(j_lo, eng.J_REACH)
+    want = O.dickson_survivors(n, j_lo, eng.J_REACH, q2=q2)
+    assert j_lo > 1 << 64 and len(want) >= 20
+    assert got == want
+    with pytest.raises(ValueError):
+        eng.survivors_j_deep(j_lo, eng.J_REACH + 1)
+    with pytest.raises(ValueError):
+        eng.survivors_j(j_lo, eng.J_REACH)
+
+
+def test_dickson_k_line_view_is_inclusive():
+    import ladder_gpu
+    eng = ladder_gpu.GpuEngine(10, q2=4096)
+    j = int(eng.survivors_j(10**9, 10**9 + 10**6)[3])
+    k = j * eng.W
+    say("dickson k-line", k)
+    assert j in eng.survivors_pre_mr(k, k).tolist()
+    assert j in eng.survivors_pre_mr(k - eng.W + 1, k).tolist()
+    assert j not in eng.survivors_pre_mr(k + 1, k + eng.W - 1).tolist()
+
+
+def test_dickson_runs_match_oracle():
+    import ladder_gpu
+    n, q2 = 10, 2048
+    eng = ladder_gpu.GpuEngine(n, q2=q2)
+    j_lo, span = 10**12, 1 << 25
+    say("dickson runs")
+    full = eng.sweep_classified(j_lo, j_lo + span, cap=n + 8,
+                                two_pass=False)
+    two = eng.sweep_classified(j_lo, j_lo + span, cap=n + 8, two_pass=True)
+    assert [j for j, _ in full] == [j for j, _ in two]
+    assert len(full) > 300
+    rng = random.Random(11)
+    sample = rng.sample(range(len(full)), 200)
+    for i in sample:
+        j, r = full[i]
+        k = j * eng.W
+        assert r == O.dickson_run(k, cap=n + 8), (j, r)
+        assert two[i][1] == O.dickson_sprp_run(k, n + 8), j
+    # the deep runs in the window are all exact
+    for j, r in full:
+        if r >= 4:
+            assert r == O.dickson_run(j * eng.W, cap=n + 8)
+
+
+def test_dickson_evidence_runs_on_device(golden):
+    import ladder_gpu
+    for rec in golden["evidence"]["dickson"]:
+        k, run = int(rec["k"]), rec["run"]
+        eng = ladder_gpu.GpuEngine(10, q2=1024)
+        say("dickson evidence", k)
+        assert eng._eng.classify([k], N.CLASSIFY_FULL, run + 8)[0] == run
+
+
+# --------------------------------------------------------------- Euler
+
+EULER_WINDOWS = [
+    # (n, lo, hi, wheel, min population)
+    (5, 10**5, 4 * 10**5, None, 20),
+    (5, 10**5, 4 * 10**5, [2, 3, 5, 7, 11, 13], 20),
+    (9, 10**6, 6 * 10**7, None, 2),
+    (7, 10**15, 10**15 + 10**9, [2, 3, 5, 7, 11, 13, 17], 20),
+    (6, 2**64 - 10**8, 2**64 + 10**8, [2, 3, 5, 7, 11, 13], 20),
+    (6, 10**24 - 2 * 10**8, 10**24, [2, 3, 5, 7, 11, 13], 20),
+]
+
+
+@pytest.mark.parametrize("case", EULER_WINDOWS)
+def test_euler_stream_matches_oracle(case):
+    import euler_gpu
+    n, lo, hi, wheel, need = case
+    say("euler", case)
+    eng = euler_gpu.GpuEngine(n, wheel_primes=wheel)
+    got = eng.survivors_pre_mr(lo, hi)
+    want = O.euler_survivors(n, lo, hi,
+                             wheel_primes=[2, 3, 5, 7, 11, 13, 17, 19, 23])
+    assert len(want) >= need
+    assert got == want
+
+
+def test_euler_canary_a13_first_occurrence(golden):
+    import euler_gpu
+    target = int(golden["euler_reference"]["KNOWN"]["value"]["13"])
+    say("euler a(13)")
+    hits = euler_gpu.GpuEngine(13).hunt(euler_gpu.P_FLOOR, target + 1000)
+    firsts = [p for p, r in hits if r == 13]
+    assert firsts and firsts[0] == target
+
+
+def test_euler_runs_match_oracle(golden):
+    import euler_gpu
+    eng = euler_gpu.GpuEngine(6, wheel_primes=[2, 3, 5, 7, 11, 13])
+    lo = 10**12
+    say("euler runs")
+    recs = eng._eng.sweep(lo, lo + 10**9, N.CLASSIFY_FULL, 11)
+    vals = N.record_values(recs)
+    assert len(vals) > 100
+    for p, r in zip(vals[:300], recs["run"].tolist()[:300]):
+        assert r == O.euler_run(p, cap=11), p
+    for rec in golden["evidence"]["euler"]:
+        p = int(rec["p"])
+        assert eng.mr_run_length(p, cap=40) == rec["run"]
+
+
+# ------------------------------------------------------------------ AP
+
+def test_ap_gate_windows(golden):
+    import ap_gpu
+    for w in golden["ap_gate_windows"]:
+        base, span = int(w["base"]), int(w["span"])
+        say("ap", w["name"])
+        eng = ap_gpu.GpuEngine(w["n"], w["q2"], launch_u=1 << 16)
+        got = np.concatenate([c for c in eng.survivors(base, span)] or
+                             [np.zeros(0, np.uint64)])
+        want = O.ap_survivors(w["n"], base, span, q2=w["q2"])
+        assert len(want) == w["count"]
+        assert got.tolist() == want
+
+
+def test_ap_depths_match_oracle(golden):
+    import ap_gpu
+    n, q2 = 8, 4096
+    eng = ap_gpu.GpuEngine(n, q2)
+    base, span = 4 * 10**13 - 10, 3 * 10**6
+    say("ap depths")
+    hits = eng.hunt(base, span, min_depth=0)
+    assert len(hits) > 100
+    for p, dep in hits:
+        assert dep == O.ap_chain_depth(p, n), p
+    for rec in golden["evidence"]["ap"]:
+        e = ap_gpu.GpuEngine(rec["n"], 2048)
+        p = int(rec["p"])
+        assert e.chain_depth(p, cap=40) == O.ap_chain_depth(p, rec["n"],
+                                                            cap=40)
+
+
+# ------------------------------------------------------- slicing
+
+def test_streams_do_not_depend_on_cuts_or_tiles():
+    import ladder_gpu
+    n, q2, lo, hi = 10, 4096, 10**12 + 7, 10**12 + 7 + 30_000_017
+    say("slicing")
+    ref = ladder_gpu.GpuEngine(n, q2=q2).survivors_j(lo, hi).tolist()
+    assert len(ref) > 100
+    for kw in ({"threads": 512, "words_per_thread": 4},
+               {"threads": 1024, "words_per_thread": 1},
+               {"threads": 256, "words_per_thread": 16},
+               {"q_med": 1024, "q_pat": 64},
+               {"q_med": 16384, "q_pat": 256},
+               {"nwheel": 1}, {"nwheel": 2},
+               {"round_growth": 4, "round_chunk": 16}):
+        eng = ladder_gpu.GpuEngine(n, q2=q2, **kw)
+        assert eng.survivors_j(lo, hi).tolist() == ref, kw
+    eng = ladder_gpu.GpuEngine(n, q2=q2)
+    for cut in (lo + 1, lo + 12_345_678, hi - 1):
+        a = eng.survivors_j(lo, cut).tolist()
+        b = eng.survivors_j(cut, hi).tolist()
+        assert a + b == ref, cut
