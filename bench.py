@@ -1,0 +1,277 @@
+#!/usr/bin/env python3
+"""Benchmark: candidates swept per second at a bit-exact survivor stream.
+
+Headline workload (one step = one seeded window): the plain GPU engine on
+the Dickson ladder sweep, n = 10, every prime below 2^22 sieved, a window of
+2^32 consecutive j near k = 2310 j = 1e15, and every survivor classified on
+the device (its full seven-base ladder depth, cap 18).  value = j swept per
+second over the timed steps, inputs resident on the device (the window is
+two integers; nothing is copied in per step).
+
+Beside it, in the same JSON line:
+  roofline      the LDS tile-sieve kernel (k_sieve), timed with HIP events
+                on its own stream inside the timed region
+  cpu_baseline  the oracle's restatement of the reference CPU sieve
+                (dickson-ladders/ladder_search.py:92-112) on a bounded sample
+                of the same window on this host's cores, plus the
+                classification of its survivors, with the sample's survivor
+                stream checked equal to the GPU's
+  batched       256 mixed windows of the three families (lengths 2^20..2^30,
+                n 8..14), one persistent batched launch vs the plain engine
+                window by window, with the two streams checked equal
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+import argparse
+import json
+import math
+import os
+import pathlib
+import random
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "first-knowns_amd"))
+
+METRIC = ("candidates swept/sec at bit-exact survivor stream; "
+          "batched vs plain GPU engine")
+LDS_PEAK = 256 * 128 * 2.4e9          # B/s: 256 CUs x 128 B/clk x 2.4 GHz
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64(x):
+    x = (x + 0x9E3779B97F4A7C15) & MASK64
+    z = x
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+    return z ^ (z >> 31)
+
+
+# ------------------------------------------------------------- config 1
+
+C1_N, C1_Q2, C1_SPAN, C1_CAP = 10, 1 << 22, 1 << 32, 18
+
+
+def c1_window(seed):
+    j_lo = 432_900_432_900 + splitmix64(seed) % 10**11
+    return j_lo, j_lo + C1_SPAN
+
+
+def run_plain(steps, warmup):
+    import fk_native as N
+    import ladder_gpu
+    eng = ladder_gpu.GpuEngine(C1_N, q2=C1_Q2)
+    for s in range(warmup):
+        lo, hi = c1_window(1000 + s)
+        eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, C1_CAP)
+    surv, sieve_ms, whole_ms, cand = 0, 0.0, 0.0, 0
+    stage = [0.0] * 5
+    deepest = (0, None)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        lo, hi = c1_window(s)
+        recs = eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, C1_CAP)
+        ms, tiles, c = eng._eng.last_timing()
+        for i in range(5):
+            stage[i] += ms[i]
+        cand += c
+        surv += int(recs.size)
+        if recs.size and int(recs["run"].max()) > deepest[0]:
+            i = int(recs["run"].argmax())
+            deepest = (int(recs["run"][i]), int(recs["v0"][i]))
+    wall = time.perf_counter() - t0
+    return eng, wall, surv, stage, cand, deepest
+
+
+def cpu_baseline(eng, sample_log2, threads):
+    import fk_native as N
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import fk_oracle as O
+    lo, _ = c1_window(0)
+    hi = lo + (1 << sample_log2)
+    sieve = O.DicksonSieve(C1_N, C1_Q2)            # tables outside the clock
+    t0 = time.perf_counter()
+    js = sieve.survivors(lo, hi, nthreads=threads)
+    runs = [O.dickson_run(j * eng.W, cap=C1_CAP) for j in js]
+    wall = time.perf_counter() - t0
+    recs = eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, C1_CAP)
+    same = (N.record_values(recs) == js and
+            [int(r) for r in recs["run"].tolist()] == runs)
+    return {"value": (hi - lo) / wall, "unit": "j-candidates/s",
+            "cores": threads, "kind": "port",
+            "sample": f"2^{sample_log2} j from the step-0 window, n={C1_N}, "
+                      f"q2=2^22, survivors classified (cap {C1_CAP}); "
+                      f"{len(js)} survivors, {wall:.1f} s",
+            "sample_stream_equal": bool(same)}
+
+
+# ------------------------------------------------------------- config 3
+
+FAMILY_Q2 = {"euler": 1 << 16, "dickson": 1 << 22, "ap": 1 << 24}
+
+
+def c3_windows(seed=3, count=256):
+    """256 windows: family uniform, length 2^(20 + 10 U) on the family's
+    line, n uniform in 8..14, heights spread so the strong tests run on
+    64-, 128- and 192-bit values."""
+    rng = random.Random(seed)
+    out = []
+    for _ in range(count):
+        fam = rng.choice(["euler", "dickson", "ap"])
+        n = rng.randint(8, 14)
+        length = int(2 ** (20 + 10 * rng.random()))
+        wide = rng.random() < 0.5
+        if fam == "euler":
+            lo = (rng.randrange(10**12, 10**13) if not wide
+                  else rng.randrange(10**20, 10**22))
+        elif fam == "dickson":
+            W = 30030 if n >= 12 else 2310
+            k = (rng.randrange(10**14, 10**16) if not wide
+                 else rng.randrange(10**21, 10**22))
+            lo = k // W
+        else:
+            lo = (rng.randrange(10**15, 10**16) if not wide
+                  else rng.randrange(10**19, 10**20))
+            lo -= lo % 30
+            length -= length % 30
+        out.append((fam, n, lo, lo + length))
+    return out
+
+
+def c3_spec(fam, n):
+    import fk_native as N
+    if fam == "euler":
+        return (N.CLASSIFY_FULL, n + 3, 0)
+    if fam == "dickson":
+        return (N.CLASSIFY_FULL, n + 8, 0)
+    return (N.CLASSIFY_TWO_PASS, n, 6)
+
+
+def run_batched():
+    import fk_native as N
+    import fk_families as F
+    wins = c3_windows()
+    engines, index = [], {}
+    fam_id = {"euler": N.FAMILY_EULER, "dickson": N.FAMILY_DICKSON,
+              "ap": N.FAMILY_AP}
+    for fam, n, lo, hi in wins:
+        if (fam, n) not in index:
+            t = F.tuning(fam)
+            kw = {}
+            if fam == "euler":
+                kw["wheel_primes"] = [2, 3, 5, 7, 11, 13]
+            index[(fam, n)] = len(engines)
+            engines.append(N.Engine(fam_id[fam], n, FAMILY_Q2[fam], **kw,
+                                    **t))
+    eo = [index[(f, n)] for f, n, _, _ in wins]
+    ws = [(lo, hi) for _, _, lo, hi in wins]
+    specs = [c3_spec(f, n) for f, n, _, _ in wins]
+    cand = sum(hi - lo for lo, hi in ws)
+    # plain: one engine call per window
+    for i in range(3):
+        engines[eo[i]].sweep(*ws[i], *specs[i])
+    t0 = time.perf_counter()
+    plain = [engines[eo[i]].sweep(*ws[i], *specs[i]) for i in range(len(ws))]
+    t_plain = time.perf_counter() - t0
+    # batched: one submission
+    b = N.Batch()
+    b.submit(engines, eo, ws, specs)
+    b.results()
+    t0 = time.perf_counter()
+    b.submit(engines, eo, ws, specs)
+    recs, counts = b.results()
+    t_batch = time.perf_counter() - t0
+    same = True
+    pos = 0
+    for i, p in enumerate(plain):
+        mine = recs[pos:pos + int(counts[i])]
+        pos += int(counts[i])
+        if (N.record_values(mine) != N.record_values(p) or
+                mine["run"].tolist() != p["run"].tolist()):
+            same = False
+    return {"windows": len(ws), "engines": len(engines),
+            "candidates": cand, "survivors": int(recs.size),
+            "plain_s": t_plain, "batched_s": t_batch,
+            "plain_rate": cand / t_plain, "batched_rate": cand / t_batch,
+            "speedup": t_plain / t_batch, "streams_equal": same}
+
+
+def rocprof_traffic():
+    """HBM bytes per k_sieve dispatch from the committed PMC summary."""
+    p = ROOT / "profiles" / "k_sieve_dickson_c1_traffic.json"
+    if p.exists():
+        try:
+            return json.loads(p.read_text()).get("bytes_per_dispatch")
+        except (OSError, ValueError):
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=12)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-sample-log2", type=int, default=31)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path "
+              f"(windows shard across GPUs by range; see DESIGN.md). "
+              f"--gpus {args.gpus} is not supported.", file=sys.stderr)
+        return 2
+
+    eng, wall, surv, stage, cand, deepest = run_plain(args.steps,
+                                                     args.warmup)
+    j_total = args.steps * C1_SPAN
+    value = j_total / wall
+    info = eng._eng.info
+    # algorithmic LDS bytes of k_sieve: 8 B per pattern word read per
+    # pattern prime, 8 B (read + write) per stride mark
+    per_cand = 8.0 * info.n_pattern_primes / 64 + 8.0 * info.mark_density
+    sieve_s = stage[1] / 1e3
+    achieved = cand * per_cand / sieve_s if sieve_s > 0 else 0.0
+    out = {
+        "metric": METRIC, "value": value, "unit": "j-candidates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "dickson-ladder sweep, plain GPU engine: "
+                               "2^32 consecutive j near k=1e15 per step, "
+                               "n=10, primes < 2^22 sieved, full-bases "
+                               "ladder depth of every survivor",
+                   "n": C1_N, "q2": C1_Q2, "window_j": C1_SPAN,
+                   "classify_cap": C1_CAP,
+                   "wheel_lines": int(info.nlines),
+                   "wheel_period": int(info.line_step),
+                   "q_pat": int(info.q_pat), "q_med": int(info.q_med),
+                   "tile_bits": None},
+        "survivors": surv,
+        "deepest_ladder": {"run": deepest[0], "j": deepest[1]},
+        "stages_ms_per_step": {k: v / args.steps for k, v in
+                               zip(("prep", "sieve", "rounds", "classify",
+                                    "whole"), stage)},
+        "roofline": {"bound": "lds", "kernel": "k_sieve",
+                     "achieved": achieved / 1e9, "peak": LDS_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / LDS_PEAK,
+                     "traffic": rocprof_traffic(),
+                     "algorithmic_bytes_per_candidate": per_cand,
+                     "candidates_per_launch": cand / args.steps},
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(eng, args.cpu_sample_log2,
+                                           threads)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    if not args.no_batched:
+        out["batched"] = run_batched()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -141,6 +141,7 @@ struct fk_engine {
     uint32_t round_first = 0, round_end = 0;
     uint32_t nrounds = 0;
     double surv_lds = 1.0;
+    double mark_density = 0.0;       // sum |K_q| / q over marked primes
     double line_density = 1.0;       // nlines / D
     DevEngine host{};                // with device pointers
     DevEngine* d_engine = nullptr;   // device copy of `host`
@@ -271,6 +272,7 @@ static int build_engine(fk_engine* e)
             e->npat = idx + 1;
         } else {
             for (uint32_t r : kl) pairs.push_back((idx << 16) | r);
+            e->mark_density += (double)kl.size() / q;
         }
     }
     // patterns must be a prefix of the LDS primes
@@ -888,6 +890,7 @@ int fk_engine_get_info(const fk_engine* e, fk_engine_info* info)
     info->n_rounds = e->nrounds;
     info->device_bytes = e->device_bytes;
     info->survival_lds = e->surv_lds;
+    info->mark_density = e->mark_density;
     return FK_OK;
 }
 

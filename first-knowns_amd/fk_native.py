@@ -50,7 +50,8 @@ class EngineInfo(ctypes.Structure):
                 ("words_per_thread", u32), ("n_lds_primes", u32),
                 ("n_pattern_primes", u32), ("n_round_primes", u32),
                 ("n_rounds", u32), ("device_bytes", u64),
-                ("survival_lds", ctypes.c_double)]
+                ("survival_lds", ctypes.c_double),
+                ("mark_density", ctypes.c_double)]
 
 
 class Window(ctypes.Structure):

@@ -90,6 +90,8 @@ typedef struct fk_engine_info {
     uint64_t device_bytes;   /* device memory the engine holds */
     double survival_lds;     /* expected fraction of wheel candidates that
                                 survive the LDS stage */
+    double mark_density;     /* sum over the stride-marked primes of |K_q|/q:
+                                LDS marks per wheel candidate */
 } fk_engine_info;
 
 typedef struct fk_window {   /* [lo, hi) on the family's search line */

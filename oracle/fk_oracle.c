@@ -368,33 +368,62 @@ static uint64_t wheel_modulus(int n)
 }
 
 /*
+ * ladder_search.py:56-69 (__init__: primes and the root table) as a reusable
+ * context, so a timed sweep excludes table construction exactly as the
+ * reference's CpuEngine builds its tables once in __init__.
+ */
+typedef struct {
+    int n;
+    int64_t np;
+    uint32_t *pr, *roots, *rcnt;
+} dickson_ctx;
+
+void *fko_dickson_ctx_create(int n, uint32_t q2)
+{
+    uint64_t W = wheel_modulus(n);
+    int64_t nall = 0;
+    uint32_t *all;
+    dickson_ctx *c;
+    if (n < 1) return NULL;
+    all = primes_below(q2, &nall);
+    if (!all) return NULL;
+    c = calloc(1, sizeof *c);
+    c->n = n;
+    c->pr = malloc((nall + 1) * sizeof(uint32_t));
+    for (int64_t i = 0; i < nall; i++)
+        if (all[i] >= (uint32_t)(n + 2)) c->pr[c->np++] = all[i];
+    free(all);
+    c->roots = malloc((c->np + 1) * 2 * n * sizeof(uint32_t));
+    c->rcnt = malloc((c->np + 1) * sizeof(uint32_t));
+    for (int64_t i = 0; i < c->np; i++)
+        c->rcnt[i] = dickson_forbidden(c->pr[i], n, W, c->roots + i * 2 * n);
+    return c;
+}
+
+void fko_dickson_ctx_free(void *ctx)
+{
+    dickson_ctx *c = ctx;
+    if (!c) return;
+    free(c->pr); free(c->roots); free(c->rcnt); free(c);
+}
+
+/*
  * ladder_search.py:92-112: segmented byte sieve over j, one block at a time,
  * alive[first::q] = False for every killed residue of every prime in
  * [n+2, q2).  Blocks run in parallel; output is the ascending j.
  */
-int64_t fko_dickson_survivors(int n, uint32_t q2,
-                              uint64_t jlo_lo, uint64_t jlo_hi,
-                              uint64_t jhi_lo, uint64_t jhi_hi,
-                              uint64_t *out, int64_t cap, int nthreads,
-                              uint32_t block)
+int64_t fko_dickson_ctx_survivors(void *ctx, uint64_t jlo_lo, uint64_t jlo_hi,
+                                  uint64_t jhi_lo, uint64_t jhi_hi,
+                                  uint64_t *out, int64_t cap, int nthreads,
+                                  uint32_t block)
 {
+    dickson_ctx *c = ctx;
     u128 jlo = mk(jlo_lo, jlo_hi), jhi = mk(jhi_lo, jhi_hi);
-    uint64_t W = wheel_modulus(n);
-    int64_t nall = 0, np = 0, nblocks;
-    uint32_t *all, *pr, *roots, *rcnt;
+    int64_t nblocks;
     vec128 *parts;
-    int fail = 0;
-    if (n < 1 || jhi < jlo || block == 0) return ORC_EARG;
-    all = primes_below(q2, &nall);
-    if (!all) return ORC_ENOMEM;
-    pr = malloc((nall + 1) * sizeof(uint32_t));
-    for (int64_t i = 0; i < nall; i++)
-        if (all[i] >= (uint32_t)(n + 2)) pr[np++] = all[i];
-    free(all);
-    roots = malloc((np + 1) * 2 * n * sizeof(uint32_t));
-    rcnt = malloc((np + 1) * sizeof(uint32_t));
-    for (int64_t i = 0; i < np; i++)
-        rcnt[i] = dickson_forbidden(pr[i], n, W, roots + i * 2 * n);
+    int fail = 0, n;
+    if (!c || jhi < jlo || block == 0) return ORC_EARG;
+    n = c->n;
     nblocks = (int64_t)((jhi - jlo + block - 1) / block);
     parts = calloc(nblocks ? nblocks : 1, sizeof(vec128));
     if (nthreads < 1) nthreads = omp_get_max_threads();
@@ -406,11 +435,11 @@ int64_t fko_dickson_survivors(int n, uint32_t q2,
             u128 j0 = jlo + (u128)b * block;
             uint64_t size = (uint64_t)((jhi - j0) < block ? (jhi - j0) : block);
             memset(alive, 1, size);
-            for (int64_t i = 0; i < np; i++) {
-                uint32_t q = pr[i];
+            for (int64_t i = 0; i < c->np; i++) {
+                uint32_t q = c->pr[i];
                 uint32_t start = (uint32_t)(j0 % q);
-                const uint32_t *t = roots + i * 2 * n;
-                for (uint32_t r = 0; r < rcnt[i]; r++) {
+                const uint32_t *t = c->roots + i * 2 * n;
+                for (uint32_t r = 0; r < c->rcnt[i]; r++) {
                     uint64_t first = (t[r] + q - start) % q;
                     for (uint64_t x = first; x < size; x += q) alive[x] = 0;
                 }
@@ -420,7 +449,6 @@ int64_t fko_dickson_survivors(int n, uint32_t q2,
         }
         free(alive);
     }
-    free(pr); free(roots); free(rcnt);
     if (fail) {
         for (int64_t i = 0; i < nblocks; i++) free(parts[i].v);
         free(parts);
@@ -431,6 +459,21 @@ int64_t fko_dickson_survivors(int n, uint32_t q2,
         free(parts);
         return r;
     }
+}
+
+int64_t fko_dickson_survivors(int n, uint32_t q2,
+                              uint64_t jlo_lo, uint64_t jlo_hi,
+                              uint64_t jhi_lo, uint64_t jhi_hi,
+                              uint64_t *out, int64_t cap, int nthreads,
+                              uint32_t block)
+{
+    void *c = fko_dickson_ctx_create(n, q2);
+    int64_t r;
+    if (!c) return ORC_EARG;
+    r = fko_dickson_ctx_survivors(c, jlo_lo, jlo_hi, jhi_lo, jhi_hi, out,
+                                  cap, nthreads, block);
+    fko_dickson_ctx_free(c);
+    return r;
 }
 
 /* ---------------------------------------------------------------------- AP */

@@ -77,6 +77,13 @@ def lib():
         L.fko_ap_survivors.argtypes = [i32, u32, u64, u64, u64, p64, i64,
                                        i32, u32]
         L.fko_ap_survivors.restype = i64
+        L.fko_dickson_ctx_create.argtypes = [i32, u32]
+        L.fko_dickson_ctx_create.restype = ctypes.c_void_p
+        L.fko_dickson_ctx_free.argtypes = [ctypes.c_void_p]
+        L.fko_dickson_ctx_free.restype = None
+        L.fko_dickson_ctx_survivors.argtypes = [ctypes.c_void_p, u64, u64,
+                                                u64, u64, p64, i64, i32, u32]
+        L.fko_dickson_ctx_survivors.restype = i64
         _lib = L
     return _lib
 
@@ -305,6 +312,30 @@ def dickson_survivors(n, j_lo, j_hi, q2=65536, nthreads=None,
                                            *_split(j_hi), buf, cap,
                                            _threads(nthreads), int(block))
     return _call_growing(call, pairs=True)
+
+
+class DicksonSieve:
+    """ladder_search.CpuEngine's tables built once (ladder_search.py:56-69),
+    then survivors_j (ladder_search.py:92-112) as often as asked."""
+
+    def __init__(self, n, q2=65536):
+        self.n, self.q2 = int(n), int(q2)
+        self._c = lib().fko_dickson_ctx_create(self.n, self.q2)
+        if not self._c:
+            raise RuntimeError("oracle table build failed")
+
+    def survivors(self, j_lo, j_hi, nthreads=None, block=1 << 20):
+        def call(buf, cap):
+            return lib().fko_dickson_ctx_survivors(
+                self._c, *_split(j_lo), *_split(j_hi), buf, cap,
+                _threads(nthreads), int(block))
+        return _call_growing(call, pairs=True)
+
+    def __del__(self):
+        try:
+            lib().fko_dickson_ctx_free(self._c)
+        except Exception:
+            pass
 
 
 def ap_forbidden(q, n):
