@@ -253,7 +253,7 @@ FK_HD void mw_shr(uint32_t* a, int bits)
 // for.  huntlib/primes.py:31-47 (the loop over bases) and :61-80 (base 2
 // alone) are the two uses: nbases = 7 or nbases = 1.
 template <int L>
-FK_HD bool sprp_bases(const uint32_t* m, int nbases)
+FK_HD bool sprp_bases(const uint32_t* m, int first, int nbases)
 {
     const uint32_t minv = neg_inv32(m[0]);
     uint32_t one[L], mone[L], r2[L], d[L];
@@ -278,7 +278,7 @@ FK_HD bool sprp_bases(const uint32_t* m, int nbases)
 #pragma unroll
     for (int i = 1; i < L; i++) one_limb = one_limb && (m[i] == 0);
 
-    for (int b = 0; b < nbases; b++) {
+    for (int b = first; b < first + nbases; b++) {
         // a %= m; a == 0 -> skip (a < 2^31, so only a one-limb m reduces it)
         uint32_t a = mr_base(b);
         if (one_limb) a = a % m[0];
@@ -315,10 +315,13 @@ FK_HD bool sprp_bases(const uint32_t* m, int nbases)
     return true;
 }
 
-// huntlib/primes.py:23-47 (nbases = 7) and :50-80 (nbases = 1) for a value
-// of up to 8 limbs: the small-prime pre-check answers m == sp, then the
-// strong test runs at the narrowest limb count that holds m.
-FK_HD bool is_sprp(const uint32_t* v8, int nbases)
+// huntlib/primes.py:23-47 (bases 0..6) and :50-80 (base 0 alone) for a
+// value of up to 8 limbs: the small-prime pre-check answers m == sp, then
+// the strong test to bases first..first+nbases-1 runs at the narrowest
+// limb count that holds m.  The seven-base verdict is the AND of the
+// single-base verdicts, which is what lets the device test bases in
+// parallel.
+FK_HD bool is_sprp(const uint32_t* v8, int first, int nbases)
 {
     int top = 0;
 #pragma unroll
@@ -329,10 +332,10 @@ FK_HD bool is_sprp(const uint32_t* v8, int nbases)
         const uint32_t sp = small_prime(i);
         if (mw_mod_small<8>(v8, sp) == 0) return top == 0 && v8[0] == sp;
     }
-    if (top < 2) return sprp_bases<2>(v8, nbases);
-    if (top < 4) return sprp_bases<4>(v8, nbases);
-    if (top < 6) return sprp_bases<6>(v8, nbases);
-    return sprp_bases<8>(v8, nbases);
+    if (top < 2) return sprp_bases<2>(v8, first, nbases);
+    if (top < 4) return sprp_bases<4>(v8, first, nbases);
+    if (top < 6) return sprp_bases<6>(v8, first, nbases);
+    return sprp_bases<8>(v8, first, nbases);
 }
 
 // --------------------------------------------- small multiword helpers

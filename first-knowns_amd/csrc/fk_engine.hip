@@ -361,6 +361,8 @@ struct fk_batch {
     uint32_t* d_bq = nullptr;         size_t cap_bq = 0;
     DevRec* d_q[2] = {nullptr, nullptr}; size_t cap_q = 0;
     uint32_t* d_flags = nullptr;
+    uint64_t* d_fail2 = nullptr;      // per record: base-2 failures
+    uint64_t* d_failx = nullptr;      // per record: other-base failures
     uint32_t* d_counts = nullptr;     size_t cap_counts = 0;
     // pinned host mirrors
     uint32_t* h_counts = nullptr;
@@ -384,7 +386,8 @@ static int batch_free(fk_batch* b)
 {
     (void)hipSetDevice(b->device);
     void* ds[] = {b->d_engines, b->d_pieces, b->d_win_engine, b->d_wins,
-                  b->d_bq, b->d_q[0], b->d_q[1], b->d_flags, b->d_counts};
+                  b->d_bq, b->d_q[0], b->d_q[1], b->d_flags, b->d_fail2,
+                  b->d_failx, b->d_counts};
     for (void* p : ds) if (p) (void)hipFree(p);
     if (b->h_counts) (void)hipHostFree(b->h_counts);
     if (b->h_recs) (void)hipHostFree(b->h_recs);
@@ -559,6 +562,14 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         if (b->d_flags) (void)hipFree(b->d_flags);
         b->d_flags = nullptr;
         if ((rc = grow(&b->d_flags, &cf, c))) return rc;
+        for (uint64_t** m : {&b->d_fail2, &b->d_failx}) {
+            size_t cm = 0;
+            if (*m) (void)hipFree(*m);
+            *m = nullptr;
+            if ((rc = grow(m, &cm, c))) return rc;
+            // the classification passes leave these zero after every use
+            HIPCHK(hipMemset(*m, 0, cm * sizeof(uint64_t)));
+        }
         b->cap_q = c;
     }
     if (ncounts > b->cap_counts || !b->d_counts) {
@@ -632,11 +643,11 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         ra.flags = b->d_flags;
         ra.p_begin = prime_index(ctx->primes, rounds[r].first);
         ra.p_end = prime_index(ctx->primes, rounds[r].second);
-        ra.chunk = e0->round_chunk;
+        ra.chunk = 256;
         const uint32_t np = ra.p_end > ra.p_begin ? ra.p_end - ra.p_begin : 0;
-        ra.nchunks = (np + ra.chunk - 1) / ra.chunk;
+        ra.nchunks = (np + 255) / 256;
         if (ra.nchunks == 0) continue;
-        uint32_t gy = (uint32_t)std::min(256.0, std::ceil(alive / 256.0));
+        uint32_t gy = (uint32_t)std::min(4096.0, std::ceil(alive / 16.0));
         HIPCHK(launch_round_test(ra, gy, s));
         CompactArgs ca{};
         ca.in = b->d_q[cur];
@@ -658,7 +669,11 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
 
     // ---- classification
     bool any_cls = false;
-    for (size_t w = 0; w < nwin; w++) any_cls = any_cls || b->cls[w].mode != 0;
+    int cap_max = 1;
+    for (size_t w = 0; w < nwin; w++) {
+        any_cls = any_cls || b->cls[w].mode != 0;
+        if (b->cls[w].mode) cap_max = std::max(cap_max, b->cls[w].cap);
+    }
     if (any_cls) {
         ClassifyArgs ca{};
         ca.engines = b->d_engines;
@@ -666,7 +681,11 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         ca.recs = b->d_q[cur];
         ca.count = b->d_counts + cslot;
         ca.cap = queue_cap;
-        uint32_t g = (uint32_t)std::min(8192.0, std::ceil(alive / 128.0) + 1);
+        ca.cap_max = (uint32_t)cap_max;
+        ca.fail2 = b->d_fail2;
+        ca.failx = b->d_failx;
+        uint32_t g = (uint32_t)std::min(
+            16384.0, std::ceil(alive * cap_max / 128.0) + 4);
         HIPCHK(launch_classify(ca, g, s));
     }
     HIPCHK(hipEventRecord(b->ev[4], s));
@@ -923,31 +942,60 @@ int fk_engine_classify(fk_engine* e, const uint64_t* values, size_t n,
     if (n == 0) return FK_OK;
     if (cls->mode < 1 || cls->mode > 2 || cls->cap < 0 || cls->cap > 64)
         return fail(FK_E_ARG, "bad classification spec");
-    std::vector<uint32_t> limbs(4 * n);
+    if (n > (1u << 26)) return fail(FK_E_ARG, "at most 2^26 values per call");
+    std::vector<DevRec> recs(n);
     for (size_t i = 0; i < n; i++) {
         const u128 v = ((u128)values[2 * i + 1] << 64) | values[2 * i];
-        for (int k = 0; k < 4; k++) limbs[4 * i + k] = (uint32_t)(v >> (32 * k));
+        for (int k = 0; k < 4; k++) recs[i].t[k] = (uint32_t)(v >> (32 * k));
+        recs[i].window = 0;
+        recs[i].run = -1;
     }
+    DevWinInfo win{0, cls->mode, cls->cap, cls->exact_from};
+    const uint32_t cnt = (uint32_t)n;
     HIPCHK(hipSetDevice(e->device));
-    uint32_t* d_vals = nullptr;
-    int32_t* d_out = nullptr;
-    HIPCHK(hipMalloc(&d_vals, limbs.size() * sizeof(uint32_t)));
-    hipError_t err = hipMalloc(&d_out, n * sizeof(int32_t));
+    hipStream_t s = e->own->stream;
+    DevRec* d_recs = nullptr;
+    DevWinInfo* d_win = nullptr;
+    uint32_t* d_cnt = nullptr;
+    uint64_t* d_f = nullptr;
+    hipError_t err = hipMalloc(&d_recs, n * sizeof(DevRec));
+    if (err == hipSuccess) err = hipMalloc(&d_win, sizeof(DevWinInfo));
+    if (err == hipSuccess) err = hipMalloc(&d_cnt, sizeof(uint32_t));
+    if (err == hipSuccess) err = hipMalloc(&d_f, 2 * n * sizeof(uint64_t));
+    if (err == hipSuccess) err = hipMemset(d_f, 0, 2 * n * sizeof(uint64_t));
     if (err == hipSuccess)
-        err = hipMemcpy(d_vals, limbs.data(), limbs.size() * sizeof(uint32_t),
+        err = hipMemcpy(d_recs, recs.data(), n * sizeof(DevRec),
                         hipMemcpyHostToDevice);
     if (err == hipSuccess)
-        err = launch_classify_values(e->d_engine, d_vals, (uint32_t)n,
-                                     cls->mode, cls->cap, cls->exact_from,
-                                     d_out, e->own->stream);
-    if (err == hipSuccess) err = hipStreamSynchronize(e->own->stream);
+        err = hipMemcpy(d_win, &win, sizeof win, hipMemcpyHostToDevice);
     if (err == hipSuccess)
-        err = hipMemcpy(runs, d_out, n * sizeof(int32_t), hipMemcpyDeviceToHost);
-    (void)hipFree(d_vals);
-    if (d_out) (void)hipFree(d_out);
+        err = hipMemcpy(d_cnt, &cnt, sizeof cnt, hipMemcpyHostToDevice);
+    if (err == hipSuccess) {
+        ClassifyArgs ca{};
+        ca.engines = e->d_engine;
+        ca.wins = d_win;
+        ca.recs = d_recs;
+        ca.count = d_cnt;
+        ca.cap = cnt;
+        ca.cap_max = (uint32_t)std::max(1, cls->cap);
+        ca.fail2 = d_f;
+        ca.failx = d_f + n;
+        uint32_t g = (uint32_t)std::min<uint64_t>(
+            16384, (n * ca.cap_max + 127) / 128 + 4);
+        err = launch_classify(ca, g, s);
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
+    if (err == hipSuccess)
+        err = hipMemcpy(recs.data(), d_recs, n * sizeof(DevRec),
+                        hipMemcpyDeviceToHost);
+    if (d_recs) (void)hipFree(d_recs);
+    if (d_win) (void)hipFree(d_win);
+    if (d_cnt) (void)hipFree(d_cnt);
+    if (d_f) (void)hipFree(d_f);
     if (err != hipSuccess)
         return fail(FK_E_DEVICE, "classification failed: %s",
                     hipGetErrorString(err));
+    for (size_t i = 0; i < n; i++) runs[i] = recs[i].run;
     return FK_OK;
 }
 

@@ -9,10 +9,11 @@
 //                   then popcount/prefix compaction of the surviving bits
 //                   into the candidate queue
 //   k_round_test    per-candidate kill tests for one range of large primes
-//                   (workgroup = prime chunk, lanes = candidates), killed
+//                   (lane = prime, candidates staged through LDS), killed
 //                   candidates flagged
 //   k_round_compact wave-ballot compaction of the unflagged candidates
-//   k_classify      strong-probable-prime chains (Montgomery, 2..8 limbs)
+//   k_cls_*         strong-probable-prime chains (Montgomery, 2..8 limbs),
+//                   parallel over (candidate, value, base)
 //
 // Reference counterparts: euler-prime-runs/euler_gpu.py:438-575 and :108-273
 // (bit-sieve, rounds, cold stage), dickson-ladders/ladder_gpu.py:254-407
@@ -261,25 +262,43 @@ __device__ __forceinline__ bool kill_test(const DevEngine& E,
     }
 }
 
+// A workgroup owns 256 consecutive primes of the round (one per lane, its
+// constants held in registers) and walks the candidate queue RT_RECS
+// records at a time, staged in LDS; every lane tests its prime against each
+// staged record, so no lane runs a serial chain of dependent loads.
+constexpr int RT_RECS = 16;
+
 __global__ __launch_bounds__(256) void k_round_test(RoundArgs a)
 {
+    __shared__ DevRec srec[RT_RECS];
+    __shared__ uint32_t seng[RT_RECS];
+    __shared__ uint32_t slive[RT_RECS];
     const uint32_t count = min(*a.count, a.cap);
-    for (uint32_t chunk = blockIdx.x; chunk < a.nchunks;
-         chunk += gridDim.x) {
-        const uint32_t c0 = a.p_begin + chunk * a.chunk;
-        const uint32_t c1 = min(c0 + a.chunk, a.p_end);
-        for (uint32_t i = blockIdx.y * blockDim.x + threadIdx.x; i < count;
-             i += gridDim.y * blockDim.x) {
-            if (a.flags[i]) continue;
-            const DevRec rec = a.recs[i];
-            const DevEngine& E = a.engines[a.win_engine[rec.window]];
-            const uint32_t lo = max(c0, E.round_first);
-            const uint32_t hi = min(c1, E.round_end);
-            for (uint32_t p = lo; p < hi; p++) {
-                if (kill_test(E, rec.t, a.primes[p])) {
-                    a.flags[i] = 1;
-                    break;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t pb = blockIdx.x; pb < a.nchunks; pb += gridDim.x) {
+        const uint32_t p = a.p_begin + pb * 256 + tid;
+        const bool have = p < a.p_end;
+        PrimeK pk = have ? a.primes[p] : a.primes[a.p_begin];
+        for (uint32_t r0 = blockIdx.y * RT_RECS; r0 < count;
+             r0 += gridDim.y * RT_RECS) {
+            __syncthreads();
+            if (tid < RT_RECS) {
+                const uint32_t i = r0 + tid;
+                uint32_t live = 0;
+                if (i < count && a.flags[i] == 0) {
+                    srec[tid] = a.recs[i];
+                    seng[tid] = a.win_engine[srec[tid].window];
+                    live = 1;
                 }
+                slive[tid] = live;
+            }
+            __syncthreads();
+            for (int k = 0; k < RT_RECS; k++) {
+                if (!slive[k]) continue;
+                const DevEngine& E = a.engines[seng[k]];
+                if (have && p >= E.round_first && p < E.round_end &&
+                    kill_test(E, srec[k].t, pk))
+                    a.flags[r0 + k] = 1;
             }
         }
     }
@@ -339,61 +358,91 @@ __device__ __forceinline__ void chain_value(const DevEngine& E,
     }
 }
 
-__device__ int chain_length(const DevEngine& E, const uint32_t t[4],
-                            const uint32_t kk[8], int cap, int nbases)
+// Classification in three passes, parallel over (record, value, base):
+//   k_cls_base2   every value i < cap of every record, base 2 alone;
+//                 failures set bit i of fail2[record]
+//   k_cls_full    bases 2..7 (indices 1..6) for the values before the first
+//                 base-2 failure; failures set bit i of failx[record]
+//   k_cls_finish  run = first failing value (FULL), or the base-2 bound
+//                 when it is below exact_from (TWO_PASS); clears the masks
+// A sequential chain stops at its first composite; here the later values'
+// base-2 tests run too, in parallel, and are ignored.  The verdicts are the
+// sequential chain's exactly, because the seven-base test is the AND of the
+// single-base tests (huntlib/primes.py:31-47).
+
+__device__ __forceinline__ void rec_chain_value(const DevEngine& E,
+                                                const DevRec& rec, int i,
+                                                uint32_t v[8])
 {
-    int i = 0;
-    while (i < cap) {
-        uint32_t v[8];
-        chain_value(E, t, kk, i, v);
-        if (!is_sprp(v, nbases)) break;
-        i++;
-    }
-    return i;
+    uint32_t kk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (E.family == kDickson) mw_mul44(kk, rec.t, rec.t);
+    chain_value(E, rec.t, kk, i, v);
 }
 
-__global__ __launch_bounds__(128) void k_classify(ClassifyArgs a)
+__device__ __forceinline__ int first_set(uint64_t m, int cap)
+{
+    return m ? min(cap, __builtin_ctzll(m)) : cap;
+}
+
+__global__ __launch_bounds__(128) void k_cls_base2(ClassifyArgs a)
 {
     const uint32_t count = min(*a.count, a.cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count;
-         i += gridDim.x * blockDim.x) {
-        DevRec rec = a.recs[i];
+    const uint64_t total = (uint64_t)count * a.cap_max;
+    for (uint64_t idx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+         idx < total; idx += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t r = (uint32_t)(idx / a.cap_max);
+        const int i = (int)(idx - (uint64_t)r * a.cap_max);
+        const DevRec rec = a.recs[r];
         const DevWinInfo W = a.wins[rec.window];
-        if (W.mode == 0) continue;
+        if (W.mode == 0 || i >= W.cap) continue;
         const DevEngine& E = a.engines[W.engine];
-        uint32_t kk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (E.family == kDickson) mw_mul44(kk, rec.t, rec.t);
-        int run;
-        if (W.mode == 2) {
-            run = chain_length(E, rec.t, kk, W.cap, 1);
-            if (run >= W.exact_from) run = chain_length(E, rec.t, kk, W.cap, 7);
-        } else {
-            run = chain_length(E, rec.t, kk, W.cap, 7);
-        }
-        a.recs[i].run = run;
+        uint32_t v[8];
+        rec_chain_value(E, rec, i, v);
+        if (!is_sprp(v, 0, 1))
+            atomicOr((unsigned long long*)&a.fail2[r], 1ull << i);
     }
 }
 
-// Standalone classification of caller-supplied values (the wrapper's
-// run_length / chain_depth entry points).
-__global__ __launch_bounds__(128) void k_classify_values(
-    const DevEngine* __restrict__ E, const uint32_t* __restrict__ vals,
-    uint32_t n, int mode, int cap, int exact_from, int32_t* __restrict__ out)
+__global__ __launch_bounds__(128) void k_cls_full(ClassifyArgs a)
 {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += gridDim.x * blockDim.x) {
-        uint32_t t[4] = {vals[4 * i], vals[4 * i + 1], vals[4 * i + 2],
-                         vals[4 * i + 3]};
-        uint32_t kk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (E->family == kDickson) mw_mul44(kk, t, t);
+    const uint32_t count = min(*a.count, a.cap);
+    const uint64_t per = (uint64_t)a.cap_max * 6;
+    const uint64_t total = (uint64_t)count * per;
+    for (uint64_t idx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+         idx < total; idx += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t r = (uint32_t)(idx / per);
+        const uint32_t rem = (uint32_t)(idx - (uint64_t)r * per);
+        const int i = (int)(rem / 6);
+        const int b = 1 + (int)(rem - (uint32_t)i * 6);
+        const DevRec rec = a.recs[r];
+        const DevWinInfo W = a.wins[rec.window];
+        if (W.mode == 0 || i >= W.cap) continue;
+        const int u = first_set(a.fail2[r], W.cap);
+        if (i >= u) continue;
+        if (W.mode == 2 && u < W.exact_from) continue;
+        const DevEngine& E = a.engines[W.engine];
+        uint32_t v[8];
+        rec_chain_value(E, rec, i, v);
+        if (!is_sprp(v, b, 1))
+            atomicOr((unsigned long long*)&a.failx[r], 1ull << i);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cls_finish(ClassifyArgs a)
+{
+    const uint32_t count = min(*a.count, a.cap);
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < count;
+         r += gridDim.x * blockDim.x) {
+        const DevWinInfo W = a.wins[a.recs[r].window];
+        const uint64_t f2 = a.fail2[r], fx = a.failx[r];
+        a.fail2[r] = 0;
+        a.failx[r] = 0;
+        if (W.mode == 0) continue;
+        const int u = first_set(f2, W.cap);
         int run;
-        if (mode == 2) {
-            run = chain_length(*E, t, kk, cap, 1);
-            if (run >= exact_from) run = chain_length(*E, t, kk, cap, 7);
-        } else {
-            run = chain_length(*E, t, kk, cap, 7);
-        }
-        out[i] = run;
+        if (W.mode == 2 && u < W.exact_from) run = u;
+        else run = min(u, first_set(fx, W.cap));
+        a.recs[r].run = run;
     }
 }
 
@@ -476,7 +525,7 @@ hipError_t launch_round_test(const RoundArgs& a, uint32_t gy,
                              hipStream_t stream)
 {
     uint32_t gx = a.nchunks ? a.nchunks : 1;
-    if (gx > 65535) gx = 65535;
+    if (gx > 1u << 20) gx = 1u << 20;
     if (gy < 1) gy = 1;
     if (gy > 65535) gy = 65535;
     hipLaunchKernelGGL(k_round_test, dim3(gx, gy), dim3(256), 0, stream, a);
@@ -494,21 +543,15 @@ hipError_t launch_round_compact(const CompactArgs& a, uint32_t grid,
 hipError_t launch_classify(const ClassifyArgs& a, uint32_t grid,
                            hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_classify, dim3(grid ? grid : 1), dim3(128), 0,
-                       stream, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_classify_values(const DevEngine* E, const uint32_t* vals,
-                                  uint32_t n, int mode, int cap,
-                                  int exact_from, int32_t* out,
-                                  hipStream_t stream)
-{
-    uint32_t grid = (n + 127) / 128;
     if (grid < 1) grid = 1;
-    if (grid > 4096) grid = 4096;
-    hipLaunchKernelGGL(k_classify_values, dim3(grid), dim3(128), 0, stream,
-                       E, vals, n, mode, cap, exact_from, out);
+    hipLaunchKernelGGL(k_cls_base2, dim3(grid), dim3(128), 0, stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cls_full, dim3(grid), dim3(128), 0, stream, a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    uint32_t g2 = grid / 4 + 1;
+    hipLaunchKernelGGL(k_cls_finish, dim3(g2), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 

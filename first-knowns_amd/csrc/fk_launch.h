@@ -37,8 +37,8 @@ struct RoundArgs {
     uint32_t cap;
     uint32_t* flags;
     uint32_t p_begin, p_end;  // global prime indices of this round
-    uint32_t chunk;           // primes per work item
-    uint32_t nchunks;
+    uint32_t chunk;           // unused: a workgroup takes 256 primes
+    uint32_t nchunks;         // workgroups of 256 primes in the round
 };
 
 struct CompactArgs {
@@ -56,6 +56,9 @@ struct ClassifyArgs {
     DevRec* recs;
     const uint32_t* count;
     uint32_t cap;
+    uint32_t cap_max;         // largest chain cap of any window
+    uint64_t* fail2;          // per record: values failing base 2
+    uint64_t* failx;          // per record: values failing bases 3..7
 };
 
 hipError_t launch_prep(const DevEngine* engines, const DevPiece* pieces,
@@ -71,10 +74,6 @@ hipError_t launch_round_compact(const CompactArgs& a, uint32_t grid,
                                 hipStream_t stream);
 hipError_t launch_classify(const ClassifyArgs& a, uint32_t grid,
                            hipStream_t stream);
-hipError_t launch_classify_values(const DevEngine* E, const uint32_t* vals,
-                                  uint32_t n, int mode, int cap,
-                                  int exact_from, int32_t* out,
-                                  hipStream_t stream);
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t stream);
 
 }  // namespace fk
