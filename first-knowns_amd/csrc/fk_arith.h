@@ -207,23 +207,113 @@ FK_HD void mont_mul(uint32_t* r, const uint32_t* a, const uint32_t* b,
     for (int i = 0; i < L; i++) r[i] = use_t ? t[i] : d[i];
 }
 
-// 2^(32 L * k) mod m by doublings: used for R mod m (k = 1) and R^2 mod m
-// (k = 2), where R = 2^(32 L).
+// Montgomery square r = a^2 * 2^(-32 L) mod m: the cross products are
+// formed once and doubled, then the 2L-limb square is reduced limb by limb
+// (separated operand scanning).  a < m; r < m.
 template <int L>
-FK_HD void pow2_mod(uint32_t* r, const uint32_t* m, int bits)
+FK_HD void mont_sqr(uint32_t* r, const uint32_t* a, const uint32_t* m,
+                    uint32_t minv)
 {
-    for (int i = 0; i < L; i++) r[i] = 0;
-    r[0] = 1;
-    for (int k = 0; k < bits; k++) {
-        uint32_t top = r[L - 1] >> 31;
-        for (int i = L - 1; i > 0; i--) r[i] = (r[i] << 1) | (r[i - 1] >> 31);
-        r[0] <<= 1;
-        uint32_t d[L];
-        uint32_t borrow = mw_sub<L>(d, r, m);
-        if (top || !borrow) {
-            for (int i = 0; i < L; i++) r[i] = d[i];
+    uint32_t t[2 * L + 1];
+#pragma unroll
+    for (int i = 0; i < 2 * L + 1; i++) t[i] = 0;
+    // cross products a_i a_j, i < j
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        uint64_t c = 0;
+#pragma unroll
+        for (int j = i + 1; j < L; j++) {
+            uint64_t s = (uint64_t)a[i] * a[j] + t[i + j] + c;
+            t[i + j] = (uint32_t)s;
+            c = s >> 32;
+        }
+        t[i + L] = (uint32_t)c;
+    }
+    // double them
+    uint32_t top = 0;
+#pragma unroll
+    for (int i = 1; i < 2 * L; i++) {
+        uint32_t nt = t[i] >> 31;
+        t[i] = (t[i] << 1) | top;
+        top = nt;
+    }
+    // add the squares a_i^2
+    uint64_t c = 0;
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        uint64_t sq = (uint64_t)a[i] * a[i];
+        uint64_t s0 = (uint64_t)t[2 * i] + (uint32_t)sq + c;
+        t[2 * i] = (uint32_t)s0;
+        uint64_t s1 = (uint64_t)t[2 * i + 1] + (sq >> 32) + (s0 >> 32);
+        t[2 * i + 1] = (uint32_t)s1;
+        c = s1 >> 32;
+    }
+    // reduce: add u_i m 2^(32 i) so the low L limbs vanish
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        const uint32_t u = t[i] * minv;
+        uint64_t cc = 0;
+#pragma unroll
+        for (int j = 0; j < L; j++) {
+            uint64_t s = (uint64_t)u * m[j] + t[i + j] + cc;
+            t[i + j] = (uint32_t)s;
+            cc = s >> 32;
+        }
+#pragma unroll
+        for (int k = i + L; k < 2 * L + 1; k++) {
+            uint64_t s = (uint64_t)t[k] + cc;
+            t[k] = (uint32_t)s;
+            cc = s >> 32;
         }
     }
+    // result t[L..2L] < 2m: one conditional subtract
+    uint32_t d[L];
+    uint32_t borrow = mw_sub<L>(d, t + L, m);
+    const bool use_t = (t[2 * L] == 0) && borrow;
+#pragma unroll
+    for (int i = 0; i < L; i++) r[i] = use_t ? t[L + i] : d[i];
+}
+
+// r = 2 r mod m for r < m (m < 2^(32 L))
+template <int L>
+FK_HD void mw_dbl_mod(uint32_t* r, const uint32_t* m)
+{
+    const uint32_t top = r[L - 1] >> 31;
+#pragma unroll
+    for (int i = L - 1; i > 0; i--) r[i] = (r[i] << 1) | (r[i - 1] >> 31);
+    r[0] <<= 1;
+    uint32_t d[L];
+    const uint32_t borrow = mw_sub<L>(d, r, m);
+    const bool sub = top || !borrow;
+#pragma unroll
+    for (int i = 0; i < L; i++) r[i] = sub ? d[i] : r[i];
+}
+
+// r = (r + b) mod m for r, b < m
+template <int L>
+FK_HD void mw_add_mod(uint32_t* r, const uint32_t* b, const uint32_t* m)
+{
+    const uint32_t carry = mw_add<L>(r, r, b);
+    uint32_t d[L];
+    const uint32_t borrow = mw_sub<L>(d, r, m);
+    const bool sub = carry || !borrow;
+#pragma unroll
+    for (int i = 0; i < L; i++) r[i] = sub ? d[i] : r[i];
+}
+
+// R mod m with R = 2^(32 L): start from the top bit of m (2^(B-1) < m for
+// odd m > 1 of bit length B) and double the remaining 32 L - B + 1 times.
+template <int L>
+FK_HD void mont_one(uint32_t* r, const uint32_t* m)
+{
+    int B = 0;
+#pragma unroll
+    for (int i = 0; i < L; i++)
+        if (m[i]) B = 32 * i + 32 - __builtin_clz(m[i]);
+    const int tb = B - 1;
+#pragma unroll
+    for (int i = 0; i < L; i++) r[i] = (i == (tb >> 5)) ? (1u << (tb & 31)) : 0u;
+    for (int k = 0; k < 32 * L - tb; k++) mw_dbl_mod<L>(r, m);
 }
 
 // Shift right by `bits` (< 32 L), with only compile-time limb indexes so
@@ -248,17 +338,18 @@ FK_HD void mw_shr(uint32_t* a, int bits)
     }
 }
 
-// Strong probable-prime test of odd m > 37 (no factor <= 37) to the first
-// `nbases` of the seven bases.  Returns true when m passes every base asked
-// for.  huntlib/primes.py:31-47 (the loop over bases) and :61-80 (base 2
-// alone) are the two uses: nbases = 7 or nbases = 1.
+// Strong probable-prime test of odd m > 37 (no factor <= 37) to the bases
+// first..first+nbases-1 of the seven.  Returns true when m passes every
+// base asked for.  huntlib/primes.py:31-47 (the loop over bases) and
+// :61-80 (base 2 alone) are the two uses.  In the Montgomery domain
+// (R = 2^(32 L)): one = R mod m, a R mod m by double-and-add of one, and
+// for base 2 the "multiply by the base" step of the ladder is a doubling.
 template <int L>
 FK_HD bool sprp_bases(const uint32_t* m, int first, int nbases)
 {
     const uint32_t minv = neg_inv32(m[0]);
-    uint32_t one[L], mone[L], r2[L], d[L];
-    pow2_mod<L>(one, m, 32 * L);        // R mod m
-    pow2_mod<L>(r2, m, 64 * L);         // R^2 mod m
+    uint32_t one[L], mone[L], d[L];
+    mont_one<L>(one, m);
     mw_sub<L>(mone, m, one);            // (m - 1) R mod m == m - (R mod m)
     // m - 1 = d * 2^s
 #pragma unroll
@@ -286,8 +377,12 @@ FK_HD bool sprp_bases(const uint32_t* m, int first, int nbases)
         uint32_t am[L], x[L];
 #pragma unroll
         for (int i = 0; i < L; i++) { am[i] = 0; x[i] = 0; }
-        am[0] = a;
-        mont_mul<L>(am, am, r2, m, minv);     // a R mod m
+        // a R mod m by double-and-add over the bits of a
+        for (int k = 31 - __builtin_clz(a); k >= 0; k--) {
+            mw_dbl_mod<L>(am, m);
+            if ((a >> k) & 1) mw_add_mod<L>(am, one, m);
+        }
+        const bool two = (a == 2);
         bool started = false;
 #pragma unroll
         for (int w = L - 1; w >= 0; w--) {
@@ -295,8 +390,11 @@ FK_HD bool sprp_bases(const uint32_t* m, int first, int nbases)
             for (int bit = 31; bit >= 0; bit--) {
                 const bool set = (word >> bit) & 1;
                 if (started) {
-                    mont_mul<L>(x, x, x, m, minv);
-                    if (set) mont_mul<L>(x, x, am, m, minv);
+                    mont_sqr<L>(x, x, m, minv);
+                    if (set) {
+                        if (two) mw_dbl_mod<L>(x, m);
+                        else mont_mul<L>(x, x, am, m, minv);
+                    }
                 } else if (set) {
                     started = true;
 #pragma unroll
@@ -307,7 +405,7 @@ FK_HD bool sprp_bases(const uint32_t* m, int first, int nbases)
         if (mw_cmp<L>(x, one) == 0 || mw_cmp<L>(x, mone) == 0) continue;
         bool ok = false;
         for (int r = 1; r < s; r++) {
-            mont_mul<L>(x, x, x, m, minv);
+            mont_sqr<L>(x, x, m, minv);
             if (mw_cmp<L>(x, mone) == 0) { ok = true; break; }
         }
         if (!ok) return false;

@@ -134,6 +134,7 @@ struct fk_engine {
     u128 dval = 0;
     uint32_t q_pat = 0, q_med = 0, threads = 256, wpt = 8;
     uint32_t round_chunk = 64, growth = 2;
+    int bytes = 0;                   // medium-prime marks as bytes
     uint32_t npat = 0;
     std::vector<uint32_t> lds_q;
     std::vector<uint32_t> kill_w;    // |K_q| per LDS prime
@@ -239,6 +240,7 @@ static int build_engine(fk_engine* e)
     if (e->q_med > d.q2) e->q_med = d.q2;
     if (e->q_pat > e->q_med) e->q_pat = e->q_med;
     e->round_chunk = d.round_chunk ? d.round_chunk : 64;
+    e->bytes = d.mark_bytes ? 1 : 0;
     e->growth = d.round_growth >= 2 ? d.round_growth : 2;
 
     // ---- LDS-stage primes: past the wheel, below q_med
@@ -617,15 +619,17 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         a.out = b->d_q[0];
         a.out_count = b->d_counts + 0;
         a.out_cap = queue_cap;
-        const size_t lds = 8 * (S / 64) + 8 * ((nlds_max + 1) / 2) + 8 * (size_t)pat_max;
+        const int bytes = e0->bytes;
+        const size_t lds = 8 * (S / 64) + (bytes ? S : 0) +
+                           8 * ((nlds_max + 1) / 2) + 8 * (size_t)pat_max;
         if (lds > 160 * 1024)
             return fail(FK_E_ARG, "LDS need %zu bytes > 160 KiB (lower q_med"
                         " or q_pat)", lds);
-        int occ = sieve_occupancy((int)T, (int)wpt, lds);
+        int occ = sieve_occupancy((int)T, (int)wpt, bytes, lds);
         if (occ < 1) occ = 1;
         uint64_t grid = (uint64_t)occ * ctx->cus;
         grid = std::min<uint64_t>(grid, tiles);
-        HIPCHK(launch_sieve(a, (int)T, (int)wpt, (uint32_t)grid, lds, s));
+        HIPCHK(launch_sieve(a, (int)T, (int)wpt, bytes, (uint32_t)grid, lds, s));
     }
     HIPCHK(hipEventRecord(b->ev[2], s));
 
@@ -684,8 +688,10 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         ca.cap_max = (uint32_t)cap_max;
         ca.fail2 = b->d_fail2;
         ca.failx = b->d_failx;
+        // few candidates: every base at once, one latency instead of two
+        ca.single = (alive * cap_max <= 8192.0) ? 1u : 0u;
         uint32_t g = (uint32_t)std::min(
-            16384.0, std::ceil(alive * cap_max / 128.0) + 4);
+            16384.0, std::ceil(alive * cap_max * (ca.single ? 7 : 1) / 128.0) + 4);
         HIPCHK(launch_classify(ca, g, s));
     }
     HIPCHK(hipEventRecord(b->ev[4], s));
@@ -980,6 +986,7 @@ int fk_engine_classify(fk_engine* e, const uint64_t* values, size_t n,
         ca.cap_max = (uint32_t)std::max(1, cls->cap);
         ca.fail2 = d_f;
         ca.failx = d_f + n;
+        ca.single = (n * (uint64_t)ca.cap_max <= 8192) ? 1u : 0u;
         uint32_t g = (uint32_t)std::min<uint64_t>(
             16384, (n * ca.cap_max + 127) / 128 + 4);
         err = launch_classify(ca, g, s);

@@ -83,14 +83,21 @@ __device__ __forceinline__ uint64_t edge_mask(uint64_t pos0, uint64_t vlo,
     return m;
 }
 
-template <int T, int WPT>
+// BYTES = 0: medium primes set bits with ds_or (atomic read-modify-write).
+// BYTES = 1: medium primes store a 1 into a byte per position (plain
+//            ds_write_b8: idempotent, no read, two VALU per mark); the
+//            pattern stage keeps the bit array and compaction consults the
+//            bytes only for the few positions the patterns left alive.
+template <int T, int WPT, int BYTES>
 __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
 {
     extern __shared__ uint64_t lds64[];
     constexpr uint32_t S = 64u * T * WPT;          // line positions per tile
+    constexpr uint32_t MARK64 = BYTES ? S / 8 : 0;  // u64 words of marks
     uint32_t* bits = reinterpret_cast<uint32_t*>(lds64);     // S/32 words
-    uint32_t* s0 = bits + S / 32;                             // [nlds_max]
-    uint64_t* pat = lds64 + (S / 64) + ((a.nlds_max + 1) / 2);  // patterns
+    uint8_t* marks = reinterpret_cast<uint8_t*>(lds64 + S / 64);  // [S]
+    uint32_t* s0 = reinterpret_cast<uint32_t*>(lds64 + S / 64 + MARK64);
+    uint64_t* pat = lds64 + (S / 64) + MARK64 + ((a.nlds_max + 1) / 2);
     __shared__ uint32_t sh_count, sh_base;
 
     const uint32_t tid = threadIdx.x;
@@ -153,6 +160,11 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
             bits[2 * w] = (uint32_t)acc[k];
             bits[2 * w + 1] = (uint32_t)(acc[k] >> 32);
         }
+        if (BYTES) {
+            uint4* m4 = reinterpret_cast<uint4*>(marks);
+            const uint32_t nz = ((uint32_t)len + 15) / 16;
+            for (uint32_t i = tid; i < nz; i += T) m4[i] = make_uint4(0, 0, 0, 0);
+        }
         if (tid == 0) sh_count = 0;
         __syncthreads();
 
@@ -165,8 +177,12 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
             const uint32_t q = E.lds_q[pidx];
             const uint32_t s = s0[pidx];
             uint32_t pos = (res >= s) ? res - s : res + q - s;
-            for (; pos < ulen; pos += q)
-                atomicOr(&bits[pos >> 5], 1u << (pos & 31));
+            if (BYTES) {
+                for (; pos < ulen; pos += q) marks[pos] = 1;
+            } else {
+                for (; pos < ulen; pos += q)
+                    atomicOr(&bits[pos >> 5], 1u << (pos & 31));
+            }
         }
         __syncthreads();
 
@@ -177,6 +193,14 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
         for (int k = 0; k < WPT; k++) {
             const uint32_t w = tid + (uint32_t)k * T;
             live[k] = ~(((uint64_t)bits[2 * w + 1] << 32) | bits[2 * w]);
+            if (BYTES) {
+                uint64_t m = live[k];
+                while (m) {
+                    const int b = __builtin_ctzll(m);
+                    m &= m - 1;
+                    if (marks[64 * w + b]) live[k] &= ~(1ull << b);
+                }
+            }
             cnt += __popcll(live[k]);
         }
         uint32_t mine = 0;
@@ -218,31 +242,39 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
 
 // ---------------------------------------------------------------- rounds
 
-// Does prime p divide one of the family's values for this candidate?
-__device__ __forceinline__ bool kill_test(const DevEngine& E,
-                                          const uint32_t t[4],
-                                          const PrimeK& pk)
+// A candidate staged for the kill tests, with the few engine constants the
+// tests read, so the per-prime loop touches LDS only.
+struct Staged {
+    uint32_t t[4];
+    int32_t family, n;
+    uint32_t round_first, round_end, xxmax;
+    uint32_t d[4];
+    const uint32_t* xxbits;
+};
+
+// Does prime q divide one of the family's values for this candidate?
+__device__ __forceinline__ bool kill_test(const Staged& c, const PrimeK& pk)
 {
     const uint32_t q = pk.q;
-    const uint32_t r = red128(t, pk);
-    if (E.family == kEuler) {
+    const uint32_t r = red128(c.t, pk);
+    if (c.family == kEuler) {
         // q | x^2+x+p  <=>  (q - p mod q) mod q == x^2+x (mod q)
-        if (q > E.xxmax) {
+        if (q > c.xxmax) {
             const uint32_t dd = r ? q - r : 0;
-            return dd <= E.xxmax && ((E.xxbits[dd >> 5] >> (dd & 31)) & 1);
+            return dd <= c.xxmax && ((c.xxbits[dd >> 5] >> (dd & 31)) & 1);
         }
-        for (int x = 0; x < E.n; x++) {
+        for (int x = 0; x < c.n; x++) {
             const uint64_t v = (uint64_t)x * x + x + r;
             if (v % q == 0) return true;
         }
         return false;
-    } else if (E.family == kDickson) {
-        // t = k mod q; q | m k^2 + 1 for some m <= n
+    } else if (c.family == kDickson) {
+        // r = k mod q; q | m k^2 + 1 for some m <= n
         const uint32_t kk = mulmod32(r, r, q, pk.m64);
         if (kk == 0) return false;
         uint32_t v = kk + 1;
         if (v >= q) v -= q;
-        for (int m = 1; m <= E.n; m++) {
+        for (int m = 1; m <= c.n; m++) {
             if (v == 0) return true;
             v += kk;
             if (v >= q) v -= q;
@@ -250,10 +282,10 @@ __device__ __forceinline__ bool kill_test(const DevEngine& E,
         return false;
     } else {
         // q | p + j d for some j < n
-        const uint32_t dq = red128(E.d, pk);
+        const uint32_t dq = red128(c.d, pk);
         if (dq == 0) return r == 0;
         uint32_t f = r;
-        for (int j = 0; j < E.n; j++) {
+        for (int j = 0; j < c.n; j++) {
             if (f == 0) return true;
             f += dq;
             if (f >= q) f -= q;
@@ -270,15 +302,14 @@ constexpr int RT_RECS = 16;
 
 __global__ __launch_bounds__(256) void k_round_test(RoundArgs a)
 {
-    __shared__ DevRec srec[RT_RECS];
-    __shared__ uint32_t seng[RT_RECS];
+    __shared__ Staged st[RT_RECS];
     __shared__ uint32_t slive[RT_RECS];
     const uint32_t count = min(*a.count, a.cap);
     const uint32_t tid = threadIdx.x;
     for (uint32_t pb = blockIdx.x; pb < a.nchunks; pb += gridDim.x) {
         const uint32_t p = a.p_begin + pb * 256 + tid;
         const bool have = p < a.p_end;
-        PrimeK pk = have ? a.primes[p] : a.primes[a.p_begin];
+        const PrimeK pk = have ? a.primes[p] : a.primes[a.p_begin];
         for (uint32_t r0 = blockIdx.y * RT_RECS; r0 < count;
              r0 += gridDim.y * RT_RECS) {
             __syncthreads();
@@ -286,18 +317,30 @@ __global__ __launch_bounds__(256) void k_round_test(RoundArgs a)
                 const uint32_t i = r0 + tid;
                 uint32_t live = 0;
                 if (i < count && a.flags[i] == 0) {
-                    srec[tid] = a.recs[i];
-                    seng[tid] = a.win_engine[srec[tid].window];
+                    const DevRec rec = a.recs[i];
+                    const DevEngine& E = a.engines[a.win_engine[rec.window]];
+                    Staged c;
+                    for (int k = 0; k < 4; k++) {
+                        c.t[k] = rec.t[k];
+                        c.d[k] = E.d[k];
+                    }
+                    c.family = E.family;
+                    c.n = E.n;
+                    c.round_first = E.round_first;
+                    c.round_end = E.round_end;
+                    c.xxmax = E.xxmax;
+                    c.xxbits = E.xxbits;
+                    st[tid] = c;
                     live = 1;
                 }
                 slive[tid] = live;
             }
             __syncthreads();
+            if (!have) continue;
             for (int k = 0; k < RT_RECS; k++) {
                 if (!slive[k]) continue;
-                const DevEngine& E = a.engines[seng[k]];
-                if (have && p >= E.round_first && p < E.round_end &&
-                    kill_test(E, srec[k].t, pk))
+                const Staged& c = st[k];
+                if (p >= c.round_first && p < c.round_end && kill_test(c, pk))
                     a.flags[r0 + k] = 1;
             }
         }
@@ -406,20 +449,36 @@ __global__ __launch_bounds__(128) void k_cls_base2(ClassifyArgs a)
 __global__ __launch_bounds__(128) void k_cls_full(ClassifyArgs a)
 {
     const uint32_t count = min(*a.count, a.cap);
-    const uint64_t per = (uint64_t)a.cap_max * 6;
+    // single pass: all seven bases (base 2 into fail2) with no dependence
+    // on a base-2 pass; otherwise bases 3..7 after it
+    const uint32_t nb = a.single ? 7 : 6;
+    const uint64_t per = (uint64_t)a.cap_max * nb;
     const uint64_t total = (uint64_t)count * per;
     for (uint64_t idx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
          idx < total; idx += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t r = (uint32_t)(idx / per);
         const uint32_t rem = (uint32_t)(idx - (uint64_t)r * per);
-        const int i = (int)(rem / 6);
-        const int b = 1 + (int)(rem - (uint32_t)i * 6);
+        const int i = (int)(rem / nb);
+        const int b = (int)(rem - (uint32_t)i * nb) + (a.single ? 0 : 1);
+        if (b == 0) {
+            const DevRec rec = a.recs[r];
+            const DevWinInfo W = a.wins[rec.window];
+            if (W.mode == 0 || i >= W.cap) continue;
+            const DevEngine& E = a.engines[W.engine];
+            uint32_t v[8];
+            rec_chain_value(E, rec, i, v);
+            if (!is_sprp(v, 0, 1))
+                atomicOr((unsigned long long*)&a.fail2[r], 1ull << i);
+            continue;
+        }
         const DevRec rec = a.recs[r];
         const DevWinInfo W = a.wins[rec.window];
         if (W.mode == 0 || i >= W.cap) continue;
-        const int u = first_set(a.fail2[r], W.cap);
-        if (i >= u) continue;
-        if (W.mode == 2 && u < W.exact_from) continue;
+        if (!a.single) {
+            const int u = first_set(a.fail2[r], W.cap);
+            if (i >= u) continue;
+            if (W.mode == 2 && u < W.exact_from) continue;
+        }
         const DevEngine& E = a.engines[W.engine];
         uint32_t v[8];
         rec_chain_value(E, rec, i, v);
@@ -455,33 +514,34 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n)
 
 // --------------------------------------------------------------- launchers
 
+#define FK_SIEVE_SHAPES(X)                                                 \
+    X(256, 1) X(256, 2) X(256, 4) X(256, 8) X(256, 16)                       \
+    X(512, 1) X(512, 2) X(512, 4) X(512, 8)                                  \
+    X(1024, 1) X(1024, 2) X(1024, 4)
+
+hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt, int bytes,
+                        uint32_t grid, size_t lds_bytes, hipStream_t stream)
+{
 #define FK_SIEVE_CASE(TT, WW)                                              \
     if (threads == TT && wpt == WW) {                                      \
-        auto fn = k_sieve<TT, WW>;                                         \
+        auto fn = bytes ? k_sieve<TT, WW, 1> : k_sieve<TT, WW, 0>;         \
         if (lds_bytes > 65536)                                             \
             (void)hipFuncSetAttribute((const void*)fn,                     \
                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes); \
         hipLaunchKernelGGL(fn, dim3(grid), dim3(TT), lds_bytes, stream, a); \
         return hipGetLastError();                                          \
     }
-
-hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,
-                        uint32_t grid, size_t lds_bytes, hipStream_t stream)
-{
-    FK_SIEVE_CASE(256, 1) FK_SIEVE_CASE(256, 2) FK_SIEVE_CASE(256, 4)
-    FK_SIEVE_CASE(256, 8) FK_SIEVE_CASE(256, 16)
-    FK_SIEVE_CASE(512, 1) FK_SIEVE_CASE(512, 2) FK_SIEVE_CASE(512, 4)
-    FK_SIEVE_CASE(512, 8)
-    FK_SIEVE_CASE(1024, 1) FK_SIEVE_CASE(1024, 2) FK_SIEVE_CASE(1024, 4)
+    FK_SIEVE_SHAPES(FK_SIEVE_CASE)
+#undef FK_SIEVE_CASE
     return hipErrorInvalidValue;
 }
 
-int sieve_occupancy(int threads, int wpt, size_t lds_bytes)
+int sieve_occupancy(int threads, int wpt, int bytes, size_t lds_bytes)
 {
     int blocks = 0;
 #define FK_OCC_CASE(TT, WW)                                                 \
     if (threads == TT && wpt == WW) {                                       \
-        auto fn = k_sieve<TT, WW>;                                          \
+        auto fn = bytes ? k_sieve<TT, WW, 1> : k_sieve<TT, WW, 0>;          \
         if (lds_bytes > 65536)                                              \
             (void)hipFuncSetAttribute((const void*)fn,                      \
                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);  \
@@ -489,23 +549,17 @@ int sieve_occupancy(int threads, int wpt, size_t lds_bytes)
                 &blocks, fn, TT, lds_bytes) != hipSuccess) blocks = 0;      \
         return blocks;                                                      \
     }
-    FK_OCC_CASE(256, 1) FK_OCC_CASE(256, 2) FK_OCC_CASE(256, 4)
-    FK_OCC_CASE(256, 8) FK_OCC_CASE(256, 16)
-    FK_OCC_CASE(512, 1) FK_OCC_CASE(512, 2) FK_OCC_CASE(512, 4)
-    FK_OCC_CASE(512, 8)
-    FK_OCC_CASE(1024, 1) FK_OCC_CASE(1024, 2) FK_OCC_CASE(1024, 4)
+    FK_SIEVE_SHAPES(FK_OCC_CASE)
 #undef FK_OCC_CASE
     return 0;
 }
 
 bool sieve_shape_ok(int threads, int wpt)
 {
-    return sieve_occupancy(threads, wpt, 1024) >= 0 &&
-           ((threads == 256 && (wpt == 1 || wpt == 2 || wpt == 4 ||
-                                wpt == 8 || wpt == 16)) ||
-            (threads == 512 && (wpt == 1 || wpt == 2 || wpt == 4 ||
-                                wpt == 8)) ||
-            (threads == 1024 && (wpt == 1 || wpt == 2 || wpt == 4)));
+#define FK_OK_CASE(TT, WW) if (threads == TT && wpt == WW) return true;
+    FK_SIEVE_SHAPES(FK_OK_CASE)
+#undef FK_OK_CASE
+    return false;
 }
 
 hipError_t launch_prep(const DevEngine* engines, const DevPiece* pieces,
@@ -544,9 +598,12 @@ hipError_t launch_classify(const ClassifyArgs& a, uint32_t grid,
                            hipStream_t stream)
 {
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(k_cls_base2, dim3(grid), dim3(128), 0, stream, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    hipError_t e;
+    if (!a.single) {
+        hipLaunchKernelGGL(k_cls_base2, dim3(grid), dim3(128), 0, stream, a);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_cls_full, dim3(grid), dim3(128), 0, stream, a);
     e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -59,14 +59,15 @@ struct ClassifyArgs {
     uint32_t cap_max;         // largest chain cap of any window
     uint64_t* fail2;          // per record: values failing base 2
     uint64_t* failx;          // per record: values failing bases 3..7
+    uint32_t single;          // 1: all bases in one pass (few candidates)
 };
 
 hipError_t launch_prep(const DevEngine* engines, const DevPiece* pieces,
                        uint32_t npieces, uint32_t max_nlds, uint32_t* bq,
                        hipStream_t stream);
-hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,
+hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt, int bytes,
                         uint32_t grid, size_t lds_bytes, hipStream_t stream);
-int sieve_occupancy(int threads, int wpt, size_t lds_bytes);
+int sieve_occupancy(int threads, int wpt, int bytes, size_t lds_bytes);
 bool sieve_shape_ok(int threads, int wpt);
 hipError_t launch_round_test(const RoundArgs& a, uint32_t gy,
                              hipStream_t stream);

@@ -1,0 +1,163 @@
+"""Autotuner for the sieve engine's free parameters.
+
+The survivor stream does not depend on any of them (the GPU parity tests
+check this across tile shapes, cutoffs, wheels and round growth), so the
+tuner is free to pick whatever is fastest:
+
+  threads, words_per_thread   workgroup size and tile length of k_sieve
+  q_pat                       primes below it are OR-ed in as pattern words
+  q_med                       primes below it are stride-marked in LDS;
+                              above it they are tested per candidate in
+                              compaction rounds
+  nwheel                      kill primes enumerated by the wheel
+  round_growth                prime-range growth factor between rounds
+
+Coordinate descent from the family defaults: each pass tries every value
+of one parameter with the others held, keeps the fastest, and moves on.
+Timing is the device time of whole sweeps (HIP events around the batch)
+over a representative window, median of `reps`.  The result is written to
+fk_tuning.json next to this file, which fk_families.tuning() reads.
+
+python fk_autotune.py --family dickson --n 10 --q2 4194304 \
+                      --lo 432900432900 --span 4294967296
+"""
+
+import argparse
+import json
+import pathlib
+import statistics
+import sys
+import time
+
+HERE = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE))
+
+import fk_families as F   # noqa: E402
+import fk_native as N     # noqa: E402
+
+FAMILY = {"euler": N.FAMILY_EULER, "dickson": N.FAMILY_DICKSON,
+          "ap": N.FAMILY_AP}
+
+SPACE = {
+    "threads": [256, 512, 1024],
+    "words_per_thread": [1, 2, 4, 8],
+    "q_pat": [64, 128, 192, 256, 320],
+    "q_med": [2048, 4096, 8192, 16384, 32768],
+    "nwheel": [0, 3, 4, 5, 6, 7],
+    "round_growth": [2, 3, 4],
+    "mark_bytes": [0, 1],
+}
+
+VALID_SHAPES = {(256, 1), (256, 2), (256, 4), (256, 8), (256, 16),
+                (512, 1), (512, 2), (512, 4), (512, 8),
+                (1024, 1), (1024, 2), (1024, 4)}
+
+
+def measure(family, n, q2, lo, span, params, reps=3, classify=None,
+            wheel_primes=None):
+    """Median device milliseconds of one sweep, or None if the
+    configuration is not buildable (LDS, shape, wheel size)."""
+    if (params["threads"], params["words_per_thread"]) not in VALID_SHAPES:
+        return None, None
+    kw = dict(params)
+    if wheel_primes:
+        kw.pop("nwheel", None)
+        kw["wheel_primes"] = wheel_primes
+    try:
+        eng = N.Engine(FAMILY[family], n, q2, **kw)
+    except (ValueError, RuntimeError):
+        return None, None
+    mode, cap, ef = classify or (N.CLASSIFY_NONE, 0, 0)
+    try:
+        eng.sweep(lo, lo + span, mode, cap, ef)       # warm
+        times, stages = [], []
+        for _ in range(reps):
+            eng.sweep(lo, lo + span, mode, cap, ef)
+            ms, _, _ = eng.last_timing()
+            times.append(ms[4])
+            stages.append(ms)
+    except (ValueError, RuntimeError):
+        return None, None
+    finally:
+        eng.close()
+    i = times.index(statistics.median(times))
+    return statistics.median(times), stages[i]
+
+
+def tune(family, n, q2, lo, span, passes=2, reps=3, classify=None,
+         wheel_primes=None, log=print):
+    cur = dict(F.tuning(family))
+    cur.setdefault("nwheel", 0)
+    cur.setdefault("round_growth", 2)
+    cur.setdefault("mark_bytes", 0)
+    cur.pop("round_chunk", None)
+    best, st = measure(family, n, q2, lo, span, cur, reps, classify,
+                       wheel_primes)
+    log(f"start {cur} -> {best:.3f} ms {st}")
+    keys = ["mark_bytes", "q_med", "q_pat", "threads", "words_per_thread",
+            "round_growth"]
+    if not wheel_primes:
+        keys.append("nwheel")
+    for _ in range(passes):
+        improved = False
+        for key in keys:
+            for val in SPACE[key]:
+                if val == cur.get(key):
+                    continue
+                trial = dict(cur)
+                trial[key] = val
+                if key == "threads":
+                    # keep the tile length when the workgroup changes
+                    bits = cur["threads"] * cur["words_per_thread"]
+                    trial["words_per_thread"] = max(1, bits // val)
+                t, st = measure(family, n, q2, lo, span, trial, reps,
+                                classify, wheel_primes)
+                log(f"  {key}={val}: "
+                    + ("unbuildable" if t is None else f"{t:.3f} ms {st}"))
+                if t is not None and t < best * 0.98:
+                    best, cur, improved = t, trial, True
+        if not improved:
+            break
+    log(f"best {cur} -> {best:.3f} ms")
+    return cur, best
+
+
+def save(family, params, path=F.TUNING_PATH):
+    data = {}
+    if path.exists():
+        try:
+            data = json.loads(path.read_text())
+        except ValueError:
+            data = {}
+    data[family] = params
+    path.write_text(json.dumps(data, indent=1, sort_keys=True) + "\n")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--family", required=True, choices=sorted(FAMILY))
+    ap.add_argument("--n", type=int, required=True)
+    ap.add_argument("--q2", type=int, required=True)
+    ap.add_argument("--lo", type=int, required=True)
+    ap.add_argument("--span", type=int, required=True)
+    ap.add_argument("--passes", type=int, default=2)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--cap", type=int, default=0)
+    ap.add_argument("--wheel", type=str, default="")
+    ap.add_argument("--save", action="store_true")
+    args = ap.parse_args()
+    cls = (N.CLASSIFY_FULL, args.cap, 0) if args.cap else None
+    wheel = [int(x) for x in args.wheel.split(",") if x] or None
+    t0 = time.time()
+    params, best = tune(args.family, args.n, args.q2, args.lo, args.span,
+                        args.passes, args.reps, cls, wheel,
+                        log=lambda s: print(s, flush=True))
+    print(json.dumps({"family": args.family, "params": params,
+                      "ms": best, "tuning_s": time.time() - t0}), flush=True)
+    if args.save:
+        save(args.family, params)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -74,6 +74,8 @@ typedef struct fk_engine_desc {
     uint32_t round_chunk;    /* primes per round work item (0 = default) */
     uint32_t round_growth;   /* prime bound grows by this factor per round
                                 (0 = default 2) */
+    uint32_t mark_bytes;     /* 1: medium-prime marks as LDS bytes (plain
+                                stores); 0: as LDS bits (atomic OR) */
 } fk_engine_desc;
 
 typedef struct fk_engine_info {
