@@ -8,9 +8,10 @@
 // stands for.  For a sieve prime q the killed values F_q (family-specific)
 // become the killed line positions K_q = F_q * D^-1 (mod q), shifted per
 // line and window by t = (base + off_i) * D^-1 (mod q).  Small primes are
-// OR-ed in as precomputed 64-bit pattern words, medium primes are
-// stride-marked in LDS, and primes above the LDS cutoff are tested per
-// surviving candidate in compaction rounds.
+// OR-ed in as precomputed 64-bit pattern words, medium primes are either
+// stride-marked or tested per surviving position against a kill bitmap,
+// both in LDS, and primes above the LDS cutoff are tested per surviving
+// candidate in compaction rounds.
 #pragma once
 
 #include <stdint.h>
@@ -27,10 +28,13 @@ struct DevEngine {
     int32_t family;
     int32_t n;
     uint32_t nlines;          // wheel residues
-    uint32_t nlds;            // LDS-stage primes (pattern + marking)
-    uint32_t npat;            // the first npat of them use pattern words
+    uint32_t nlds;            // LDS-stage primes (pattern, marking, test)
+    uint32_t npat;            // [0, npat): pattern words
+    uint32_t nmark;           // [npat, nmark): stride marking;
+                              // [nmark, nlds): per-survivor bitmap tests
     uint32_t npairs;          // (prime, residue) marking pairs
     uint32_t pat_words;       // total pattern words (sum of q)
+    uint32_t bm_words;        // total kill-bitmap words of the test primes
     uint32_t round_first;     // global prime-table index of the first round prime
     uint32_t round_end;       // one past the last prime below q2
     uint32_t xxmax;           // Euler: (n-1)^2 + (n-1)
@@ -45,6 +49,12 @@ struct DevEngine {
     const uint32_t* pat_off;  // [npat] first word of each prime's table
     const uint64_t* pat;      // [pat_words] pattern words
     const uint32_t* pairs;    // [npairs] (prime index << 16) | residue
+    const uint32_t* bm_off;   // [nlds - nmark] first bitmap word per prime
+    const uint32_t* bm;       // [bm_words] line-domain kill bitmaps
+    uint32_t ntr;             // in-tile rounds over [nmark, nlds)
+    const uint32_t* tr_end;   // [ntr] LDS-prime index where round r ends
+    const uint32_t* kc;       // AP: [round_end - round_first] P(n)^-1 mod q
+                              // (0 when q | P(n)); unused otherwise
     const uint32_t* xxbits;   // Euler: bitset of x^2+x, x < n
 };
 

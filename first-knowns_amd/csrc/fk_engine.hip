@@ -134,7 +134,7 @@ struct fk_engine {
     u128 dval = 0;
     uint32_t q_pat = 0, q_med = 0, threads = 256, wpt = 8;
     uint32_t round_chunk = 64, growth = 2;
-    int bytes = 0;                   // medium-prime marks as bytes
+    uint32_t q_mark = 0;             // [q_pat, q_mark) stride-marked
     uint32_t npat = 0;
     std::vector<uint32_t> lds_q;
     std::vector<uint32_t> kill_w;    // |K_q| per LDS prime
@@ -235,17 +235,21 @@ static int build_engine(fk_engine* e)
         return fail(FK_E_ARG, "unsupported sieve shape threads=%u wpt=%u",
                     e->threads, e->wpt);
     e->q_pat = d.q_pat ? d.q_pat : 128;
-    e->q_med = d.q_med ? d.q_med : 4096;
+    e->q_med = d.q_med ? d.q_med : 2048;
     if (e->q_med > 65536) e->q_med = 65536;     // pairs pack q in 16 bits
     if (e->q_med > d.q2) e->q_med = d.q2;
     if (e->q_pat > e->q_med) e->q_pat = e->q_med;
     e->round_chunk = d.round_chunk ? d.round_chunk : 64;
-    e->bytes = d.mark_bytes ? 1 : 0;
+    e->q_mark = d.q_mark ? d.q_mark : e->q_pat;
+    if (e->q_mark < e->q_pat) e->q_mark = e->q_pat;
+    if (e->q_mark > e->q_med) e->q_mark = e->q_med;
     e->growth = d.round_growth >= 2 ? d.round_growth : 2;
 
-    // ---- LDS-stage primes: past the wheel, below q_med
-    std::vector<uint32_t> lq, lm32, ldinv, pat_off, pairs;
+    // ---- LDS-stage primes: past the wheel, below q_med; in order the
+    // pattern primes, the stride-marked primes, the bitmap-tested primes
+    std::vector<uint32_t> lq, lm32, ldinv, pat_off, pairs, bm_off, bm;
     std::vector<uint64_t> lm64, pat;
+    uint32_t nmark = 0;
     double surv = 1.0;
     for (size_t i = wheel.size(); i < kp.size() && kp[i] < e->q_med; i++) {
         const uint32_t q = kp[i];
@@ -272,12 +276,45 @@ static int build_engine(fk_engine* e)
                 pat.push_back(w);
             }
             e->npat = idx + 1;
-        } else {
+        } else if (q < e->q_mark) {
             for (uint32_t r : kl) pairs.push_back((idx << 16) | r);
             e->mark_density += (double)kl.size() / q;
+            nmark = idx + 1;
+        } else {
+            bm_off.push_back((uint32_t)bm.size());
+            const size_t base = bm.size();
+            bm.resize(base + (q + 31) / 32, 0u);
+            for (uint32_t r : kl) bm[base + (r >> 5)] |= 1u << (r & 31);
         }
     }
-    // patterns must be a prefix of the LDS primes
+    if (nmark < e->npat) nmark = e->npat;
+    // in-tile rounds: cut the tested primes where the expected survival of
+    // the round reaches one half
+    std::vector<uint32_t> tr_end;
+    {
+        double lsv = 0.0;
+        for (uint32_t i = nmark; i < (uint32_t)lq.size(); i++) {
+            lsv += std::log(1.0 - (double)e->kill_w[i] / lq[i]);
+            if (lsv <= std::log(0.5)) {
+                tr_end.push_back(i + 1);
+                lsv = 0.0;
+            }
+        }
+        if (tr_end.empty() || tr_end.back() != (uint32_t)lq.size())
+            if ((uint32_t)lq.size() > nmark) tr_end.push_back((uint32_t)lq.size());
+    }
+    // the tile's LDS: bits, s0, per-prime info, patterns, kill bitmaps
+    {
+        const size_t S = (size_t)64 * e->threads * e->wpt;
+        const size_t need = S / 8 + 16 * ((lq.size() + 3) / 4) +
+                            16 * lq.size() + 8 * pat.size() +
+                            4 * ((bm.size() + 3) & ~(size_t)3) +
+                            2 * std::min<size_t>(S / 16, 4096) *
+                                (S <= 65536 ? 2 : 4);
+        if (need > 160 * 1024)
+            return fail(FK_E_ARG, "tile needs %zu bytes of LDS (> 160 KiB): "
+                        "lower q_med, q_pat or the tile", need);
+    }
     e->lds_q = lq;
     e->pat_words = (uint32_t)pat.size();
     e->surv_lds = surv;
@@ -319,8 +356,11 @@ static int build_engine(fk_engine* e)
     h.nlines = (uint32_t)e->offs.size();
     h.nlds = (uint32_t)lq.size();
     h.npat = e->npat;
+    h.nmark = nmark;
     h.npairs = (uint32_t)pairs.size();
     h.pat_words = e->pat_words;
+    h.bm_words = (uint32_t)bm.size();
+    h.ntr = (uint32_t)tr_end.size();
     h.round_first = e->round_first;
     h.round_end = e->round_end;
     h.xxmax = xxmax;
@@ -338,6 +378,22 @@ static int build_engine(fk_engine* e)
     if ((rc = eng_upload(e, pat_off, &h.pat_off))) return rc;
     if ((rc = eng_upload(e, pat, &h.pat))) return rc;
     if ((rc = eng_upload(e, pairs, &h.pairs))) return rc;
+    if ((rc = eng_upload(e, bm_off, &h.bm_off))) return rc;
+    if ((rc = eng_upload(e, bm, &h.bm))) return rc;
+    if ((rc = eng_upload(e, tr_end, &h.tr_end))) return rc;
+    {
+        // AP kill constants of the round primes: P(n)^-1 mod q, 0 if q | P(n)
+        std::vector<uint32_t> kc;
+        if (d.family == FK_FAMILY_AP) {
+            kc.resize(e->round_end - e->round_first);
+            for (uint32_t i = e->round_first; i < e->round_end; i++) {
+                const uint32_t q = ctx->primes[i];
+                const uint32_t dq = (uint32_t)(e->dval % q);
+                kc[i - e->round_first] = dq ? inv_mod(dq, q) : 0;
+            }
+        }
+        if ((rc = eng_upload(e, kc, &h.kc))) return rc;
+    }
     if (xx.empty()) xx.push_back(0);
     if ((rc = eng_upload(e, xx, &h.xxbits))) return rc;
     std::vector<DevEngine> one(1, h);
@@ -481,11 +537,12 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
     std::vector<DevWinInfo> wins(nwin);
     std::vector<uint32_t> win_engine(nwin);
     uint64_t tiles = 0, bq_total = 0;
-    uint32_t nlds_max = 0, pat_max = 0;
+    uint32_t nlds_max = 0, pat_max = 0, bm_max = 0;
     double expect = 0.0, cand = 0.0;
     for (fk_engine* e : b->engines) {
         nlds_max = std::max(nlds_max, e->host.nlds);
         pat_max = std::max(pat_max, e->pat_words);
+        bm_max = std::max(bm_max, e->host.bm_words);
     }
     for (size_t w = 0; w < nwin; w++) {
         const fk_engine* e = b->engines[b->engine_of[w]];
@@ -615,21 +672,27 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         a.npieces = (uint32_t)pieces.size();
         a.ntiles = (uint32_t)tiles;
         a.nlds_max = nlds_max;
+        a.pat_max = pat_max;
+        a.bm_max = bm_max;
         a.bq = b->d_bq;
         a.out = b->d_q[0];
         a.out_count = b->d_counts + 0;
         a.out_cap = queue_cap;
-        const int bytes = e0->bytes;
-        const size_t lds = 8 * (S / 64) + (bytes ? S : 0) +
-                           8 * ((nlds_max + 1) / 2) + 8 * (size_t)pat_max;
+        // layout of k_sieve's dynamic LDS: bits, s0, per-prime info,
+        // pattern words, kill bitmaps
+        const size_t lds = S / 8 + 16 * (size_t)((nlds_max + 3) / 4) +
+                           16 * (size_t)nlds_max + 8 * (size_t)pat_max +
+                           4 * (size_t)((bm_max + 3) & ~3u) +
+                           2 * std::min<size_t>(S / 16, 4096) *
+                               (S <= 65536 ? 2 : 4);
         if (lds > 160 * 1024)
             return fail(FK_E_ARG, "LDS need %zu bytes > 160 KiB (lower q_med"
                         " or q_pat)", lds);
-        int occ = sieve_occupancy((int)T, (int)wpt, bytes, lds);
+        int occ = sieve_occupancy((int)T, (int)wpt, lds);
         if (occ < 1) occ = 1;
         uint64_t grid = (uint64_t)occ * ctx->cus;
         grid = std::min<uint64_t>(grid, tiles);
-        HIPCHK(launch_sieve(a, (int)T, (int)wpt, bytes, (uint32_t)grid, lds, s));
+        HIPCHK(launch_sieve(a, (int)T, (int)wpt, (uint32_t)grid, lds, s));
     }
     HIPCHK(hipEventRecord(b->ev[2], s));
 
@@ -639,6 +702,7 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
     for (size_t r = 0; r < rounds.size(); r++) {
         RoundArgs ra{};
         ra.engines = b->d_engines;
+        ra.nengines = (uint32_t)b->engines.size();
         ra.win_engine = b->d_win_engine;
         ra.primes = ctx->d_primes;
         ra.recs = b->d_q[cur];
@@ -647,12 +711,21 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         ra.flags = b->d_flags;
         ra.p_begin = prime_index(ctx->primes, rounds[r].first);
         ra.p_end = prime_index(ctx->primes, rounds[r].second);
-        ra.chunk = 256;
+        ra.chunk = 0;
         const uint32_t np = ra.p_end > ra.p_begin ? ra.p_end - ra.p_begin : 0;
         ra.nchunks = (np + 255) / 256;
         if (ra.nchunks == 0) continue;
-        uint32_t gy = (uint32_t)std::min(4096.0, std::ceil(alive / 16.0));
-        HIPCHK(launch_round_test(ra, gy, s));
+        // dense (lane = candidate) when candidates outnumber primes
+        const bool dense = alive > (double)np;
+        uint32_t gx, gy;
+        if (dense) {
+            gx = (uint32_t)std::min(2048.0, std::ceil(alive / 256.0));
+            gy = (np + 63) / 64;
+        } else {
+            gx = ra.nchunks;
+            gy = (uint32_t)std::min(1024.0, std::ceil(alive / 256.0));
+        }
+        HIPCHK(launch_round_test(ra, dense, gx, gy, s));
         CompactArgs ca{};
         ca.in = b->d_q[cur];
         ca.in_count = b->d_counts + cslot;
@@ -906,6 +979,7 @@ int fk_engine_get_info(const fk_engine* e, fk_engine_info* info)
     info->d[0] = (uint64_t)e->dval;
     info->d[1] = (uint64_t)(e->dval >> 64);
     info->q_pat = e->q_pat;
+    info->q_mark = e->q_mark;
     info->q_med = e->q_med;
     info->threads = e->threads;
     info->words_per_thread = e->wpt;

@@ -5,11 +5,13 @@
 //   k_prep          B mod q for every (piece, LDS prime)
 //   k_sieve         one LDS tile (S line positions of one wheel line) per
 //                   workgroup iteration: pattern words for the smallest
-//                   primes, ds_or stride marking for the medium primes,
-//                   then popcount/prefix compaction of the surviving bits
-//                   into the candidate queue
-//   k_round_test    per-candidate kill tests for one range of large primes
-//                   (lane = prime, candidates staged through LDS), killed
+//                   primes, optional ds_or stride marking, per-survivor
+//                   tests against LDS kill bitmaps for the medium primes,
+//                   then compaction of the surviving bits into the queue
+//   k_round_dense / k_round_sparse
+//                   per-candidate kill tests for one range of large primes,
+//                   lane = candidate (dense rounds) or lane = prime with the
+//                   candidates staged through LDS (sparse rounds); killed
 //                   candidates flagged
 //   k_round_compact wave-ballot compaction of the unflagged candidates
 //   k_cls_*         strong-probable-prime chains (Montgomery, 2..8 limbs),
@@ -22,6 +24,8 @@
 // huntlib/primes.py:23-80.  None of them is translated: the tile layout,
 // the shared line model and the on-device classification are this build's.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "fk_device.h"
 #include "fk_launch.h"
@@ -83,24 +87,66 @@ __device__ __forceinline__ uint64_t edge_mask(uint64_t pos0, uint64_t vlo,
     return m;
 }
 
-// BYTES = 0: medium primes set bits with ds_or (atomic read-modify-write).
-// BYTES = 1: medium primes store a 1 into a byte per position (plain
-//            ds_write_b8: idempotent, no read, two VALU per mark); the
-//            pattern stage keeps the bit array and compaction consults the
-//            bytes only for the few positions the patterns left alive.
-template <int T, int WPT, int BYTES>
+// One tile = S consecutive positions of one wheel line, in LDS as bits
+// (1 = dead).  Per tile:
+//   1. s0[p] = line position of the tile start, shifted into prime p's
+//      kill coordinates, for every LDS-stage prime
+//   2. pattern stage: the first npat primes OR in precomputed 64-bit words
+//   3. marking stage: primes [npat, nmark) stride-marked with ds_or
+//   4. test stage: the positions still alive are listed in LDS and run
+//      through in-tile rounds over the primes [nmark, nlds); in a round
+//      every lane tests the same prime (broadcast LDS reads) against its
+//      own candidate -- one Barrett step and one bit of the prime's
+//      line-domain kill bitmap -- and the survivors are compacted with a
+//      wave ballot into the other list
+//   5. the survivors become candidate records
+// Per-engine LDS (loaded when a workgroup meets a new engine): pattern
+// words, {q, floor(2^32/q), bitmap offset} per prime, kill bitmaps.
+template <uint32_t S>
+struct TileList {
+    typedef typename std::conditional<(S <= 65536), uint16_t, uint32_t>::type T;
+};
+
+__device__ __forceinline__ void emit_record(const SieveArgs& a,
+                                            const DevPiece& P,
+                                            const DevEngine& E,
+                                            unsigned __int128 B, uint64_t off,
+                                            uint64_t v, uint32_t slot)
+{
+    unsigned __int128 x = B + off + (unsigned __int128)E.D * v;
+    if (E.family == kDickson) x *= E.W;
+    if (slot < a.out_cap) {
+        DevRec r;
+        r.t[0] = (uint32_t)x;
+        r.t[1] = (uint32_t)(x >> 32);
+        r.t[2] = (uint32_t)(x >> 64);
+        r.t[3] = (uint32_t)(x >> 96);
+        r.window = P.window;
+        r.run = -1;
+        a.out[slot] = r;
+    }
+}
+
+template <int T, int WPT>
 __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
 {
     extern __shared__ uint64_t lds64[];
     constexpr uint32_t S = 64u * T * WPT;          // line positions per tile
-    constexpr uint32_t MARK64 = BYTES ? S / 8 : 0;  // u64 words of marks
-    uint32_t* bits = reinterpret_cast<uint32_t*>(lds64);     // S/32 words
-    uint8_t* marks = reinterpret_cast<uint8_t*>(lds64 + S / 64);  // [S]
-    uint32_t* s0 = reinterpret_cast<uint32_t*>(lds64 + S / 64 + MARK64);
-    uint64_t* pat = lds64 + (S / 64) + MARK64 + ((a.nlds_max + 1) / 2);
-    __shared__ uint32_t sh_count, sh_base;
+    // listed survivors per tile: past this the tile tests word by word
+    constexpr uint32_t LCAP = (S / 16 < 4096) ? S / 16 : 4096;
+    typedef typename TileList<S>::T PosT;
+    uint32_t* bits = reinterpret_cast<uint32_t*>(lds64);          // S/32
+    uint32_t* s0 = reinterpret_cast<uint32_t*>(lds64 + S / 64);   // nlds_max
+    uint4* pinfo = reinterpret_cast<uint4*>(lds64 + S / 64 +
+                                            ((a.nlds_max + 3) / 4) * 2);
+    uint64_t* pat = reinterpret_cast<uint64_t*>(pinfo + a.nlds_max);
+    uint32_t* bm = reinterpret_cast<uint32_t*>(pat + a.pat_max);
+    PosT* listA = reinterpret_cast<PosT*>(bm + ((a.bm_max + 3) & ~3u));
+    PosT* listB = listA + LCAP;
+    __shared__ uint32_t sh_count, sh_base, sh_next, sh_n, sh_m;
 
     const uint32_t tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
     uint32_t cur_engine = 0xffffffffu;
 
     for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
@@ -123,6 +169,11 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
         if (P.engine != cur_engine) {
             __syncthreads();
             for (uint32_t w = tid; w < E.pat_words; w += T) pat[w] = E.pat[w];
+            for (uint32_t w = tid; w < E.bm_words; w += T) bm[w] = E.bm[w];
+            for (uint32_t p = tid; p < E.nlds; p += T)
+                pinfo[p] = make_uint4(E.lds_q[p], E.lds_m32[p],
+                                      p >= E.nmark ? E.bm_off[p - E.nmark] : 0,
+                                      0);
             cur_engine = P.engine;
         }
         // s0[p] = (v0 + t_p) mod q with t_p = (B + off) D^-1 mod q
@@ -134,16 +185,17 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
             t = mulmod32(t, E.lds_dinv[p], q, m64);
             s0[p] = red64(v0 + t, q, m64);
         }
+        if (tid == 0) { sh_count = 0; sh_next = 0; sh_n = 0; }
         __syncthreads();
 
-        // ---- pattern words: OR of every small prime's precomputed word
+        // ---- 2. pattern words: OR of every small prime's precomputed word
         uint64_t acc[WPT];
 #pragma unroll
         for (int k = 0; k < WPT; k++)
             acc[k] = edge_mask(v0 + 64ull * (tid + (uint32_t)k * T), vlo, vend);
         for (uint32_t p = 0; p < E.npat; p++) {
-            const uint32_t q = E.lds_q[p];
-            const uint32_t m32 = E.lds_m32[p];
+            const uint4 pq = pinfo[p];
+            const uint32_t q = pq.x, m32 = pq.y;
             const uint32_t step = red32(64u * T, q, m32);
             const uint64_t* tab = pat + E.pat_off[p];
             uint32_t s = red32(s0[p] + 64u * tid, q, m32);
@@ -160,79 +212,168 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
             bits[2 * w] = (uint32_t)acc[k];
             bits[2 * w + 1] = (uint32_t)(acc[k] >> 32);
         }
-        if (BYTES) {
-            uint4* m4 = reinterpret_cast<uint4*>(marks);
-            const uint32_t nz = ((uint32_t)len + 15) / 16;
-            for (uint32_t i = tid; i < nz; i += T) m4[i] = make_uint4(0, 0, 0, 0);
-        }
-        if (tid == 0) sh_count = 0;
         __syncthreads();
 
-        // ---- medium primes: stride marking with LDS atomics
+        // ---- 3. marking stage: stride marking with LDS atomics
         const uint32_t ulen = (uint32_t)len;
-        for (uint32_t i = tid; i < E.npairs; i += T) {
-            const uint32_t pr = E.pairs[i];
-            const uint32_t pidx = pr >> 16;
-            const uint32_t res = pr & 0xffffu;
-            const uint32_t q = E.lds_q[pidx];
-            const uint32_t s = s0[pidx];
-            uint32_t pos = (res >= s) ? res - s : res + q - s;
-            if (BYTES) {
-                for (; pos < ulen; pos += q) marks[pos] = 1;
-            } else {
+        if (E.npairs) {
+            for (uint32_t i = tid; i < E.npairs; i += T) {
+                const uint32_t pr = E.pairs[i];
+                const uint32_t pidx = pr >> 16;
+                const uint32_t res = pr & 0xffffu;
+                const uint32_t q = pinfo[pidx].x;
+                const uint32_t s = s0[pidx];
+                uint32_t pos = (res >= s) ? res - s : res + q - s;
                 for (; pos < ulen; pos += q)
                     atomicOr(&bits[pos >> 5], 1u << (pos & 31));
             }
+            __syncthreads();
         }
-        __syncthreads();
 
-        // ---- compaction: surviving bits -> candidate records
         uint64_t live[WPT];
         uint32_t cnt = 0;
 #pragma unroll
         for (int k = 0; k < WPT; k++) {
             const uint32_t w = tid + (uint32_t)k * T;
             live[k] = ~(((uint64_t)bits[2 * w + 1] << 32) | bits[2 * w]);
-            if (BYTES) {
-                uint64_t m = live[k];
-                while (m) {
-                    const int b = __builtin_ctzll(m);
-                    m &= m - 1;
-                    if (marks[64 * w + b]) live[k] &= ~(1ull << b);
-                }
-            }
             cnt += __popcll(live[k]);
         }
-        uint32_t mine = 0;
-        if (cnt) mine = atomicAdd(&sh_count, cnt);
-        __syncthreads();
-        if (tid == 0 && sh_count) sh_base = atomicAdd(a.out_count, sh_count);
-        __syncthreads();
-        if (cnt) {
-            uint32_t slot = sh_base + mine;
-            const unsigned __int128 B =
-                ((unsigned __int128)(((uint64_t)P.base[3] << 32) | P.base[2]) << 64)
-                | (((uint64_t)P.base[1] << 32) | P.base[0]);
+
+        // ---- 4. test stage
+        bool listed = false;
+        PosT* cur = listA;
+        uint32_t n = 0;
+        if (E.nmark < E.nlds) {
+            // 4a. list the live positions
+            const uint32_t at = cnt ? atomicAdd(&sh_n, cnt) : 0;
+            if (cnt && at + cnt <= LCAP) {
+                uint32_t k2 = at;
 #pragma unroll
-            for (int k = 0; k < WPT; k++) {
-                uint64_t m = live[k];
-                while (m) {
-                    const int b = __builtin_ctzll(m);
-                    m &= m - 1;
-                    const uint64_t v = v0 + 64ull * (tid + (uint32_t)k * T) + b;
-                    unsigned __int128 x = B + off + (unsigned __int128)D * v;
-                    if (E.family == kDickson) x *= E.W;
-                    if (slot < a.out_cap) {
-                        DevRec r;
-                        r.t[0] = (uint32_t)x;
-                        r.t[1] = (uint32_t)(x >> 32);
-                        r.t[2] = (uint32_t)(x >> 64);
-                        r.t[3] = (uint32_t)(x >> 96);
-                        r.window = P.window;
-                        r.run = -1;
-                        a.out[slot] = r;
+                for (int k = 0; k < WPT; k++) {
+                    uint64_t m = live[k];
+                    const uint32_t w = tid + (uint32_t)k * T;
+                    while (m) {
+                        const int b = __builtin_ctzll(m);
+                        m &= m - 1;
+                        cur[k2++] = (PosT)(64u * w + b);
                     }
-                    slot++;
+                }
+            }
+            __syncthreads();
+            n = sh_n;
+            if (n <= LCAP) {
+                listed = true;
+                // 4b. in-tile rounds over the listed candidates
+                PosT* nxt = listB;
+                uint32_t p = E.nmark;
+                for (uint32_t r = 0; r < E.ntr && n; r++) {
+                    const uint32_t pend = E.tr_end[r];
+                    if (tid == 0) sh_m = 0;
+                    __syncthreads();
+                    for (uint32_t b0 = 0; b0 < n; b0 += T) {
+                        const uint32_t i = b0 + tid;
+                        bool alive = false;
+                        uint32_t pos = 0;
+                        if (i < n) {
+                            pos = cur[i];
+                            alive = true;
+                            for (uint32_t pp = p; pp < pend; pp++) {
+                                const uint4 pq = pinfo[pp];
+                                const uint32_t x = red32(s0[pp] + pos, pq.x, pq.y);
+                                if ((bm[pq.z + (x >> 5)] >> (x & 31)) & 1) {
+                                    alive = false;
+                                    break;
+                                }
+                            }
+                        }
+                        const uint64_t mask = __ballot(alive);
+                        if (mask) {
+                            const int leader = __ffsll((unsigned long long)mask) - 1;
+                            uint32_t wb = 0;
+                            if (lane == (uint32_t)leader)
+                                wb = atomicAdd(&sh_m, (uint32_t)__popcll(mask));
+                            wb = __shfl(wb, leader);
+                            if (alive) {
+                                const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                                    (uint32_t)(mask >> 32),
+                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                                nxt[wb + below] = (PosT)pos;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    n = sh_m;
+                    PosT* t = cur; cur = nxt; nxt = t;
+                    p = pend;
+                    __syncthreads();        // every lane has read sh_m
+                }
+            } else {
+                // 4c. too many to list: test word by word (divergent but
+                // correct), then fall through to the bit compaction
+                const uint32_t nwords = (ulen + 63) / 64;
+                const uint32_t p0 = E.nmark, p1 = E.nlds;
+                for (;;) {
+                    const uint32_t w = atomicAdd(&sh_next, 1u);
+                    if (w >= nwords) break;
+                    uint64_t lv = ~(((uint64_t)bits[2 * w + 1] << 32) | bits[2 * w]);
+                    uint64_t dead = 0, m = lv;
+                    while (m) {
+                        const int b = __builtin_ctzll(m);
+                        m &= m - 1;
+                        const uint32_t pos = 64u * w + b;
+                        for (uint32_t pp = p0; pp < p1; pp++) {
+                            const uint4 pq = pinfo[pp];
+                            const uint32_t x = red32(s0[pp] + pos, pq.x, pq.y);
+                            if ((bm[pq.z + (x >> 5)] >> (x & 31)) & 1) {
+                                dead |= 1ull << b;
+                                break;
+                            }
+                        }
+                    }
+                    if (dead) {
+                        const uint64_t nw = ~(lv & ~dead);
+                        bits[2 * w] = (uint32_t)nw;
+                        bits[2 * w + 1] = (uint32_t)(nw >> 32);
+                    }
+                }
+                __syncthreads();
+                cnt = 0;
+#pragma unroll
+                for (int k = 0; k < WPT; k++) {
+                    const uint32_t w = tid + (uint32_t)k * T;
+                    live[k] = ~(((uint64_t)bits[2 * w + 1] << 32) | bits[2 * w]);
+                    cnt += __popcll(live[k]);
+                }
+            }
+        }
+
+        // ---- 5. survivors -> candidate records
+        const unsigned __int128 B =
+            ((unsigned __int128)(((uint64_t)P.base[3] << 32) | P.base[2]) << 64)
+            | (((uint64_t)P.base[1] << 32) | P.base[0]);
+        if (listed) {
+            if (tid == 0 && n) sh_base = atomicAdd(a.out_count, n);
+            __syncthreads();
+            for (uint32_t i = tid; i < n; i += T)
+                emit_record(a, P, E, B, off, v0 + cur[i], sh_base + i);
+        } else {
+            uint32_t mine = 0;
+            if (cnt) mine = atomicAdd(&sh_count, cnt);
+            __syncthreads();
+            if (tid == 0 && sh_count) sh_base = atomicAdd(a.out_count, sh_count);
+            __syncthreads();
+            if (cnt) {
+                uint32_t slot = sh_base + mine;
+#pragma unroll
+                for (int k = 0; k < WPT; k++) {
+                    uint64_t m = live[k];
+                    while (m) {
+                        const int b = __builtin_ctzll(m);
+                        m &= m - 1;
+                        emit_record(a, P, E, B, off,
+                                    v0 + 64ull * (tid + (uint32_t)k * T) + b,
+                                    slot++);
+                    }
                 }
             }
         }
@@ -242,23 +383,50 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
 
 // ---------------------------------------------------------------- rounds
 
-// A candidate staged for the kill tests, with the few engine constants the
-// tests read, so the per-prime loop touches LDS only.
-struct Staged {
-    uint32_t t[4];
+// The engine constants a kill test reads.
+struct KillEng {
     int32_t family, n;
-    uint32_t round_first, round_end, xxmax;
+    uint32_t round_first, round_end, xxmax, nround;
     uint32_t d[4];
     const uint32_t* xxbits;
+    const uint32_t* kc;
 };
 
-// Does prime q divide one of the family's values for this candidate?
-__device__ __forceinline__ bool kill_test(const Staged& c, const PrimeK& pk)
+__device__ __forceinline__ KillEng kill_eng(const DevEngine& E)
+{
+    KillEng k;
+    k.family = E.family;
+    k.n = E.n;
+    k.round_first = E.round_first;
+    k.round_end = E.round_end;
+    k.xxmax = E.xxmax;
+    k.nround = E.round_end - E.round_first;
+    for (int i = 0; i < 4; i++) k.d[i] = E.d[i];
+    k.xxbits = E.xxbits;
+    k.kc = E.kc;
+    return k;
+}
+
+// value mod q, with the one-step form for values below 2^64
+__device__ __forceinline__ uint32_t value_mod(const uint32_t t[4],
+                                              const PrimeK& pk)
+{
+    if ((t[2] | t[3]) == 0)
+        return red64(((uint64_t)t[1] << 32) | t[0], pk.q, pk.m64);
+    return red128(t, pk);
+}
+
+// Does prime q (global index p) divide one of the family's values for this
+// candidate?  Euler: q | x^2+x+p; Dickson (t = k): q | m k^2 + 1, m <= n;
+// AP: q | p + j d, j < n, via j0 = -p d^-1 mod q (kc holds d^-1 mod q, or 0
+// when q | d and only p == 0 mod q kills).
+__device__ __forceinline__ bool kill_test(const KillEng& c,
+                                          const uint32_t t[4],
+                                          const PrimeK& pk, uint32_t p)
 {
     const uint32_t q = pk.q;
-    const uint32_t r = red128(c.t, pk);
+    const uint32_t r = value_mod(t, pk);
     if (c.family == kEuler) {
-        // q | x^2+x+p  <=>  (q - p mod q) mod q == x^2+x (mod q)
         if (q > c.xxmax) {
             const uint32_t dd = r ? q - r : 0;
             return dd <= c.xxmax && ((c.xxbits[dd >> 5] >> (dd & 31)) & 1);
@@ -269,7 +437,6 @@ __device__ __forceinline__ bool kill_test(const Staged& c, const PrimeK& pk)
         }
         return false;
     } else if (c.family == kDickson) {
-        // r = k mod q; q | m k^2 + 1 for some m <= n
         const uint32_t kk = mulmod32(r, r, q, pk.m64);
         if (kk == 0) return false;
         uint32_t v = kk + 1;
@@ -281,67 +448,94 @@ __device__ __forceinline__ bool kill_test(const Staged& c, const PrimeK& pk)
         }
         return false;
     } else {
-        // q | p + j d for some j < n
-        const uint32_t dq = red128(c.d, pk);
-        if (dq == 0) return r == 0;
-        uint32_t f = r;
-        for (int j = 0; j < c.n; j++) {
-            if (f == 0) return true;
-            f += dq;
-            if (f >= q) f -= q;
-        }
-        return false;
+        const uint32_t dinv = c.kc[p - c.round_first];
+        if (dinv == 0) return r == 0;
+        const uint32_t j0 = mulmod32(r ? q - r : 0, dinv, q, pk.m64);
+        return j0 < (uint32_t)c.n;
     }
 }
 
-// A workgroup owns 256 consecutive primes of the round (one per lane, its
-// constants held in registers) and walks the candidate queue RT_RECS
-// records at a time, staged in LDS; every lane tests its prime against each
-// staged record, so no lane runs a serial chain of dependent loads.
-constexpr int RT_RECS = 16;
+// Dense rounds (more candidates than primes): lane = candidate; the
+// workgroup stages a block of RD_PRIMES primes (and, for AP, their kill
+// constants) and the batch's engine constants in LDS, then every lane
+// walks the block for its own candidate.
+constexpr uint32_t RD_PRIMES = 64;
+constexpr uint32_t RD_ENGINES = 64;
 
-__global__ __launch_bounds__(256) void k_round_test(RoundArgs a)
+__global__ __launch_bounds__(256) void k_round_dense(RoundArgs a)
 {
-    __shared__ Staged st[RT_RECS];
-    __shared__ uint32_t slive[RT_RECS];
+    __shared__ PrimeK sp[RD_PRIMES];
+    __shared__ KillEng se[RD_ENGINES];
+    const uint32_t count = min(*a.count, a.cap);
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t e = tid; e < a.nengines && e < RD_ENGINES; e += blockDim.x)
+        se[e] = kill_eng(a.engines[e]);
+    const uint32_t nblk = (a.p_end - a.p_begin + RD_PRIMES - 1) / RD_PRIMES;
+    for (uint32_t pb = blockIdx.y; pb < nblk; pb += gridDim.y) {
+        const uint32_t p0 = a.p_begin + pb * RD_PRIMES;
+        const uint32_t p1 = min(p0 + RD_PRIMES, a.p_end);
+        __syncthreads();
+        if (tid < p1 - p0) sp[tid] = a.primes[p0 + tid];
+        __syncthreads();
+        for (uint32_t i = blockIdx.x * blockDim.x + tid; i < count;
+             i += gridDim.x * blockDim.x) {
+            if (a.flags[i]) continue;
+            const DevRec rec = a.recs[i];
+            const uint32_t eidx = a.win_engine[rec.window];
+            const KillEng c = eidx < RD_ENGINES ? se[eidx]
+                                                : kill_eng(a.engines[eidx]);
+            const uint32_t lo = max(p0, c.round_first);
+            const uint32_t hi = min(p1, c.round_end);
+            for (uint32_t p = lo; p < hi; p++) {
+                if (kill_test(c, rec.t, sp[p - p0], p)) {
+                    a.flags[i] = 1;
+                    break;
+                }
+            }
+        }
+    }
+}
+
+// Sparse rounds (more primes than candidates): lane = prime; RS_RECS
+// candidates at a time are staged in LDS by all lanes at once, and every
+// lane tests its prime against each of them.
+constexpr int RS_RECS = 256;
+
+__global__ __launch_bounds__(256) void k_round_sparse(RoundArgs a)
+{
+    __shared__ uint4 st_t[RS_RECS];
+    __shared__ KillEng st_e[RS_RECS];
+    __shared__ uint32_t st_live[RS_RECS];
     const uint32_t count = min(*a.count, a.cap);
     const uint32_t tid = threadIdx.x;
     for (uint32_t pb = blockIdx.x; pb < a.nchunks; pb += gridDim.x) {
         const uint32_t p = a.p_begin + pb * 256 + tid;
         const bool have = p < a.p_end;
         const PrimeK pk = have ? a.primes[p] : a.primes[a.p_begin];
-        for (uint32_t r0 = blockIdx.y * RT_RECS; r0 < count;
-             r0 += gridDim.y * RT_RECS) {
+        for (uint32_t r0 = blockIdx.y * RS_RECS; r0 < count;
+             r0 += gridDim.y * RS_RECS) {
             __syncthreads();
-            if (tid < RT_RECS) {
-                const uint32_t i = r0 + tid;
+            for (uint32_t k = tid; k < RS_RECS; k += blockDim.x) {
+                const uint32_t i = r0 + k;
                 uint32_t live = 0;
                 if (i < count && a.flags[i] == 0) {
                     const DevRec rec = a.recs[i];
-                    const DevEngine& E = a.engines[a.win_engine[rec.window]];
-                    Staged c;
-                    for (int k = 0; k < 4; k++) {
-                        c.t[k] = rec.t[k];
-                        c.d[k] = E.d[k];
-                    }
-                    c.family = E.family;
-                    c.n = E.n;
-                    c.round_first = E.round_first;
-                    c.round_end = E.round_end;
-                    c.xxmax = E.xxmax;
-                    c.xxbits = E.xxbits;
-                    st[tid] = c;
+                    st_t[k] = make_uint4(rec.t[0], rec.t[1], rec.t[2], rec.t[3]);
+                    st_e[k] = kill_eng(a.engines[a.win_engine[rec.window]]);
                     live = 1;
                 }
-                slive[tid] = live;
+                st_live[k] = live;
             }
             __syncthreads();
             if (!have) continue;
-            for (int k = 0; k < RT_RECS; k++) {
-                if (!slive[k]) continue;
-                const Staged& c = st[k];
-                if (p >= c.round_first && p < c.round_end && kill_test(c, pk))
-                    a.flags[r0 + k] = 1;
+            const uint32_t nk = min((uint32_t)RS_RECS, count - r0);
+            for (uint32_t k = 0; k < nk; k++) {
+                if (!st_live[k]) continue;
+                const KillEng& c = st_e[k];
+                if (p < c.round_first || p >= c.round_end) continue;
+                const uint4 tv = st_t[k];
+                const uint32_t t[4] = {tv.x, tv.y, tv.z, tv.w};
+                if (kill_test(c, t, pk, p)) a.flags[r0 + k] = 1;
             }
         }
     }
@@ -519,12 +713,12 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n)
     X(512, 1) X(512, 2) X(512, 4) X(512, 8)                                  \
     X(1024, 1) X(1024, 2) X(1024, 4)
 
-hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt, int bytes,
+hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,
                         uint32_t grid, size_t lds_bytes, hipStream_t stream)
 {
 #define FK_SIEVE_CASE(TT, WW)                                              \
     if (threads == TT && wpt == WW) {                                      \
-        auto fn = bytes ? k_sieve<TT, WW, 1> : k_sieve<TT, WW, 0>;         \
+        auto fn = k_sieve<TT, WW>;                                         \
         if (lds_bytes > 65536)                                             \
             (void)hipFuncSetAttribute((const void*)fn,                     \
                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes); \
@@ -536,12 +730,12 @@ hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt, int bytes,
     return hipErrorInvalidValue;
 }
 
-int sieve_occupancy(int threads, int wpt, int bytes, size_t lds_bytes)
+int sieve_occupancy(int threads, int wpt, size_t lds_bytes)
 {
     int blocks = 0;
 #define FK_OCC_CASE(TT, WW)                                                 \
     if (threads == TT && wpt == WW) {                                       \
-        auto fn = bytes ? k_sieve<TT, WW, 1> : k_sieve<TT, WW, 0>;          \
+        auto fn = k_sieve<TT, WW>;                                          \
         if (lds_bytes > 65536)                                              \
             (void)hipFuncSetAttribute((const void*)fn,                      \
                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);  \
@@ -575,14 +769,16 @@ hipError_t launch_prep(const DevEngine* engines, const DevPiece* pieces,
     return hipGetLastError();
 }
 
-hipError_t launch_round_test(const RoundArgs& a, uint32_t gy,
-                             hipStream_t stream)
+hipError_t launch_round_test(const RoundArgs& a, bool dense, uint32_t gx,
+                             uint32_t gy, hipStream_t stream)
 {
-    uint32_t gx = a.nchunks ? a.nchunks : 1;
-    if (gx > 1u << 20) gx = 1u << 20;
+    if (gx < 1) gx = 1;
     if (gy < 1) gy = 1;
     if (gy > 65535) gy = 65535;
-    hipLaunchKernelGGL(k_round_test, dim3(gx, gy), dim3(256), 0, stream, a);
+    if (dense)
+        hipLaunchKernelGGL(k_round_dense, dim3(gx, gy), dim3(256), 0, stream, a);
+    else
+        hipLaunchKernelGGL(k_round_sparse, dim3(gx, gy), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 

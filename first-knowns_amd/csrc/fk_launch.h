@@ -21,7 +21,9 @@ struct SieveArgs {
     const DevPiece* pieces;
     uint32_t npieces;
     uint32_t ntiles;
-    uint32_t nlds_max;        // LDS room reserved for s0[] (max over engines)
+    uint32_t nlds_max;        // LDS room for per-prime arrays (max over engines)
+    uint32_t pat_max;         // LDS room for pattern words (max over engines)
+    uint32_t bm_max;          // LDS room for kill-bitmap words (max over engines)
     const uint32_t* bq;       // per piece, per LDS prime: base mod q
     DevRec* out;
     uint32_t* out_count;
@@ -30,6 +32,7 @@ struct SieveArgs {
 
 struct RoundArgs {
     const DevEngine* engines;
+    uint32_t nengines;
     const uint32_t* win_engine;
     const PrimeK* primes;     // the device-wide prime table
     const DevRec* recs;
@@ -37,8 +40,8 @@ struct RoundArgs {
     uint32_t cap;
     uint32_t* flags;
     uint32_t p_begin, p_end;  // global prime indices of this round
-    uint32_t chunk;           // unused: a workgroup takes 256 primes
-    uint32_t nchunks;         // workgroups of 256 primes in the round
+    uint32_t chunk;           // unused
+    uint32_t nchunks;         // sparse rounds: blocks of 256 primes
 };
 
 struct CompactArgs {
@@ -65,12 +68,12 @@ struct ClassifyArgs {
 hipError_t launch_prep(const DevEngine* engines, const DevPiece* pieces,
                        uint32_t npieces, uint32_t max_nlds, uint32_t* bq,
                        hipStream_t stream);
-hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt, int bytes,
+hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,
                         uint32_t grid, size_t lds_bytes, hipStream_t stream);
-int sieve_occupancy(int threads, int wpt, int bytes, size_t lds_bytes);
+int sieve_occupancy(int threads, int wpt, size_t lds_bytes);
 bool sieve_shape_ok(int threads, int wpt);
-hipError_t launch_round_test(const RoundArgs& a, uint32_t gy,
-                             hipStream_t stream);
+hipError_t launch_round_test(const RoundArgs& a, bool dense, uint32_t gx,
+                             uint32_t gy, hipStream_t stream);
 hipError_t launch_round_compact(const CompactArgs& a, uint32_t grid,
                                 hipStream_t stream);
 hipError_t launch_classify(const ClassifyArgs& a, uint32_t grid,

@@ -6,9 +6,11 @@ tuner is free to pick whatever is fastest:
 
   threads, words_per_thread   workgroup size and tile length of k_sieve
   q_pat                       primes below it are OR-ed in as pattern words
-  q_med                       primes below it are stride-marked in LDS;
-                              above it they are tested per candidate in
-                              compaction rounds
+  q_mark                      primes in [q_pat, q_mark) are stride-marked
+                              in LDS
+  q_med                       primes in [q_mark, q_med) are tested per
+                              surviving position against LDS kill bitmaps;
+                              above it per candidate in compaction rounds
   nwheel                      kill primes enumerated by the wheel
   round_growth                prime-range growth factor between rounds
 
@@ -42,10 +44,10 @@ SPACE = {
     "threads": [256, 512, 1024],
     "words_per_thread": [1, 2, 4, 8],
     "q_pat": [64, 128, 192, 256, 320],
-    "q_med": [2048, 4096, 8192, 16384, 32768],
+    "q_med": [512, 1024, 1536, 2048, 3072, 4096, 8192, 16384],
     "nwheel": [0, 3, 4, 5, 6, 7],
     "round_growth": [2, 3, 4],
-    "mark_bytes": [0, 1],
+    "q_mark": [0, 192, 256, 512, 1024],
 }
 
 VALID_SHAPES = {(256, 1), (256, 2), (256, 4), (256, 8), (256, 16),
@@ -89,12 +91,12 @@ def tune(family, n, q2, lo, span, passes=2, reps=3, classify=None,
     cur = dict(F.tuning(family))
     cur.setdefault("nwheel", 0)
     cur.setdefault("round_growth", 2)
-    cur.setdefault("mark_bytes", 0)
+    cur.setdefault("q_mark", 0)
     cur.pop("round_chunk", None)
     best, st = measure(family, n, q2, lo, span, cur, reps, classify,
                        wheel_primes)
     log(f"start {cur} -> {best:.3f} ms {st}")
-    keys = ["mark_bytes", "q_med", "q_pat", "threads", "words_per_thread",
+    keys = ["q_med", "q_mark", "q_pat", "threads", "words_per_thread",
             "round_growth"]
     if not wheel_primes:
         keys.append("nwheel")

@@ -123,11 +123,11 @@ def ap_difference(n):
 # Defaults measured on MI355X (see the autotuner, fk_autotune.py); a
 # fk_tuning.json written by the autotuner overrides them per family.
 DEFAULT_TUNING = {
-    "euler": {"q_pat": 128, "q_med": 2048, "threads": 256,
+    "euler": {"q_pat": 128, "q_mark": 0, "q_med": 1024, "threads": 256,
               "words_per_thread": 8, "round_chunk": 64},
-    "dickson": {"q_pat": 128, "q_med": 4096, "threads": 256,
+    "dickson": {"q_pat": 128, "q_mark": 0, "q_med": 2048, "threads": 256,
                 "words_per_thread": 8, "round_chunk": 64},
-    "ap": {"q_pat": 128, "q_med": 4096, "threads": 256,
+    "ap": {"q_pat": 128, "q_mark": 0, "q_med": 2048, "threads": 256,
            "words_per_thread": 8, "round_chunk": 64},
 }
 

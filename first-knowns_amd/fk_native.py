@@ -39,15 +39,16 @@ class EngineDesc(ctypes.Structure):
                 ("nwheel", i32), ("wheel_primes", u32 * 12),
                 ("q_pat", u32), ("q_med", u32), ("threads", u32),
                 ("words_per_thread", u32), ("round_chunk", u32),
-                ("round_growth", u32), ("mark_bytes", u32)]
+                ("round_growth", u32), ("q_mark", u32)]
 
 
 class EngineInfo(ctypes.Structure):
     _fields_ = [("family", i32), ("n", i32), ("q2", u32), ("nwheel", u32),
                 ("wheel_primes", u32 * 16), ("line_step", u64),
                 ("nlines", u64), ("W", u64), ("d", u64 * 4),
-                ("q_pat", u32), ("q_med", u32), ("threads", u32),
-                ("words_per_thread", u32), ("n_lds_primes", u32),
+                ("q_pat", u32), ("q_mark", u32), ("q_med", u32),
+                ("threads", u32), ("words_per_thread", u32),
+                ("n_lds_primes", u32),
                 ("n_pattern_primes", u32), ("n_round_primes", u32),
                 ("n_rounds", u32), ("device_bytes", u64),
                 ("survival_lds", ctypes.c_double),
@@ -163,7 +164,7 @@ class Engine:
 
     def __init__(self, family, n, q2, device=0, nwheel=0, wheel_primes=None,
                  q_pat=0, q_med=0, threads=0, words_per_thread=0,
-                 round_chunk=0, round_growth=0, mark_bytes=0):
+                 round_chunk=0, round_growth=0, q_mark=0):
         d = EngineDesc()
         d.family, d.n, d.q2, d.device = int(family), int(n), int(q2), device
         d.nwheel = int(nwheel)
@@ -172,7 +173,7 @@ class Engine:
         d.q_pat, d.q_med = int(q_pat), int(q_med)
         d.threads, d.words_per_thread = int(threads), int(words_per_thread)
         d.round_chunk, d.round_growth = int(round_chunk), int(round_growth)
-        d.mark_bytes = int(mark_bytes)
+        d.q_mark = int(q_mark)
         h = ctypes.c_void_p()
         check(lib().fk_engine_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h

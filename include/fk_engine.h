@@ -67,15 +67,17 @@ typedef struct fk_engine_desc {
     int32_t nwheel;          /* kill primes enumerated by the wheel (0 = default) */
     uint32_t wheel_primes[12];  /* explicit wheel (Euler), else zeros */
     uint32_t q_pat;          /* primes below: 64-bit pattern words (0 = default) */
-    uint32_t q_med;          /* primes below: LDS stride marking; at and above:
-                                per-candidate rounds (0 = default) */
+    uint32_t q_med;          /* primes below: LDS stage (marking or bitmap
+                                tests); at and above: per-candidate rounds
+                                (0 = default) */
     uint32_t threads;        /* sieve workgroup size: 256, 512, 1024 (0 = default) */
     uint32_t words_per_thread; /* tile = 64 * threads * this bits (0 = default) */
     uint32_t round_chunk;    /* primes per round work item (0 = default) */
     uint32_t round_growth;   /* prime bound grows by this factor per round
                                 (0 = default 2) */
-    uint32_t mark_bytes;     /* 1: medium-prime marks as LDS bytes (plain
-                                stores); 0: as LDS bits (atomic OR) */
+    uint32_t q_mark;         /* primes in [q_pat, q_mark) are stride-marked
+                                in LDS; [q_mark, q_med) are tested per
+                                surviving position (0 = q_pat: no marking) */
 } fk_engine_desc;
 
 typedef struct fk_engine_info {
@@ -87,7 +89,7 @@ typedef struct fk_engine_info {
     uint64_t nlines;         /* admissible residues of the wheel */
     uint64_t W;              /* Dickson k-wheel W(n); 1 otherwise */
     uint64_t d[4];           /* AP difference P(n) (256-bit LE); 0 otherwise */
-    uint32_t q_pat, q_med, threads, words_per_thread;
+    uint32_t q_pat, q_mark, q_med, threads, words_per_thread;
     uint32_t n_lds_primes, n_pattern_primes, n_round_primes, n_rounds;
     uint64_t device_bytes;   /* device memory the engine holds */
     double survival_lds;     /* expected fraction of wheel candidates that

@@ -40,7 +40,7 @@ def test_struct_sizes_match_header():
     assert ctypes.sizeof(N.Window) == 32
     assert ctypes.sizeof(N.ClassifySpec) == 12
     assert N.RECORD_DTYPE.itemsize == 24
-    assert ctypes.sizeof(N.EngineInfo) == 16 + 64 + 8 * 7 + 32 + 8 + 8 + 8
+    assert ctypes.sizeof(N.EngineInfo) == 16 + 64 + 8 * 7 + 36 + 4 + 8 + 8 + 8
 
 
 def test_status_codes_match_header():

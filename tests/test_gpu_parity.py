@@ -202,12 +202,12 @@ def test_streams_do_not_depend_on_cuts_or_tiles():
                {"threads": 1024, "words_per_thread": 1},
                {"threads": 256, "words_per_thread": 16},
                {"q_med": 1024, "q_pat": 64},
-               {"q_med": 16384, "q_pat": 256},
+               {"q_med": 2048, "q_pat": 256},
                {"nwheel": 1}, {"nwheel": 2},
                {"round_growth": 4, "round_chunk": 16},
-               {"mark_bytes": 1},
-               {"mark_bytes": 1, "threads": 1024, "words_per_thread": 1},
-               {"mark_bytes": 1, "q_pat": 256, "q_med": 32768}):
+               {"q_mark": 1024}, {"q_mark": 4096},
+               {"q_mark": 256, "threads": 1024, "words_per_thread": 1},
+               {"q_mark": 512, "q_med": 3072, "q_pat": 64}):
         eng = ladder_gpu.GpuEngine(n, q2=q2, **kw)
         assert eng.survivors_j(lo, hi).tolist() == ref, kw
     eng = ladder_gpu.GpuEngine(n, q2=q2)
