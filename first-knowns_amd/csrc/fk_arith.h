@@ -10,6 +10,7 @@
 // what gfx950 executes as v_mad_u64_u32.
 #pragma once
 
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -106,6 +107,51 @@ FK_HD PrimeK make_primek(uint32_t q)
     // for an odd prime; q = 2 gives 2^63 either way.
     if (q == 2) p.m64 = (uint64_t)1 << 63;
     return p;
+}
+
+// ------------------------------------------- floating-point residues
+// A value split into 16-bit limbs a_i is congruent mod q to
+// s = sum a_i * c_i with c_i = 2^(16 i) mod q.  For q < 2^30 and at most
+// eight limbs every product is below 2^46 and s below 2^49, so fused
+// multiply-adds in double precision form s exactly.  With qi the double
+// nearest 1/q, floor(s * qi) is within one of floor(s / q) (the relative
+// error of s * qi is below 2^-52 and s / q below 2^49), so s - floor(s*qi) q,
+// formed exactly by one more fma, lies in [-q, 2q) and two unsigned min
+// steps bring it into [0, q).  The double-precision pipe runs these at full
+// rate, where 32-bit integer multiplies are quarter rate.
+struct PrimeD {
+    uint32_t q;
+    uint32_t pad;
+    double qd;             // q
+    double qi;             // nearest double to 1/q
+    double c[7];           // 2^16, 2^32, ..., 2^112 mod q
+};
+
+FK_HD PrimeD make_primed(uint32_t q)
+{
+    PrimeD p;
+    p.qd = (double)q;
+    p.qi = 1.0 / (double)q;
+    uint64_t c = 1;
+    for (int i = 0; i < 7; i++) {
+        c = (c << 16) % q;
+        p.c[i] = (double)c;
+    }
+    p.q = q;
+    p.pad = 0;
+    return p;
+}
+
+// s (an integer below 2^49 held exactly) mod q, for q < 2^30.
+FK_HD uint32_t resid_d(double s, double qd, double qi, uint32_t q)
+{
+    const double qt = floor(s * qi);
+    const double r = fma(-qt, qd, s);            // exact, in [-q, 2q)
+    uint32_t u = (uint32_t)(int32_t)r;
+    const uint32_t up = u + q;                   // fixes [-q, 0)
+    u = u < up ? u : up;
+    const uint32_t dn = u - q;                   // fixes [q, 2q)
+    return u < dn ? u : dn;
 }
 
 // ----------------------------------------------------- multiword values

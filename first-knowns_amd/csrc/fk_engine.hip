@@ -19,6 +19,7 @@
 
 #include "../../include/fk_engine.h"
 #include "fk_device.h"
+#include "fk_kill.h"
 #include "fk_launch.h"
 #include "fk_plan.h"
 
@@ -56,7 +57,8 @@ struct DeviceCtx {
     int cus = 0;
     uint32_t limit = 0;             // primes below this are in the table
     std::vector<uint32_t> primes;   // host copy
-    PrimeK* d_primes = nullptr;
+    PrimeK* d_primes = nullptr;     // Barrett form (table builds)
+    PrimeD* d_primed = nullptr;     // double-precision form (rounds)
 };
 
 std::mutex g_ctx_mu;
@@ -80,16 +82,26 @@ int ctx_for(int device, uint32_t need_limit, DeviceCtx** out)
         uint32_t lim = std::max(need_limit, 1u << 16);
         std::vector<uint32_t> ps = primes_below(lim);
         std::vector<PrimeK> pk(ps.size());
-        for (size_t i = 0; i < ps.size(); i++) pk[i] = make_primek(ps[i]);
+        std::vector<PrimeD> pd(ps.size());
+        for (size_t i = 0; i < ps.size(); i++) {
+            pk[i] = make_primek(ps[i]);
+            pd[i] = make_primed(ps[i]);
+        }
         PrimeK* d = nullptr;
+        PrimeD* dd = nullptr;
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipMalloc(&d, pk.size() * sizeof(PrimeK)));
         HIPCHK(hipMemcpy(d, pk.data(), pk.size() * sizeof(PrimeK),
                          hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&dd, pd.size() * sizeof(PrimeD)));
+        HIPCHK(hipMemcpy(dd, pd.data(), pd.size() * sizeof(PrimeD),
+                         hipMemcpyHostToDevice));
         // engines still hold indices only, which stay valid: the table is
         // the same ascending prime list, extended
         if (c->d_primes) (void)hipFree(c->d_primes);
+        if (c->d_primed) (void)hipFree(c->d_primed);
         c->d_primes = d;
+        c->d_primed = dd;
         c->primes.swap(ps);
         c->limit = lim;
     }
@@ -337,16 +349,6 @@ static int build_engine(fk_engine* e)
     // ---- family constants
     e->W = (d.family == FK_FAMILY_DICKSON) ? dickson_wheel(d.n) : 1;
     e->dval = (d.family == FK_FAMILY_AP) ? ap_difference(d.n) : 0;
-    std::vector<uint32_t> xx;
-    uint32_t xxmax = 0;
-    if (d.family == FK_FAMILY_EULER) {
-        xxmax = (uint32_t)((d.n - 1) * (d.n - 1) + (d.n - 1));
-        xx.assign(xxmax / 32 + 1, 0);
-        for (int x = 0; x < d.n; x++) {
-            const uint32_t v = (uint32_t)(x * x + x);
-            xx[v >> 5] |= 1u << (v & 31);
-        }
-    }
     if (e->W > 0xffffffffull) return fail(FK_E_ARG, "W(n) exceeds 32 bits");
 
     HIPCHK(hipSetDevice(d.device));
@@ -363,7 +365,7 @@ static int build_engine(fk_engine* e)
     h.ntr = (uint32_t)tr_end.size();
     h.round_first = e->round_first;
     h.round_end = e->round_end;
-    h.xxmax = xxmax;
+    h.kstride = std::max(16u, (1u + kill_max(d.family, d.n) + 3u) & ~3u);
     h.D = e->D;
     h.W = (uint32_t)e->W;
     for (int i = 0; i < 4; i++) {
@@ -382,20 +384,18 @@ static int build_engine(fk_engine* e)
     if ((rc = eng_upload(e, bm, &h.bm))) return rc;
     if ((rc = eng_upload(e, tr_end, &h.tr_end))) return rc;
     {
-        // AP kill constants of the round primes: P(n)^-1 mod q, 0 if q | P(n)
-        std::vector<uint32_t> kc;
-        if (d.family == FK_FAMILY_AP) {
-            kc.resize(e->round_end - e->round_first);
-            for (uint32_t i = e->round_first; i < e->round_end; i++) {
-                const uint32_t q = ctx->primes[i];
-                const uint32_t dq = (uint32_t)(e->dval % q);
-                kc[i - e->round_first] = dq ? inv_mod(dq, q) : 0;
-            }
-        }
-        if ((rc = eng_upload(e, kc, &h.kc))) return rc;
+        // kill sets of the round primes, built on the device
+        const uint32_t np = e->round_end - e->round_first;
+        const size_t nt = std::max<size_t>((size_t)np * h.kstride, 1);
+        uint32_t* kt = nullptr;
+        HIPCHK(hipMalloc(&kt, nt * sizeof(uint32_t)));
+        e->allocs.push_back(kt);
+        e->device_bytes += nt * sizeof(uint32_t);
+        HIPCHK(launch_kill_table(ctx->d_primes, e->round_first, np, d.family,
+                                 d.n, h.d, h.kstride, kt, nullptr));
+        HIPCHK(hipDeviceSynchronize());
+        h.kt = kt;
     }
-    if (xx.empty()) xx.push_back(0);
-    if ((rc = eng_upload(e, xx, &h.xxbits))) return rc;
     std::vector<DevEngine> one(1, h);
     const DevEngine* de = nullptr;
     if ((rc = eng_upload(e, one, &de))) return rc;
@@ -704,28 +704,23 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         ra.engines = b->d_engines;
         ra.nengines = (uint32_t)b->engines.size();
         ra.win_engine = b->d_win_engine;
-        ra.primes = ctx->d_primes;
+        ra.primed = ctx->d_primed;
         ra.recs = b->d_q[cur];
         ra.count = b->d_counts + cslot;
         ra.cap = queue_cap;
         ra.flags = b->d_flags;
         ra.p_begin = prime_index(ctx->primes, rounds[r].first);
         ra.p_end = prime_index(ctx->primes, rounds[r].second);
-        ra.chunk = 0;
         const uint32_t np = ra.p_end > ra.p_begin ? ra.p_end - ra.p_begin : 0;
-        ra.nchunks = (np + 255) / 256;
-        if (ra.nchunks == 0) continue;
-        // dense (lane = candidate) when candidates outnumber primes
-        const bool dense = alive > (double)np;
-        uint32_t gx, gy;
-        if (dense) {
-            gx = (uint32_t)std::min(2048.0, std::ceil(alive / 256.0));
-            gy = (np + 63) / 64;
-        } else {
-            gx = ra.nchunks;
-            gy = (uint32_t)std::min(1024.0, std::ceil(alive / 256.0));
-        }
-        HIPCHK(launch_round_test(ra, dense, gx, gy, s));
+        if (np == 0) continue;
+        // about 2048 workgroups: candidate blocks times prime blocks of at
+        // least 16 primes
+        const uint32_t gx = (uint32_t)std::min(1024.0, std::max(1.0, std::ceil(alive / 256.0)));
+        uint32_t gy = std::min((np + 15) / 16, (2048 + gx - 1) / gx);
+        gy = std::max(1u, std::min(gy, 65535u));
+        ra.pblock = (np + gy - 1) / gy;
+        gy = (np + ra.pblock - 1) / ra.pblock;
+        HIPCHK(launch_round_test(ra, gx, gy, s));
         CompactArgs ca{};
         ca.in = b->d_q[cur];
         ca.in_count = b->d_counts + cslot;

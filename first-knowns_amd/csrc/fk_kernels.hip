@@ -8,11 +8,13 @@
 //                   primes, optional ds_or stride marking, per-survivor
 //                   tests against LDS kill bitmaps for the medium primes,
 //                   then compaction of the surviving bits into the queue
-//   k_round_dense / k_round_sparse
-//                   per-candidate kill tests for one range of large primes,
-//                   lane = candidate (dense rounds) or lane = prime with the
-//                   candidates staged through LDS (sparse rounds); killed
-//                   candidates flagged
+//   k_round         per-candidate kill tests for one range of large primes:
+//                   lane = candidate, the prime uniform across the wave (its
+//                   constants and kill set in scalar registers), the residue
+//                   formed exactly in double precision; killed candidates
+//                   flagged
+//   k_kill_table    (engine creation) the value-domain kill set of every
+//                   round prime
 //   k_round_compact wave-ballot compaction of the unflagged candidates
 //   k_cls_*         strong-probable-prime chains (Montgomery, 2..8 limbs),
 //                   parallel over (candidate, value, base)
@@ -28,6 +30,7 @@
 #include <type_traits>
 
 #include "fk_device.h"
+#include "fk_kill.h"
 #include "fk_launch.h"
 
 namespace fk {
@@ -383,161 +386,130 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
 
 // ---------------------------------------------------------------- rounds
 
-// The engine constants a kill test reads.
-struct KillEng {
-    int32_t family, n;
-    uint32_t round_first, round_end, xxmax, nround;
-    uint32_t d[4];
-    const uint32_t* xxbits;
-    const uint32_t* kc;
-};
-
-__device__ __forceinline__ KillEng kill_eng(const DevEngine& E)
+// Kill-set table of each engine's round primes, built once at engine
+// creation: one thread per prime.  Row layout: [count, residues...,
+// kKillPad...], stride words per row.
+__global__ void k_kill_table(const PrimeK* __restrict__ primes, uint32_t p0,
+                             uint32_t np, int family, int n, uint4 d,
+                             uint32_t stride, uint32_t* __restrict__ kt)
 {
-    KillEng k;
-    k.family = E.family;
-    k.n = E.n;
-    k.round_first = E.round_first;
-    k.round_end = E.round_end;
-    k.xxmax = E.xxmax;
-    k.nround = E.round_end - E.round_first;
-    for (int i = 0; i < 4; i++) k.d[i] = E.d[i];
-    k.xxbits = E.xxbits;
-    k.kc = E.kc;
-    return k;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const uint32_t dl[4] = {d.x, d.y, d.z, d.w};
+    uint32_t* row = kt + (size_t)i * stride;
+    row[0] = kill_values(family, n, dl, primes[p0 + i], row + 1, stride - 1);
 }
 
-// value mod q, with the one-step form for values below 2^64
-__device__ __forceinline__ uint32_t value_mod(const uint32_t t[4],
-                                              const PrimeK& pk)
+// Read-only tables the kernels index with wave-uniform indices, viewed in
+// the constant address space: loads through it become scalar loads into
+// SGPRs (the compiler cannot otherwise prove that the kernel's own stores
+// leave them alone).
+#define FK_CONST __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const FK_CONST T* uniform_view(const T* p)
 {
-    if ((t[2] | t[3]) == 0)
-        return red64(((uint64_t)t[1] << 32) | t[0], pk.q, pk.m64);
-    return red128(t, pk);
+    return (const FK_CONST T*)p;
 }
 
-// Does prime q (global index p) divide one of the family's values for this
-// candidate?  Euler: q | x^2+x+p; Dickson (t = k): q | m k^2 + 1, m <= n;
-// AP: q | p + j d, j < n, via j0 = -p d^-1 mod q (kc holds d^-1 mod q, or 0
-// when q | d and only p == 0 mod q kills).
-__device__ __forceinline__ bool kill_test(const KillEng& c,
-                                          const uint32_t t[4],
-                                          const PrimeK& pk, uint32_t p)
+// The candidate's residue mod the prime from its NL 16-bit limbs.
+template <int NL>
+__device__ __forceinline__ uint32_t limb_residue(const double (&a)[8],
+                                                 const FK_CONST PrimeD* P)
 {
-    const uint32_t q = pk.q;
-    const uint32_t r = value_mod(t, pk);
-    if (c.family == kEuler) {
-        if (q > c.xxmax) {
-            const uint32_t dd = r ? q - r : 0;
-            return dd <= c.xxmax && ((c.xxbits[dd >> 5] >> (dd & 31)) & 1);
-        }
-        for (int x = 0; x < c.n; x++) {
-            const uint64_t v = (uint64_t)x * x + x + r;
-            if (v % q == 0) return true;
-        }
-        return false;
-    } else if (c.family == kDickson) {
-        const uint32_t kk = mulmod32(r, r, q, pk.m64);
-        if (kk == 0) return false;
-        uint32_t v = kk + 1;
-        if (v >= q) v -= q;
-        for (int m = 1; m <= c.n; m++) {
-            if (v == 0) return true;
-            v += kk;
-            if (v >= q) v -= q;
-        }
-        return false;
-    } else {
-        const uint32_t dinv = c.kc[p - c.round_first];
-        if (dinv == 0) return r == 0;
-        const uint32_t j0 = mulmod32(r ? q - r : 0, dinv, q, pk.m64);
-        return j0 < (uint32_t)c.n;
+    double s = a[0];
+#pragma unroll
+    for (int k = 1; k < NL; k++) s = fma(a[k], P->c[k - 1], s);
+    return resid_d(s, P->qd, P->qi, P->q);
+}
+
+__device__ __forceinline__ uint4 load4(const FK_CONST uint32_t* p)
+{
+    return make_uint4(p[0], p[1], p[2], p[3]);
+}
+
+// Lanes whose residue r matches one of the four words.
+__device__ __forceinline__ uint64_t match4(uint32_t r, uint4 v)
+{
+    return __ballot(r == v.x) | __ballot(r == v.y) | __ballot(r == v.z) |
+           __ballot(r == v.w);
+}
+
+// Primes [lo, hi) of one engine against this lane's candidate; returns the
+// mask of lanes killed.  The prime index is uniform across the wave, so the
+// prime's constants and kill-set row are scalar loads; the lane does the
+// reduction and the compares (one v_cmp into a lane mask per residue).  The
+// first 16 words of a row (count and up to 15 residues, padded) are read
+// unconditionally; longer rows continue in a loop.
+template <int NL>
+__device__ __forceinline__ uint64_t round_scan(const double (&a)[8],
+                                               const FK_CONST PrimeD* pd,
+                                               const FK_CONST uint32_t* kt,
+                                               uint32_t stride, uint32_t rf,
+                                               uint32_t lo, uint32_t hi)
+{
+    uint64_t hit = 0;
+    for (uint32_t p = lo; p < hi; p++) {
+        const FK_CONST uint32_t* row = kt + (size_t)(p - rf) * stride;
+        const uint4 v0 = load4(row), v1 = load4(row + 4), v2 = load4(row + 8),
+                    v3 = load4(row + 12);
+        const uint32_t r = limb_residue<NL>(a, pd + p);
+        hit |= __ballot(r == v0.y) | __ballot(r == v0.z) |
+               __ballot(r == v0.w) | match4(r, v1) | match4(r, v2) |
+               match4(r, v3);
+        for (uint32_t c = 16; c <= v0.x; c += 4) hit |= match4(r, load4(row + c));
     }
+    return hit;
 }
 
-// Dense rounds (more candidates than primes): lane = candidate; the
-// workgroup stages a block of RD_PRIMES primes (and, for AP, their kill
-// constants) and the batch's engine constants in LDS, then every lane
-// walks the block for its own candidate.
-constexpr uint32_t RD_PRIMES = 64;
-constexpr uint32_t RD_ENGINES = 64;
-
-__global__ __launch_bounds__(256) void k_round_dense(RoundArgs a)
+// One round: every queued candidate against the primes [p_begin, p_end),
+// lane = candidate, blockIdx.y = a block of pblock primes.  A wave whose
+// lanes come from several engines takes them one engine at a time (the
+// engine index is made uniform with readlane), so the prime-side data
+// stays uniform.  A killed candidate's flag is set; the compaction that
+// follows drops it.
+__global__ __launch_bounds__(256) void k_round(RoundArgs a)
 {
-    __shared__ PrimeK sp[RD_PRIMES];
-    __shared__ KillEng se[RD_ENGINES];
     const uint32_t count = min(*a.count, a.cap);
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t e = tid; e < a.nengines && e < RD_ENGINES; e += blockDim.x)
-        se[e] = kill_eng(a.engines[e]);
-    const uint32_t nblk = (a.p_end - a.p_begin + RD_PRIMES - 1) / RD_PRIMES;
-    for (uint32_t pb = blockIdx.y; pb < nblk; pb += gridDim.y) {
-        const uint32_t p0 = a.p_begin + pb * RD_PRIMES;
-        const uint32_t p1 = min(p0 + RD_PRIMES, a.p_end);
-        __syncthreads();
-        if (tid < p1 - p0) sp[tid] = a.primes[p0 + tid];
-        __syncthreads();
-        for (uint32_t i = blockIdx.x * blockDim.x + tid; i < count;
-             i += gridDim.x * blockDim.x) {
-            if (a.flags[i]) continue;
+    const uint32_t pb0 = a.p_begin + blockIdx.y * a.pblock;
+    const uint32_t pb1 = min(pb0 + a.pblock, a.p_end);
+    if (pb0 >= pb1) return;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count;
+         base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        const bool live = i < count && a.flags[i] == 0;
+        uint32_t t[4] = {0u, 0u, 0u, 0u};
+        uint32_t eidx = 0;
+        if (live) {
             const DevRec rec = a.recs[i];
-            const uint32_t eidx = a.win_engine[rec.window];
-            const KillEng c = eidx < RD_ENGINES ? se[eidx]
-                                                : kill_eng(a.engines[eidx]);
-            const uint32_t lo = max(p0, c.round_first);
-            const uint32_t hi = min(p1, c.round_end);
-            for (uint32_t p = lo; p < hi; p++) {
-                if (kill_test(c, rec.t, sp[p - p0], p)) {
-                    a.flags[i] = 1;
-                    break;
-                }
+            for (int k = 0; k < 4; k++) t[k] = rec.t[k];
+            eidx = a.win_engine[rec.window];
+        }
+        double l[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            l[k] = (double)((t[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
+        const bool wide = __ballot(live && (t[2] | t[3]) != 0) != 0;
+        bool killed = false;
+        uint64_t todo = __ballot(live);
+        while (todo) {
+            const int lead = __ffsll((unsigned long long)todo) - 1;
+            const uint32_t e = __builtin_amdgcn_readlane(eidx, lead);
+            const bool mine = live && eidx == e;
+            todo &= ~(uint64_t)__ballot(mine);
+            const FK_CONST DevEngine* E = uniform_view(a.engines) + e;
+            const uint32_t rf = E->round_first;
+            const uint32_t lo = max(pb0, rf);
+            const uint32_t hi = min(pb1, E->round_end);
+            const FK_CONST PrimeD* pd = uniform_view(a.primed);
+            const FK_CONST uint32_t* kt = uniform_view(E->kt);
+            if (mine && lo < hi) {
+                const uint64_t m =
+                    wide ? round_scan<8>(l, pd, kt, E->kstride, rf, lo, hi)
+                         : round_scan<4>(l, pd, kt, E->kstride, rf, lo, hi);
+                killed = killed || ((m >> __lane_id()) & 1);
             }
         }
-    }
-}
-
-// Sparse rounds (more primes than candidates): lane = prime; RS_RECS
-// candidates at a time are staged in LDS by all lanes at once, and every
-// lane tests its prime against each of them.
-constexpr int RS_RECS = 256;
-
-__global__ __launch_bounds__(256) void k_round_sparse(RoundArgs a)
-{
-    __shared__ uint4 st_t[RS_RECS];
-    __shared__ KillEng st_e[RS_RECS];
-    __shared__ uint32_t st_live[RS_RECS];
-    const uint32_t count = min(*a.count, a.cap);
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t pb = blockIdx.x; pb < a.nchunks; pb += gridDim.x) {
-        const uint32_t p = a.p_begin + pb * 256 + tid;
-        const bool have = p < a.p_end;
-        const PrimeK pk = have ? a.primes[p] : a.primes[a.p_begin];
-        for (uint32_t r0 = blockIdx.y * RS_RECS; r0 < count;
-             r0 += gridDim.y * RS_RECS) {
-            __syncthreads();
-            for (uint32_t k = tid; k < RS_RECS; k += blockDim.x) {
-                const uint32_t i = r0 + k;
-                uint32_t live = 0;
-                if (i < count && a.flags[i] == 0) {
-                    const DevRec rec = a.recs[i];
-                    st_t[k] = make_uint4(rec.t[0], rec.t[1], rec.t[2], rec.t[3]);
-                    st_e[k] = kill_eng(a.engines[a.win_engine[rec.window]]);
-                    live = 1;
-                }
-                st_live[k] = live;
-            }
-            __syncthreads();
-            if (!have) continue;
-            const uint32_t nk = min((uint32_t)RS_RECS, count - r0);
-            for (uint32_t k = 0; k < nk; k++) {
-                if (!st_live[k]) continue;
-                const KillEng& c = st_e[k];
-                if (p < c.round_first || p >= c.round_end) continue;
-                const uint4 tv = st_t[k];
-                const uint32_t t[4] = {tv.x, tv.y, tv.z, tv.w};
-                if (kill_test(c, t, pk, p)) a.flags[r0 + k] = 1;
-            }
-        }
+        if (killed) a.flags[i] = 1;
     }
 }
 
@@ -769,16 +741,27 @@ hipError_t launch_prep(const DevEngine* engines, const DevPiece* pieces,
     return hipGetLastError();
 }
 
-hipError_t launch_round_test(const RoundArgs& a, bool dense, uint32_t gx,
-                             uint32_t gy, hipStream_t stream)
+hipError_t launch_round_test(const RoundArgs& a, uint32_t gx, uint32_t gy,
+                             hipStream_t stream)
 {
-    if (gx < 1) gx = 1;
-    if (gy < 1) gy = 1;
-    if (gy > 65535) gy = 65535;
-    if (dense)
-        hipLaunchKernelGGL(k_round_dense, dim3(gx, gy), dim3(256), 0, stream, a);
-    else
-        hipLaunchKernelGGL(k_round_sparse, dim3(gx, gy), dim3(256), 0, stream, a);
+    // every prime block must be launched: the host sizes pblock for gy
+    if (gx < 1 || gy < 1 || gy > 65535 || a.pblock == 0 ||
+        (uint64_t)gy * a.pblock < (uint64_t)(a.p_end - a.p_begin))
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_round, dim3(gx, gy), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_kill_table(const PrimeK* primes, uint32_t p0, uint32_t np,
+                             int family, int n, const uint32_t d[4],
+                             uint32_t stride, uint32_t* kt, hipStream_t stream)
+{
+    if (np == 0) return hipSuccess;
+    if (stride < 16 || (stride & 3) || stride < 1 + kill_max(family, n))
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_kill_table, dim3((np + 255) / 256), dim3(256), 0,
+                       stream, primes, p0, np, family, n,
+                       make_uint4(d[0], d[1], d[2], d[3]), stride, kt);
     return hipGetLastError();
 }
 

@@ -1,0 +1,115 @@
+// fk_kill.h -- value-domain kill sets of the round primes.
+//
+// A round prime q kills a candidate when q divides one of the family's n
+// values built from it; that depends only on the candidate's value mod q:
+//   Euler   (value p)      q | x^2 + x + p   for some 0 <= x < n
+//   Dickson (value k=W j)  q | m k^2 + 1     for some 1 <= m <= n
+//   AP      (value p)      q | p + j d       for some 0 <= j < n, d = P(n)
+// (euler-prime-runs/euler_search.py:85-87, dickson-ladders/
+// ladder_gpu.py:597-614, primorial-ap/ap_gpu.py:88-97 state the same
+// conditions).  The engine stores, per round prime, the killing residues
+// without repeats, padded to a fixed stride with kKillPad (never a residue),
+// and the kernels compare a candidate's residue against them.
+#pragma once
+
+#include "fk_arith.h"
+
+namespace fk {
+
+constexpr uint32_t kKillPad = 0xffffffffu;
+
+// Largest kill set: Dickson has up to two roots per m, the others one
+// residue per value.
+FK_HD uint32_t kill_max(int family, int n)
+{
+    return family == 2 ? 2u * (uint32_t)n : (uint32_t)n;
+}
+
+FK_HD uint32_t pow_mod(uint32_t a, uint32_t e, const PrimeK& p)
+{
+    uint32_t r = 1u % p.q, b = a % p.q;
+    while (e) {
+        if (e & 1) r = mulmod32(r, b, p.q, p.m64);
+        b = mulmod32(b, b, p.q, p.m64);
+        e >>= 1;
+    }
+    return r;
+}
+
+// A square root of the quadratic residue a (0 < a < q) modulo an odd prime
+// q, by Tonelli-Shanks.  Every loop is bounded: the exponent of 2 in q - 1
+// is below 32, so is M, and M falls on each pass.
+FK_HD uint32_t sqrt_mod(uint32_t a, const PrimeK& p)
+{
+    const uint32_t q = p.q;
+    if ((q & 3u) == 3u) return pow_mod(a, (q + 1) / 4, p);
+    uint32_t Q = q - 1, S = 0;
+    while (!(Q & 1u)) {
+        Q >>= 1;
+        S++;
+    }
+    uint32_t z = 2;
+    while (z < q && pow_mod(z, (q - 1) / 2, p) != q - 1) z++;
+    uint32_t M = S;
+    uint32_t c = pow_mod(z, Q, p);
+    uint32_t t = pow_mod(a, Q, p);
+    uint32_t R = pow_mod(a, (Q + 1) / 2, p);
+    for (int pass = 0; pass < 32 && t != 1; pass++) {
+        uint32_t i = 0, tt = t;
+        while (tt != 1 && i < M) {
+            tt = mulmod32(tt, tt, q, p.m64);
+            i++;
+        }
+        if (i >= M) break;                 // a was not a residue
+        uint32_t b = c;
+        for (uint32_t k = 0; k + i + 1 < M; k++) b = mulmod32(b, b, q, p.m64);
+        M = i;
+        c = mulmod32(b, b, q, p.m64);
+        t = mulmod32(t, c, q, p.m64);
+        R = mulmod32(R, b, q, p.m64);
+    }
+    return R;
+}
+
+FK_HD void kill_add(uint32_t* out, uint32_t& cnt, uint32_t v)
+{
+    for (uint32_t i = 0; i < cnt; i++)
+        if (out[i] == v) return;
+    out[cnt++] = v;
+}
+
+// The kill set of the odd prime q (> n) for family/n, d = P(n) in four
+// limbs (AP only).  Writes out[0, count), pads out[count, stride) and
+// returns count.
+FK_HD uint32_t kill_values(int family, int n, const uint32_t d[4],
+                           const PrimeK& p, uint32_t* out, uint32_t stride)
+{
+    const uint32_t q = p.q;
+    uint32_t cnt = 0;
+    if (family == 1) {
+        for (int x = 0; x < n; x++) {
+            const uint32_t v = red64((uint64_t)x * x + (uint64_t)x, q, p.m64);
+            kill_add(out, cnt, v ? q - v : 0u);
+        }
+    } else if (family == 2) {
+        for (int m = 1; m <= n; m++) {
+            const uint32_t mq = (uint32_t)m % q;
+            if (mq == 0) continue;             // m k^2 + 1 == 1 (mod q)
+            const uint32_t a = q - pow_mod(mq, q - 2, p);    // -1/m
+            if (pow_mod(a, (q - 1) / 2, p) != 1) continue;
+            const uint32_t r = sqrt_mod(a, p);
+            kill_add(out, cnt, r);
+            kill_add(out, cnt, q - r);
+        }
+    } else {
+        const uint32_t dq = red128(d, p);
+        for (int j = 0; j < n; j++) {
+            const uint32_t v = mulmod32((uint32_t)j % q, dq, q, p.m64);
+            kill_add(out, cnt, v ? q - v : 0u);
+        }
+    }
+    for (uint32_t i = cnt; i < stride; i++) out[i] = kKillPad;
+    return cnt;
+}
+
+}  // namespace fk

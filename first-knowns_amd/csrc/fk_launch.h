@@ -34,14 +34,13 @@ struct RoundArgs {
     const DevEngine* engines;
     uint32_t nengines;
     const uint32_t* win_engine;
-    const PrimeK* primes;     // the device-wide prime table
+    const PrimeD* primed;     // the device-wide prime table (residue form)
     const DevRec* recs;
     const uint32_t* count;
     uint32_t cap;
     uint32_t* flags;
     uint32_t p_begin, p_end;  // global prime indices of this round
-    uint32_t chunk;           // unused
-    uint32_t nchunks;         // sparse rounds: blocks of 256 primes
+    uint32_t pblock;          // primes per blockIdx.y
 };
 
 struct CompactArgs {
@@ -72,8 +71,11 @@ hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,
                         uint32_t grid, size_t lds_bytes, hipStream_t stream);
 int sieve_occupancy(int threads, int wpt, size_t lds_bytes);
 bool sieve_shape_ok(int threads, int wpt);
-hipError_t launch_round_test(const RoundArgs& a, bool dense, uint32_t gx,
-                             uint32_t gy, hipStream_t stream);
+hipError_t launch_round_test(const RoundArgs& a, uint32_t gx, uint32_t gy,
+                             hipStream_t stream);
+hipError_t launch_kill_table(const PrimeK* primes, uint32_t p0, uint32_t np,
+                             int family, int n, const uint32_t d[4],
+                             uint32_t stride, uint32_t* kt, hipStream_t stream);
 hipError_t launch_round_compact(const CompactArgs& a, uint32_t grid,
                                 hipStream_t stream);
 hipError_t launch_classify(const ClassifyArgs& a, uint32_t grid,

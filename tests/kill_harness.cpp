@@ -1,0 +1,43 @@
+// Host build of the round-prime arithmetic the kernels use
+// (first-knowns_amd/csrc/fk_kill.h, fk_arith.h).  Reads lines
+//   K family n q d0 d1 d2 d3      -> count and the kill set (value domain)
+//   R q nl t0 t1 t2 t3            -> the residue of the value t (four hex
+//                                    32-bit limbs) by the 16-bit-limb
+//                                    double-precision path with nl limbs
+#include "../first-knowns_amd/csrc/fk_kill.h"
+#include <cstdio>
+#include <cstring>
+#include <vector>
+int main()
+{
+    char line[512];
+    while (fgets(line, sizeof line, stdin)) {
+        if (line[0] == 'K') {
+            int fam, n;
+            unsigned q;
+            uint32_t d[4];
+            if (sscanf(line + 1, "%d %d %u %x %x %x %x", &fam, &n, &q, &d[0],
+                       &d[1], &d[2], &d[3]) != 7) continue;
+            const fk::PrimeK pk = fk::make_primek(q);
+            const uint32_t stride = fk::kill_max(fam, n) + 3;
+            std::vector<uint32_t> out(stride);
+            const uint32_t c = fk::kill_values(fam, n, d, pk, out.data(), stride);
+            printf("%u", c);
+            for (uint32_t i = 0; i < stride; i++) printf(" %u", out[i]);
+            printf("\n");
+        } else if (line[0] == 'R') {
+            unsigned q;
+            int nl;
+            uint32_t t[4];
+            if (sscanf(line + 1, "%u %d %x %x %x %x", &q, &nl, &t[0], &t[1],
+                       &t[2], &t[3]) != 6) continue;
+            const fk::PrimeD P = fk::make_primed(q);
+            double a[8];
+            for (int k = 0; k < 8; k++)
+                a[k] = (double)((t[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
+            double s = a[0];
+            for (int k = 1; k < nl; k++) s = fma(a[k], P.c[k - 1], s);
+            printf("%u\n", fk::resid_d(s, P.qd, P.qi, P.q));
+        }
+    }
+}

@@ -1,0 +1,126 @@
+"""The round-prime arithmetic of the kernels (first-knowns_amd/csrc/fk_kill.h
+and the double-precision residue in fk_arith.h), compiled for the host with
+g++, against Python integers: the residue of values up to 2^128 - 1 by
+primes up to 2^30 through the 16-bit-limb fused-multiply-add path, and the
+value-domain kill set of each family by brute force over every residue
+(small q) and by checking each member and the count (large q)."""
+import pathlib
+import random
+import subprocess
+
+import pytest
+
+import fk_oracle as O
+
+HERE = pathlib.Path(__file__).resolve().parent
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    exe = tmp_path_factory.mktemp("kill") / "kill_harness"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", str(exe),
+                    str(HERE / "kill_harness.cpp")], check=True)
+    return exe
+
+
+def call(exe, lines):
+    out = subprocess.run([str(exe)], input="\n".join(lines) + "\n",
+                         capture_output=True, text=True, check=True)
+    return out.stdout.splitlines()
+
+
+def is_prime(q):
+    return q >= 2 and O.mr_is_prime(q)
+
+
+def random_prime(rng, lo, hi):
+    while True:
+        q = rng.randrange(lo, hi) | 1
+        if is_prime(q):
+            return q
+
+
+def limbs(v):
+    return " ".join("%x" % ((v >> (32 * i)) & 0xFFFFFFFF) for i in range(4))
+
+
+def test_residue_matches_python(harness):
+    rng = random.Random(21)
+    qs = [3, 5, 65521, 65537, 2**22 - 3, 2**29 - 3, 2**30 - 35]
+    qs += [random_prime(rng, 3, 2**30) for _ in range(40)]
+    vals = [0, 1, 2**16 - 1, 2**16, 2**32 - 1, 2**63, 2**64 - 1, 2**64,
+            2**96 + 12345, 2**128 - 1]
+    vals += [rng.getrandbits(b) for b in (20, 40, 63, 64, 80, 100, 127, 128)
+             for _ in range(6)]
+    cases, want = [], []
+    for q in qs:
+        for v in vals:
+            nl = 4 if v < 2**64 else 8
+            for use in {nl, 8}:
+                cases.append(f"R {q} {use} {limbs(v)}")
+                want.append(v % q)
+    got = [int(x) for x in call(harness, cases)]
+    assert got == want
+
+
+def kill_brute(family, n, q):
+    d = O.primorial(n) if family == 3 else 0
+    out = set()
+    for r in range(q):
+        if family == 1:
+            dead = any((x * x + x + r) % q == 0 for x in range(n))
+        elif family == 2:
+            dead = any((m * r * r + 1) % q == 0 for m in range(1, n + 1))
+        else:
+            dead = any((r + j * d) % q == 0 for j in range(n))
+        if dead:
+            out.add(r)
+    return out
+
+
+def run_kill(exe, family, n, qs):
+    d = O.primorial(n) if family == 3 else 0
+    lines = [f"K {family} {n} {q} {limbs(d)}" for q in qs]
+    res = []
+    for q, row in zip(qs, call(exe, lines)):
+        w = [int(x) for x in row.split()]
+        c = w[0]
+        vals = w[1:1 + c]
+        pad = w[1 + c:]
+        assert len(set(vals)) == c
+        assert all(x == 0xFFFFFFFF for x in pad)
+        res.append(set(vals))
+    return res
+
+
+@pytest.mark.parametrize("family,n", [(1, 5), (1, 17), (2, 6), (2, 10),
+                                      (2, 13), (3, 8), (3, 16)])
+def test_kill_sets_small_primes_brute_force(harness, family, n):
+    qs = [q for q in range(3, 700) if is_prime(q) and q > n + 1]
+    got = run_kill(harness, family, n, qs)
+    for q, g in zip(qs, got):
+        assert g == kill_brute(family, n, q), q
+
+
+@pytest.mark.parametrize("family,n", [(1, 17), (2, 10), (2, 13), (3, 17)])
+def test_kill_sets_large_primes(harness, family, n):
+    rng = random.Random(family * 100 + n)
+    qs = [random_prime(rng, 2**16, 2**30) for _ in range(60)]
+    qs += [2**30 - 35, 2**22 - 3, 1048573]
+    got = run_kill(harness, family, n, qs)
+    d = O.primorial(n)
+    for q, g in zip(qs, got):
+        if family == 1:
+            assert g == {(-(x * x + x)) % q for x in range(n)}
+        elif family == 3:
+            assert g == {(-j * d) % q for j in range(n)}
+        else:
+            want = 0
+            for m in range(1, n + 1):
+                a = (-pow(m, -1, q)) % q
+                if pow(a, (q - 1) // 2, q) == 1:
+                    want += 2
+            for r in g:
+                assert any((m * r * r + 1) % q == 0 for m in range(1, n + 1))
+            # distinct m give distinct roots: -1/m differ for m < q
+            assert len(g) == want
