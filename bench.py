@@ -209,6 +209,20 @@ def rocprof_traffic():
     return None
 
 
+def copy_probe(mib):
+    """One device-to-device copy of a known size: the calibration run for
+    the FETCH_SIZE / WRITE_SIZE counters (bytes read and written are both
+    mib MiB)."""
+    import torch
+    n = mib << 20
+    x = torch.ones(n, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,7 +231,12 @@ def main():
     ap.add_argument("--cpu-sample-log2", type=int, default=31)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run one device copy of MIB MiB (counter "
+                         "calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path "
               f"(windows shard across GPUs by range; see DESIGN.md). "
