@@ -37,7 +37,7 @@ struct DevEngine {
     uint32_t bm_words;        // total kill-bitmap words of the test primes
     uint32_t round_first;     // global prime-table index of the first round prime
     uint32_t round_end;       // one past the last prime below q2
-    uint32_t kstride;         // kill-set row stride (multiple of 4, >= 16)
+    uint32_t kstride;         // round-table row stride in words
     uint64_t D;               // wheel period of the line
     uint32_t W;               // Dickson: k = W j (product of primes <= n+1)
     uint32_t d[8];            // AP: P(n), the common difference, in limbs
@@ -53,9 +53,10 @@ struct DevEngine {
     const uint32_t* bm;       // [bm_words] line-domain kill bitmaps
     uint32_t ntr;             // in-tile rounds over [nmark, nlds)
     const uint32_t* tr_end;   // [ntr] LDS-prime index where round r ends
-    const uint32_t* kt;       // [round_end - round_first][kstride] per
-                              // round prime: count, then the killing
-                              // residues of the value, padded
+    const uint32_t* kt;       // [round_end - round_first][kstride] round
+                              // table: per round prime a RoundRow header
+                              // (fk_kill.h), then the killing residues of
+                              // the value, padded
 };
 
 // One piece of one window: a contiguous range of line positions of every

@@ -145,7 +145,7 @@ struct fk_engine {
     uint64_t W = 1;
     u128 dval = 0;
     uint32_t q_pat = 0, q_med = 0, threads = 256, wpt = 8;
-    uint32_t round_chunk = 64, growth = 2;
+    uint32_t round_lane_prime = 8192, growth = 2;
     uint32_t q_mark = 0;             // [q_pat, q_mark) stride-marked
     uint32_t npat = 0;
     std::vector<uint32_t> lds_q;
@@ -251,7 +251,7 @@ static int build_engine(fk_engine* e)
     if (e->q_med > 65536) e->q_med = 65536;     // pairs pack q in 16 bits
     if (e->q_med > d.q2) e->q_med = d.q2;
     if (e->q_pat > e->q_med) e->q_pat = e->q_med;
-    e->round_chunk = d.round_chunk ? d.round_chunk : 64;
+    e->round_lane_prime = d.round_lane_prime ? d.round_lane_prime : 8192;
     e->q_mark = d.q_mark ? d.q_mark : e->q_pat;
     if (e->q_mark < e->q_pat) e->q_mark = e->q_pat;
     if (e->q_mark > e->q_med) e->q_mark = e->q_med;
@@ -365,7 +365,7 @@ static int build_engine(fk_engine* e)
     h.ntr = (uint32_t)tr_end.size();
     h.round_first = e->round_first;
     h.round_end = e->round_end;
-    h.kstride = std::max(16u, (1u + kill_max(d.family, d.n) + 3u) & ~3u);
+    h.kstride = round_row_stride(d.family, d.n);
     h.D = e->D;
     h.W = (uint32_t)e->W;
     for (int i = 0; i < 4; i++) {
@@ -431,6 +431,8 @@ struct fk_batch {
     std::vector<fk_window> windows;
     std::vector<fk_classify_spec> cls;
     uint32_t final_slot = 0;          // which counter holds the final count
+    std::vector<uint32_t> round_slots;  // counter of each executed round
+    std::vector<uint32_t> stage_counts; // queue after the sieve and rounds
     uint32_t final_q = 0;             // which queue holds the final records
     uint32_t queue_cap = 0;
     uint32_t copied = 0;
@@ -497,10 +499,15 @@ static double expected_lds(const fk_engine* e, double span)
 static double round_survival(const fk_engine* e, double lo, double hi)
 {
     if (hi <= lo) return 1.0;
-    // sum over primes in [lo, hi) of w/q ~ w ln(ln hi / ln lo)
-    const double w = (e->family == FK_FAMILY_DICKSON) ? e->n : e->n;
+    // sum over primes in [lo, hi) of w/q ~ w ln(ln hi / ln lo), with about
+    // n killing residues per prime in every family
+    const double w = e->n;
     return std::pow(std::log(lo) / std::log(hi), w);
 }
+
+// records x chain cap at or below which classification runs every base in
+// one pass
+static constexpr uint32_t kSingleLimit = 8192;
 
 static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
 {
@@ -539,6 +546,7 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
     uint64_t tiles = 0, bq_total = 0;
     uint32_t nlds_max = 0, pat_max = 0, bm_max = 0;
     double expect = 0.0, cand = 0.0;
+    std::vector<double> e_alive(b->engines.size(), 0.0);  // expected per engine
     for (fk_engine* e : b->engines) {
         nlds_max = std::max(nlds_max, e->host.nlds);
         pat_max = std::max(pat_max, e->pat_words);
@@ -575,6 +583,7 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         }
         const double span = (double)(hi - lo);
         expect += expected_lds(e, span);
+        e_alive[b->engine_of[w]] += expected_lds(e, span);
         cand += span * e->line_density;
     }
     b->tiles = tiles;
@@ -698,8 +707,11 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
 
     // ---- rounds
     uint32_t cur = 0, cslot = 0;
+    b->round_slots.clear();
     double alive = expect;
     for (size_t r = 0; r < rounds.size(); r++) {
+        alive = 0.0;
+        for (double x : e_alive) alive += x;
         RoundArgs ra{};
         ra.engines = b->d_engines;
         ra.nengines = (uint32_t)b->engines.size();
@@ -713,14 +725,27 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         ra.p_end = prime_index(ctx->primes, rounds[r].second);
         const uint32_t np = ra.p_end > ra.p_begin ? ra.p_end - ra.p_begin : 0;
         if (np == 0) continue;
-        // about 2048 workgroups: candidate blocks times prime blocks of at
-        // least 16 primes
-        const uint32_t gx = (uint32_t)std::min(1024.0, std::max(1.0, std::ceil(alive / 256.0)));
-        uint32_t gy = std::min((np + 15) / 16, (2048 + gx - 1) / gx);
-        gy = std::max(1u, std::min(gy, 65535u));
-        ra.pblock = (np + gy - 1) / gy;
-        gy = (np + ra.pblock - 1) / ra.pblock;
-        HIPCHK(launch_round_test(ra, gx, gy, s));
+        // Many primes and enough candidates: lane = prime (256 primes per
+        // workgroup column, the queue cut into about 4096 / columns slices
+        // of at least 32).  Otherwise lane = candidate (candidate blocks
+        // times prime blocks of at least 16, about 2048 workgroups).  Both
+        // read the true count on the device; grids sized from an estimate
+        // that is off still cover every candidate.
+        const bool lane_prime = np >= e0->round_lane_prime &&
+                                alive >= e0->round_lane_prime / 8.0;
+        uint32_t gx, gy;
+        if (lane_prime) {
+            gx = (np + 255) / 256;
+            gy = (uint32_t)std::min(std::ceil(alive / 32.0), std::ceil(4096.0 / gx));
+            gy = std::max(1u, std::min(gy, 65535u));
+        } else {
+            gx = (uint32_t)std::min(1024.0, std::max(1.0, std::ceil(alive / 256.0)));
+            gy = std::min((np + 15) / 16, (2048 + gx - 1) / gx);
+            gy = std::max(1u, std::min(gy, 65535u));
+            ra.pblock = (np + gy - 1) / gy;
+            gy = (np + ra.pblock - 1) / ra.pblock;
+        }
+        HIPCHK(launch_round_test(ra, lane_prime, gx, gy, s));
         CompactArgs ca{};
         ca.in = b->d_q[cur];
         ca.in_count = b->d_counts + cslot;
@@ -728,15 +753,20 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         ca.flags = b->d_flags;
         ca.out = b->d_q[cur ^ 1];
         ca.out_count = b->d_counts + 2 + r;
-        uint32_t cg = (uint32_t)std::min(1024.0, std::ceil(alive / 256.0) + 1);
+        uint32_t cg = (uint32_t)std::min(1024.0, std::ceil(alive / 1024.0) + 1);
         HIPCHK(launch_round_compact(ca, cg, s));
         cur ^= 1;
         cslot = 2 + (uint32_t)r;
-        double sv = 1.0;
-        for (fk_engine* e : b->engines)
-            sv = std::min(sv, round_survival(e, rounds[r].first, rounds[r].second));
-        alive *= sv;
+        b->round_slots.push_back(cslot);
+        for (size_t k = 0; k < b->engines.size(); k++) {
+            const fk_engine* e = b->engines[k];
+            const double lo = std::max<double>(rounds[r].first, e->q_med);
+            const double hi = std::min<double>(rounds[r].second, e->q2);
+            if (hi > lo) e_alive[k] *= round_survival(e, lo, hi);
+        }
     }
+    alive = 0.0;
+    for (double x : e_alive) alive += x;
     HIPCHK(hipEventRecord(b->ev[3], s));
 
     // ---- classification
@@ -757,9 +787,11 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         ca.fail2 = b->d_fail2;
         ca.failx = b->d_failx;
         // few candidates: every base at once, one latency instead of two
-        ca.single = (alive * cap_max <= 8192.0) ? 1u : 0u;
-        uint32_t g = (uint32_t)std::min(
-            16384.0, std::ceil(alive * cap_max * (ca.single ? 7 : 1) / 128.0) + 4);
+        // (the kernels decide from the true count)
+        ca.single_limit = kSingleLimit;
+        const double work = alive * cap_max * 7.0;
+        uint32_t g = (uint32_t)std::min(16384.0,
+                                        std::max(1024.0, std::ceil(work / 128.0) + 4));
         HIPCHK(launch_classify(ca, g, s));
     }
     HIPCHK(hipEventRecord(b->ev[4], s));
@@ -810,6 +842,8 @@ static int batch_collect(fk_batch* b)
         return batch_collect(b);
     }
     const uint32_t n = b->h_counts[b->final_slot];
+    b->stage_counts.assign(1, tile_count);
+    for (uint32_t sl : b->round_slots) b->stage_counts.push_back(b->h_counts[sl]);
     HIPCHK(hipEventElapsedTime(&b->ms[0], b->ev[0], b->ev[1]));
     HIPCHK(hipEventElapsedTime(&b->ms[1], b->ev[1], b->ev[2]));
     HIPCHK(hipEventElapsedTime(&b->ms[2], b->ev[2], b->ev[3]));
@@ -1055,9 +1089,10 @@ int fk_engine_classify(fk_engine* e, const uint64_t* values, size_t n,
         ca.cap_max = (uint32_t)std::max(1, cls->cap);
         ca.fail2 = d_f;
         ca.failx = d_f + n;
-        ca.single = (n * (uint64_t)ca.cap_max <= 8192) ? 1u : 0u;
-        uint32_t g = (uint32_t)std::min<uint64_t>(
-            16384, (n * ca.cap_max + 127) / 128 + 4);
+        ca.single_limit = kSingleLimit;
+        const uint64_t work = n * (uint64_t)ca.cap_max *
+                              (n * (uint64_t)ca.cap_max <= kSingleLimit ? 7 : 1);
+        uint32_t g = (uint32_t)std::min<uint64_t>(16384, (work + 127) / 128 + 4);
         err = launch_classify(ca, g, s);
     }
     if (err == hipSuccess) err = hipStreamSynchronize(s);
@@ -1138,6 +1173,19 @@ int fk_batch_timing(fk_batch* b, float* ms5, uint64_t* tiles,
     if (tiles) *tiles = b->tiles;
     if (candidates) *candidates = b->candidates;
     return FK_OK;
+}
+
+int fk_batch_counts(fk_batch* b, uint32_t* counts, size_t cap, size_t* n)
+{
+    if (!b || !n) return fail(FK_E_ARG, "null argument");
+    *n = b->stage_counts.size();
+    if (counts)
+        std::memcpy(counts, b->stage_counts.data(),
+                    std::min(cap, b->stage_counts.size()) * sizeof(uint32_t));
+    return cap < b->stage_counts.size() && counts
+               ? fail(FK_E_OVERFLOW, "%zu stage counts, room for %zu",
+                      b->stage_counts.size(), cap)
+               : FK_OK;
 }
 
 int fk_engine_batch(fk_engine* eng, fk_batch** out)

@@ -386,9 +386,8 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
 
 // ---------------------------------------------------------------- rounds
 
-// Kill-set table of each engine's round primes, built once at engine
-// creation: one thread per prime.  Row layout: [count, residues...,
-// kKillPad...], stride words per row.
+// Round table of each engine, built once at engine creation: one thread
+// per round prime writes its RoundRow and kill set.
 __global__ void k_kill_table(const PrimeK* __restrict__ primes, uint32_t p0,
                              uint32_t np, int family, int n, uint4 d,
                              uint32_t stride, uint32_t* __restrict__ kt)
@@ -396,8 +395,7 @@ __global__ void k_kill_table(const PrimeK* __restrict__ primes, uint32_t p0,
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const uint32_t dl[4] = {d.x, d.y, d.z, d.w};
-    uint32_t* row = kt + (size_t)i * stride;
-    row[0] = kill_values(family, n, dl, primes[p0 + i], row + 1, stride - 1);
+    round_row(family, n, dl, primes[p0 + i], kt + (size_t)i * stride, stride);
 }
 
 // Read-only tables the kernels index with wave-uniform indices, viewed in
@@ -411,20 +409,14 @@ __device__ __forceinline__ const FK_CONST T* uniform_view(const T* p)
     return (const FK_CONST T*)p;
 }
 
-// The candidate's residue mod the prime from its NL 16-bit limbs.
-template <int NL>
-__device__ __forceinline__ uint32_t limb_residue(const double (&a)[8],
-                                                 const FK_CONST PrimeD* P)
-{
-    double s = a[0];
-#pragma unroll
-    for (int k = 1; k < NL; k++) s = fma(a[k], P->c[k - 1], s);
-    return resid_d(s, P->qd, P->qi, P->q);
-}
-
 __device__ __forceinline__ uint4 load4(const FK_CONST uint32_t* p)
 {
     return make_uint4(p[0], p[1], p[2], p[3]);
+}
+
+__device__ __forceinline__ double load_f64(const FK_CONST uint32_t* p)
+{
+    return __hiloint2double((int)p[1], (int)p[0]);
 }
 
 // Lanes whose residue r matches one of the four words.
@@ -434,12 +426,61 @@ __device__ __forceinline__ uint64_t match4(uint32_t r, uint4 v)
            __ballot(r == v.w);
 }
 
-// Primes [lo, hi) of one engine against this lane's candidate; returns the
-// mask of lanes killed.  The prime index is uniform across the wave, so the
-// prime's constants and kill-set row are scalar loads; the lane does the
-// reduction and the compares (one v_cmp into a lane mask per residue).  The
-// first 16 words of a row (count and up to 15 residues, padded) are read
-// unconditionally; longer rows continue in a loop.
+// ---- lane = candidate (many candidates, few primes): the prime is
+// wave-uniform, its row is read with scalar loads into SGPRs.
+
+// One round prime's row as scalar values: the header and the first 20
+// killing residues, 32 words in all.
+struct RowView {
+    double qd, qi, c16, c32, c48;
+    uint32_t q, cnt;
+    uint4 k[5];
+};
+
+__device__ __forceinline__ RowView load_row(const FK_CONST uint32_t* row)
+{
+    RowView v;
+    v.qd = load_f64(row + 0);
+    v.qi = load_f64(row + 2);
+    v.c16 = load_f64(row + 4);
+    v.c32 = load_f64(row + 6);
+    v.c48 = load_f64(row + 8);
+    v.q = row[10];
+    v.cnt = row[11];
+#pragma unroll
+    for (int i = 0; i < 5; i++) v.k[i] = load4(row + kRowHead + 4 * i);
+    return v;
+}
+
+// The lane's candidate residue mod the row's prime from its 16-bit limbs;
+// limbs past the fourth take 2^64..2^112 mod q from the prime table.
+template <int NL>
+__device__ __forceinline__ uint32_t row_residue(const double (&a)[8],
+                                                const RowView& v,
+                                                const FK_CONST PrimeD* P)
+{
+    double s = fma(a[3], v.c48, fma(a[2], v.c32, fma(a[1], v.c16, a[0])));
+    if (NL > 4) {
+#pragma unroll
+        for (int k = 4; k < NL; k++) s = fma(a[k], P->c[k - 1], s);
+    }
+    return resid_d(s, v.qd, v.qi, v.q);
+}
+
+// Lanes whose residue matches the row's kill set (the first 20 residues
+// from registers; longer rows continue in a loop).
+__device__ __forceinline__ uint64_t row_match(uint32_t r, const RowView& v,
+                                              const FK_CONST uint32_t* row)
+{
+    uint64_t hit = match4(r, v.k[0]) | match4(r, v.k[1]) | match4(r, v.k[2]) |
+                   match4(r, v.k[3]) | match4(r, v.k[4]);
+    if (v.cnt > 20) {
+        for (uint32_t c = 20; c < v.cnt; c += 4)
+            hit |= match4(r, load4(row + kRowHead + c));
+    }
+    return hit;
+}
+
 template <int NL>
 __device__ __forceinline__ uint64_t round_scan(const double (&a)[8],
                                                const FK_CONST PrimeD* pd,
@@ -447,27 +488,32 @@ __device__ __forceinline__ uint64_t round_scan(const double (&a)[8],
                                                uint32_t stride, uint32_t rf,
                                                uint32_t lo, uint32_t hi)
 {
+    // two primes per pass, straight-line
     uint64_t hit = 0;
-    for (uint32_t p = lo; p < hi; p++) {
-        const FK_CONST uint32_t* row = kt + (size_t)(p - rf) * stride;
-        const uint4 v0 = load4(row), v1 = load4(row + 4), v2 = load4(row + 8),
-                    v3 = load4(row + 12);
-        const uint32_t r = limb_residue<NL>(a, pd + p);
-        hit |= __ballot(r == v0.y) | __ballot(r == v0.z) |
-               __ballot(r == v0.w) | match4(r, v1) | match4(r, v2) |
-               match4(r, v3);
-        for (uint32_t c = 16; c <= v0.x; c += 4) hit |= match4(r, load4(row + c));
+    uint32_t p = lo;
+    for (; p + 1 < hi; p += 2) {
+        const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * stride;
+        const FK_CONST uint32_t* r1 = r0 + stride;
+        const RowView v0 = load_row(r0);
+        const RowView v1 = load_row(r1);
+        const uint32_t x0 = row_residue<NL>(a, v0, pd + p);
+        const uint32_t x1 = row_residue<NL>(a, v1, pd + p + 1);
+        hit |= row_match(x0, v0, r0) | row_match(x1, v1, r1);
+    }
+    if (p < hi) {
+        const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * stride;
+        const RowView v0 = load_row(r0);
+        hit |= row_match(row_residue<NL>(a, v0, pd + p), v0, r0);
     }
     return hit;
 }
 
-// One round: every queued candidate against the primes [p_begin, p_end),
-// lane = candidate, blockIdx.y = a block of pblock primes.  A wave whose
+// One round, lane = candidate: blockIdx.x walks the queue in blocks of 256
+// (grid-stride), blockIdx.y takes a block of pblock primes.  A wave whose
 // lanes come from several engines takes them one engine at a time (the
-// engine index is made uniform with readlane), so the prime-side data
-// stays uniform.  A killed candidate's flag is set; the compaction that
-// follows drops it.
-__global__ __launch_bounds__(256) void k_round(RoundArgs a)
+// engine index made uniform with readlane), so the prime-side data stays
+// uniform.  A killed candidate's flag is set.
+__global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
 {
     const uint32_t count = min(*a.count, a.cap);
     const uint32_t pb0 = a.p_begin + blockIdx.y * a.pblock;
@@ -513,31 +559,214 @@ __global__ __launch_bounds__(256) void k_round(RoundArgs a)
     }
 }
 
+// ---- lane = prime (many primes): each lane holds one prime's row in
+// VGPRs and the wave walks the queue.
+
+// One round prime's row in a lane's registers: the residue constants and
+// up to kRegResidues killing residues (kKillPad past the count).
+constexpr int kRegResidues = 32;
+
+struct LaneRow {
+    double qd, qi, c16, c32, c48;
+    double cw[4];                 // 2^64 .. 2^112 mod q (values past 2^64)
+    uint32_t q, cnt;
+    uint32_t k[kRegResidues];
+    const uint32_t* row;          // the row in memory (residues past 32)
+};
+
+__device__ __forceinline__ void load_lane_row(LaneRow& L, const uint32_t* row,
+                                              const PrimeD* pd)
+{
+    RoundRow h;
+    const uint4* r4 = reinterpret_cast<const uint4*>(row);
+    uint4 w0 = r4[0], w1 = r4[1], w2 = r4[2];
+    const uint32_t hw[kRowHead] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y,
+                                   w1.z, w1.w, w2.x, w2.y, w2.z, w2.w};
+    __builtin_memcpy(&h, hw, sizeof h);
+    L.qd = h.qd;
+    L.qi = h.qi;
+    L.c16 = h.c16;
+    L.c32 = h.c32;
+    L.c48 = h.c48;
+    L.q = h.q;
+    L.cnt = h.cnt;
+    L.row = row;
+#pragma unroll
+    for (int i = 0; i < kRegResidues / 4; i++) {
+        const uint4 v = (4 * i + 4 <= 20) ? r4[3 + i] : make_uint4(kKillPad, kKillPad, kKillPad, kKillPad);
+        L.k[4 * i] = v.x;
+        L.k[4 * i + 1] = v.y;
+        L.k[4 * i + 2] = v.z;
+        L.k[4 * i + 3] = v.w;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) L.cw[i] = pd->c[3 + i];
+}
+
+// Round prime rows hold at least 20 residues (32 words); rows of engines
+// with larger kill sets (Dickson n > 10, the others n > 20) carry more.
+__device__ __forceinline__ void load_lane_row_wide(LaneRow& L, uint32_t kmax)
+{
+    const uint32_t* res = L.row + kRowHead;
+#pragma unroll
+    for (int i = 20; i < kRegResidues; i++)
+        L.k[i] = ((uint32_t)i < kmax) ? res[i] : kKillPad;
+}
+
+// Does the lane's prime kill a candidate of residue-limbs a[]?  ncmp (wave
+// uniform, a multiple of 4) residues are compared from registers; residues
+// past 32 (kill sets larger than 32) from memory.
+template <int NL>
+__device__ __forceinline__ bool lane_kills(const LaneRow& L, const double (&a)[8],
+                                           uint32_t ncmp)
+{
+    double s = fma(a[3], L.c48, fma(a[2], L.c32, fma(a[1], L.c16, a[0])));
+    if (NL > 4) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) s = fma(a[4 + k], L.cw[k], s);
+    }
+    const uint32_t r = resid_d(s, L.qd, L.qi, L.q);
+    bool hit = false;
+#pragma unroll
+    for (int i = 0; i < kRegResidues; i += 4) {
+        if ((uint32_t)i < ncmp)
+            hit = hit | (r == L.k[i]) | (r == L.k[i + 1]) | (r == L.k[i + 2]) |
+                  (r == L.k[i + 3]);
+    }
+    if (ncmp > (uint32_t)kRegResidues) {
+        for (uint32_t c = kRegResidues; c < L.cnt; c++)
+            hit = hit || r == L.row[kRowHead + c];
+    }
+    return hit;
+}
+
+// One round: the queued candidates against the primes [p_begin, p_end).
+// Lane = prime: blockIdx.x picks 256 primes (one per lane), blockIdx.y a
+// slice of the queue.  Each lane holds its prime's row (residue constants
+// and kill set) in registers, loaded once per engine; the wave walks the
+// slice, a candidate at a time, wave-uniform (scalar loads of the record,
+// four candidates per load batch), and a candidate that some lane's prime
+// kills is flagged by the lowest such lane.  The arithmetic per test is
+// the exact double-precision residue (resid_d) and one compare per
+// killing residue; there is no memory access per (candidate, prime).
+__global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
+{
+    const uint32_t count = min(*a.count, a.cap);
+    const uint32_t per = (count + gridDim.y - 1) / gridDim.y;
+    const uint32_t c0 = blockIdx.y * per;
+    const uint32_t c1 = min(count, c0 + per);
+    if (c0 >= c1) return;
+    const uint32_t p = a.p_begin + blockIdx.x * 256 + threadIdx.x;
+    const uint32_t wave_p0 = a.p_begin + blockIdx.x * 256 + (threadIdx.x & ~63u);
+    if (wave_p0 >= a.p_end) return;              // whole wave past the round
+    const FK_CONST DevRec* recs = uniform_view(a.recs);
+    const FK_CONST uint32_t* wine = uniform_view(a.win_engine);
+    const FK_CONST DevEngine* engs = uniform_view(a.engines);
+    const uint32_t lane = __lane_id();
+
+    LaneRow L;
+    uint32_t cur_e = 0xffffffffu;
+    bool mine = false;               // this lane's prime is in the engine's rounds
+    uint32_t ncmp = 0;
+    // four candidates per load batch: their records (96 contiguous bytes)
+    // and then their engines are fetched together, one latency each
+    for (uint32_t i0 = c0; i0 < c1; i0 += 4) {
+        DevRec r[4];
+        uint32_t e[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const FK_CONST DevRec* src = recs + min(i0 + j, c1 - 1);
+            for (int k = 0; k < 4; k++) r[j].t[k] = src->t[k];
+            r[j].window = src->window;
+            r[j].run = src->run;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) e[j] = wine[r[j].window];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (i0 + j >= c1) break;
+            if (e[j] != cur_e) {
+                cur_e = e[j];
+                const uint32_t rf = engs[cur_e].round_first;
+                const uint32_t re = engs[cur_e].round_end;
+                const uint32_t stride = engs[cur_e].kstride;
+                // compares per test: the family's largest kill set, in fours
+                ncmp = (kill_max(engs[cur_e].family, engs[cur_e].n) + 3u) & ~3u;
+                mine = p >= rf && p < re && p < a.p_end;
+                if (mine) {
+                    load_lane_row(L, engs[cur_e].kt + (size_t)(p - rf) * stride,
+                                  a.primed + p);
+                    if (ncmp > 20) load_lane_row_wide(L, stride - kRowHead);
+                }
+            }
+            if (!__ballot(mine)) continue;
+            double l[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                l[k] = (double)((r[j].t[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
+            const bool wide = (r[j].t[2] | r[j].t[3]) != 0;
+            bool kill = false;
+            if (mine)
+                kill = wide ? lane_kills<8>(L, l, ncmp) : lane_kills<4>(L, l, ncmp);
+            const uint64_t m = __ballot(kill);
+            if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1))
+                a.flags[i0 + j] = 1;
+        }
+    }
+}
+
+// Compaction of the unflagged candidates: each workgroup takes 1024 queue
+// entries (four per thread, coalesced), counts its survivors with ballots,
+// and reserves room for all of them with one atomic.  One atomic per wave
+// serialises on the counter when the queue holds many records.
+constexpr uint32_t CP_PER = 4;
+
 __global__ __launch_bounds__(256) void k_round_compact(CompactArgs a)
 {
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t wbase;
     const uint32_t count = min(*a.in_count, a.cap);
     const uint32_t lane = __lane_id();
-    for (uint32_t base = blockIdx.x * blockDim.x; base < count;
-         base += gridDim.x * blockDim.x) {
-        const uint32_t i = base + threadIdx.x;
-        bool alive = false;
-        if (i < count) {
-            alive = a.flags[i] == 0;
-            a.flags[i] = 0;
+    const uint32_t wave = threadIdx.x >> 6;
+    for (uint32_t base = blockIdx.x * (256 * CP_PER); base < count;
+         base += gridDim.x * (256 * CP_PER)) {
+        uint64_t m[CP_PER];
+        uint32_t tot = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < CP_PER; k++) {
+            const uint32_t i = base + k * 256 + threadIdx.x;
+            bool alive = false;
+            if (i < count) {
+                alive = a.flags[i] == 0;
+                a.flags[i] = 0;
+            }
+            m[k] = __ballot(alive);
+            tot += (uint32_t)__popcll((unsigned long long)m[k]);
         }
-        const uint64_t mask = __ballot(alive);
-        if (mask == 0) continue;
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        uint32_t wbase = 0;
-        if (lane == (uint32_t)leader)
-            wbase = atomicAdd(a.out_count, (uint32_t)__popcll(mask));
-        wbase = __shfl(wbase, leader);
-        if (alive) {
-            const uint32_t below = __builtin_amdgcn_mbcnt_hi(
-                (uint32_t)(mask >> 32),
-                __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-            a.out[wbase + below] = a.in[i];
+        if (lane == 0) wsum[wave] = tot;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t run = 0;
+            for (int w = 0; w < 4; w++) {
+                const uint32_t t = wsum[w];
+                wsum[w] = run;
+                run += t;
+            }
+            wbase = run ? atomicAdd(a.out_count, run) : 0u;
         }
+        __syncthreads();
+        uint32_t off = wbase + wsum[wave];
+#pragma unroll
+        for (uint32_t k = 0; k < CP_PER; k++) {
+            if ((m[k] >> lane) & 1) {
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(m[k] >> 32),
+                    __builtin_amdgcn_mbcnt_lo((uint32_t)m[k], 0u));
+                a.out[off + below] = a.in[base + k * 256 + threadIdx.x];
+            }
+            off += (uint32_t)__popcll((unsigned long long)m[k]);
+        }
+        __syncthreads();           // wsum and wbase are reused
     }
 }
 
@@ -593,9 +822,18 @@ __device__ __forceinline__ int first_set(uint64_t m, int cap)
     return m ? min(cap, __builtin_ctzll(m)) : cap;
 }
 
+// Few candidates: every base in one pass (one latency instead of two),
+// decided from the true count.
+__device__ __forceinline__ bool cls_single(const ClassifyArgs& a,
+                                           uint32_t count)
+{
+    return (uint64_t)count * a.cap_max <= a.single_limit;
+}
+
 __global__ __launch_bounds__(128) void k_cls_base2(ClassifyArgs a)
 {
     const uint32_t count = min(*a.count, a.cap);
+    if (cls_single(a, count)) return;
     const uint64_t total = (uint64_t)count * a.cap_max;
     for (uint64_t idx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
          idx < total; idx += (uint64_t)gridDim.x * blockDim.x) {
@@ -617,7 +855,8 @@ __global__ __launch_bounds__(128) void k_cls_full(ClassifyArgs a)
     const uint32_t count = min(*a.count, a.cap);
     // single pass: all seven bases (base 2 into fail2) with no dependence
     // on a base-2 pass; otherwise bases 3..7 after it
-    const uint32_t nb = a.single ? 7 : 6;
+    const bool single = cls_single(a, count);
+    const uint32_t nb = single ? 7 : 6;
     const uint64_t per = (uint64_t)a.cap_max * nb;
     const uint64_t total = (uint64_t)count * per;
     for (uint64_t idx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -625,7 +864,7 @@ __global__ __launch_bounds__(128) void k_cls_full(ClassifyArgs a)
         const uint32_t r = (uint32_t)(idx / per);
         const uint32_t rem = (uint32_t)(idx - (uint64_t)r * per);
         const int i = (int)(rem / nb);
-        const int b = (int)(rem - (uint32_t)i * nb) + (a.single ? 0 : 1);
+        const int b = (int)(rem - (uint32_t)i * nb) + (single ? 0 : 1);
         if (b == 0) {
             const DevRec rec = a.recs[r];
             const DevWinInfo W = a.wins[rec.window];
@@ -640,7 +879,7 @@ __global__ __launch_bounds__(128) void k_cls_full(ClassifyArgs a)
         const DevRec rec = a.recs[r];
         const DevWinInfo W = a.wins[rec.window];
         if (W.mode == 0 || i >= W.cap) continue;
-        if (!a.single) {
+        if (!single) {
             const int u = first_set(a.fail2[r], W.cap);
             if (i >= u) continue;
             if (W.mode == 2 && u < W.exact_from) continue;
@@ -741,14 +980,21 @@ hipError_t launch_prep(const DevEngine* engines, const DevPiece* pieces,
     return hipGetLastError();
 }
 
-hipError_t launch_round_test(const RoundArgs& a, uint32_t gx, uint32_t gy,
-                             hipStream_t stream)
+hipError_t launch_round_test(const RoundArgs& a, bool lane_prime, uint32_t gx,
+                             uint32_t gy, hipStream_t stream)
 {
-    // every prime block must be launched: the host sizes pblock for gy
-    if (gx < 1 || gy < 1 || gy > 65535 || a.pblock == 0 ||
-        (uint64_t)gy * a.pblock < (uint64_t)(a.p_end - a.p_begin))
-        return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_round, dim3(gx, gy), dim3(256), 0, stream, a);
+    const uint64_t np = a.p_end - a.p_begin;
+    if (gx < 1 || gy < 1 || gy > 65535) return hipErrorInvalidValue;
+    if (lane_prime) {
+        // every prime must have its lane: 256 primes per blockIdx.x
+        if ((uint64_t)gx * 256 < np) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_round_p, dim3(gx, gy), dim3(256), 0, stream, a);
+    } else {
+        // every prime block must be launched
+        if (a.pblock == 0 || (uint64_t)gy * a.pblock < np)
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_round_c, dim3(gx, gy), dim3(256), 0, stream, a);
+    }
     return hipGetLastError();
 }
 
@@ -757,8 +1003,7 @@ hipError_t launch_kill_table(const PrimeK* primes, uint32_t p0, uint32_t np,
                              uint32_t stride, uint32_t* kt, hipStream_t stream)
 {
     if (np == 0) return hipSuccess;
-    if (stride < 16 || (stride & 3) || stride < 1 + kill_max(family, n))
-        return hipErrorInvalidValue;
+    if (stride != round_row_stride(family, n)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_kill_table, dim3((np + 255) / 256), dim3(256), 0,
                        stream, primes, p0, np, family, n,
                        make_uint4(d[0], d[1], d[2], d[3]), stride, kt);
@@ -778,11 +1023,10 @@ hipError_t launch_classify(const ClassifyArgs& a, uint32_t grid,
 {
     if (grid < 1) grid = 1;
     hipError_t e;
-    if (!a.single) {
-        hipLaunchKernelGGL(k_cls_base2, dim3(grid), dim3(128), 0, stream, a);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
+    // exits at once when the count makes it a single pass
+    hipLaunchKernelGGL(k_cls_base2, dim3(grid), dim3(128), 0, stream, a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_cls_full, dim3(grid), dim3(128), 0, stream, a);
     e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -112,4 +112,45 @@ FK_HD uint32_t kill_values(int family, int n, const uint32_t d[4],
     return cnt;
 }
 
+// One round prime's row in an engine's round table: everything a wave
+// needs to test its candidates against the prime, contiguous, so that two
+// 64-byte scalar loads fetch it.  Words [0, 12) are this header, words
+// [12, stride) the killing residues (kKillPad past `cnt`).
+struct RoundRow {
+    double qd;        // q
+    double qi;        // nearest double to 1/q
+    double c16;       // 2^16 mod q
+    double c32;       // 2^32 mod q
+    double c48;       // 2^48 mod q
+    uint32_t q;
+    uint32_t cnt;     // killing residues
+};
+constexpr uint32_t kRowHead = 12;           // words of RoundRow
+static_assert(sizeof(RoundRow) == 4 * kRowHead, "RoundRow layout");
+
+// Row stride in words: the header plus the largest kill set, rounded up to
+// whole 64-byte loads, at least 32 words (20 residues).
+FK_HD uint32_t round_row_stride(int family, int n)
+{
+    const uint32_t w = kRowHead + kill_max(family, n);
+    return w <= 32 ? 32u : (w + 15u) & ~15u;
+}
+
+FK_HD void round_row(int family, int n, const uint32_t d[4], const PrimeK& p,
+                     uint32_t* row, uint32_t stride)
+{
+    const PrimeD pd = make_primed(p.q);
+    RoundRow h;
+    h.qd = pd.qd;
+    h.qi = pd.qi;
+    h.c16 = pd.c[0];
+    h.c32 = pd.c[1];
+    h.c48 = pd.c[2];
+    h.q = p.q;
+    h.cnt = kill_values(family, n, d, p, row + kRowHead, stride - kRowHead);
+    uint32_t hw[kRowHead];
+    __builtin_memcpy(hw, &h, sizeof h);          // no type punning
+    for (uint32_t i = 0; i < kRowHead; i++) row[i] = hw[i];
+}
+
 }  // namespace fk

@@ -40,7 +40,7 @@ struct RoundArgs {
     uint32_t cap;
     uint32_t* flags;
     uint32_t p_begin, p_end;  // global prime indices of this round
-    uint32_t pblock;          // primes per blockIdx.y
+    uint32_t pblock;          // lane = candidate: primes per blockIdx.y
 };
 
 struct CompactArgs {
@@ -61,7 +61,8 @@ struct ClassifyArgs {
     uint32_t cap_max;         // largest chain cap of any window
     uint64_t* fail2;          // per record: values failing base 2
     uint64_t* failx;          // per record: values failing bases 3..7
-    uint32_t single;          // 1: all bases in one pass (few candidates)
+    uint32_t single_limit;    // records x cap_max at or below which all
+                              // bases run in one pass
 };
 
 hipError_t launch_prep(const DevEngine* engines, const DevPiece* pieces,
@@ -71,8 +72,8 @@ hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,
                         uint32_t grid, size_t lds_bytes, hipStream_t stream);
 int sieve_occupancy(int threads, int wpt, size_t lds_bytes);
 bool sieve_shape_ok(int threads, int wpt);
-hipError_t launch_round_test(const RoundArgs& a, uint32_t gx, uint32_t gy,
-                             hipStream_t stream);
+hipError_t launch_round_test(const RoundArgs& a, bool lane_prime, uint32_t gx,
+                             uint32_t gy, hipStream_t stream);
 hipError_t launch_kill_table(const PrimeK* primes, uint32_t p0, uint32_t np,
                              int family, int n, const uint32_t d[4],
                              uint32_t stride, uint32_t* kt, hipStream_t stream);

@@ -92,7 +92,7 @@ def tune(family, n, q2, lo, span, passes=2, reps=3, classify=None,
     cur.setdefault("nwheel", 0)
     cur.setdefault("round_growth", 2)
     cur.setdefault("q_mark", 0)
-    cur.pop("round_chunk", None)
+    cur.pop("round_lane_prime", None)
     best, st = measure(family, n, q2, lo, span, cur, reps, classify,
                        wheel_primes)
     log(f"start {cur} -> {best:.3f} ms {st}")

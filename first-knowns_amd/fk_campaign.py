@@ -1,0 +1,182 @@
+"""Campaign segment loop over the MI355X engine.
+
+A campaign sweeps a range of one family's search line in segments. Every
+survivor is classified on the device, inside the sweep. Each segment's
+records are appended to a durable ledger, and the cursor is persisted at
+every segment boundary.
+
+The reference loops are euler-prime-runs/launch.py:648-781,
+dickson-ladders/launch.py:1072-1117 and primorial-ap/launch.py:583-642
+(`_finalize` :686-770). They keep one segment in flight on the GPU while a
+host process pool classifies the previous one. Here the strong tests run on
+the device, so the overlap is different: two batches alternate, and the
+device sweeps segment i+1 while the host persists segment i.
+
+Durability follows huntlib/checkpoint.py (see fk_checkpoint):
+* A segment's records are appended to the ledger and fsynced before the
+  cursor that covers them is saved.
+* The cursor stores the ledger's byte length at its boundary, and a resume
+  truncates the ledger to that length. A crash between the two writes
+  therefore redoes that one segment and never duplicates its records.
+* The cursor's key names what its position is counted in: family, depth,
+  sieve bound, wheel and classification. A cursor under another key is
+  refused, not reinterpreted.
+"""
+
+import json
+import os
+
+import fk_checkpoint as C
+import fk_native as N
+
+FAMILY_NAME = {N.FAMILY_EULER: "euler", N.FAMILY_DICKSON: "dickson",
+               N.FAMILY_AP: "ap"}
+
+
+def default_classify(family, n):
+    """(mode, cap, exact_from) the reference's launchers record. Dickson
+    uses base-2 chains redone with every base from run 5
+    (dickson-ladders/launch.py:307-342). AP uses two-pass depth from 6
+    (primorial-ap/ap_search.py:79-86, 160-194). Euler uses full-base runs
+    (euler-prime-runs/euler_search.py:258-262)."""
+    if family == N.FAMILY_DICKSON:
+        return (N.CLASSIFY_TWO_PASS, n + 8, 5)
+    if family == N.FAMILY_AP:
+        return (N.CLASSIFY_TWO_PASS, n, 6)
+    return (N.CLASSIFY_FULL, n + 8, 0)
+
+
+def engine_key(engine, classify):
+    """The parameters that give a cursor position its meaning."""
+    return {"family": FAMILY_NAME.get(engine.family, str(engine.family)),
+            "n": int(engine.n), "q2": int(engine.q2),
+            "line_step": int(engine.line_step),
+            "nlines": int(engine.nlines),
+            "classify": [int(x) for x in classify]}
+
+
+class Campaign:
+    """Sweep [lo, hi) of `engine`'s line in segments of `seg`, with the
+    cursor at `path` and the records in `ledger` (JSON lines of
+    {"value", "run"}; default `path` + ".records.jsonl").
+
+    `batches` is for tests: two objects with the fk_native.Batch interface.
+    """
+
+    def __init__(self, engine, lo, hi, seg, path, ledger=None,
+                 classify=None, fresh=False, batches=None, warn=None):
+        lo, hi, seg = int(lo), int(hi), int(seg)
+        if not lo < hi:
+            raise ValueError(f"empty range [{lo}, {hi})")
+        if seg <= 0:
+            raise ValueError(f"segment length {seg} must be positive")
+        self.engine = engine
+        self.lo, self.hi, self.seg = lo, hi, seg
+        self.path = os.fspath(path)
+        self.ledger = os.fspath(ledger) if ledger else self.path + ".records.jsonl"
+        self.classify = tuple(classify or default_classify(engine.family,
+                                                           engine.n))
+        self.key = engine_key(engine, self.classify)
+        self.warn = warn
+        C.refuse_mismatch(self.path, self.key, fresh=fresh,
+                          describe=lambda s: f"cursor at {s.get('next')}")
+        state = None if fresh else C.load(self.path, self.key, warn=warn)
+        if state is None:
+            state = {"key": self.key, "next": str(lo), "segments": 0,
+                     "survivors": 0, "swept": "0", "census": {},
+                     "best_run": -1, "best_value": None, "ledger_bytes": 0}
+            self._truncate_ledger(0)
+        else:
+            self._truncate_ledger(int(state["ledger_bytes"]))
+        self.state = state
+        self.batches = batches if batches is not None else [
+            N.Batch(getattr(engine, "device", 0)) for _ in range(2)]
+
+    # ------------------------------------------------------------ ledger
+
+    def _truncate_ledger(self, nbytes):
+        if nbytes == 0:
+            with open(self.ledger, "w"):
+                pass
+            return
+        if not os.path.exists(self.ledger) or \
+                os.path.getsize(self.ledger) < nbytes:
+            raise C.CheckpointCorrupt(
+                f"{self.ledger} is shorter than the {nbytes} bytes its cursor "
+                f"covers: records below the cursor were lost")
+        with open(self.ledger, "r+") as f:
+            f.truncate(nbytes)
+            f.flush()
+            os.fsync(f.fileno())
+
+    def _append(self, values, runs):
+        with open(self.ledger, "a", newline="\n") as f:
+            for v, r in zip(values, runs):
+                f.write(json.dumps({"value": int(v), "run": int(r)}) + "\n")
+            f.flush()
+            os.fsync(f.fileno())
+        return os.path.getsize(self.ledger)
+
+    # -------------------------------------------------------------- loop
+
+    @property
+    def cursor(self):
+        return int(self.state["next"])
+
+    def _consume(self, s0, s1, batch, on_segment):
+        recs, _ = batch.results()
+        values = N.record_values(recs)
+        runs = [int(r) for r in recs["run"].tolist()]
+        size = self._append(values, runs)
+        st = dict(self.state)
+        census = dict(st["census"])
+        best_run, best_value = st["best_run"], st["best_value"]
+        for v, r in zip(values, runs):
+            census[str(r)] = census.get(str(r), 0) + 1
+            if r > best_run:
+                best_run, best_value = r, str(v)
+        st.update(next=str(s1), segments=st["segments"] + 1,
+                  survivors=st["survivors"] + len(values),
+                  swept=str(int(st["swept"]) + (s1 - s0)), census=census,
+                  best_run=best_run, best_value=best_value,
+                  ledger_bytes=size)
+        C.save_json(self.path, st)
+        self.state = st
+        if on_segment:
+            on_segment(s0, s1, values, runs, st)
+
+    def run(self, max_segments=None, on_segment=None):
+        """Sweep from the cursor to hi (or max_segments more segments).
+        Returns the final cursor state. An interrupt leaves the cursor at
+        the last completed segment boundary; so does any other error, and
+        then nothing more is written."""
+        eng = self.engine
+        cur = self.cursor
+        pending, slot, submitted = None, 0, 0
+        while True:
+            new = None
+            if cur < self.hi and (max_segments is None or
+                                  submitted < max_segments):
+                s1 = min(cur + self.seg, self.hi)
+                b = self.batches[slot]
+                slot ^= 1
+                b.submit([eng], [0], [(cur, s1)], [self.classify])
+                new = (cur, s1, b)
+                cur = s1
+                submitted += 1
+            if pending is not None:
+                self._consume(*pending, on_segment)
+            elif new is None:
+                break
+            pending = new
+        return self.state
+
+    def records(self):
+        """Every (value, run) in the ledger, in sweep order."""
+        out = []
+        with open(self.ledger) as f:
+            for line in f:
+                if line.strip():
+                    r = json.loads(line)
+                    out.append((int(r["value"]), int(r["run"])))
+        return out

@@ -124,11 +124,11 @@ def ap_difference(n):
 # fk_tuning.json written by the autotuner overrides them per family.
 DEFAULT_TUNING = {
     "euler": {"q_pat": 128, "q_mark": 0, "q_med": 1024, "threads": 256,
-              "words_per_thread": 8, "round_chunk": 64},
+              "words_per_thread": 8},
     "dickson": {"q_pat": 128, "q_mark": 0, "q_med": 2048, "threads": 256,
-                "words_per_thread": 8, "round_chunk": 64},
+                "words_per_thread": 8},
     "ap": {"q_pat": 128, "q_mark": 0, "q_med": 2048, "threads": 256,
-           "words_per_thread": 8, "round_chunk": 64},
+           "words_per_thread": 8},
 }
 
 

@@ -29,7 +29,8 @@ EXPORTS = ("fk_abi_version", "fk_last_error", "fk_device_count",
            "fk_engine_create", "fk_engine_destroy", "fk_engine_get_info",
            "fk_engine_sweep", "fk_engine_classify", "fk_batch_create",
            "fk_batch_destroy", "fk_batch_submit", "fk_batch_query",
-           "fk_batch_results", "fk_batch_timing", "fk_engine_batch")
+           "fk_batch_results", "fk_batch_timing", "fk_batch_counts",
+           "fk_engine_batch")
 
 u32, i32, u64 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
 
@@ -38,7 +39,7 @@ class EngineDesc(ctypes.Structure):
     _fields_ = [("family", i32), ("n", i32), ("q2", u32), ("device", i32),
                 ("nwheel", i32), ("wheel_primes", u32 * 12),
                 ("q_pat", u32), ("q_med", u32), ("threads", u32),
-                ("words_per_thread", u32), ("round_chunk", u32),
+                ("words_per_thread", u32), ("round_lane_prime", u32),
                 ("round_growth", u32), ("q_mark", u32)]
 
 
@@ -104,6 +105,7 @@ def lib():
     L.fk_batch_query.argtypes = [vp, P(ctypes.c_int)]
     L.fk_batch_results.argtypes = [vp, vp, sz, P(sz), vp]
     L.fk_batch_timing.argtypes = [vp, vp, P(u64), P(u64)]
+    L.fk_batch_counts.argtypes = [vp, vp, sz, P(sz)]
     L.fk_engine_batch.argtypes = [vp, P(vp)]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
@@ -164,7 +166,7 @@ class Engine:
 
     def __init__(self, family, n, q2, device=0, nwheel=0, wheel_primes=None,
                  q_pat=0, q_med=0, threads=0, words_per_thread=0,
-                 round_chunk=0, round_growth=0, q_mark=0):
+                 round_lane_prime=0, round_growth=0, q_mark=0):
         d = EngineDesc()
         d.family, d.n, d.q2, d.device = int(family), int(n), int(q2), device
         d.nwheel = int(nwheel)
@@ -172,7 +174,8 @@ class Engine:
             d.wheel_primes[i] = int(q)
         d.q_pat, d.q_med = int(q_pat), int(q_med)
         d.threads, d.words_per_thread = int(threads), int(words_per_thread)
-        d.round_chunk, d.round_growth = int(round_chunk), int(round_growth)
+        d.round_lane_prime = int(round_lane_prime)
+        d.round_growth = int(round_growth)
         d.q_mark = int(q_mark)
         h = ctypes.c_void_p()
         check(lib().fk_engine_create(ctypes.byref(d), ctypes.byref(h)))
@@ -240,6 +243,22 @@ class Engine:
         check(lib().fk_engine_batch(self._h, ctypes.byref(b)))
         return batch_timing(b)
 
+    def last_counts(self):
+        """Queue sizes after the sieve and each round of the last sweep."""
+        b = ctypes.c_void_p()
+        check(lib().fk_engine_batch(self._h, ctypes.byref(b)))
+        return batch_counts(b)
+
+
+def batch_counts(handle):
+    """Queue sizes after the sieve and after each round of the last run."""
+    n = ctypes.c_size_t(0)
+    check(lib().fk_batch_counts(handle, None, 0, ctypes.byref(n)))
+    out = (u32 * max(n.value, 1))()
+    check(lib().fk_batch_counts(handle, ctypes.cast(out, ctypes.c_void_p),
+                                n.value, ctypes.byref(n)))
+    return [int(x) for x in out[:n.value]]
+
 
 def batch_timing(handle):
     ms = (ctypes.c_float * 5)()
@@ -304,3 +323,6 @@ class Batch:
 
     def timing(self):
         return batch_timing(self._h)
+
+    def counts(self):
+        return batch_counts(self._h)

@@ -72,7 +72,10 @@ typedef struct fk_engine_desc {
                                 (0 = default) */
     uint32_t threads;        /* sieve workgroup size: 256, 512, 1024 (0 = default) */
     uint32_t words_per_thread; /* tile = 64 * threads * this bits (0 = default) */
-    uint32_t round_chunk;    /* primes per round work item (0 = default) */
+    uint32_t round_lane_prime; /* rounds of at least this many primes, with at
+                                least 1/8 as many expected candidates, test
+                                one prime per lane; the others one candidate
+                                per lane (0 = default 8192) */
     uint32_t round_growth;   /* prime bound grows by this factor per round
                                 (0 = default 2) */
     uint32_t q_mark;         /* primes in [q_pat, q_mark) are stride-marked
@@ -207,6 +210,15 @@ int fk_batch_results(fk_batch* b, fk_record* out, size_t out_cap,
  */
 int fk_batch_timing(fk_batch* b, float* ms5, uint64_t* tiles,
                     uint64_t* candidates);
+
+/*
+ * Candidates in the queue after each stage of the last completed batch or
+ * sweep: [0] after the sieve, then after each compaction round, in order.
+ * *n = the number of stages; FK_E_OVERFLOW if cap is smaller (the first
+ * cap are written).  No reference counterpart: the autotuner and the
+ * roofline accounting (tests per round = candidates x primes) read it.
+ */
+int fk_batch_counts(fk_batch* b, uint32_t* counts, size_t cap, size_t* n);
 
 /* The engine's own internal batch, for timing its last sweep. */
 int fk_engine_batch(fk_engine* eng, fk_batch** out);

@@ -1,6 +1,7 @@
 // Host build of the round-prime arithmetic the kernels use
 // (first-knowns_amd/csrc/fk_kill.h, fk_arith.h).  Reads lines
 //   K family n q d0 d1 d2 d3      -> count and the kill set (value domain)
+//   W family n q d0 d1 d2 d3      -> the round-table row of q
 //   R q nl t0 t1 t2 t3            -> the residue of the value t (four hex
 //                                    32-bit limbs) by the 16-bit-limb
 //                                    double-precision path with nl limbs
@@ -24,6 +25,22 @@ int main()
             const uint32_t c = fk::kill_values(fam, n, d, pk, out.data(), stride);
             printf("%u", c);
             for (uint32_t i = 0; i < stride; i++) printf(" %u", out[i]);
+            printf("\n");
+        } else if (line[0] == 'W') {
+            int fam, n;
+            unsigned q;
+            uint32_t d[4];
+            if (sscanf(line + 1, "%d %d %u %x %x %x %x", &fam, &n, &q, &d[0],
+                       &d[1], &d[2], &d[3]) != 7) continue;
+            const fk::PrimeK pk = fk::make_primek(q);
+            const uint32_t stride = fk::round_row_stride(fam, n);
+            std::vector<uint32_t> row(stride);
+            fk::round_row(fam, n, d, pk, row.data(), stride);
+            fk::RoundRow h;
+            memcpy(&h, row.data(), sizeof h);
+            printf("%u %.17g %.17g %.17g %.17g %.17g %u %u", stride, h.qd, h.qi,
+                   h.c16, h.c32, h.c48, h.q, h.cnt);
+            for (uint32_t i = fk::kRowHead; i < stride; i++) printf(" %u", row[i]);
             printf("\n");
         } else if (line[0] == 'R') {
             unsigned q;

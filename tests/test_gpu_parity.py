@@ -204,7 +204,9 @@ def test_streams_do_not_depend_on_cuts_or_tiles():
                {"q_med": 1024, "q_pat": 64},
                {"q_med": 2048, "q_pat": 256},
                {"nwheel": 1}, {"nwheel": 2},
-               {"round_growth": 4, "round_chunk": 16},
+               {"round_growth": 4, "round_lane_prime": 1},
+               {"round_lane_prime": 1}, {"round_lane_prime": 1 << 30},
+               {"round_lane_prime": 1, "q_med": 256},
                {"q_mark": 1024}, {"q_mark": 4096},
                {"q_mark": 256, "threads": 1024, "words_per_thread": 1},
                {"q_mark": 512, "q_med": 3072, "q_pat": 64}):
@@ -215,3 +217,44 @@ def test_streams_do_not_depend_on_cuts_or_tiles():
         a = eng.survivors_j(lo, cut).tolist()
         b = eng.survivors_j(cut, hi).tolist()
         assert a + b == ref, cut
+
+
+# --------------------------------------------- both round kernels
+
+ROUND_CASES = [
+    # (label, family, n, q2, lo, hi, engine kwargs, min population)
+    ("dickson", N.FAMILY_DICKSON, 10, 1 << 16, 10**12, 10**12 + (1 << 30),
+     {}, 250),
+    ("dickson-wide", N.FAMILY_DICKSON, 13, 1 << 16, 4 * 10**18 - 10**9,
+     4 * 10**18, {}, 12),
+    ("euler-wide", N.FAMILY_EULER, 8, 1 << 16, 10**20, 10**20 + 10**9,
+     {"wheel_primes": [2, 3, 5, 7, 11, 13]}, 50),
+    ("ap", N.FAMILY_AP, 12, 1 << 16, 4 * 10**13 - 10,
+     4 * 10**13 - 10 + 3 * 10**8, {}, 80),
+]
+
+
+def round_case_oracle(fam, n, q2, lo, hi, kw):
+    if fam == N.FAMILY_DICKSON:
+        return O.dickson_survivors(n, lo, hi, q2=q2)
+    if fam == N.FAMILY_EULER:
+        return O.euler_survivors(n, lo, hi, wheel_primes=kw["wheel_primes"],
+                                 q2=q2)
+    return [lo + o for o in O.ap_survivors(n, lo, hi - lo, q2=q2)]
+
+
+@pytest.mark.parametrize("case", ROUND_CASES, ids=[c[0] for c in ROUND_CASES])
+def test_round_kernels_match_oracle(case):
+    """Lane = prime and lane = candidate rounds (forced by round_lane_prime),
+    at small and default LDS cutoffs, against the oracle."""
+    label, fam, n, q2, lo, hi, kw, need = case
+    say("round kernels", label)
+    want = round_case_oracle(fam, n, q2, lo, hi, kw)
+    assert len(want) >= need
+    for extra in ({"round_lane_prime": 1}, {"round_lane_prime": 1 << 30},
+                  {"round_lane_prime": 1, "q_med": 256, "q_pat": 64},
+                  {"round_lane_prime": 1 << 30, "q_med": 256, "q_pat": 64}):
+        eng = N.Engine(fam, n, q2, **kw, **extra)
+        got = N.record_values(eng.sweep(lo, hi))
+        eng.close()
+        assert got == want, (label, extra)

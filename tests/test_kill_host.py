@@ -124,3 +124,26 @@ def test_kill_sets_large_primes(harness, family, n):
                 assert any((m * r * r + 1) % q == 0 for m in range(1, n + 1))
             # distinct m give distinct roots: -1/m differ for m < q
             assert len(g) == want
+
+
+@pytest.mark.parametrize("family,n", [(1, 17), (2, 10), (2, 13), (3, 8),
+                                      (3, 17), (2, 24)])
+def test_round_rows(harness, family, n):
+    rng = random.Random(7 * family + n)
+    qs = [random_prime(rng, 2**11, 2**30) for _ in range(30)] + [2053]
+    d = O.primorial(n) if family == 3 else 0
+    lines = [f"W {family} {n} {q} {limbs(d)}" for q in qs]
+    kmax = 2 * n if family == 2 else n
+    for q, row in zip(qs, call(harness, lines)):
+        w = row.split()
+        stride = int(w[0])
+        assert stride >= 12 + kmax and stride % 16 == 0 and stride >= 32
+        qd, qi, c16, c32, c48 = (float(x) for x in w[1:6])
+        assert (qd, qi) == (float(q), 1.0 / q)
+        assert (c16, c32, c48) == (2**16 % q, 2**32 % q, 2**48 % q)
+        assert int(w[6]) == q
+        cnt = int(w[7])
+        res = [int(x) for x in w[8:]]
+        assert len(res) == stride - 12
+        assert len(set(res[:cnt])) == cnt
+        assert all(x == 0xFFFFFFFF for x in res[cnt:])
