@@ -38,6 +38,10 @@ struct DevEngine {
     uint32_t round_first;     // global prime-table index of the first round prime
     uint32_t round_end;       // one past the last prime below q2
     uint32_t kstride;         // round-table row stride in words
+    uint32_t nround;          // compaction rounds of this engine
+    uint32_t rb[33];          // [nround + 1] prime-table index where each
+                              // round starts (rb[0] = round_first, the last
+                              // = round_end): q_med * growth^r
     uint64_t D;               // wheel period of the line
     uint32_t W;               // Dickson: k = W j (product of primes <= n+1)
     uint32_t d[8];            // AP: P(n), the common difference, in limbs

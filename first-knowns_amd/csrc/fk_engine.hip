@@ -340,11 +340,22 @@ static int build_engine(fk_engine* e)
                               prime_index(ctx->primes, wheel.back() + 1));
     e->round_end = prime_index(ctx->primes, d.q2);
     if (e->round_end < e->round_first) e->round_end = e->round_first;
-    {
-        uint32_t r = 0;
-        for (uint64_t b = e->q_med; b < d.q2; b *= e->growth) r++;
-        e->nrounds = r;
+    // round boundaries: prime values q_med * growth^r, as table indices
+    std::vector<uint32_t> rb(1, e->round_first);
+    if (e->round_end > e->round_first) {
+        for (uint64_t v = (uint64_t)e->q_med * e->growth;; v *= e->growth) {
+            const uint32_t idx = std::min(
+                e->round_end,
+                prime_index(ctx->primes, (uint32_t)std::min<uint64_t>(v, d.q2)));
+            if (idx > rb.back()) rb.push_back(idx);
+            if (v >= d.q2) break;
+        }
+        if (rb.back() < e->round_end) rb.push_back(e->round_end);
     }
+    if (rb.size() > 33)
+        return fail(FK_E_ARG, "more than 32 compaction rounds (raise "
+                    "round_growth)");
+    e->nrounds = (uint32_t)rb.size() - 1;
 
     // ---- family constants
     e->W = (d.family == FK_FAMILY_DICKSON) ? dickson_wheel(d.n) : 1;
@@ -366,6 +377,9 @@ static int build_engine(fk_engine* e)
     h.round_first = e->round_first;
     h.round_end = e->round_end;
     h.kstride = round_row_stride(d.family, d.n);
+    h.nround = e->nrounds;
+    for (size_t i = 0; i < 33; i++)
+        h.rb[i] = i < rb.size() ? rb[i] : e->round_end;
     h.D = e->D;
     h.W = (uint32_t)e->W;
     for (int i = 0; i < 4; i++) {
@@ -434,7 +448,9 @@ struct fk_batch {
     std::vector<uint32_t> round_slots;  // counter of each executed round
     std::vector<uint32_t> stage_counts; // queue after the sieve and rounds
     uint32_t final_q = 0;             // which queue holds the final records
-    uint32_t queue_cap = 0;
+    uint32_t queue_cap = 0;           // entries per queue (all segments)
+    std::vector<uint32_t> seg_caps;   // per engine: its queue segment
+    uint2* d_segs = nullptr;          size_t cap_segs = 0;
     uint32_t copied = 0;
     uint64_t tiles = 0, candidates = 0;
     std::vector<fk_record> out;       // sorted results
@@ -445,7 +461,7 @@ struct fk_batch {
 static int batch_free(fk_batch* b)
 {
     (void)hipSetDevice(b->device);
-    void* ds[] = {b->d_engines, b->d_pieces, b->d_win_engine, b->d_wins,
+    void* ds[] = {b->d_engines, b->d_pieces, b->d_win_engine, b->d_wins, b->d_segs,
                   b->d_bq, b->d_q[0], b->d_q[1], b->d_flags, b->d_fail2,
                   b->d_failx, b->d_counts};
     for (void* p : ds) if (p) (void)hipFree(p);
@@ -509,10 +525,24 @@ static double round_survival(const fk_engine* e, double lo, double hi)
 // one pass
 static constexpr uint32_t kSingleLimit = 8192;
 
-static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
+static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
 {
     const size_t nwin = b->windows.size();
+    // the engine with the most candidates sets the tile threads, the
+    // longest tile and the round-kernel crossover of the whole batch
     fk_engine* e0 = b->engines[0];
+    {
+        std::vector<double> work(b->engines.size(), 0.0);
+        for (size_t w = 0; w < nwin; w++) {
+            const fk_engine* e = b->engines[b->engine_of[w]];
+            const u128 lo = mk128(b->windows[w].lo), hi = mk128(b->windows[w].hi);
+            if (hi > lo) work[b->engine_of[w]] += (double)(hi - lo) * e->line_density;
+        }
+        size_t best = 0;
+        for (size_t k = 1; k < work.size(); k++)
+            if (work[k] > work[best]) best = k;
+        e0 = b->engines[best];
+    }
     DeviceCtx* ctx = nullptr;
     uint32_t need_limit = 0;
     for (fk_engine* e : b->engines) need_limit = std::max(need_limit, e->q2);
@@ -520,25 +550,31 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
     if (rc) return rc;
     HIPCHK(hipSetDevice(b->device));
 
-    // ---- pieces
+    // ---- pieces, grouped by tile shape: each window's tile is no longer
+    // than its lines need (words_per_thread halves while half a tile still
+    // holds a line), and each group of windows with one shape is one sieve
+    // launch over a contiguous range of tiles
     const uint32_t T = e0->threads;
-    uint64_t max_npos = 1;
-    struct WinPlan { u128 B; uint64_t H; };
+    struct WinPlan { u128 B; uint64_t H; uint32_t wpt; };
     std::vector<WinPlan> wp(nwin);
     for (size_t w = 0; w < nwin; w++) {
         const fk_engine* e = b->engines[b->engine_of[w]];
         const u128 lo = mk128(b->windows[w].lo), hi = mk128(b->windows[w].hi);
         const u128 B = lo - lo % e->D;
         wp[w].B = B;
-        wp[w].H = (uint64_t)(hi - B);
+        wp[w].H = hi > B ? (uint64_t)(hi - B) : 0;
         const uint64_t npos = wp[w].H / e->D + 1;
-        max_npos = std::max(max_npos, npos);
+        uint32_t wpt = e0->wpt;
+        while (wpt > 1 && (uint64_t)64 * T * (wpt / 2) >= npos) wpt /= 2;
+        wp[w].wpt = wpt;
     }
-    // tile length: no longer than the longest line needs
-    uint32_t wpt = e0->wpt;
-    while (wpt > 1 && (uint64_t)64 * T * (wpt / 2) >= max_npos) wpt /= 2;
-    const uint64_t S = (uint64_t)64 * T * wpt;
-    const uint64_t PIECE = std::max<uint64_t>(S, ((uint64_t)1 << 31) / S * S);
+    std::vector<uint32_t> order(nwin);
+    for (size_t w = 0; w < nwin; w++) order[w] = (uint32_t)w;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        return wp[x].wpt > wp[y].wpt;
+    });
+    struct Group { uint32_t wpt; uint64_t tile_begin, tile_end; };
+    std::vector<Group> groups;
 
     std::vector<DevPiece> pieces;
     std::vector<DevWinInfo> wins(nwin);
@@ -553,14 +589,20 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         bm_max = std::max(bm_max, e->host.bm_words);
     }
     for (size_t w = 0; w < nwin; w++) {
-        const fk_engine* e = b->engines[b->engine_of[w]];
         win_engine[w] = b->engine_of[w];
         wins[w].engine = b->engine_of[w];
         wins[w].mode = b->cls[w].mode;
         wins[w].cap = b->cls[w].cap;
         wins[w].exact_from = b->cls[w].exact_from;
+    }
+    for (uint32_t w : order) {
+        const fk_engine* e = b->engines[b->engine_of[w]];
         const u128 lo = mk128(b->windows[w].lo), hi = mk128(b->windows[w].hi);
         if (hi <= lo) continue;
+        const uint64_t S = (uint64_t)64 * T * wp[w].wpt;
+        const uint64_t PIECE = std::max<uint64_t>(S, ((uint64_t)1 << 31) / S * S);
+        if (groups.empty() || groups.back().wpt != wp[w].wpt)
+            groups.push_back({wp[w].wpt, tiles, tiles});
         const uint64_t npos_all = wp[w].H / e->D + 1;
         for (uint64_t p0 = 0; p0 < npos_all; p0 += PIECE) {
             DevPiece P{};
@@ -581,6 +623,7 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
             bq_total += e->host.nlds;
             pieces.push_back(P);
         }
+        groups.back().tile_end = tiles;
         const double span = (double)(hi - lo);
         expect += expected_lds(e, span);
         e_alive[b->engine_of[w]] += expected_lds(e, span);
@@ -588,24 +631,42 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
     }
     b->tiles = tiles;
     b->candidates = (uint64_t)cand;
-    if (queue_cap == 0) {
-        const double c = 1.5 * expect + 8.0 * std::sqrt(expect) + 4096.0;
-        queue_cap = (uint32_t)std::min(c, 1.0e9);
-    }
-    b->queue_cap = queue_cap;
+    std::vector<char> used(b->engines.size(), 0);
+    for (size_t w = 0; w < nwin; w++) used[b->engine_of[w]] = 1;
 
-    // ---- rounds (shared boundaries by prime value)
-    uint32_t qmin = 0xffffffffu, qmax = 0;
-    for (fk_engine* e : b->engines) {
-        qmin = std::min(qmin, e->q_med);
-        qmax = std::max(qmax, e->q2);
+    // ---- queue segments: each engine's from its expected LDS-stage
+    // survivors (or from the counts of a run that overflowed)
+    const size_t ne = b->engines.size();
+    std::vector<uint32_t> caps(ne, 0);
+    std::vector<uint2> segs(ne);
+    uint64_t qtot = 0;
+    for (size_t k = 0; k < ne; k++) {
+        if (used[k]) {
+            double c = 1.5 * e_alive[k] + 8.0 * std::sqrt(e_alive[k]) + 1024.0;
+            if (caps_in && k < caps_in->size() && (*caps_in)[k])
+                c = (*caps_in)[k];
+            caps[k] = (uint32_t)std::min(c, 1.0e9);
+            caps[k] = (caps[k] + 255u) & ~255u;
+        }
+        segs[k] = make_uint2((uint32_t)qtot, caps[k]);
+        qtot += caps[k];
     }
-    const uint32_t growth = e0->growth;
-    std::vector<std::pair<uint32_t, uint32_t>> rounds;   // prime values
-    for (uint64_t lo = qmin; lo < qmax; lo *= growth)
-        rounds.push_back({(uint32_t)lo,
-                          (uint32_t)std::min<uint64_t>(lo * growth, qmax)});
-    const size_t ncounts = 2 + rounds.size();
+    if (qtot >= 0xf0000000ull)
+        return fail(FK_E_OVERFLOW, "survivor queues need %llu entries",
+                    (unsigned long long)qtot);
+    const uint32_t queue_cap = (uint32_t)std::max<uint64_t>(qtot, 256);
+    b->queue_cap = queue_cap;
+    b->seg_caps = caps;
+
+    // ---- rounds: round r of every engine together (each engine keeps its
+    // own q_med * growth^r boundaries, DevEngine::rb)
+    uint32_t nround_max = 0;
+    for (size_t k = 0; k < b->engines.size(); k++)
+        if (used[k]) nround_max = std::max(nround_max, b->engines[k]->host.nround);
+    // counters: per stage (the sieve, then each round) one per engine,
+    // then the packed total
+    const size_t nstage = 1 + nround_max;
+    const size_t ncounts = nstage * ne + 1;
 
     // ---- buffers
     if ((rc = grow(&b->d_engines, &b->cap_engines, b->engines.size()))) return rc;
@@ -618,6 +679,7 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         b->cap_win = c;
     }
     if ((rc = grow(&b->d_bq, &b->cap_bq, std::max<uint64_t>(bq_total, 1)))) return rc;
+    if ((rc = grow(&b->d_segs, &b->cap_segs, ne))) return rc;
     if (queue_cap > b->cap_q || !b->d_q[0]) {
         size_t c = 0;
         for (int i = 0; i < 2; i++) {
@@ -662,6 +724,8 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
                           nwin * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(b->d_wins, wins.data(), nwin * sizeof(DevWinInfo),
                           hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->d_segs, segs.data(), ne * sizeof(uint2),
+                          hipMemcpyHostToDevice, s));
     // the host vectors must outlive the async copies: wait for them here
     // (they are small; the device work below is what stays asynchronous)
     HIPCHK(hipStreamSynchronize(s));
@@ -673,20 +737,23 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
                        nlds_max, b->d_bq, s));
     HIPCHK(hipEventRecord(b->ev[1], s));
 
-    // ---- sieve
-    if (tiles) {
+    // ---- sieve: one launch per tile shape
+    for (const Group& g : groups) {
+        if (g.tile_end == g.tile_begin) continue;
+        const uint64_t S = (uint64_t)64 * T * g.wpt;
         SieveArgs a{};
         a.engines = b->d_engines;
         a.pieces = b->d_pieces;
         a.npieces = (uint32_t)pieces.size();
-        a.ntiles = (uint32_t)tiles;
+        a.tile_begin = (uint32_t)g.tile_begin;
+        a.ntiles = (uint32_t)g.tile_end;
         a.nlds_max = nlds_max;
         a.pat_max = pat_max;
         a.bm_max = bm_max;
         a.bq = b->d_bq;
+        a.segs = b->d_segs;
         a.out = b->d_q[0];
-        a.out_count = b->d_counts + 0;
-        a.out_cap = queue_cap;
+        a.counts = b->d_counts;               // stage 0
         // layout of k_sieve's dynamic LDS: bits, s0, per-prime info,
         // pattern words, kill bitmaps
         const size_t lds = S / 8 + 16 * (size_t)((nlds_max + 3) / 4) +
@@ -697,37 +764,46 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         if (lds > 160 * 1024)
             return fail(FK_E_ARG, "LDS need %zu bytes > 160 KiB (lower q_med"
                         " or q_pat)", lds);
-        int occ = sieve_occupancy((int)T, (int)wpt, lds);
+        int occ = sieve_occupancy((int)T, (int)g.wpt, lds);
         if (occ < 1) occ = 1;
         uint64_t grid = (uint64_t)occ * ctx->cus;
-        grid = std::min<uint64_t>(grid, tiles);
-        HIPCHK(launch_sieve(a, (int)T, (int)wpt, (uint32_t)grid, lds, s));
+        grid = std::min<uint64_t>(grid, g.tile_end - g.tile_begin);
+        HIPCHK(launch_sieve(a, (int)T, (int)g.wpt, (uint32_t)grid, lds, s));
     }
     HIPCHK(hipEventRecord(b->ev[2], s));
 
     // ---- rounds
-    uint32_t cur = 0, cslot = 0;
+    uint32_t cur = 0, cstage = 0;
     b->round_slots.clear();
     double alive = expect;
-    for (size_t r = 0; r < rounds.size(); r++) {
+    for (uint32_t r = 0; r < nround_max; r++) {
+        // the primes of round r of every engine that has one
+        uint32_t P0 = 0xffffffffu, P1 = 0;
         alive = 0.0;
-        for (double x : e_alive) alive += x;
+        for (size_t k = 0; k < b->engines.size(); k++) {
+            const DevEngine& h = b->engines[k]->host;
+            if (!used[k] || r >= h.nround || h.rb[r + 1] <= h.rb[r]) continue;
+            P0 = std::min(P0, h.rb[r]);
+            P1 = std::max(P1, h.rb[r + 1]);
+            alive += e_alive[k];
+        }
+        if (P0 >= P1) continue;
         RoundArgs ra{};
         ra.engines = b->d_engines;
-        ra.nengines = (uint32_t)b->engines.size();
-        ra.win_engine = b->d_win_engine;
+        ra.nengines = (uint32_t)ne;
+        ra.segs = b->d_segs;
         ra.primed = ctx->d_primed;
         ra.recs = b->d_q[cur];
-        ra.count = b->d_counts + cslot;
-        ra.cap = queue_cap;
+        ra.counts = b->d_counts + cstage * ne;
         ra.flags = b->d_flags;
-        ra.p_begin = prime_index(ctx->primes, rounds[r].first);
-        ra.p_end = prime_index(ctx->primes, rounds[r].second);
-        const uint32_t np = ra.p_end > ra.p_begin ? ra.p_end - ra.p_begin : 0;
-        if (np == 0) continue;
+        ra.round = r;
+        ra.p_begin = P0;
+        ra.p_end = P1;
+        const uint32_t np = P1 - P0;
         // Many primes and enough candidates: lane = prime (256 primes per
-        // workgroup column, the queue cut into about 4096 / columns slices
-        // of at least 32).  Otherwise lane = candidate (candidate blocks
+        // workgroup column, the queue cut into slices of about 256
+        // candidates, so that no wave walks a long chain of candidates).
+        // Otherwise lane = candidate (candidate blocks
         // times prime blocks of at least 16, about 2048 workgroups).  Both
         // read the true count on the device; grids sized from an estimate
         // that is off still cover every candidate.
@@ -736,7 +812,8 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         uint32_t gx, gy;
         if (lane_prime) {
             gx = (np + 255) / 256;
-            gy = (uint32_t)std::min(std::ceil(alive / 32.0), std::ceil(4096.0 / gx));
+            gy = (uint32_t)std::min(std::ceil(alive / 256.0),
+                                    std::max(1.0, std::floor((1u << 22) / (double)gx)));
             gy = std::max(1u, std::min(gy, 65535u));
         } else {
             gx = (uint32_t)std::min(1024.0, std::max(1.0, std::ceil(alive / 256.0)));
@@ -747,26 +824,44 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         }
         HIPCHK(launch_round_test(ra, lane_prime, gx, gy, s));
         CompactArgs ca{};
+        ca.nengines = (uint32_t)ne;
+        ca.segs = b->d_segs;
         ca.in = b->d_q[cur];
-        ca.in_count = b->d_counts + cslot;
-        ca.cap = queue_cap;
+        ca.in_counts = b->d_counts + cstage * ne;
         ca.flags = b->d_flags;
         ca.out = b->d_q[cur ^ 1];
-        ca.out_count = b->d_counts + 2 + r;
-        uint32_t cg = (uint32_t)std::min(1024.0, std::ceil(alive / 1024.0) + 1);
+        ca.out_counts = b->d_counts + (size_t)(r + 1) * ne;
+        uint32_t cg = (uint32_t)std::min(1024.0, std::ceil(alive / 1024.0) + ne);
         HIPCHK(launch_round_compact(ca, cg, s));
         cur ^= 1;
-        cslot = 2 + (uint32_t)r;
-        b->round_slots.push_back(cslot);
+        cstage = r + 1;
+        b->round_slots.push_back(cstage);
         for (size_t k = 0; k < b->engines.size(); k++) {
             const fk_engine* e = b->engines[k];
-            const double lo = std::max<double>(rounds[r].first, e->q_med);
-            const double hi = std::min<double>(rounds[r].second, e->q2);
-            if (hi > lo) e_alive[k] *= round_survival(e, lo, hi);
+            const DevEngine& h = e->host;
+            if (r >= h.nround) continue;
+            const double lo = ctx->primes[h.rb[r]];
+            const double hi = h.rb[r + 1] < ctx->primes.size()
+                                  ? (double)ctx->primes[h.rb[r + 1]] : (double)e->q2;
+            e_alive[k] *= round_survival(e, lo, hi);
         }
     }
     alive = 0.0;
     for (double x : e_alive) alive += x;
+    // ---- pack the segments for the classification and the copy back
+    {
+        GatherArgs ga{};
+        ga.nengines = (uint32_t)ne;
+        ga.segs = b->d_segs;
+        ga.in = b->d_q[cur];
+        ga.counts = b->d_counts + cstage * ne;
+        ga.out = b->d_q[cur ^ 1];
+        ga.total = b->d_counts + nstage * ne;
+        uint32_t gg = (uint32_t)std::min(1024.0, std::ceil(alive / 256.0) + ne);
+        HIPCHK(launch_gather(ga, gg, s));
+        cur ^= 1;
+    }
+    const uint32_t total_slot = (uint32_t)(nstage * ne);
     HIPCHK(hipEventRecord(b->ev[3], s));
 
     // ---- classification
@@ -781,7 +876,7 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
         ca.engines = b->d_engines;
         ca.wins = b->d_wins;
         ca.recs = b->d_q[cur];
-        ca.count = b->d_counts + cslot;
+        ca.count = b->d_counts + total_slot;
         ca.cap = queue_cap;
         ca.cap_max = (uint32_t)cap_max;
         ca.fail2 = b->d_fail2;
@@ -811,7 +906,7 @@ static int batch_enqueue(fk_batch* b, uint32_t queue_cap)
     HIPCHK(hipMemcpyAsync(b->h_recs, b->d_q[cur], (size_t)copy * sizeof(DevRec),
                           hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(b->ev[5], s));
-    b->final_slot = cslot;
+    b->final_slot = total_slot;
     b->final_q = cur;
     b->copied = copy;
     b->pending = true;
@@ -828,22 +923,34 @@ static int batch_collect(fk_batch* b)
     HIPCHK(hipSetDevice(b->device));
     HIPCHK(hipEventSynchronize(b->ev[5]));
     b->pending = false;
-    const uint32_t tile_count = b->h_counts[0];
-    if (tile_count > b->queue_cap) {
-        // the LDS stage found more survivors than the queue holds: size the
-        // queue from the true count and run the whole batch again
-        uint32_t cap = (uint32_t)std::min<double>(1.25 * tile_count + 65536.0,
-                                                  4.0e9);
-        if (tile_count >= 0xf0000000u)
-            return fail(FK_E_OVERFLOW, "survivor queue overflow: %u",
-                        tile_count);
-        int rc = batch_enqueue(b, cap);
+    const size_t ne = b->seg_caps.size();
+    bool over = false;
+    std::vector<uint32_t> caps = b->seg_caps;
+    for (size_t k = 0; k < ne; k++) {
+        const uint32_t c = b->h_counts[k];          // stage 0: the sieve
+        if (c > caps[k]) {
+            // the LDS stage found more survivors than the engine's segment
+            // holds: size it from the true count and run the batch again
+            if (c >= 0xf0000000u)
+                return fail(FK_E_OVERFLOW, "survivor queue overflow: %u", c);
+            caps[k] = (uint32_t)std::min<double>(1.25 * c + 65536.0, 4.0e9);
+            over = true;
+        }
+    }
+    if (over) {
+        int rc = batch_enqueue(b, &caps);
         if (rc) return rc;
         return batch_collect(b);
     }
     const uint32_t n = b->h_counts[b->final_slot];
-    b->stage_counts.assign(1, tile_count);
-    for (uint32_t sl : b->round_slots) b->stage_counts.push_back(b->h_counts[sl]);
+    auto stage_total = [&](uint32_t st) {
+        uint64_t t = 0;
+        for (size_t k = 0; k < ne; k++)
+            t += std::min(b->h_counts[st * ne + k], b->seg_caps[k]);
+        return (uint32_t)t;
+    };
+    b->stage_counts.assign(1, stage_total(0));
+    for (uint32_t st : b->round_slots) b->stage_counts.push_back(stage_total(st));
     HIPCHK(hipEventElapsedTime(&b->ms[0], b->ev[0], b->ev[1]));
     HIPCHK(hipEventElapsedTime(&b->ms[1], b->ev[1], b->ev[2]));
     HIPCHK(hipEventElapsedTime(&b->ms[2], b->ev[2], b->ev[3]));
@@ -925,7 +1032,7 @@ static int batch_submit(fk_batch* b, fk_engine* const* engines,
                         c.cap);
         b->cls[w] = c;
     }
-    return batch_enqueue(b, 0);
+    return batch_enqueue(b, nullptr);
 }
 
 // ------------------------------------------------------------------ ABI

@@ -118,7 +118,8 @@ __device__ __forceinline__ void emit_record(const SieveArgs& a,
 {
     unsigned __int128 x = B + off + (unsigned __int128)E.D * v;
     if (E.family == kDickson) x *= E.W;
-    if (slot < a.out_cap) {
+    const uint2 seg = a.segs[P.engine];
+    if (slot < seg.y) {
         DevRec r;
         r.t[0] = (uint32_t)x;
         r.t[1] = (uint32_t)(x >> 32);
@@ -126,7 +127,7 @@ __device__ __forceinline__ void emit_record(const SieveArgs& a,
         r.t[3] = (uint32_t)(x >> 96);
         r.window = P.window;
         r.run = -1;
-        a.out[slot] = r;
+        a.out[seg.x + slot] = r;
     }
 }
 
@@ -152,7 +153,14 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
     const uint32_t lane = __lane_id();
     uint32_t cur_engine = 0xffffffffu;
 
-    for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    // a contiguous run of tiles per workgroup: consecutive tiles are
+    // consecutive lines of one window, so the engine tables loaded into LDS
+    // change only at window boundaries
+    const uint64_t tiles_per =
+        ((uint64_t)(a.ntiles - a.tile_begin) + gridDim.x - 1) / gridDim.x;
+    const uint64_t tile_lo = a.tile_begin + (uint64_t)blockIdx.x * tiles_per;
+    const uint64_t tile_hi = min((uint64_t)a.ntiles, tile_lo + tiles_per);
+    for (uint32_t tile = (uint32_t)tile_lo; tile < tile_hi; tile++) {
         const uint32_t pi = find_piece(a.pieces, a.npieces, tile);
         const DevPiece& P = a.pieces[pi];
         const DevEngine& E = a.engines[P.engine];
@@ -196,7 +204,10 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
 #pragma unroll
         for (int k = 0; k < WPT; k++)
             acc[k] = edge_mask(v0 + 64ull * (tid + (uint32_t)k * T), vlo, vend);
-        for (uint32_t p = 0; p < E.npat; p++) {
+        // a thread whose first word is past the line's end has only dead
+        // words (edge_mask); short lines leave whole waves idle here
+        const uint32_t npat_here = (v0 + 64ull * tid < vend) ? E.npat : 0u;
+        for (uint32_t p = 0; p < npat_here; p++) {
             const uint4 pq = pinfo[p];
             const uint32_t q = pq.x, m32 = pq.y;
             const uint32_t step = red32(64u * T, q, m32);
@@ -355,7 +366,7 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
             ((unsigned __int128)(((uint64_t)P.base[3] << 32) | P.base[2]) << 64)
             | (((uint64_t)P.base[1] << 32) | P.base[0]);
         if (listed) {
-            if (tid == 0 && n) sh_base = atomicAdd(a.out_count, n);
+            if (tid == 0 && n) sh_base = atomicAdd(a.counts + P.engine, n);
             __syncthreads();
             for (uint32_t i = tid; i < n; i += T)
                 emit_record(a, P, E, B, off, v0 + cur[i], sh_base + i);
@@ -363,7 +374,8 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
             uint32_t mine = 0;
             if (cnt) mine = atomicAdd(&sh_count, cnt);
             __syncthreads();
-            if (tid == 0 && sh_count) sh_base = atomicAdd(a.out_count, sh_count);
+            if (tid == 0 && sh_count)
+                sh_base = atomicAdd(a.counts + P.engine, sh_count);
             __syncthreads();
             if (cnt) {
                 uint32_t slot = sh_base + mine;
@@ -407,6 +419,37 @@ template <class T>
 __device__ __forceinline__ const FK_CONST T* uniform_view(const T* p)
 {
     return (const FK_CONST T*)p;
+}
+
+// Global block b of a stage, in blocks of `per` entries of each engine's
+// segment, engine by engine: the engine, the block within it, the engine's
+// entry count and the entries of the engines before it.  False past the
+// last block.  The loop is over the batch's engines (a few dozen), with
+// wave-uniform scalar loads.
+struct SegBlock {
+    uint32_t e, lb, n, before;
+};
+
+__device__ __forceinline__ bool seg_block(const FK_CONST uint2* segs,
+                                          const FK_CONST uint32_t* counts,
+                                          uint32_t ne, uint32_t per,
+                                          uint32_t b, SegBlock& out)
+{
+    uint32_t acc = 0, before = 0;
+    for (uint32_t e = 0; e < ne; e++) {
+        const uint32_t n = min(counts[e], segs[e].y);
+        const uint32_t nb = (n + per - 1) / per;
+        if (b < acc + nb) {
+            out.e = e;
+            out.lb = b - acc;
+            out.n = n;
+            out.before = before;
+            return true;
+        }
+        acc += nb;
+        before += n;
+    }
+    return false;
 }
 
 __device__ __forceinline__ uint4 load4(const FK_CONST uint32_t* p)
@@ -508,54 +551,46 @@ __device__ __forceinline__ uint64_t round_scan(const double (&a)[8],
     return hit;
 }
 
-// One round, lane = candidate: blockIdx.x walks the queue in blocks of 256
-// (grid-stride), blockIdx.y takes a block of pblock primes.  A wave whose
-// lanes come from several engines takes them one engine at a time (the
-// engine index made uniform with readlane), so the prime-side data stays
-// uniform.  A killed candidate's flag is set.
+// One round, lane = candidate: blockIdx.x walks the engines' segments in
+// blocks of 256 candidates (grid-stride), blockIdx.y takes a block of
+// pblock primes.  The engine is uniform per block, so the prime-side data
+// is too.  A killed candidate's flag is set.
 __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
 {
-    const uint32_t count = min(*a.count, a.cap);
     const uint32_t pb0 = a.p_begin + blockIdx.y * a.pblock;
     const uint32_t pb1 = min(pb0 + a.pblock, a.p_end);
     if (pb0 >= pb1) return;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < count;
-         base += gridDim.x * blockDim.x) {
-        const uint32_t i = base + threadIdx.x;
-        const bool live = i < count && a.flags[i] == 0;
+    const FK_CONST uint2* segs = uniform_view(a.segs);
+    const FK_CONST uint32_t* counts = uniform_view(a.counts);
+    SegBlock sb;
+    for (uint32_t blk = blockIdx.x;
+         seg_block(segs, counts, a.nengines, 256, blk, sb); blk += gridDim.x) {
+        const FK_CONST DevEngine* E = uniform_view(a.engines) + sb.e;
+        if (a.round >= E->nround) continue;
+        const uint32_t lo = max(pb0, E->rb[a.round]);
+        const uint32_t hi = min(pb1, E->rb[a.round + 1]);
+        if (lo >= hi) continue;
+        const uint32_t li = sb.lb * 256 + threadIdx.x;
+        const uint32_t i = segs[sb.e].x + li;
+        const bool live = li < sb.n && a.flags[i] == 0;
         uint32_t t[4] = {0u, 0u, 0u, 0u};
-        uint32_t eidx = 0;
         if (live) {
             const DevRec rec = a.recs[i];
             for (int k = 0; k < 4; k++) t[k] = rec.t[k];
-            eidx = a.win_engine[rec.window];
         }
+        if (!__ballot(live)) continue;
         double l[8];
 #pragma unroll
         for (int k = 0; k < 8; k++)
             l[k] = (double)((t[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
         const bool wide = __ballot(live && (t[2] | t[3]) != 0) != 0;
-        bool killed = false;
-        uint64_t todo = __ballot(live);
-        while (todo) {
-            const int lead = __ffsll((unsigned long long)todo) - 1;
-            const uint32_t e = __builtin_amdgcn_readlane(eidx, lead);
-            const bool mine = live && eidx == e;
-            todo &= ~(uint64_t)__ballot(mine);
-            const FK_CONST DevEngine* E = uniform_view(a.engines) + e;
-            const uint32_t rf = E->round_first;
-            const uint32_t lo = max(pb0, rf);
-            const uint32_t hi = min(pb1, E->round_end);
-            const FK_CONST PrimeD* pd = uniform_view(a.primed);
-            const FK_CONST uint32_t* kt = uniform_view(E->kt);
-            if (mine && lo < hi) {
-                const uint64_t m =
-                    wide ? round_scan<8>(l, pd, kt, E->kstride, rf, lo, hi)
-                         : round_scan<4>(l, pd, kt, E->kstride, rf, lo, hi);
-                killed = killed || ((m >> __lane_id()) & 1);
-            }
-        }
-        if (killed) a.flags[i] = 1;
+        const FK_CONST PrimeD* pd = uniform_view(a.primed);
+        const FK_CONST uint32_t* kt = uniform_view(E->kt);
+        const uint32_t rf = E->round_first;
+        const uint64_t m =
+            wide ? round_scan<8>(l, pd, kt, E->kstride, rf, lo, hi)
+                 : round_scan<4>(l, pd, kt, E->kstride, rf, lo, hi);
+        if (live && ((m >> __lane_id()) & 1)) a.flags[i] = 1;
     }
 }
 
@@ -640,105 +675,104 @@ __device__ __forceinline__ bool lane_kills(const LaneRow& L, const double (&a)[8
     return hit;
 }
 
-// One round: the queued candidates against the primes [p_begin, p_end).
-// Lane = prime: blockIdx.x picks 256 primes (one per lane), blockIdx.y a
-// slice of the queue.  Each lane holds its prime's row (residue constants
-// and kill set) in registers, loaded once per engine; the wave walks the
-// slice, a candidate at a time, wave-uniform (scalar loads of the record,
-// four candidates per load batch), and a candidate that some lane's prime
-// kills is flagged by the lowest such lane.  The arithmetic per test is
-// the exact double-precision residue (resid_d) and one compare per
-// killing residue; there is no memory access per (candidate, prime).
+// One round, lane = prime: blockIdx.x picks 256 primes (one per lane),
+// blockIdx.y walks slices of kRoundSlice candidates of the engines'
+// segments (grid-stride).  Each lane holds its prime's row (residue
+// constants and kill set) for the slice's engine in registers; the wave
+// walks the slice a candidate at a time, wave-uniform (scalar loads of the
+// records, four per load batch), and a candidate that some lane's prime
+// kills is flagged by the lowest such lane.  The arithmetic per test is the
+// exact double-precision residue (resid_d) and one compare per killing
+// residue; there is no memory access per (candidate, prime).
+constexpr uint32_t kRoundSlice = 256;
+
 __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
 {
-    const uint32_t count = min(*a.count, a.cap);
-    const uint32_t per = (count + gridDim.y - 1) / gridDim.y;
-    const uint32_t c0 = blockIdx.y * per;
-    const uint32_t c1 = min(count, c0 + per);
-    if (c0 >= c1) return;
     const uint32_t p = a.p_begin + blockIdx.x * 256 + threadIdx.x;
     const uint32_t wave_p0 = a.p_begin + blockIdx.x * 256 + (threadIdx.x & ~63u);
     if (wave_p0 >= a.p_end) return;              // whole wave past the round
+    const FK_CONST uint2* segs = uniform_view(a.segs);
+    const FK_CONST uint32_t* counts = uniform_view(a.counts);
     const FK_CONST DevRec* recs = uniform_view(a.recs);
-    const FK_CONST uint32_t* wine = uniform_view(a.win_engine);
     const FK_CONST DevEngine* engs = uniform_view(a.engines);
     const uint32_t lane = __lane_id();
-
     LaneRow L;
-    uint32_t cur_e = 0xffffffffu;
-    bool mine = false;               // this lane's prime is in the engine's rounds
-    uint32_t ncmp = 0;
-    // four candidates per load batch: their records (96 contiguous bytes)
-    // and then their engines are fetched together, one latency each
-    for (uint32_t i0 = c0; i0 < c1; i0 += 4) {
-        DevRec r[4];
-        uint32_t e[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const FK_CONST DevRec* src = recs + min(i0 + j, c1 - 1);
-            for (int k = 0; k < 4; k++) r[j].t[k] = src->t[k];
-            r[j].window = src->window;
-            r[j].run = src->run;
+    SegBlock sb;
+    for (uint32_t blk = blockIdx.y;
+         seg_block(segs, counts, a.nengines, kRoundSlice, blk, sb);
+         blk += gridDim.y) {
+        const uint32_t e = sb.e;
+        if (a.round >= engs[e].nround) continue;
+        const uint32_t rlo = engs[e].rb[a.round], rhi = engs[e].rb[a.round + 1];
+        const bool mine = p >= rlo && p < rhi && p < a.p_end;
+        if (!__ballot(mine)) continue;
+        // compares per test: the family's largest kill set, in fours
+        const uint32_t ncmp = (kill_max(engs[e].family, engs[e].n) + 3u) & ~3u;
+        const uint32_t stride = engs[e].kstride;
+        if (mine) {
+            load_lane_row(L, engs[e].kt + (size_t)(p - engs[e].round_first) * stride,
+                          a.primed + p);
+            if (ncmp > 20) load_lane_row_wide(L, stride - kRowHead);
         }
+        const uint32_t base = segs[e].x;
+        const uint32_t c0 = sb.lb * kRoundSlice;
+        const uint32_t c1 = min(sb.n, c0 + kRoundSlice);
+        for (uint32_t i0 = c0; i0 < c1; i0 += 4) {
+            DevRec r[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) e[j] = wine[r[j].window];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if (i0 + j >= c1) break;
-            if (e[j] != cur_e) {
-                cur_e = e[j];
-                const uint32_t rf = engs[cur_e].round_first;
-                const uint32_t re = engs[cur_e].round_end;
-                const uint32_t stride = engs[cur_e].kstride;
-                // compares per test: the family's largest kill set, in fours
-                ncmp = (kill_max(engs[cur_e].family, engs[cur_e].n) + 3u) & ~3u;
-                mine = p >= rf && p < re && p < a.p_end;
-                if (mine) {
-                    load_lane_row(L, engs[cur_e].kt + (size_t)(p - rf) * stride,
-                                  a.primed + p);
-                    if (ncmp > 20) load_lane_row_wide(L, stride - kRowHead);
-                }
+            for (int j = 0; j < 4; j++) {
+                const FK_CONST DevRec* src = recs + base + min(i0 + j, c1 - 1);
+                for (int k = 0; k < 4; k++) r[j].t[k] = src->t[k];
             }
-            if (!__ballot(mine)) continue;
-            double l[8];
 #pragma unroll
-            for (int k = 0; k < 8; k++)
-                l[k] = (double)((r[j].t[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
-            const bool wide = (r[j].t[2] | r[j].t[3]) != 0;
-            bool kill = false;
-            if (mine)
-                kill = wide ? lane_kills<8>(L, l, ncmp) : lane_kills<4>(L, l, ncmp);
-            const uint64_t m = __ballot(kill);
-            if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1))
-                a.flags[i0 + j] = 1;
+            for (int j = 0; j < 4; j++) {
+                if (i0 + j >= c1) break;
+                double l[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    l[k] = (double)((r[j].t[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
+                const bool wide = (r[j].t[2] | r[j].t[3]) != 0;
+                bool kill = false;
+                if (mine)
+                    kill = wide ? lane_kills<8>(L, l, ncmp) : lane_kills<4>(L, l, ncmp);
+                const uint64_t m = __ballot(kill);
+                if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1))
+                    a.flags[base + i0 + j] = 1;
+            }
         }
     }
 }
 
-// Compaction of the unflagged candidates: each workgroup takes 1024 queue
-// entries (four per thread, coalesced), counts its survivors with ballots,
-// and reserves room for all of them with one atomic.  One atomic per wave
-// serialises on the counter when the queue holds many records.
+// Compaction of the unflagged candidates, within each engine's segment:
+// each workgroup takes 1024 entries of one segment (four per thread,
+// coalesced), counts its survivors with ballots, and reserves room for all
+// of them in the segment with one atomic.  One atomic per wave serialises
+// on the counter when the queue holds many records.
 constexpr uint32_t CP_PER = 4;
 
 __global__ __launch_bounds__(256) void k_round_compact(CompactArgs a)
 {
     __shared__ uint32_t wsum[4];
     __shared__ uint32_t wbase;
-    const uint32_t count = min(*a.in_count, a.cap);
+    const FK_CONST uint2* segs = uniform_view(a.segs);
+    const FK_CONST uint32_t* counts = uniform_view(a.in_counts);
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
-    for (uint32_t base = blockIdx.x * (256 * CP_PER); base < count;
-         base += gridDim.x * (256 * CP_PER)) {
+    SegBlock sb;
+    for (uint32_t blk = blockIdx.x;
+         seg_block(segs, counts, a.nengines, 256 * CP_PER, blk, sb);
+         blk += gridDim.x) {
+        const uint32_t base = segs[sb.e].x;
+        const uint32_t l0 = sb.lb * (256 * CP_PER);
         uint64_t m[CP_PER];
         uint32_t tot = 0;
 #pragma unroll
         for (uint32_t k = 0; k < CP_PER; k++) {
-            const uint32_t i = base + k * 256 + threadIdx.x;
+            const uint32_t li = l0 + k * 256 + threadIdx.x;
             bool alive = false;
-            if (i < count) {
-                alive = a.flags[i] == 0;
-                a.flags[i] = 0;
+            if (li < sb.n) {
+                alive = a.flags[base + li] == 0;
+                a.flags[base + li] = 0;
             }
             m[k] = __ballot(alive);
             tot += (uint32_t)__popcll((unsigned long long)m[k]);
@@ -752,21 +786,40 @@ __global__ __launch_bounds__(256) void k_round_compact(CompactArgs a)
                 wsum[w] = run;
                 run += t;
             }
-            wbase = run ? atomicAdd(a.out_count, run) : 0u;
+            wbase = run ? atomicAdd(a.out_counts + sb.e, run) : 0u;
         }
         __syncthreads();
-        uint32_t off = wbase + wsum[wave];
+        uint32_t off = base + wbase + wsum[wave];
 #pragma unroll
         for (uint32_t k = 0; k < CP_PER; k++) {
             if ((m[k] >> lane) & 1) {
                 const uint32_t below = __builtin_amdgcn_mbcnt_hi(
                     (uint32_t)(m[k] >> 32),
                     __builtin_amdgcn_mbcnt_lo((uint32_t)m[k], 0u));
-                a.out[off + below] = a.in[base + k * 256 + threadIdx.x];
+                a.out[off + below] = a.in[base + l0 + k * 256 + threadIdx.x];
             }
             off += (uint32_t)__popcll((unsigned long long)m[k]);
         }
         __syncthreads();           // wsum and wbase are reused
+    }
+}
+
+// Pack the engines' segments into one array for the classification and
+// the copy back.
+__global__ __launch_bounds__(256) void k_gather(GatherArgs a)
+{
+    const FK_CONST uint2* segs = uniform_view(a.segs);
+    const FK_CONST uint32_t* counts = uniform_view(a.counts);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t e = 0; e < a.nengines; e++) tot += min(counts[e], segs[e].y);
+        *a.total = tot;
+    }
+    SegBlock sb;
+    for (uint32_t blk = blockIdx.x;
+         seg_block(segs, counts, a.nengines, 256, blk, sb); blk += gridDim.x) {
+        const uint32_t li = sb.lb * 256 + threadIdx.x;
+        if (li < sb.n) a.out[sb.before + li] = a.in[segs[sb.e].x + li];
     }
 }
 
@@ -1032,6 +1085,13 @@ hipError_t launch_classify(const ClassifyArgs& a, uint32_t grid,
     if (e != hipSuccess) return e;
     uint32_t g2 = grid / 4 + 1;
     hipLaunchKernelGGL(k_cls_finish, dim3(g2), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const GatherArgs& a, uint32_t grid,
+                         hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_gather, dim3(grid ? grid : 1), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 

@@ -16,40 +16,58 @@ struct DevWinInfo {
     int32_t exact_from;
 };
 
+// Queue segments: engine e owns entries [segs[e].x, segs[e].x + segs[e].y)
+// of both queues (and of the flag array) for a whole batch, so that every
+// wave of the round kernels sees one engine.  A stage's counters hold one
+// count per engine (capped at the segment size where read).
 struct SieveArgs {
     const DevEngine* engines;
     const DevPiece* pieces;
     uint32_t npieces;
+    uint32_t tile_begin;      // this launch's tiles: [tile_begin, ntiles)
     uint32_t ntiles;
     uint32_t nlds_max;        // LDS room for per-prime arrays (max over engines)
     uint32_t pat_max;         // LDS room for pattern words (max over engines)
     uint32_t bm_max;          // LDS room for kill-bitmap words (max over engines)
     const uint32_t* bq;       // per piece, per LDS prime: base mod q
+    const uint2* segs;        // per engine: queue segment (base, size)
     DevRec* out;
-    uint32_t* out_count;
-    uint32_t out_cap;
+    uint32_t* counts;         // per engine: records emitted
 };
 
 struct RoundArgs {
     const DevEngine* engines;
     uint32_t nengines;
-    const uint32_t* win_engine;
+    const uint2* segs;
     const PrimeD* primed;     // the device-wide prime table (residue form)
     const DevRec* recs;
-    const uint32_t* count;
-    uint32_t cap;
+    const uint32_t* counts;   // per engine: candidates in the queue
     uint32_t* flags;
-    uint32_t p_begin, p_end;  // global prime indices of this round
+    uint32_t round;           // round index (each engine's rb[round..])
+    uint32_t p_begin, p_end;  // global prime indices: the union of the
+                              // engines' round-`round` primes
     uint32_t pblock;          // lane = candidate: primes per blockIdx.y
 };
 
 struct CompactArgs {
+    uint32_t nengines;
+    const uint2* segs;
     const DevRec* in;
-    const uint32_t* in_count;
-    uint32_t cap;
+    const uint32_t* in_counts;
     uint32_t* flags;
     DevRec* out;
-    uint32_t* out_count;
+    uint32_t* out_counts;
+};
+
+// The final queue packed for classification and the copy back: every
+// engine's segment, in engine order, from out[0]; *total = the record count.
+struct GatherArgs {
+    uint32_t nengines;
+    const uint2* segs;
+    const DevRec* in;
+    const uint32_t* counts;
+    DevRec* out;
+    uint32_t* total;
 };
 
 struct ClassifyArgs {
@@ -79,6 +97,8 @@ hipError_t launch_kill_table(const PrimeK* primes, uint32_t p0, uint32_t np,
                              uint32_t stride, uint32_t* kt, hipStream_t stream);
 hipError_t launch_round_compact(const CompactArgs& a, uint32_t grid,
                                 hipStream_t stream);
+hipError_t launch_gather(const GatherArgs& a, uint32_t grid,
+                         hipStream_t stream);
 hipError_t launch_classify(const ClassifyArgs& a, uint32_t grid,
                            hipStream_t stream);
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t stream);

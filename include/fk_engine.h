@@ -183,8 +183,11 @@ int fk_batch_destroy(fk_batch* b);
 
 /*
  * Enqueue windows (window i runs on engines[engine_of[i]] with cls[i]) and
- * return without waiting.  Tuning (threads, words_per_thread) comes from
- * engines[0]; FK_E_BUSY if the batch has unfinished work.
+ * return without waiting.  The sieve workgroup size, the longest tile and
+ * the round-kernel crossover come from the engine with the most candidates
+ * in the submission; each window's tile is then as short as its lines
+ * allow, and each engine keeps its own cutoffs and round schedule.
+ * FK_E_BUSY if the batch has unfinished work.
  */
 int fk_batch_submit(fk_batch* b, fk_engine* const* engines, size_t nengines,
                     const uint32_t* engine_of, const fk_window* windows,
