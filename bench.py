@@ -19,6 +19,9 @@ Beside it, in the same JSON line:
   batched       256 mixed windows of the three families (lengths 2^20..2^30,
                 n 8..14), one persistent batched launch vs the plain engine
                 window by window, with the two streams checked equal
+  serving       Poisson arrivals of requests of the three families (spans
+                2^18..2^32, n 8..14, Dickson k to 1e22), continuous batching
+                over 4 slots: p50/p99 completion latency and candidates/s
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -198,6 +201,27 @@ def run_batched():
             "speedup": t_plain / t_batch, "streams_equal": same}
 
 
+# ------------------------------------------------------------- config 4
+
+def run_serving(count, rate, seed=4):
+    """Poisson arrivals of range-sweep requests of all three families,
+    spans log-uniform in [2^18, 2^32] line values, depths 8..14, Dickson k
+    up to about 1e22 (192-bit values); continuous batching over 4 slots."""
+    import fk_serve as S
+    reqs = S.poisson_requests(seed, count, rate)
+    srv = S.Server(slots=4, budget=64 << 30)
+    # engines are built (resident tables) before the clock starts
+    for r in reqs:
+        srv.engine(r.family, r.n, r.q2)
+    st = srv.run(reqs)
+    st["rate_per_s"] = rate
+    st["slots"] = 4
+    st["submissions"] = len(srv.submissions)
+    st["mean_requests_per_submission"] = (
+        sum(srv.submissions) / len(srv.submissions) if srv.submissions else 0)
+    return st
+
+
 def rocprof_traffic():
     """HBM bytes per k_sieve dispatch from the committed PMC summary."""
     p = ROOT / "profiles" / "k_sieve_dickson_c1_traffic.json"
@@ -231,6 +255,9 @@ def main():
     ap.add_argument("--cpu-sample-log2", type=int, default=31)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device copy of MIB MiB (counter "
                          "calibration) and exit")
@@ -288,6 +315,8 @@ def main():
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     if not args.no_batched:
         out["batched"] = run_batched()
+    if not args.no_serving:
+        out["serving"] = run_serving(args.serve_requests, args.serve_rate)
     print(json.dumps(out), flush=True)
     return 0
 
