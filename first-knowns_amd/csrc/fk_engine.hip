@@ -419,6 +419,91 @@ static int build_engine(fk_engine* e)
 
 // ---------------------------------------------------------------- batch
 
+// One step of a batch's launch sequence.  batch_enqueue records the whole
+// sequence first (every allocation and occupancy query done), then either
+// issues it on the batch's stream, with stage events, or captures it once
+// into a HIP graph and replays the graph while the recorded sequence stays
+// byte-identical (same grids, same argument blocks, same buffers; only the
+// device-side inputs uploaded before it differ).  Steps are zero-filled
+// before they are set, so that byte comparison is exact.
+struct Step {
+    enum Kind : int32_t { kMark, kZero, kPrep, kSieve, kRound, kCompact,
+                          kGather, kClassify, kD2H };
+    int32_t kind;
+    int32_t ev;                      // kMark: event index
+    uint32_t grid, gy;
+    int32_t threads, wpt;
+    size_t lds;
+    int32_t lane_prime;
+    void* dst;                       // kZero / kD2H
+    const void* src;                 // kD2H
+    size_t n;                        // kZero: words; kD2H: bytes
+    const DevEngine* p_engines;      // kPrep
+    const DevPiece* p_pieces;
+    uint32_t p_npieces, p_nlds;
+    uint32_t* p_bq;
+    SieveArgs sa;
+    RoundArgs ra;
+    CompactArgs ca;
+    GatherArgs ga;
+    ClassifyArgs cla;
+};
+
+static Step new_step(int32_t kind)
+{
+    Step st;
+    std::memset(&st, 0, sizeof st);
+    st.kind = kind;
+    return st;
+}
+
+static bool same_plan(const std::vector<Step>& a, const std::vector<Step>& b)
+{
+    return a.size() == b.size() &&
+           (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(Step)) == 0);
+}
+
+// Issue the recorded steps on `s`; stage events only when `events`.
+static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
+                             bool events, hipStream_t s)
+{
+    hipError_t e = hipSuccess;
+    for (const Step& st : plan) {
+        switch (st.kind) {
+        case Step::kMark:
+            if (events) e = hipEventRecord(ev[st.ev], s);
+            break;
+        case Step::kZero:
+            e = launch_zero((uint32_t*)st.dst, (uint32_t)st.n, s);
+            break;
+        case Step::kPrep:
+            e = launch_prep(st.p_engines, st.p_pieces, st.p_npieces, st.p_nlds,
+                            st.p_bq, s);
+            break;
+        case Step::kSieve:
+            e = launch_sieve(st.sa, st.threads, st.wpt, st.grid, st.lds, s);
+            break;
+        case Step::kRound:
+            e = launch_round_test(st.ra, st.lane_prime != 0, st.grid, st.gy, s);
+            break;
+        case Step::kCompact:
+            e = launch_round_compact(st.ca, st.grid, s);
+            break;
+        case Step::kGather:
+            e = launch_gather(st.ga, st.grid, s);
+            break;
+        case Step::kClassify:
+            e = launch_classify(st.cla, st.grid, s);
+            break;
+        case Step::kD2H:
+            e = hipMemcpyAsync(st.dst, st.src, st.n, hipMemcpyDeviceToHost, s);
+            break;
+        }
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 struct fk_batch {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -451,6 +536,13 @@ struct fk_batch {
     uint32_t queue_cap = 0;           // entries per queue (all segments)
     std::vector<uint32_t> seg_caps;   // per engine: its queue segment
     uint2* d_segs = nullptr;          size_t cap_segs = 0;
+    // graph mode: the captured launch sequence and the plan it was made from
+    bool use_graph = false;
+    bool graph_ran = false;           // the last submission ran as a graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<Step> graph_plan;
+    uint64_t graph_captures = 0, graph_launches = 0;
     uint32_t copied = 0;
     uint64_t tiles = 0, candidates = 0;
     std::vector<fk_record> out;       // sorted results
@@ -460,6 +552,10 @@ struct fk_batch {
 
 static int batch_free(fk_batch* b)
 {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
     (void)hipSetDevice(b->device);
     void* ds[] = {b->d_engines, b->d_pieces, b->d_win_engine, b->d_wins, b->d_segs,
                   b->d_bq, b->d_q[0], b->d_q[1], b->d_flags, b->d_fail2,
@@ -730,18 +826,37 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     // (they are small; the device work below is what stays asynchronous)
     HIPCHK(hipStreamSynchronize(s));
 
-    HIPCHK(hipEventRecord(b->ev[0], s));
-    HIPCHK(launch_zero(b->d_counts, (uint32_t)ncounts, s));
-    HIPCHK(launch_zero(b->d_flags, queue_cap, s));
-    HIPCHK(launch_prep(b->d_engines, b->d_pieces, (uint32_t)pieces.size(),
-                       nlds_max, b->d_bq, s));
-    HIPCHK(hipEventRecord(b->ev[1], s));
+    std::vector<Step> plan;
+    auto mark = [&](int i) {
+        Step st = new_step(Step::kMark);
+        st.ev = i;
+        plan.push_back(st);
+    };
+    mark(0);
+    {
+        Step st = new_step(Step::kZero);
+        st.dst = b->d_counts;
+        st.n = ncounts;
+        plan.push_back(st);
+        st.dst = b->d_flags;
+        st.n = queue_cap;
+        plan.push_back(st);
+        Step pr = new_step(Step::kPrep);
+        pr.p_engines = b->d_engines;
+        pr.p_pieces = b->d_pieces;
+        pr.p_npieces = (uint32_t)pieces.size();
+        pr.p_nlds = nlds_max;
+        pr.p_bq = b->d_bq;
+        plan.push_back(pr);
+    }
+    mark(1);
 
     // ---- sieve: one launch per tile shape
     for (const Group& g : groups) {
         if (g.tile_end == g.tile_begin) continue;
         const uint64_t S = (uint64_t)64 * T * g.wpt;
-        SieveArgs a{};
+        Step st = new_step(Step::kSieve);
+        SieveArgs& a = st.sa;
         a.engines = b->d_engines;
         a.pieces = b->d_pieces;
         a.npieces = (uint32_t)pieces.size();
@@ -768,9 +883,13 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         if (occ < 1) occ = 1;
         uint64_t grid = (uint64_t)occ * ctx->cus;
         grid = std::min<uint64_t>(grid, g.tile_end - g.tile_begin);
-        HIPCHK(launch_sieve(a, (int)T, (int)g.wpt, (uint32_t)grid, lds, s));
+        st.threads = (int)T;
+        st.wpt = (int)g.wpt;
+        st.grid = (uint32_t)grid;
+        st.lds = lds;
+        plan.push_back(st);
     }
-    HIPCHK(hipEventRecord(b->ev[2], s));
+    mark(2);
 
     // ---- rounds
     uint32_t cur = 0, cstage = 0;
@@ -788,7 +907,8 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             alive += e_alive[k];
         }
         if (P0 >= P1) continue;
-        RoundArgs ra{};
+        Step rst = new_step(Step::kRound);
+        RoundArgs& ra = rst.ra;
         ra.engines = b->d_engines;
         ra.nengines = (uint32_t)ne;
         ra.segs = b->d_segs;
@@ -822,8 +942,12 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             ra.pblock = (np + gy - 1) / gy;
             gy = (np + ra.pblock - 1) / ra.pblock;
         }
-        HIPCHK(launch_round_test(ra, lane_prime, gx, gy, s));
-        CompactArgs ca{};
+        rst.lane_prime = lane_prime ? 1 : 0;
+        rst.grid = gx;
+        rst.gy = gy;
+        plan.push_back(rst);
+        Step cst = new_step(Step::kCompact);
+        CompactArgs& ca = cst.ca;
         ca.nengines = (uint32_t)ne;
         ca.segs = b->d_segs;
         ca.in = b->d_q[cur];
@@ -832,7 +956,8 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         ca.out = b->d_q[cur ^ 1];
         ca.out_counts = b->d_counts + (size_t)(r + 1) * ne;
         uint32_t cg = (uint32_t)std::min(1024.0, std::ceil(alive / 1024.0) + ne);
-        HIPCHK(launch_round_compact(ca, cg, s));
+        cst.grid = cg;
+        plan.push_back(cst);
         cur ^= 1;
         cstage = r + 1;
         b->round_slots.push_back(cstage);
@@ -850,7 +975,8 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     for (double x : e_alive) alive += x;
     // ---- pack the segments for the classification and the copy back
     {
-        GatherArgs ga{};
+        Step gst = new_step(Step::kGather);
+        GatherArgs& ga = gst.ga;
         ga.nengines = (uint32_t)ne;
         ga.segs = b->d_segs;
         ga.in = b->d_q[cur];
@@ -858,11 +984,12 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         ga.out = b->d_q[cur ^ 1];
         ga.total = b->d_counts + nstage * ne;
         uint32_t gg = (uint32_t)std::min(1024.0, std::ceil(alive / 256.0) + ne);
-        HIPCHK(launch_gather(ga, gg, s));
+        gst.grid = gg;
+        plan.push_back(gst);
         cur ^= 1;
     }
     const uint32_t total_slot = (uint32_t)(nstage * ne);
-    HIPCHK(hipEventRecord(b->ev[3], s));
+    mark(3);
 
     // ---- classification
     bool any_cls = false;
@@ -872,7 +999,8 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         if (b->cls[w].mode) cap_max = std::max(cap_max, b->cls[w].cap);
     }
     if (any_cls) {
-        ClassifyArgs ca{};
+        Step kst = new_step(Step::kClassify);
+        ClassifyArgs& ca = kst.cla;
         ca.engines = b->d_engines;
         ca.wins = b->d_wins;
         ca.recs = b->d_q[cur];
@@ -887,13 +1015,12 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         const double work = alive * cap_max * 7.0;
         uint32_t g = (uint32_t)std::min(16384.0,
                                         std::max(1024.0, std::ceil(work / 128.0) + 4));
-        HIPCHK(launch_classify(ca, g, s));
+        kst.grid = g;
+        plan.push_back(kst);
     }
-    HIPCHK(hipEventRecord(b->ev[4], s));
+    mark(4);
 
     // ---- readback: counters and the first `copy` final records
-    HIPCHK(hipMemcpyAsync(b->h_counts, b->d_counts, ncounts * sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, s));
     uint32_t copy = (uint32_t)std::min<double>(queue_cap,
                                                4.0 * alive + 4096.0);
     if (copy > b->cap_hrecs || !b->h_recs) {
@@ -903,9 +1030,50 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         HIPCHK(hipHostMalloc(&b->h_recs, c * sizeof(DevRec)));
         b->cap_hrecs = c;
     }
-    HIPCHK(hipMemcpyAsync(b->h_recs, b->d_q[cur], (size_t)copy * sizeof(DevRec),
-                          hipMemcpyDeviceToHost, s));
-    HIPCHK(hipEventRecord(b->ev[5], s));
+    {
+        Step st = new_step(Step::kD2H);
+        st.dst = b->h_counts;
+        st.src = b->d_counts;
+        st.n = ncounts * sizeof(uint32_t);
+        plan.push_back(st);
+        st.dst = b->h_recs;
+        st.src = b->d_q[cur];
+        st.n = (size_t)copy * sizeof(DevRec);
+        plan.push_back(st);
+    }
+    mark(5);
+
+    // ---- run: eagerly with stage events, or as a (cached) graph
+    b->graph_ran = false;
+    if (b->use_graph) {
+        if (!b->graph_exec || !same_plan(plan, b->graph_plan)) {
+            if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            b->graph_exec = nullptr;
+            b->graph = nullptr;
+            b->graph_plan.clear();
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const hipError_t ce = emit_steps(plan, b->ev, false, s);
+            hipGraph_t g = nullptr;
+            const hipError_t ee = hipStreamEndCapture(s, &g);
+            if (ce != hipSuccess || ee != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(FK_E_DEVICE, "graph capture failed: %s",
+                            hipGetErrorString(ce != hipSuccess ? ce : ee));
+            }
+            b->graph = g;
+            HIPCHK(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+            b->graph_plan = plan;
+            b->graph_captures++;
+        }
+        HIPCHK(hipEventRecord(b->ev[0], s));
+        HIPCHK(hipGraphLaunch(b->graph_exec, s));
+        HIPCHK(hipEventRecord(b->ev[5], s));
+        b->graph_launches++;
+        b->graph_ran = true;
+    } else {
+        HIPCHK(emit_steps(plan, b->ev, true, s));
+    }
     b->final_slot = total_slot;
     b->final_q = cur;
     b->copied = copy;
@@ -951,10 +1119,15 @@ static int batch_collect(fk_batch* b)
     };
     b->stage_counts.assign(1, stage_total(0));
     for (uint32_t st : b->round_slots) b->stage_counts.push_back(stage_total(st));
-    HIPCHK(hipEventElapsedTime(&b->ms[0], b->ev[0], b->ev[1]));
-    HIPCHK(hipEventElapsedTime(&b->ms[1], b->ev[1], b->ev[2]));
-    HIPCHK(hipEventElapsedTime(&b->ms[2], b->ev[2], b->ev[3]));
-    HIPCHK(hipEventElapsedTime(&b->ms[3], b->ev[3], b->ev[4]));
+    if (b->graph_ran) {
+        // a graph has no stage events: only the whole is timed
+        for (int i = 0; i < 4; i++) b->ms[i] = 0.0f;
+    } else {
+        HIPCHK(hipEventElapsedTime(&b->ms[0], b->ev[0], b->ev[1]));
+        HIPCHK(hipEventElapsedTime(&b->ms[1], b->ev[1], b->ev[2]));
+        HIPCHK(hipEventElapsedTime(&b->ms[2], b->ev[2], b->ev[3]));
+        HIPCHK(hipEventElapsedTime(&b->ms[3], b->ev[3], b->ev[4]));
+    }
     HIPCHK(hipEventElapsedTime(&b->ms[4], b->ev[0], b->ev[5]));
     std::vector<DevRec> extra;
     const DevRec* recs = b->h_recs;
@@ -1279,6 +1452,22 @@ int fk_batch_timing(fk_batch* b, float* ms5, uint64_t* tiles,
     if (ms5) std::memcpy(ms5, b->ms, sizeof b->ms);
     if (tiles) *tiles = b->tiles;
     if (candidates) *candidates = b->candidates;
+    return FK_OK;
+}
+
+int fk_batch_set_graph(fk_batch* b, int enable)
+{
+    if (!b) return fail(FK_E_ARG, "null argument");
+    if (b->pending) return fail(FK_E_BUSY, "batch has work in flight");
+    b->use_graph = enable != 0;
+    return FK_OK;
+}
+
+int fk_batch_graph_stats(fk_batch* b, uint64_t* captures, uint64_t* launches)
+{
+    if (!b) return fail(FK_E_ARG, "null argument");
+    if (captures) *captures = b->graph_captures;
+    if (launches) *launches = b->graph_launches;
     return FK_OK;
 }
 

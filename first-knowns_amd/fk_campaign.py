@@ -89,8 +89,10 @@ class Campaign:
         else:
             self._truncate_ledger(int(state["ledger_bytes"]))
         self.state = state
+        # segments of one length repeat one launch sequence: the batches
+        # run it as a captured HIP graph after the first segment
         self.batches = batches if batches is not None else [
-            N.Batch(getattr(engine, "device", 0)) for _ in range(2)]
+            N.Batch(getattr(engine, "device", 0), graph=True) for _ in range(2)]
 
     # ------------------------------------------------------------ ledger
 

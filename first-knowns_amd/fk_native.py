@@ -29,8 +29,8 @@ EXPORTS = ("fk_abi_version", "fk_last_error", "fk_device_count",
            "fk_engine_create", "fk_engine_destroy", "fk_engine_get_info",
            "fk_engine_sweep", "fk_engine_classify", "fk_batch_create",
            "fk_batch_destroy", "fk_batch_submit", "fk_batch_query",
-           "fk_batch_results", "fk_batch_timing", "fk_batch_counts",
-           "fk_engine_batch")
+           "fk_batch_results", "fk_batch_timing", "fk_batch_set_graph",
+           "fk_batch_graph_stats", "fk_batch_counts", "fk_engine_batch")
 
 u32, i32, u64 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
 
@@ -106,6 +106,8 @@ def lib():
     L.fk_batch_results.argtypes = [vp, vp, sz, P(sz), vp]
     L.fk_batch_timing.argtypes = [vp, vp, P(u64), P(u64)]
     L.fk_batch_counts.argtypes = [vp, vp, sz, P(sz)]
+    L.fk_batch_set_graph.argtypes = [vp, ctypes.c_int]
+    L.fk_batch_graph_stats.argtypes = [vp, P(u64), P(u64)]
     L.fk_engine_batch.argtypes = [vp, P(vp)]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
@@ -271,11 +273,20 @@ def batch_timing(handle):
 class Batch:
     """An fk_batch: many windows of many engines, submitted asynchronously."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, graph=False):
         h = ctypes.c_void_p()
         check(lib().fk_batch_create(int(device), ctypes.byref(h)))
         self._h = h
         self._nwin = 0
+        if graph:
+            check(lib().fk_batch_set_graph(h, 1))
+
+    def graph_stats(self):
+        """(graphs captured, graph launches) so far."""
+        c, n = u64(0), u64(0)
+        check(lib().fk_batch_graph_stats(self._h, ctypes.byref(c),
+                                         ctypes.byref(n)))
+        return int(c.value), int(n.value)
 
     def close(self):
         if getattr(self, "_h", None):

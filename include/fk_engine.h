@@ -215,6 +215,20 @@ int fk_batch_timing(fk_batch* b, float* ms5, uint64_t* tiles,
                     uint64_t* candidates);
 
 /*
+ * Graph mode (enable != 0): the batch records each submission's launch
+ * sequence and runs it as a HIP graph, captured the first time and
+ * replayed for every later submission whose sequence is byte-identical
+ * (same engines, window shapes and grids; the windows themselves may
+ * differ).  fk_batch_timing then reports only the whole ([4]).  The
+ * sequence holds no memset nodes (counters are cleared by a kernel).  No
+ * reference counterpart.  FK_E_BUSY while work is in flight.
+ */
+int fk_batch_set_graph(fk_batch* b, int enable);
+
+/* Graphs captured and graph launches so far (either pointer may be NULL). */
+int fk_batch_graph_stats(fk_batch* b, uint64_t* captures, uint64_t* launches);
+
+/*
  * Candidates in the queue after each stage of the last completed batch or
  * sweep: [0] after the sieve, then after each compaction round, in order.
  * *n = the number of stages; FK_E_OVERFLOW if cap is smaller (the first
