@@ -41,6 +41,7 @@ sys.path.insert(0, str(ROOT / "first-knowns_amd"))
 METRIC = ("candidates swept/sec at bit-exact survivor stream; "
           "batched vs plain GPU engine")
 LDS_PEAK = 256 * 128 * 2.4e9          # B/s: 256 CUs x 128 B/clk x 2.4 GHz
+HBM_PEAK = 8.0e12                      # B/s
 MASK64 = (1 << 64) - 1
 
 
@@ -280,6 +281,8 @@ def main():
     per_cand = 8.0 * info.n_pattern_primes / 64 + 8.0 * info.mark_density
     sieve_s = stage[1] / 1e3
     achieved = cand * per_cand / sieve_s if sieve_s > 0 else 0.0
+    traffic = rocprof_traffic()
+    launch_s = sieve_s / args.steps      # one k_sieve launch per window
     out = {
         "metric": METRIC, "value": value, "unit": "j-candidates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -304,9 +307,13 @@ def main():
         "roofline": {"bound": "lds", "kernel": "k_sieve",
                      "achieved": achieved / 1e9, "peak": LDS_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / LDS_PEAK,
-                     "traffic": rocprof_traffic(),
+                     "traffic": traffic,
                      "algorithmic_bytes_per_candidate": per_cand,
-                     "candidates_per_launch": cand / args.steps},
+                     "candidates_per_launch": cand / args.steps,
+                     "hbm_GBps": (traffic / launch_s / 1e9
+                                  if traffic and launch_s > 0 else None),
+                     "hbm_frac": (traffic / launch_s / HBM_PEAK
+                                  if traffic and launch_s > 0 else None)},
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
