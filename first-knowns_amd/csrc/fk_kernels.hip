@@ -510,13 +510,20 @@ __device__ __forceinline__ uint32_t row_residue(const double (&a)[8],
     return resid_d(s, v.qd, v.qi, v.q);
 }
 
-// Lanes whose residue matches the row's kill set (the first 20 residues
-// from registers; longer rows continue in a loop).
+// Lanes whose residue matches the row's kill set: only the four-residue
+// groups the row's count reaches (uniform branches), the first 20 residues
+// from registers, longer rows in a loop.  Folded rows (Dickson) take
+// min(r, q - r).
 __device__ __forceinline__ uint64_t row_match(uint32_t r, const RowView& v,
-                                              const FK_CONST uint32_t* row)
+                                              const FK_CONST uint32_t* row,
+                                              bool fold)
 {
-    uint64_t hit = match4(r, v.k[0]) | match4(r, v.k[1]) | match4(r, v.k[2]) |
-                   match4(r, v.k[3]) | match4(r, v.k[4]);
+    if (fold) r = min(r, v.q - r);
+    uint64_t hit = match4(r, v.k[0]);
+    if (v.cnt > 4) hit |= match4(r, v.k[1]);
+    if (v.cnt > 8) hit |= match4(r, v.k[2]);
+    if (v.cnt > 12) hit |= match4(r, v.k[3]);
+    if (v.cnt > 16) hit |= match4(r, v.k[4]);
     if (v.cnt > 20) {
         for (uint32_t c = 20; c < v.cnt; c += 4)
             hit |= match4(r, load4(row + kRowHead + c));
@@ -529,7 +536,8 @@ __device__ __forceinline__ uint64_t round_scan(const double (&a)[8],
                                                const FK_CONST PrimeD* pd,
                                                const FK_CONST uint32_t* kt,
                                                uint32_t stride, uint32_t rf,
-                                               uint32_t lo, uint32_t hi)
+                                               uint32_t lo, uint32_t hi,
+                                               bool fold)
 {
     // two primes per pass, straight-line
     uint64_t hit = 0;
@@ -541,12 +549,12 @@ __device__ __forceinline__ uint64_t round_scan(const double (&a)[8],
         const RowView v1 = load_row(r1);
         const uint32_t x0 = row_residue<NL>(a, v0, pd + p);
         const uint32_t x1 = row_residue<NL>(a, v1, pd + p + 1);
-        hit |= row_match(x0, v0, r0) | row_match(x1, v1, r1);
+        hit |= row_match(x0, v0, r0, fold) | row_match(x1, v1, r1, fold);
     }
     if (p < hi) {
         const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * stride;
         const RowView v0 = load_row(r0);
-        hit |= row_match(row_residue<NL>(a, v0, pd + p), v0, r0);
+        hit |= row_match(row_residue<NL>(a, v0, pd + p), v0, r0, fold);
     }
     return hit;
 }
@@ -587,9 +595,10 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
         const FK_CONST PrimeD* pd = uniform_view(a.primed);
         const FK_CONST uint32_t* kt = uniform_view(E->kt);
         const uint32_t rf = E->round_first;
+        const bool fold = kill_folded(E->family);
         const uint64_t m =
-            wide ? round_scan<8>(l, pd, kt, E->kstride, rf, lo, hi)
-                 : round_scan<4>(l, pd, kt, E->kstride, rf, lo, hi);
+            wide ? round_scan<8>(l, pd, kt, E->kstride, rf, lo, hi, fold)
+                 : round_scan<4>(l, pd, kt, E->kstride, rf, lo, hi, fold);
         if (live && ((m >> __lane_id()) & 1)) a.flags[i] = 1;
     }
 }
@@ -653,14 +662,15 @@ __device__ __forceinline__ void load_lane_row_wide(LaneRow& L, uint32_t kmax)
 // past 32 (kill sets larger than 32) from memory.
 template <int NL>
 __device__ __forceinline__ bool lane_kills(const LaneRow& L, const double (&a)[8],
-                                           uint32_t ncmp)
+                                           uint32_t ncmp, bool fold)
 {
     double s = fma(a[3], L.c48, fma(a[2], L.c32, fma(a[1], L.c16, a[0])));
     if (NL > 4) {
 #pragma unroll
         for (int k = 0; k < 4; k++) s = fma(a[4 + k], L.cw[k], s);
     }
-    const uint32_t r = resid_d(s, L.qd, L.qi, L.q);
+    uint32_t r = resid_d(s, L.qd, L.qi, L.q);
+    if (fold) r = min(r, L.q - r);
     bool hit = false;
 #pragma unroll
     for (int i = 0; i < kRegResidues; i += 4) {
@@ -709,6 +719,7 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
         // compares per test: the family's largest kill set, in fours
         const uint32_t ncmp = (kill_max(engs[e].family, engs[e].n) + 3u) & ~3u;
         const uint32_t stride = engs[e].kstride;
+        const bool fold = kill_folded(engs[e].family);
         if (mine) {
             load_lane_row(L, engs[e].kt + (size_t)(p - engs[e].round_first) * stride,
                           a.primed + p);
@@ -734,7 +745,8 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
                 const bool wide = (r[j].t[2] | r[j].t[3]) != 0;
                 bool kill = false;
                 if (mine)
-                    kill = wide ? lane_kills<8>(L, l, ncmp) : lane_kills<4>(L, l, ncmp);
+                    kill = wide ? lane_kills<8>(L, l, ncmp, fold)
+                                : lane_kills<4>(L, l, ncmp, fold);
                 const uint64_t m = __ballot(kill);
                 if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1))
                     a.flags[base + i0 + j] = 1;

@@ -10,6 +10,10 @@
 // conditions).  The engine stores, per round prime, the killing residues
 // without repeats, padded to a fixed stride with kKillPad (never a residue),
 // and the kernels compare a candidate's residue against them.
+//
+// Dickson kill sets are closed under negation (the roots of m k^2 = -1 come
+// in pairs r, q - r, and 0 is never one), so they are stored folded: one
+// value min(r, q - r) per pair, and a residue x is tested as min(x, q - x).
 #pragma once
 
 #include "fk_arith.h"
@@ -18,12 +22,16 @@ namespace fk {
 
 constexpr uint32_t kKillPad = 0xffffffffu;
 
-// Largest kill set: Dickson has up to two roots per m, the others one
-// residue per value.
+// Largest stored kill set: one residue per value (Dickson: one folded root
+// pair per m).
 FK_HD uint32_t kill_max(int family, int n)
 {
-    return family == 2 ? 2u * (uint32_t)n : (uint32_t)n;
+    (void)family;
+    return (uint32_t)n;
 }
+
+// Dickson rows hold folded residues.
+FK_HD bool kill_folded(int family) { return family == 2; }
 
 FK_HD uint32_t pow_mod(uint32_t a, uint32_t e, const PrimeK& p)
 {
@@ -79,8 +87,8 @@ FK_HD void kill_add(uint32_t* out, uint32_t& cnt, uint32_t v)
 }
 
 // The kill set of the odd prime q (> n) for family/n, d = P(n) in four
-// limbs (AP only).  Writes out[0, count), pads out[count, stride) and
-// returns count.
+// limbs (AP only), folded for Dickson.  Writes out[0, count), pads
+// out[count, stride) and returns count.
 FK_HD uint32_t kill_values(int family, int n, const uint32_t d[4],
                            const PrimeK& p, uint32_t* out, uint32_t stride)
 {
@@ -98,8 +106,7 @@ FK_HD uint32_t kill_values(int family, int n, const uint32_t d[4],
             const uint32_t a = q - pow_mod(mq, q - 2, p);    // -1/m
             if (pow_mod(a, (q - 1) / 2, p) != 1) continue;
             const uint32_t r = sqrt_mod(a, p);
-            kill_add(out, cnt, r);
-            kill_add(out, cnt, q - r);
+            kill_add(out, cnt, r < q - r ? r : q - r);
         }
     } else {
         const uint32_t dq = red128(d, p);

@@ -99,7 +99,12 @@ def test_kill_sets_small_primes_brute_force(harness, family, n):
     qs = [q for q in range(3, 700) if is_prime(q) and q > n + 1]
     got = run_kill(harness, family, n, qs)
     for q, g in zip(qs, got):
-        assert g == kill_brute(family, n, q), q
+        want = kill_brute(family, n, q)
+        if family == 2:
+            # stored folded: one min(r, q - r) per root pair
+            assert all(q - r in want for r in want) and 0 not in want
+            want = {min(r, q - r) for r in want}
+        assert g == want, q
 
 
 @pytest.mark.parametrize("family,n", [(1, 17), (2, 10), (2, 13), (3, 17)])
@@ -121,9 +126,11 @@ def test_kill_sets_large_primes(harness, family, n):
                 if pow(a, (q - 1) // 2, q) == 1:
                     want += 2
             for r in g:
+                assert 0 < r <= q // 2
                 assert any((m * r * r + 1) % q == 0 for m in range(1, n + 1))
-            # distinct m give distinct roots: -1/m differ for m < q
-            assert len(g) == want
+            # distinct m give distinct root pairs (-1/m differ for m < q),
+            # stored folded: one per pair
+            assert 2 * len(g) == want
 
 
 @pytest.mark.parametrize("family,n", [(1, 17), (2, 10), (2, 13), (3, 8),
@@ -133,7 +140,7 @@ def test_round_rows(harness, family, n):
     qs = [random_prime(rng, 2**11, 2**30) for _ in range(30)] + [2053]
     d = O.primorial(n) if family == 3 else 0
     lines = [f"W {family} {n} {q} {limbs(d)}" for q in qs]
-    kmax = 2 * n if family == 2 else n
+    kmax = n                      # Dickson rows hold folded root pairs
     for q, row in zip(qs, call(harness, lines)):
         w = row.split()
         stride = int(w[0])
