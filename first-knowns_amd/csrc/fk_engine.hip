@@ -623,7 +623,7 @@ static constexpr uint32_t kSingleLimit = 8192;
 
 // lane-per-candidate rounds: workgroups to aim for (candidate blocks times
 // prime blocks)
-static constexpr uint32_t kRoundWorkgroups = 16384;
+static constexpr uint32_t kRoundWorkgroups = 4096;
 
 static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
 {
@@ -928,9 +928,9 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         // workgroup column, the queue cut into slices of about 256
         // candidates, so that no wave walks a long chain of candidates).
         // Otherwise lane = candidate (candidate blocks times
-        // prime blocks of at least 8, about kRoundWorkgroups workgroups: a
-        // wave walks its block's primes with a scalar-load latency per pair,
-        // so short blocks and many workgroups hide it).  Both
+        // prime blocks of at least 16, about kRoundWorkgroups workgroups:
+        // a workgroup pulls its block's rows toward the caches once, then
+        // its waves walk them with a cache latency per pair).  Both
         // read the true count on the device; grids sized from an estimate
         // that is off still cover every candidate.
         const bool lane_prime = np >= e0->round_lane_prime &&
@@ -943,7 +943,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             gy = std::max(1u, std::min(gy, 65535u));
         } else {
             gx = (uint32_t)std::min(1024.0, std::max(1.0, std::ceil(alive / 256.0)));
-            gy = std::min((np + 7) / 8, (kRoundWorkgroups + gx - 1) / gx);
+            gy = std::min((np + 15) / 16, (kRoundWorkgroups + gx - 1) / gx);
             gy = std::max(1u, std::min(gy, 65535u));
             ra.pblock = (np + gy - 1) / gy;
             gy = (np + ra.pblock - 1) / ra.pblock;

@@ -570,6 +570,7 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
     if (pb0 >= pb1) return;
     const FK_CONST uint2* segs = uniform_view(a.segs);
     const FK_CONST uint32_t* counts = uniform_view(a.counts);
+    uint32_t touch = 0;
     SegBlock sb;
     for (uint32_t blk = blockIdx.x;
          seg_block(segs, counts, a.nengines, 256, blk, sb); blk += gridDim.x) {
@@ -578,15 +579,32 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
         const uint32_t lo = max(pb0, E->rb[a.round]);
         const uint32_t hi = min(pb1, E->rb[a.round + 1]);
         if (lo >= hi) continue;
+        // touch this block's rows (one vector load per 64-byte line) so that
+        // the scalar row loads below find them in the caches: vector loads
+        // complete in order, so the records' wait below also covers these,
+        // one memory latency for the whole block instead of one per pair
+        // (the first 256 lines of the block: a hint, not a requirement)
+        uint32_t tv = 0;
+        {
+            const uint32_t* rows = E->kt + (size_t)(lo - E->round_first) * E->kstride;
+            const uint32_t nlines = ((hi - lo) * E->kstride + 15u) / 16u;
+            if (threadIdx.x < nlines) tv = rows[16u * threadIdx.x];
+        }
         const uint32_t li = sb.lb * 256 + threadIdx.x;
         const uint32_t i = segs[sb.e].x + li;
-        const bool live = li < sb.n && a.flags[i] == 0;
+        // the flag and the record are loaded together (one latency)
         uint32_t t[4] = {0u, 0u, 0u, 0u};
-        if (live) {
+        uint32_t flag = 1;
+        if (li < sb.n) {
+            flag = a.flags[i];
             const DevRec rec = a.recs[i];
             for (int k = 0; k < 4; k++) t[k] = rec.t[k];
         }
-        if (!__ballot(live)) continue;
+        const bool live = flag == 0;
+        if (!__ballot(live)) {
+            touch ^= tv;
+            continue;
+        }
         double l[8];
 #pragma unroll
         for (int k = 0; k < 8; k++)
@@ -600,7 +618,10 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
             wide ? round_scan<8>(l, pd, kt, E->kstride, rf, lo, hi, fold)
                  : round_scan<4>(l, pd, kt, E->kstride, rf, lo, hi, fold);
         if (live && ((m >> __lane_id()) & 1)) a.flags[i] = 1;
+        touch ^= tv;
     }
+    // keeps the touch loads; round == ~0 never comes from the host
+    if (a.round == 0xffffffffu && touch == 1u) a.flags[0] = touch;
 }
 
 // ---- lane = prime (many primes): each lane holds one prime's row in
