@@ -183,6 +183,11 @@ static int eng_upload(fk_engine* e, const std::vector<T>& v, const T** out)
     return FK_OK;
 }
 
+// In-tile test rounds are cut where the expected survival over the round
+// reaches this fraction (fewer rounds: fewer barriers, more tests on
+// positions an earlier cut would have dropped).
+static constexpr double kTileRoundSurvival = 0.25;
+
 static int build_engine(fk_engine* e)
 {
     const fk_engine_desc& d = e->desc;
@@ -301,13 +306,13 @@ static int build_engine(fk_engine* e)
     }
     if (nmark < e->npat) nmark = e->npat;
     // in-tile rounds: cut the tested primes where the expected survival of
-    // the round reaches one half
+    // the round reaches kTileRoundSurvival
     std::vector<uint32_t> tr_end;
     {
         double lsv = 0.0;
         for (uint32_t i = nmark; i < (uint32_t)lq.size(); i++) {
             lsv += std::log(1.0 - (double)e->kill_w[i] / lq[i]);
-            if (lsv <= std::log(0.5)) {
+            if (lsv <= std::log(kTileRoundSurvival)) {
                 tr_end.push_back(i + 1);
                 lsv = 0.0;
             }

@@ -35,6 +35,17 @@
 
 namespace fk {
 
+// Read-only tables the kernels index with wave-uniform indices, viewed in
+// the constant address space: loads through it become scalar loads into
+// SGPRs (the compiler cannot otherwise prove that the kernel's own stores
+// leave them alone).
+#define FK_CONST __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const FK_CONST T* uniform_view(const T* p)
+{
+    return (const FK_CONST T*)p;
+}
+
 // ------------------------------------------------------------------ prep
 __global__ void k_prep(const DevEngine* __restrict__ engines,
                        const DevPiece* __restrict__ pieces, uint32_t npieces,
@@ -58,7 +69,7 @@ __global__ void k_prep(const DevEngine* __restrict__ engines,
 
 // ----------------------------------------------------------------- sieve
 
-__device__ __forceinline__ uint32_t find_piece(const DevPiece* pieces,
+__device__ __forceinline__ uint32_t find_piece(const FK_CONST DevPiece* pieces,
                                                uint32_t npieces,
                                                uint32_t tile)
 {
@@ -111,8 +122,8 @@ struct TileList {
 };
 
 __device__ __forceinline__ void emit_record(const SieveArgs& a,
-                                            const DevPiece& P,
-                                            const DevEngine& E,
+                                            const FK_CONST DevPiece& P,
+                                            const FK_CONST DevEngine& E,
                                             unsigned __int128 B, uint64_t off,
                                             uint64_t v, uint32_t slot)
 {
@@ -129,6 +140,22 @@ __device__ __forceinline__ void emit_record(const SieveArgs& a,
         r.run = -1;
         a.out[seg.x + slot] = r;
     }
+}
+
+// Does one of the LDS test primes [p, pend) kill tile position pos?  Each
+// prime's kill bitmap is read at (s0 + pos) mod q, in order, stopping at
+// the first kill.
+__device__ __forceinline__ bool tile_kills(const uint4* pinfo,
+                                           const uint32_t* s0,
+                                           const uint32_t* bm, uint32_t p,
+                                           uint32_t pend, uint32_t pos)
+{
+    for (uint32_t pp = p; pp < pend; pp++) {
+        const uint4 pq = pinfo[pp];
+        const uint32_t x = red32(s0[pp] + pos, pq.x, pq.y);
+        if ((bm[pq.z + (x >> 5)] >> (x & 31)) & 1) return true;
+    }
+    return false;
 }
 
 template <int T, int WPT>
@@ -161,9 +188,10 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
     const uint64_t tile_lo = a.tile_begin + (uint64_t)blockIdx.x * tiles_per;
     const uint64_t tile_hi = min((uint64_t)a.ntiles, tile_lo + tiles_per);
     for (uint32_t tile = (uint32_t)tile_lo; tile < tile_hi; tile++) {
-        const uint32_t pi = find_piece(a.pieces, a.npieces, tile);
-        const DevPiece& P = a.pieces[pi];
-        const DevEngine& E = a.engines[P.engine];
+        // the tile's piece and engine are uniform: scalar loads, cached
+        const uint32_t pi = find_piece(uniform_view(a.pieces), a.npieces, tile);
+        const FK_CONST DevPiece& P = uniform_view(a.pieces)[pi];
+        const FK_CONST DevEngine& E = uniform_view(a.engines)[P.engine];
         const uint32_t local = tile - P.tile0;
         const uint32_t line = local / P.nseg;
         const uint32_t seg = local - line * P.nseg;
@@ -181,10 +209,12 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
             __syncthreads();
             for (uint32_t w = tid; w < E.pat_words; w += T) pat[w] = E.pat[w];
             for (uint32_t w = tid; w < E.bm_words; w += T) bm[w] = E.bm[w];
+            // per LDS prime: q, 2^32/q, its kill bitmap's first word (test
+            // primes) and its pattern table's first word (pattern primes)
             for (uint32_t p = tid; p < E.nlds; p += T)
                 pinfo[p] = make_uint4(E.lds_q[p], E.lds_m32[p],
                                       p >= E.nmark ? E.bm_off[p - E.nmark] : 0,
-                                      0);
+                                      p < E.npat ? E.pat_off[p] : 0);
             cur_engine = P.engine;
         }
         // s0[p] = (v0 + t_p) mod q with t_p = (B + off) D^-1 mod q
@@ -211,7 +241,7 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
             const uint4 pq = pinfo[p];
             const uint32_t q = pq.x, m32 = pq.y;
             const uint32_t step = red32(64u * T, q, m32);
-            const uint64_t* tab = pat + E.pat_off[p];
+            const uint64_t* tab = pat + pq.w;
             uint32_t s = red32(s0[p] + 64u * tid, q, m32);
 #pragma unroll
             for (int k = 0; k < WPT; k++) {
@@ -290,15 +320,7 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
                         uint32_t pos = 0;
                         if (i < n) {
                             pos = cur[i];
-                            alive = true;
-                            for (uint32_t pp = p; pp < pend; pp++) {
-                                const uint4 pq = pinfo[pp];
-                                const uint32_t x = red32(s0[pp] + pos, pq.x, pq.y);
-                                if ((bm[pq.z + (x >> 5)] >> (x & 31)) & 1) {
-                                    alive = false;
-                                    break;
-                                }
-                            }
+                            alive = !tile_kills(pinfo, s0, bm, p, pend, pos);
                         }
                         const uint64_t mask = __ballot(alive);
                         if (mask) {
@@ -335,14 +357,8 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
                         const int b = __builtin_ctzll(m);
                         m &= m - 1;
                         const uint32_t pos = 64u * w + b;
-                        for (uint32_t pp = p0; pp < p1; pp++) {
-                            const uint4 pq = pinfo[pp];
-                            const uint32_t x = red32(s0[pp] + pos, pq.x, pq.y);
-                            if ((bm[pq.z + (x >> 5)] >> (x & 31)) & 1) {
-                                dead |= 1ull << b;
-                                break;
-                            }
-                        }
+                        if (tile_kills(pinfo, s0, bm, p0, p1, pos))
+                            dead |= 1ull << b;
                     }
                     if (dead) {
                         const uint64_t nw = ~(lv & ~dead);
@@ -410,16 +426,6 @@ __global__ void k_kill_table(const PrimeK* __restrict__ primes, uint32_t p0,
     round_row(family, n, dl, primes[p0 + i], kt + (size_t)i * stride, stride);
 }
 
-// Read-only tables the kernels index with wave-uniform indices, viewed in
-// the constant address space: loads through it become scalar loads into
-// SGPRs (the compiler cannot otherwise prove that the kernel's own stores
-// leave them alone).
-#define FK_CONST __attribute__((address_space(4)))
-template <class T>
-__device__ __forceinline__ const FK_CONST T* uniform_view(const T* p)
-{
-    return (const FK_CONST T*)p;
-}
 
 // Global block b of a stage, in blocks of `per` entries of each engine's
 // segment, engine by engine: the engine, the block within it, the engine's
