@@ -30,8 +30,10 @@ struct DevEngine {
     uint32_t nlines;          // wheel residues
     uint32_t nlds;            // LDS-stage primes (pattern, marking, test)
     uint32_t npat;            // [0, npat): pattern words
-    uint32_t nmark;           // [npat, nmark): stride marking;
-                              // [nmark, nlds): per-survivor bitmap tests
+    uint32_t nmark;           // [npat, nmark): stride marking
+    uint32_t nbits;           // [nmark, nbits): 64-bit windows of periodic
+                              // kill bitmaps ORed into the tile's words;
+                              // [nbits, nlds): per-survivor bitmap tests
     uint32_t npairs;          // (prime, residue) marking pairs
     uint32_t pat_words;       // total pattern words (sum of q)
     uint32_t bm_words;        // total kill-bitmap words of the test primes
@@ -54,7 +56,8 @@ struct DevEngine {
     const uint64_t* pat;      // [pat_words] pattern words
     const uint32_t* pairs;    // [npairs] (prime index << 16) | residue
     const uint32_t* bm_off;   // [nlds - nmark] first bitmap word per prime
-    const uint32_t* bm;       // [bm_words] line-domain kill bitmaps
+    const uint32_t* bm;       // [bm_words] line-domain kill bitmaps, each
+                              // q bits then its first 64 bits again
     uint32_t ntr;             // in-tile rounds over [nmark, nlds)
     const uint32_t* tr_end;   // [ntr] LDS-prime index where round r ends
     const uint32_t* kt;       // [round_end - round_first][kstride] round

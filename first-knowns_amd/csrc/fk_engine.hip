@@ -147,6 +147,7 @@ struct fk_engine {
     uint32_t q_pat = 0, q_med = 0, threads = 256, wpt = 8;
     uint32_t round_lane_prime = 8192, growth = 2;
     uint32_t q_mark = 0;             // [q_pat, q_mark) stride-marked
+    uint32_t q_bits = 0;             // [q_mark, q_bits) ORed in from bitmaps
     uint32_t npat = 0;
     std::vector<uint32_t> lds_q;
     std::vector<uint32_t> kill_w;    // |K_q| per LDS prime
@@ -260,13 +261,17 @@ static int build_engine(fk_engine* e)
     e->q_mark = d.q_mark ? d.q_mark : e->q_pat;
     if (e->q_mark < e->q_pat) e->q_mark = e->q_pat;
     if (e->q_mark > e->q_med) e->q_mark = e->q_med;
+    e->q_bits = d.q_bits ? d.q_bits : e->q_mark;
+    if (e->q_bits < e->q_mark) e->q_bits = e->q_mark;
+    if (e->q_bits > e->q_med) e->q_bits = e->q_med;
     e->growth = d.round_growth >= 2 ? d.round_growth : 2;
 
     // ---- LDS-stage primes: past the wheel, below q_med; in order the
-    // pattern primes, the stride-marked primes, the bitmap-tested primes
+    // pattern primes, the stride-marked primes, the bitmap primes (ORed in
+    // as 64-bit windows below q_bits, tested per surviving position above)
     std::vector<uint32_t> lq, lm32, ldinv, pat_off, pairs, bm_off, bm;
     std::vector<uint64_t> lm64, pat;
-    uint32_t nmark = 0;
+    uint32_t nmark = 0, nbits = 0;
     double surv = 1.0;
     for (size_t i = wheel.size(); i < kp.size() && kp[i] < e->q_med; i++) {
         const uint32_t q = kp[i];
@@ -298,19 +303,27 @@ static int build_engine(fk_engine* e)
             e->mark_density += (double)kl.size() / q;
             nmark = idx + 1;
         } else {
+            // periodic kill bitmap: q bits, then the first 64 again, so a
+            // 64-bit window starting at any x < q lies inside it; one word
+            // more for the window's third word
             bm_off.push_back((uint32_t)bm.size());
             const size_t base = bm.size();
-            bm.resize(base + (q + 31) / 32, 0u);
-            for (uint32_t r : kl) bm[base + (r >> 5)] |= 1u << (r & 31);
+            bm.resize(base + (q - 1) / 32 + 3, 0u);
+            for (uint32_t r : kl) {
+                bm[base + (r >> 5)] |= 1u << (r & 31);
+                if (r < 64) bm[base + ((q + r) >> 5)] |= 1u << ((q + r) & 31);
+            }
+            if (q < e->q_bits) nbits = idx + 1;
         }
     }
     if (nmark < e->npat) nmark = e->npat;
+    if (nbits < nmark) nbits = nmark;
     // in-tile rounds: cut the tested primes where the expected survival of
     // the round reaches kTileRoundSurvival
     std::vector<uint32_t> tr_end;
     {
         double lsv = 0.0;
-        for (uint32_t i = nmark; i < (uint32_t)lq.size(); i++) {
+        for (uint32_t i = nbits; i < (uint32_t)lq.size(); i++) {
             lsv += std::log(1.0 - (double)e->kill_w[i] / lq[i]);
             if (lsv <= std::log(kTileRoundSurvival)) {
                 tr_end.push_back(i + 1);
@@ -318,7 +331,7 @@ static int build_engine(fk_engine* e)
             }
         }
         if (tr_end.empty() || tr_end.back() != (uint32_t)lq.size())
-            if ((uint32_t)lq.size() > nmark) tr_end.push_back((uint32_t)lq.size());
+            if ((uint32_t)lq.size() > nbits) tr_end.push_back((uint32_t)lq.size());
     }
     // the tile's LDS: bits, s0, per-prime info, patterns, kill bitmaps
     {
@@ -375,6 +388,7 @@ static int build_engine(fk_engine* e)
     h.nlds = (uint32_t)lq.size();
     h.npat = e->npat;
     h.nmark = nmark;
+    h.nbits = nbits;
     h.npairs = (uint32_t)pairs.size();
     h.pat_words = e->pat_words;
     h.bm_words = (uint32_t)bm.size();

@@ -106,9 +106,11 @@ __device__ __forceinline__ uint64_t edge_mask(uint64_t pos0, uint64_t vlo,
 //   1. s0[p] = line position of the tile start, shifted into prime p's
 //      kill coordinates, for every LDS-stage prime
 //   2. pattern stage: the first npat primes OR in precomputed 64-bit words
+//   2b. bitmap stage: primes [nmark, nbits) OR in 64-bit windows of their
+//      periodic kill bitmaps
 //   3. marking stage: primes [npat, nmark) stride-marked with ds_or
 //   4. test stage: the positions still alive are listed in LDS and run
-//      through in-tile rounds over the primes [nmark, nlds); in a round
+//      through in-tile rounds over the primes [nbits, nlds); in a round
 //      every lane tests the same prime (broadcast LDS reads) against its
 //      own candidate -- one Barrett step and one bit of the prime's
 //      line-domain kill bitmap -- and the survivors are compacted with a
@@ -236,7 +238,8 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
             acc[k] = edge_mask(v0 + 64ull * (tid + (uint32_t)k * T), vlo, vend);
         // a thread whose first word is past the line's end has only dead
         // words (edge_mask); short lines leave whole waves idle here
-        const uint32_t npat_here = (v0 + 64ull * tid < vend) ? E.npat : 0u;
+        const bool words_here = v0 + 64ull * tid < vend;
+        const uint32_t npat_here = words_here ? E.npat : 0u;
         for (uint32_t p = 0; p < npat_here; p++) {
             const uint4 pq = pinfo[p];
             const uint32_t q = pq.x, m32 = pq.y;
@@ -248,6 +251,27 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
                 acc[k] |= tab[s];
                 s += step;
                 if (s >= q) s -= q;
+            }
+        }
+        // ---- 2b. bitmap windows: the 64 kill bits of word w are bits
+        // [x, x + 64) of the prime's periodic bitmap, x = (s0 + 64 w) mod q,
+        // funnel-shifted out of three consecutive words
+        const uint32_t nbits_here = words_here ? E.nbits : E.nmark;
+        for (uint32_t p = E.nmark; p < nbits_here; p++) {
+            const uint4 pq = pinfo[p];
+            const uint32_t q = pq.x, m32 = pq.y;
+            const uint32_t step = red32(64u * T, q, m32);
+            const uint32_t* b = bm + pq.z;
+            uint32_t x = red32(s0[p] + 64u * tid, q, m32);
+#pragma unroll
+            for (int k = 0; k < WPT; k++) {
+                const uint32_t i = x >> 5, o = x & 31u;
+                const uint32_t w0 = b[i], w1 = b[i + 1], w2 = b[i + 2];
+                const uint32_t lo = __builtin_amdgcn_alignbit(w1, w0, o);
+                const uint32_t hi = __builtin_amdgcn_alignbit(w2, w1, o);
+                acc[k] |= ((uint64_t)hi << 32) | lo;
+                x += step;
+                if (x >= q) x -= q;
             }
         }
 #pragma unroll
@@ -287,7 +311,7 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
         bool listed = false;
         PosT* cur = listA;
         uint32_t n = 0;
-        if (E.nmark < E.nlds) {
+        if (E.nbits < E.nlds) {
             // 4a. list the live positions
             const uint32_t at = cnt ? atomicAdd(&sh_n, cnt) : 0;
             if (cnt && at + cnt <= LCAP) {
@@ -309,7 +333,7 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
                 listed = true;
                 // 4b. in-tile rounds over the listed candidates
                 PosT* nxt = listB;
-                uint32_t p = E.nmark;
+                uint32_t p = E.nbits;
                 for (uint32_t r = 0; r < E.ntr && n; r++) {
                     const uint32_t pend = E.tr_end[r];
                     if (tid == 0) sh_m = 0;
@@ -347,7 +371,7 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
                 // 4c. too many to list: test word by word (divergent but
                 // correct), then fall through to the bit compaction
                 const uint32_t nwords = (ulen + 63) / 64;
-                const uint32_t p0 = E.nmark, p1 = E.nlds;
+                const uint32_t p0 = E.nbits, p1 = E.nlds;
                 for (;;) {
                     const uint32_t w = atomicAdd(&sh_next, 1u);
                     if (w >= nwords) break;

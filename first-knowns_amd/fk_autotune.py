@@ -8,6 +8,8 @@ tuner is free to pick whatever is fastest:
   q_pat                       primes below it are OR-ed in as pattern words
   q_mark                      primes in [q_pat, q_mark) are stride-marked
                               in LDS
+  q_bits                      primes in [q_mark, q_bits) OR in 64-bit
+                              windows of periodic kill bitmaps
   q_med                       primes in [q_mark, q_med) are tested per
                               surviving position against LDS kill bitmaps;
                               above it per candidate in compaction rounds
@@ -48,6 +50,7 @@ SPACE = {
     "nwheel": [0, 3, 4, 5, 6, 7],
     "round_growth": [2, 3, 4],
     "q_mark": [0, 192, 256, 512, 1024],
+    "q_bits": [0, 1, 256, 512, 1024, 2048],
 }
 
 VALID_SHAPES = {(256, 1), (256, 2), (256, 4), (256, 8), (256, 16),
@@ -92,12 +95,13 @@ def tune(family, n, q2, lo, span, passes=2, reps=3, classify=None,
     cur.setdefault("nwheel", 0)
     cur.setdefault("round_growth", 2)
     cur.setdefault("q_mark", 0)
+    cur.setdefault("q_bits", 0)
     cur.pop("round_lane_prime", None)
     best, st = measure(family, n, q2, lo, span, cur, reps, classify,
                        wheel_primes)
     log(f"start {cur} -> {best:.3f} ms {st}")
-    keys = ["q_med", "q_mark", "q_pat", "threads", "words_per_thread",
-            "round_growth"]
+    keys = ["q_med", "q_bits", "q_mark", "q_pat", "threads",
+            "words_per_thread", "round_growth"]
     if not wheel_primes:
         keys.append("nwheel")
     for _ in range(passes):

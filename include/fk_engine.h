@@ -79,8 +79,11 @@ typedef struct fk_engine_desc {
     uint32_t round_growth;   /* prime bound grows by this factor per round
                                 (0 = default 2) */
     uint32_t q_mark;         /* primes in [q_pat, q_mark) are stride-marked
-                                in LDS; [q_mark, q_med) are tested per
-                                surviving position (0 = q_pat: no marking) */
+                                in LDS (0 = q_pat: no marking) */
+    uint32_t q_bits;         /* primes in [q_mark, q_bits) are ORed in as
+                                64-bit windows of their periodic kill bitmaps;
+                                [q_bits, q_med) are tested per surviving
+                                position (0 = q_mark: no bitmap windows) */
 } fk_engine_desc;
 
 typedef struct fk_engine_info {

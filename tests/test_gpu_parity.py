@@ -206,6 +206,8 @@ def test_streams_do_not_depend_on_cuts_or_tiles():
                {"nwheel": 1}, {"nwheel": 2},
                {"round_growth": 4, "round_lane_prime": 1},
                {"round_lane_prime": 1}, {"round_lane_prime": 1 << 30},
+               {"q_bits": 1}, {"q_bits": 300, "q_med": 1024},
+               {"q_bits": 1, "q_med": 2048, "q_pat": 64},
                {"round_lane_prime": 1, "q_med": 256},
                {"q_mark": 1024}, {"q_mark": 4096},
                {"q_mark": 256, "threads": 1024, "words_per_thread": 1},
@@ -253,7 +255,8 @@ def test_round_kernels_match_oracle(case):
     assert len(want) >= need
     for extra in ({"round_lane_prime": 1}, {"round_lane_prime": 1 << 30},
                   {"round_lane_prime": 1, "q_med": 256, "q_pat": 64},
-                  {"round_lane_prime": 1 << 30, "q_med": 256, "q_pat": 64}):
+                  {"round_lane_prime": 1 << 30, "q_med": 256, "q_pat": 64},
+                  {"q_bits": 1, "q_med": 1024}):
         eng = N.Engine(fam, n, q2, **kw, **extra)
         got = N.record_values(eng.sweep(lo, hi))
         eng.close()
