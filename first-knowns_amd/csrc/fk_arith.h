@@ -216,150 +216,165 @@ FK_HD uint32_t neg_inv32(uint32_t a)
     return 0u - x;
 }
 
-// Montgomery product r = a * b * 2^(-32 L) mod m, for odd m < 2^(32 L)
-// (coarsely integrated operand scanning).  a, b < m; r < m.
-template <int L>
-FK_HD void mont_mul(uint32_t* r, const uint32_t* a, const uint32_t* b,
-                    const uint32_t* m, uint32_t minv)
+// ------------------------------------------ radix-2^26 Montgomery residues
+// A residue modulo an odd m of bit length k is held in N = ceil((k + 2) / 26)
+// limbs of 26 bits (uint32_t each, little-endian), so R = 2^(26 N) > 4 m.
+// Every limb product is below 2^53, so one column of the schoolbook product
+// plus the reduction terms (at most 2 N - 1 < 20 of each) and the carry in
+// stay far below 2^64: a column is summed in one 64-bit accumulator with one
+// v_mad_u64_u32 per product and no carry handling, and the carry out is one
+// shift.  Residues are kept lazily, below 2 m rather than m, inside the
+// exponentiation; reduce26 makes them exact where they are compared.
+
+constexpr int kRadixBits = 26;
+constexpr uint32_t kRadixMask = (1u << kRadixBits) - 1;
+constexpr int kMaxRadixLimbs = 10;        // 26 * 10 >= 256 + 2
+
+template <int N>
+struct Mod26 {
+    uint32_t m[N];
+    uint32_t minv;       // -m^-1 mod 2^26
+    double inv_top;      // 1 / top26<N>(m)
+};
+
+// The top min(N, 4) limbs of x as a double (the same scale for every x of
+// N limbs).  m's top four limbs hold at least 76 of its bits, so ratios of
+// these approximate ratios of the full values to about 2^-50.
+template <int N>
+FK_HD double top26(const uint32_t* x)
 {
-    uint32_t t[L + 2];
-    for (int i = 0; i < L + 2; i++) t[i] = 0;
-    for (int i = 0; i < L; i++) {
-        uint64_t c = 0;
-        const uint32_t bi = b[i];
-        for (int j = 0; j < L; j++) {
-            uint64_t s = (uint64_t)a[j] * bi + t[j] + c;
-            t[j] = (uint32_t)s;
-            c = s >> 32;
-        }
-        uint64_t s = (uint64_t)t[L] + c;
-        t[L] = (uint32_t)s;
-        t[L + 1] = (uint32_t)(s >> 32);
-        const uint32_t u = t[0] * minv;
-        c = ((uint64_t)u * m[0] + t[0]) >> 32;
-        for (int j = 1; j < L; j++) {
-            uint64_t s2 = (uint64_t)u * m[j] + t[j] + c;
-            t[j - 1] = (uint32_t)s2;
-            c = s2 >> 32;
-        }
-        s = (uint64_t)t[L] + c;
-        t[L - 1] = (uint32_t)s;
-        t[L] = t[L + 1] + (uint32_t)(s >> 32);
-    }
-    // t < 2m: one conditional subtract
-    uint32_t d[L];
-    uint32_t borrow = mw_sub<L>(d, t, m);
-    const bool use_t = (t[L] == 0) && borrow;
-    for (int i = 0; i < L; i++) r[i] = use_t ? t[i] : d[i];
+    constexpr int T = N < 4 ? N : 4;
+    double r = 0.0;
+#pragma unroll
+    for (int i = N - 1; i >= N - T; i--) r = r * 67108864.0 + (double)x[i];
+    return r;
 }
 
-// Montgomery square r = a^2 * 2^(-32 L) mod m: the cross products are
-// formed once and doubled, then the 2L-limb square is reduced limb by limb
-// (separated operand scanning).  a < m; r < m.
-template <int L>
-FK_HD void mont_sqr(uint32_t* r, const uint32_t* a, const uint32_t* m,
-                    uint32_t minv)
+// 26-bit limbs of the value in eight 32-bit words (bits beyond word 7 are 0)
+template <int N>
+FK_HD void to26(uint32_t* r, const uint32_t* v8)
 {
-    uint32_t t[2 * L + 1];
 #pragma unroll
-    for (int i = 0; i < 2 * L + 1; i++) t[i] = 0;
-    // cross products a_i a_j, i < j
-#pragma unroll
-    for (int i = 0; i < L; i++) {
-        uint64_t c = 0;
-#pragma unroll
-        for (int j = i + 1; j < L; j++) {
-            uint64_t s = (uint64_t)a[i] * a[j] + t[i + j] + c;
-            t[i + j] = (uint32_t)s;
-            c = s >> 32;
-        }
-        t[i + L] = (uint32_t)c;
+    for (int i = 0; i < N; i++) {
+        const int bit = kRadixBits * i, w = bit >> 5, sh = bit & 31;
+        const uint64_t lo = w < 8 ? v8[w] : 0u;
+        const uint64_t hi = w + 1 < 8 ? v8[w + 1] : 0u;
+        r[i] = (uint32_t)((lo | (hi << 32)) >> sh) & kRadixMask;
     }
-    // double them
-    uint32_t top = 0;
-#pragma unroll
-    for (int i = 1; i < 2 * L; i++) {
-        uint32_t nt = t[i] >> 31;
-        t[i] = (t[i] << 1) | top;
-        top = nt;
-    }
-    // add the squares a_i^2
-    uint64_t c = 0;
-#pragma unroll
-    for (int i = 0; i < L; i++) {
-        uint64_t sq = (uint64_t)a[i] * a[i];
-        uint64_t s0 = (uint64_t)t[2 * i] + (uint32_t)sq + c;
-        t[2 * i] = (uint32_t)s0;
-        uint64_t s1 = (uint64_t)t[2 * i + 1] + (sq >> 32) + (s0 >> 32);
-        t[2 * i + 1] = (uint32_t)s1;
-        c = s1 >> 32;
-    }
-    // reduce: add u_i m 2^(32 i) so the low L limbs vanish
-#pragma unroll
-    for (int i = 0; i < L; i++) {
-        const uint32_t u = t[i] * minv;
-        uint64_t cc = 0;
-#pragma unroll
-        for (int j = 0; j < L; j++) {
-            uint64_t s = (uint64_t)u * m[j] + t[i + j] + cc;
-            t[i + j] = (uint32_t)s;
-            cc = s >> 32;
-        }
-#pragma unroll
-        for (int k = i + L; k < 2 * L + 1; k++) {
-            uint64_t s = (uint64_t)t[k] + cc;
-            t[k] = (uint32_t)s;
-            cc = s >> 32;
-        }
-    }
-    // result t[L..2L] < 2m: one conditional subtract
-    uint32_t d[L];
-    uint32_t borrow = mw_sub<L>(d, t + L, m);
-    const bool use_t = (t[2 * L] == 0) && borrow;
-#pragma unroll
-    for (int i = 0; i < L; i++) r[i] = use_t ? t[L + i] : d[i];
 }
 
-// r = 2 r mod m for r < m (m < 2^(32 L))
-template <int L>
-FK_HD void mw_dbl_mod(uint32_t* r, const uint32_t* m)
+// x = x - m when x >= m (x < 2m on entry, exact residue on exit)
+template <int N>
+FK_HD void reduce26(uint32_t* x, const uint32_t* m)
 {
-    const uint32_t top = r[L - 1] >> 31;
+    uint32_t d[N];
+    int32_t borrow = 0;
 #pragma unroll
-    for (int i = L - 1; i > 0; i--) r[i] = (r[i] << 1) | (r[i - 1] >> 31);
-    r[0] <<= 1;
-    uint32_t d[L];
-    const uint32_t borrow = mw_sub<L>(d, r, m);
-    const bool sub = top || !borrow;
+    for (int i = 0; i < N; i++) {
+        const int32_t t = (int32_t)x[i] - (int32_t)m[i] + borrow;
+        d[i] = (uint32_t)t & kRadixMask;
+        borrow = t >> kRadixBits;            // 0 or -1
+    }
+    const bool ge = borrow == 0;
 #pragma unroll
-    for (int i = 0; i < L; i++) r[i] = sub ? d[i] : r[i];
+    for (int i = 0; i < N; i++) x[i] = ge ? d[i] : x[i];
 }
 
-// r = (r + b) mod m for r, b < m
-template <int L>
-FK_HD void mw_add_mod(uint32_t* r, const uint32_t* b, const uint32_t* m)
+template <int N>
+FK_HD bool eq26(const uint32_t* a, const uint32_t* b)
 {
-    const uint32_t carry = mw_add<L>(r, r, b);
-    uint32_t d[L];
-    const uint32_t borrow = mw_sub<L>(d, r, m);
-    const bool sub = carry || !borrow;
+    uint32_t o = 0;
 #pragma unroll
-    for (int i = 0; i < L; i++) r[i] = sub ? d[i] : r[i];
+    for (int i = 0; i < N; i++) o |= a[i] ^ b[i];
+    return o == 0;
 }
 
-// R mod m with R = 2^(32 L): start from the top bit of m (2^(B-1) < m for
-// odd m > 1 of bit length B) and double the remaining 32 L - B + 1 times.
-template <int L>
-FK_HD void mont_one(uint32_t* r, const uint32_t* m)
+// r = x y R^-1 mod m, up to one m (r < 2m), for x, y < 2m (limbs < 2^26).
+// Product scanning with the reduction interleaved column by column: u_i
+// clears column i, and the top N columns are the result (R > 4m bounds it
+// by (4 m^2 + R m) / R < 2 m).
+template <int N>
+FK_HD void mont_mul26(uint32_t* r, const uint32_t* x, const uint32_t* y,
+                      const Mod26<N>& M)
 {
-    int B = 0;
+    uint32_t u[N], o[N];
+    uint64_t acc = 0;
 #pragma unroll
-    for (int i = 0; i < L; i++)
-        if (m[i]) B = 32 * i + 32 - __builtin_clz(m[i]);
-    const int tb = B - 1;
+    for (int i = 0; i < N; i++) {
 #pragma unroll
-    for (int i = 0; i < L; i++) r[i] = (i == (tb >> 5)) ? (1u << (tb & 31)) : 0u;
-    for (int k = 0; k < 32 * L - tb; k++) mw_dbl_mod<L>(r, m);
+        for (int j = 0; j <= i; j++) acc += (uint64_t)x[j] * y[i - j];
+#pragma unroll
+        for (int j = 0; j < i; j++) acc += (uint64_t)u[j] * M.m[i - j];
+        u[i] = ((uint32_t)acc * M.minv) & kRadixMask;
+        acc += (uint64_t)u[i] * M.m[0];
+        acc >>= kRadixBits;
+    }
+#pragma unroll
+    for (int i = N; i < 2 * N - 1; i++) {
+#pragma unroll
+        for (int j = i - N + 1; j < N; j++) {
+            acc += (uint64_t)x[j] * y[i - j];
+            acc += (uint64_t)u[j] * M.m[i - j];
+        }
+        o[i - N] = (uint32_t)acc & kRadixMask;
+        acc >>= kRadixBits;
+    }
+    o[N - 1] = (uint32_t)acc;
+#pragma unroll
+    for (int i = 0; i < N; i++) r[i] = o[i];
+}
+
+// r = x^2 R^-1 mod m, up to one m, for x < 2m: mont_mul26 with each cross
+// product x_j x_k (j < k) formed once against 2 x_j.
+template <int N>
+FK_HD void mont_sqr26(uint32_t* r, const uint32_t* x, const Mod26<N>& M)
+{
+    uint32_t u[N], o[N], x2[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) x2[j] = x[j] << 1;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < 2 * N - 1; i++) {
+#pragma unroll
+        for (int j = (i < N ? 0 : i - N + 1); 2 * j < i; j++)
+            acc += (uint64_t)x2[j] * x[i - j];
+        if ((i & 1) == 0) acc += (uint64_t)x[i >> 1] * x[i >> 1];
+        if (i < N) {
+#pragma unroll
+            for (int j = 0; j < i; j++) acc += (uint64_t)u[j] * M.m[i - j];
+            u[i] = ((uint32_t)acc * M.minv) & kRadixMask;
+            acc += (uint64_t)u[i] * M.m[0];
+        } else {
+#pragma unroll
+            for (int j = i - N + 1; j < N; j++)
+                acc += (uint64_t)u[j] * M.m[i - j];
+            o[i - N] = (uint32_t)acc & kRadixMask;
+        }
+        acc >>= kRadixBits;
+    }
+    o[N - 1] = (uint32_t)acc;
+#pragma unroll
+    for (int i = 0; i < N; i++) r[i] = o[i];
+}
+
+// r = a x mod m, up to one m (r < 2m), for x < 2m and a < 2^31.  The
+// quotient q of a x by m is below 2^32; the double estimate, scaled down by
+// 1 - 2^-40, is within 2^-8 + 2^-17 below a x / m, so its floor is q or
+// q - 1 and a x - floor * m lies in [0, 2m).  One signed pass per limb.
+template <int N>
+FK_HD void mul_small26(uint32_t* r, const uint32_t* x, uint32_t a,
+                       const Mod26<N>& M)
+{
+    const double qf = (double)a * top26<N>(x) * M.inv_top * (1.0 - 0x1p-40);
+    const uint32_t q = (uint32_t)qf;
+    int64_t c = 0;
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        const int64_t v = (int64_t)((uint64_t)a * x[j])
+                        - (int64_t)((uint64_t)q * M.m[j]) + c;
+        r[j] = (uint32_t)v & kRadixMask;
+        c = v >> kRadixBits;
+    }
 }
 
 // Shift right by `bits` (< 32 L), with only compile-time limb indexes so
@@ -384,75 +399,90 @@ FK_HD void mw_shr(uint32_t* a, int bits)
     }
 }
 
-// Strong probable-prime test of odd m > 37 (no factor <= 37) to the bases
-// first..first+nbases-1 of the seven.  Returns true when m passes every
-// base asked for.  huntlib/primes.py:31-47 (the loop over bases) and
-// :61-80 (base 2 alone) are the two uses.  In the Montgomery domain
-// (R = 2^(32 L)): one = R mod m, a R mod m by double-and-add of one, and
-// for base 2 the "multiply by the base" step of the ladder is a doubling.
-template <int L>
-FK_HD bool sprp_bases(const uint32_t* m, int first, int nbases)
+// Strong probable-prime test of odd m > 37 with no factor <= 37, of bit
+// length k <= 26 N - 2, to the bases first..first+nbases-1 of the seven.
+// huntlib/primes.py:31-47 (the loop over bases) and :61-80 (base 2 alone)
+// are the two uses.  In the Montgomery domain (R = 2^(26 N)) the ladder
+// starts from one = R mod m and, per bit of d from the top, squares and
+// then multiplies by the bit's factor (the base or 1) as a plain scalar:
+// one instruction stream for every base, with no table and no branch.
+template <int N>
+FK_HD bool sprp26(const uint32_t* v8, int first, int nbases)
 {
-    const uint32_t minv = neg_inv32(m[0]);
-    uint32_t one[L], mone[L], d[L];
-    mont_one<L>(one, m);
-    mw_sub<L>(mone, m, one);            // (m - 1) R mod m == m - (R mod m)
-    // m - 1 = d * 2^s
+    Mod26<N> M;
+    to26<N>(M.m, v8);
+    M.minv = neg_inv32(v8[0]) & kRadixMask;
+    M.inv_top = 1.0 / top26<N>(M.m);
+    // one = 2^(26 N) mod m: 2^(k-1) < m, times 2^(26 N - k + 1) in factors
+    // of at most 2^28 (one factor unless N was raised to its minimum of 2)
+    int k = 0;
 #pragma unroll
-    for (int i = 0; i < L; i++) d[i] = m[i];
-    d[0] -= 1;                          // m odd: no borrow
+    for (int i = 0; i < 8; i++)
+        if (v8[i]) k = 32 * i + 32 - __builtin_clz(v8[i]);
+    const int tb = k - 1;
+    uint32_t one[N], mone[N];
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        one[i] = (i == tb / kRadixBits) ? 1u << (tb % kRadixBits) : 0u;
+    for (int sh = kRadixBits * N - tb; sh > 0; sh -= 28)
+        mul_small26<N>(one, one, 1u << (sh < 28 ? sh : 28), M);
+    reduce26<N>(one, M.m);
+    int32_t borrow = 0;                   // mone = m - one = (m - 1) R mod m
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int32_t t = (int32_t)M.m[i] - (int32_t)one[i] + borrow;
+        mone[i] = (uint32_t)t & kRadixMask;
+        borrow = t >> kRadixBits;
+    }
+    // m - 1 = d 2^s
+    constexpr int W = (kRadixBits * N + 31) / 32 < 8 ? (kRadixBits * N + 31) / 32 : 8;
+    uint32_t d[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) d[i] = v8[i];
+    d[0] -= 1;                            // m odd: no borrow
     int s = 0;
     bool found = false;
 #pragma unroll
-    for (int i = 0; i < L; i++) {
+    for (int i = 0; i < W; i++) {
         if (!found) {
             if (d[i] != 0) { s += __builtin_ctz(d[i]); found = true; }
             else s += 32;
         }
     }
-    mw_shr<L>(d, s);
-    bool one_limb = true;
+    mw_shr<W>(d, s);
+    int dtop = 0;
 #pragma unroll
-    for (int i = 1; i < L; i++) one_limb = one_limb && (m[i] == 0);
+    for (int i = 0; i < W; i++) if (d[i]) dtop = i;
 
+    const bool one_word = k <= 32;
     for (int b = first; b < first + nbases; b++) {
-        // a %= m; a == 0 -> skip (a < 2^31, so only a one-limb m reduces it)
-        uint32_t a = mr_base(b);
-        if (one_limb) a = a % m[0];
-        if (a == 0) continue;
-        uint32_t am[L], x[L];
+        // a mod m == 0 skips the base (a < 2^31, so only m < 2^32 can
+        // divide it); otherwise the unreduced a gives the same powers
+        const uint32_t a = mr_base(b);
+        if (one_word && a % v8[0] == 0) continue;
+        uint32_t x[N];
 #pragma unroll
-        for (int i = 0; i < L; i++) { am[i] = 0; x[i] = 0; }
-        // a R mod m by double-and-add over the bits of a
-        for (int k = 31 - __builtin_clz(a); k >= 0; k--) {
-            mw_dbl_mod<L>(am, m);
-            if ((a >> k) & 1) mw_add_mod<L>(am, one, m);
-        }
-        const bool two = (a == 2);
-        bool started = false;
+        for (int i = 0; i < N; i++) x[i] = one[i];
 #pragma unroll
-        for (int w = L - 1; w >= 0; w--) {
+        for (int w = W - 1; w >= 0; w--) {
+            if (w > dtop) continue;
             const uint32_t word = d[w];
-            for (int bit = 31; bit >= 0; bit--) {
-                const bool set = (word >> bit) & 1;
-                if (started) {
-                    mont_sqr<L>(x, x, m, minv);
-                    if (set) {
-                        if (two) mw_dbl_mod<L>(x, m);
-                        else mont_mul<L>(x, x, am, m, minv);
-                    }
-                } else if (set) {
-                    started = true;
-#pragma unroll
-                    for (int i = 0; i < L; i++) x[i] = am[i];
-                }
+            const int nb = w == dtop ? 32 - __builtin_clz(word) : 32;
+            for (int bit = nb - 1; bit >= 0; bit--) {
+                mont_sqr26<N>(x, x, M);
+                mul_small26<N>(x, x, ((word >> bit) & 1u) ? a : 1u, M);
             }
         }
-        if (mw_cmp<L>(x, one) == 0 || mw_cmp<L>(x, mone) == 0) continue;
+        reduce26<N>(x, M.m);
+        if (eq26<N>(x, one) || eq26<N>(x, mone)) continue;
         bool ok = false;
         for (int r = 1; r < s; r++) {
-            mont_sqr<L>(x, x, m, minv);
-            if (mw_cmp<L>(x, mone) == 0) { ok = true; break; }
+            mont_sqr26<N>(x, x, M);
+            uint32_t e[N];
+#pragma unroll
+            for (int i = 0; i < N; i++) e[i] = x[i];
+            reduce26<N>(e, M.m);
+            if (eq26<N>(e, mone)) { ok = true; break; }
         }
         if (!ok) return false;
     }
@@ -462,9 +492,9 @@ FK_HD bool sprp_bases(const uint32_t* m, int first, int nbases)
 // huntlib/primes.py:23-47 (bases 0..6) and :50-80 (base 0 alone) for a
 // value of up to 8 limbs: the small-prime pre-check answers m == sp, then
 // the strong test to bases first..first+nbases-1 runs at the narrowest
-// limb count that holds m.  The seven-base verdict is the AND of the
-// single-base verdicts, which is what lets the device test bases in
-// parallel.
+// radix-2^26 limb count that leaves R > 4 m.  The seven-base verdict is the
+// AND of the single-base verdicts, which is what lets the device test bases
+// in parallel.
 FK_HD bool is_sprp(const uint32_t* v8, int first, int nbases)
 {
     int top = 0;
@@ -476,10 +506,22 @@ FK_HD bool is_sprp(const uint32_t* v8, int first, int nbases)
         const uint32_t sp = small_prime(i);
         if (mw_mod_small<8>(v8, sp) == 0) return top == 0 && v8[0] == sp;
     }
-    if (top < 2) return sprp_bases<2>(v8, first, nbases);
-    if (top < 4) return sprp_bases<4>(v8, first, nbases);
-    if (top < 6) return sprp_bases<6>(v8, first, nbases);
-    return sprp_bases<8>(v8, first, nbases);
+    int k = 0;                            // bit length, no dynamic index
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+        if (v8[i]) k = 32 * i + 32 - __builtin_clz(v8[i]);
+    const int n = (k + 2 + kRadixBits - 1) / kRadixBits;
+    switch (n) {
+    case 0: case 1: case 2: return sprp26<2>(v8, first, nbases);
+    case 3: return sprp26<3>(v8, first, nbases);
+    case 4: return sprp26<4>(v8, first, nbases);
+    case 5: return sprp26<5>(v8, first, nbases);
+    case 6: return sprp26<6>(v8, first, nbases);
+    case 7: return sprp26<7>(v8, first, nbases);
+    case 8: return sprp26<8>(v8, first, nbases);
+    case 9: return sprp26<9>(v8, first, nbases);
+    default: return sprp26<10>(v8, first, nbases);
+    }
 }
 
 // --------------------------------------------- small multiword helpers

@@ -970,7 +970,9 @@ __global__ __launch_bounds__(128) void k_cls_full(ClassifyArgs a)
 {
     const uint32_t count = min(*a.count, a.cap);
     // single pass: all seven bases (base 2 into fail2) with no dependence
-    // on a base-2 pass; otherwise bases 3..7 after it
+    // on a base-2 pass; otherwise bases 3..7 after it.  One call site of
+    // the strong test, so a wave whose lanes hold different bases runs one
+    // instruction stream.
     const bool single = cls_single(a, count);
     const uint32_t nb = single ? 7 : 6;
     const uint64_t per = (uint64_t)a.cap_max * nb;
@@ -981,17 +983,6 @@ __global__ __launch_bounds__(128) void k_cls_full(ClassifyArgs a)
         const uint32_t rem = (uint32_t)(idx - (uint64_t)r * per);
         const int i = (int)(rem / nb);
         const int b = (int)(rem - (uint32_t)i * nb) + (single ? 0 : 1);
-        if (b == 0) {
-            const DevRec rec = a.recs[r];
-            const DevWinInfo W = a.wins[rec.window];
-            if (W.mode == 0 || i >= W.cap) continue;
-            const DevEngine& E = a.engines[W.engine];
-            uint32_t v[8];
-            rec_chain_value(E, rec, i, v);
-            if (!is_sprp(v, 0, 1))
-                atomicOr((unsigned long long*)&a.fail2[r], 1ull << i);
-            continue;
-        }
         const DevRec rec = a.recs[r];
         const DevWinInfo W = a.wins[rec.window];
         if (W.mode == 0 || i >= W.cap) continue;
@@ -1004,7 +995,8 @@ __global__ __launch_bounds__(128) void k_cls_full(ClassifyArgs a)
         uint32_t v[8];
         rec_chain_value(E, rec, i, v);
         if (!is_sprp(v, b, 1))
-            atomicOr((unsigned long long*)&a.failx[r], 1ull << i);
+            atomicOr((unsigned long long*)(b == 0 ? &a.fail2[r] : &a.failx[r]),
+                     1ull << i);
     }
 }
 

@@ -1,7 +1,9 @@
 """The device arithmetic header (first-knowns_amd/csrc/fk_arith.h), compiled
 for the host with g++, against the oracle's Python-integer strong test and
 sympy's BPSW: random odd values of 10..256 bits, primes on both sides of
-every limb boundary, strong pseudoprimes and the small-prime edge cases."""
+every 32-bit word boundary and of every radix-2^26 limb-count boundary
+(bit lengths 26 N - 2 and 26 N - 1), strong pseudoprimes to each single
+base, and the small-prime edge cases."""
 import pathlib
 import random
 import subprocess
@@ -23,10 +25,12 @@ def harness(tmp_path_factory):
 
 
 def run(exe, cases):
+    """cases: (nbases, value) from base 0, or (first, nbases, value)."""
     lines = []
-    for nb, v in cases:
+    for c in cases:
+        first, nb, v = c if len(c) == 3 else (0,) + tuple(c)
         limbs = [(v >> (32 * i)) & 0xFFFFFFFF for i in range(8)]
-        lines.append(f"{nb} " + " ".join("%x" % x for x in limbs))
+        lines.append(f"{first} {nb} " + " ".join("%x" % x for x in limbs))
     out = subprocess.run([str(exe)], input="\n".join(lines) + "\n",
                          capture_output=True, text=True, check=True)
     return [c == "1" for c in out.stdout.split()]
@@ -57,3 +61,36 @@ def test_strong_test_agrees_with_bpsw(harness):
     got = run(harness, [(7, v) for v in vals])
     for v, g in zip(vals, got):
         assert g == bool(sympy.isprime(v)), v
+
+
+def test_single_bases_and_limb_boundaries(harness):
+    """Each base alone (the device runs one base per lane): composites that
+    are strong pseudoprimes to that base, primes and random odd values at
+    the bit lengths where the radix-2^26 limb count changes."""
+    rng = random.Random(11)
+    vals = []
+    # composites with no factor <= 37 that pass some single base
+    small = [p for p in range(3, 38) if sympy.isprime(p)]
+    spsp = []
+    for n in range(41, 400000, 2):
+        if any(n % p == 0 for p in small) or sympy.isprime(n):
+            continue
+        if any(O.mr_is_prime(n, bases=[b]) for b in O.MR_BASES):
+            spsp.append(n)
+    assert len(spsp) > 20
+    vals += spsp
+    for nlimb in range(2, 11):
+        for bits in (26 * nlimb - 3, 26 * nlimb - 2, 26 * nlimb - 1):
+            if bits > 256:
+                continue
+            vals += [rng.getrandbits(bits) | 1 | (1 << (bits - 1))
+                     for _ in range(6)]
+            for _ in range(4):
+                p = int(sympy.prevprime(1 << bits))
+                vals.append(p)
+                vals.append(int(sympy.nextprime(rng.getrandbits(bits - 1)
+                                                | (1 << (bits - 2)))))
+    cases = [(b, 1, v) for b in range(7) for v in vals]
+    got = run(harness, cases)
+    for (b, _, v), g in zip(cases, got):
+        assert g == O.mr_is_prime(v, bases=[O.MR_BASES[b]]), (b, v)
