@@ -297,25 +297,31 @@ template <int N>
 FK_HD void mont_mul26(uint32_t* r, const uint32_t* x, const uint32_t* y,
                       const Mod26<N>& M)
 {
+    // two accumulators per column, the products and the reduction terms,
+    // so the two multiply-add chains run side by side
     uint32_t u[N], o[N];
     uint64_t acc = 0;
 #pragma unroll
     for (int i = 0; i < N; i++) {
+        uint64_t red = 0;
 #pragma unroll
         for (int j = 0; j <= i; j++) acc += (uint64_t)x[j] * y[i - j];
 #pragma unroll
-        for (int j = 0; j < i; j++) acc += (uint64_t)u[j] * M.m[i - j];
+        for (int j = 0; j < i; j++) red += (uint64_t)u[j] * M.m[i - j];
+        acc += red;
         u[i] = ((uint32_t)acc * M.minv) & kRadixMask;
         acc += (uint64_t)u[i] * M.m[0];
         acc >>= kRadixBits;
     }
 #pragma unroll
     for (int i = N; i < 2 * N - 1; i++) {
+        uint64_t red = 0;
 #pragma unroll
         for (int j = i - N + 1; j < N; j++) {
             acc += (uint64_t)x[j] * y[i - j];
-            acc += (uint64_t)u[j] * M.m[i - j];
+            red += (uint64_t)u[j] * M.m[i - j];
         }
+        acc += red;
         o[i - N] = (uint32_t)acc & kRadixMask;
         acc >>= kRadixBits;
     }
@@ -335,19 +341,22 @@ FK_HD void mont_sqr26(uint32_t* r, const uint32_t* x, const Mod26<N>& M)
     uint64_t acc = 0;
 #pragma unroll
     for (int i = 0; i < 2 * N - 1; i++) {
+        uint64_t red = 0;
 #pragma unroll
         for (int j = (i < N ? 0 : i - N + 1); 2 * j < i; j++)
             acc += (uint64_t)x2[j] * x[i - j];
         if ((i & 1) == 0) acc += (uint64_t)x[i >> 1] * x[i >> 1];
         if (i < N) {
 #pragma unroll
-            for (int j = 0; j < i; j++) acc += (uint64_t)u[j] * M.m[i - j];
+            for (int j = 0; j < i; j++) red += (uint64_t)u[j] * M.m[i - j];
+            acc += red;
             u[i] = ((uint32_t)acc * M.minv) & kRadixMask;
             acc += (uint64_t)u[i] * M.m[0];
         } else {
 #pragma unroll
             for (int j = i - N + 1; j < N; j++)
-                acc += (uint64_t)u[j] * M.m[i - j];
+                red += (uint64_t)u[j] * M.m[i - j];
+            acc += red;
             o[i - N] = (uint32_t)acc & kRadixMask;
         }
         acc >>= kRadixBits;
@@ -501,10 +510,21 @@ FK_HD bool is_sprp(const uint32_t* v8, int first, int nbases)
 #pragma unroll
     for (int i = 1; i < 8; i++) if (v8[i] != 0) top = i;
     if (top == 0 && v8[0] < 2) return false;
+    // a factor among the twelve small primes: then m is prime only if it is
+    // that prime.  The residues come from two 8-limb Horner passes, modulo
+    // 3*5*...*29 (below 2^32) and 31*37, and 2 from the low bit.
+    const uint32_t r1 = mw_mod_small<8>(v8, 3234846615u);
+    const uint32_t r2 = mw_mod_small<8>(v8, 1147u);
+    bool factor = (v8[0] & 1u) == 0;
 #pragma unroll
-    for (int i = 0; i < kNumSmall; i++) {
-        const uint32_t sp = small_prime(i);
-        if (mw_mod_small<8>(v8, sp) == 0) return top == 0 && v8[0] == sp;
+    for (int i = 1; i < 10; i++) factor = factor || r1 % small_prime(i) == 0;
+    factor = factor || r2 % 31u == 0 || r2 % 37u == 0;
+    if (factor) {
+        bool is_small = false;
+#pragma unroll
+        for (int i = 0; i < kNumSmall; i++)
+            is_small = is_small || (top == 0 && v8[0] == small_prime(i));
+        return is_small;
     }
     int k = 0;                            // bit length, no dynamic index
 #pragma unroll

@@ -530,6 +530,12 @@ struct fk_batch {
     bool pending = false;
     bool have_results = false;
     // device buffers (grow-only)
+    // the per-submission descriptors live in one device area, uploaded in
+    // one copy from a pinned staging area of the same layout
+    uint8_t* d_up = nullptr;
+    uint8_t* h_up = nullptr;
+    size_t up_bytes = 0;
+    size_t o_engines = 0, o_pieces = 0, o_win_engine = 0, o_wins = 0, o_segs = 0;
     DevEngine* d_engines = nullptr;   size_t cap_engines = 0;
     DevPiece* d_pieces = nullptr;     size_t cap_pieces = 0;
     uint32_t* d_win_engine = nullptr; size_t cap_win = 0;
@@ -576,10 +582,10 @@ static int batch_free(fk_batch* b)
     b->graph_exec = nullptr;
     b->graph = nullptr;
     (void)hipSetDevice(b->device);
-    void* ds[] = {b->d_engines, b->d_pieces, b->d_win_engine, b->d_wins, b->d_segs,
-                  b->d_bq, b->d_q[0], b->d_q[1], b->d_flags, b->d_fail2,
-                  b->d_failx, b->d_counts};
+    void* ds[] = {b->d_up, b->d_bq, b->d_q[0], b->d_q[1], b->d_flags,
+                  b->d_fail2, b->d_failx, b->d_counts};
     for (void* p : ds) if (p) (void)hipFree(p);
+    if (b->h_up) (void)hipHostFree(b->h_up);
     if (b->h_counts) (void)hipHostFree(b->h_counts);
     if (b->h_recs) (void)hipHostFree(b->h_recs);
     for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
@@ -597,6 +603,47 @@ static int grow(T** p, size_t* cap, size_t need)
     *p = nullptr;
     HIPCHK(hipMalloc(p, nc * sizeof(T)));
     *cap = nc;
+    return FK_OK;
+}
+
+// Lay out the descriptor areas by grow-only capacities, so the device
+// pointers (and a captured graph's kernel arguments) stay put while the
+// submissions fit.
+static int grow_uploads(fk_batch* b, size_t neng, size_t npieces,
+                        size_t nwin, size_t nseg)
+{
+    auto need = [](size_t cap, size_t n) {
+        return n > cap ? std::max<size_t>(std::max<size_t>(n, cap * 3 / 2), 16)
+                       : cap;
+    };
+    const size_t ce = need(b->cap_engines, neng), cp = need(b->cap_pieces, npieces),
+                 cw = need(b->cap_win, nwin), cs = need(b->cap_segs, nseg);
+    if (b->d_up && ce == b->cap_engines && cp == b->cap_pieces &&
+        cw == b->cap_win && cs == b->cap_segs)
+        return FK_OK;
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t off = 0;
+    b->o_engines = off;    off = align(off + ce * sizeof(DevEngine));
+    b->o_pieces = off;     off = align(off + cp * sizeof(DevPiece));
+    b->o_win_engine = off; off = align(off + cw * sizeof(uint32_t));
+    b->o_wins = off;       off = align(off + cw * sizeof(DevWinInfo));
+    b->o_segs = off;       off = align(off + cs * sizeof(uint2));
+    if (b->d_up) (void)hipFree(b->d_up);
+    if (b->h_up) (void)hipHostFree(b->h_up);
+    b->d_up = nullptr;
+    b->h_up = nullptr;
+    HIPCHK(hipMalloc(&b->d_up, off));
+    HIPCHK(hipHostMalloc(&b->h_up, off));
+    b->up_bytes = off;
+    b->d_engines = reinterpret_cast<DevEngine*>(b->d_up + b->o_engines);
+    b->d_pieces = reinterpret_cast<DevPiece*>(b->d_up + b->o_pieces);
+    b->d_win_engine = reinterpret_cast<uint32_t*>(b->d_up + b->o_win_engine);
+    b->d_wins = reinterpret_cast<DevWinInfo*>(b->d_up + b->o_wins);
+    b->d_segs = reinterpret_cast<uint2*>(b->d_up + b->o_segs);
+    b->cap_engines = ce;
+    b->cap_pieces = cp;
+    b->cap_win = cw;
+    b->cap_segs = cs;
     return FK_OK;
 }
 
@@ -788,17 +835,10 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     const size_t ncounts = nstage * ne + 1;
 
     // ---- buffers
-    if ((rc = grow(&b->d_engines, &b->cap_engines, b->engines.size()))) return rc;
-    if ((rc = grow(&b->d_pieces, &b->cap_pieces, std::max<size_t>(pieces.size(), 1)))) return rc;
-    if (nwin > b->cap_win || !b->d_win_engine) {
-        size_t c = 0;
-        if ((rc = grow(&b->d_win_engine, &c, nwin))) return rc;
-        size_t c2 = 0;
-        if ((rc = grow(&b->d_wins, &c2, c))) return rc;
-        b->cap_win = c;
-    }
+    if ((rc = grow_uploads(b, b->engines.size(), std::max<size_t>(pieces.size(), 1),
+                           nwin, ne)))
+        return rc;
     if ((rc = grow(&b->d_bq, &b->cap_bq, std::max<uint64_t>(bq_total, 1)))) return rc;
-    if ((rc = grow(&b->d_segs, &b->cap_segs, ne))) return rc;
     if (queue_cap > b->cap_q || !b->d_q[0]) {
         size_t c = 0;
         for (int i = 0; i < 2; i++) {
@@ -829,25 +869,22 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         b->cap_counts = c;
     }
 
-    // ---- uploads (small)
-    std::vector<DevEngine> des;
-    for (fk_engine* e : b->engines) des.push_back(e->host);
+    // ---- uploads (small): staged in pinned memory, one asynchronous copy.
+    // The staging area is rewritten only by the next submission, which
+    // starts after this one's work (and so this copy) has been waited for.
     hipStream_t s = b->stream;
-    HIPCHK(hipMemcpyAsync(b->d_engines, des.data(), des.size() * sizeof(DevEngine),
-                          hipMemcpyHostToDevice, s));
-    if (!pieces.empty())
-        HIPCHK(hipMemcpyAsync(b->d_pieces, pieces.data(),
-                              pieces.size() * sizeof(DevPiece),
-                              hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(b->d_win_engine, win_engine.data(),
-                          nwin * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(b->d_wins, wins.data(), nwin * sizeof(DevWinInfo),
-                          hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(b->d_segs, segs.data(), ne * sizeof(uint2),
-                          hipMemcpyHostToDevice, s));
-    // the host vectors must outlive the async copies: wait for them here
-    // (they are small; the device work below is what stays asynchronous)
-    HIPCHK(hipStreamSynchronize(s));
+    {
+        uint8_t* h = b->h_up;
+        DevEngine* he = reinterpret_cast<DevEngine*>(h + b->o_engines);
+        for (size_t k = 0; k < b->engines.size(); k++) he[k] = b->engines[k]->host;
+        if (!pieces.empty())
+            std::memcpy(h + b->o_pieces, pieces.data(), pieces.size() * sizeof(DevPiece));
+        std::memcpy(h + b->o_win_engine, win_engine.data(), nwin * sizeof(uint32_t));
+        std::memcpy(h + b->o_wins, wins.data(), nwin * sizeof(DevWinInfo));
+        std::memcpy(h + b->o_segs, segs.data(), ne * sizeof(uint2));
+        const size_t used = b->o_segs + ne * sizeof(uint2);
+        HIPCHK(hipMemcpyAsync(b->d_up, h, used, hipMemcpyHostToDevice, s));
+    }
 
     std::vector<Step> plan;
     auto mark = [&](int i) {

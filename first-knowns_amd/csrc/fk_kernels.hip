@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <mutex>
+#include <vector>
 
 #include "fk_device.h"
 #include "fk_kill.h"
@@ -1032,15 +1034,39 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n)
     X(512, 1) X(512, 2) X(512, 4) X(512, 8)                                  \
     X(1024, 1) X(1024, 2) X(1024, 4)
 
+// Dynamic LDS above 64 KiB must be allowed per kernel (and device) before
+// a launch.  The call costs tens of microseconds of host time, so it is
+// made once per kernel, device and larger size, not per launch.
+static void allow_lds(const void* fn, size_t lds_bytes)
+{
+    if (lds_bytes <= 65536) return;
+    struct Allowed { const void* fn; int device; size_t bytes; };
+    static std::mutex mu;
+    static std::vector<Allowed> done;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lock(mu);
+    for (Allowed& a : done) {
+        if (a.fn == fn && a.device == dev) {
+            if (a.bytes >= lds_bytes) return;
+            if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds_bytes) == hipSuccess)
+                a.bytes = lds_bytes;
+            return;
+        }
+    }
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes) == hipSuccess)
+        done.push_back({fn, dev, lds_bytes});
+}
+
 hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,
                         uint32_t grid, size_t lds_bytes, hipStream_t stream)
 {
 #define FK_SIEVE_CASE(TT, WW)                                              \
     if (threads == TT && wpt == WW) {                                      \
         auto fn = k_sieve<TT, WW>;                                         \
-        if (lds_bytes > 65536)                                             \
-            (void)hipFuncSetAttribute((const void*)fn,                     \
-                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes); \
+        allow_lds((const void*)fn, lds_bytes);                             \
         hipLaunchKernelGGL(fn, dim3(grid), dim3(TT), lds_bytes, stream, a); \
         return hipGetLastError();                                          \
     }
@@ -1055,9 +1081,7 @@ int sieve_occupancy(int threads, int wpt, size_t lds_bytes)
 #define FK_OCC_CASE(TT, WW)                                                 \
     if (threads == TT && wpt == WW) {                                       \
         auto fn = k_sieve<TT, WW>;                                          \
-        if (lds_bytes > 65536)                                              \
-            (void)hipFuncSetAttribute((const void*)fn,                      \
-                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);  \
+        allow_lds((const void*)fn, lds_bytes);                              \
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                   \
                 &blocks, fn, TT, lds_bytes) != hipSuccess) blocks = 0;      \
         return blocks;                                                      \
