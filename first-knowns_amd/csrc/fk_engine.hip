@@ -419,9 +419,10 @@ static int build_engine(fk_engine* e)
     {
         // kill sets of the round primes, built on the device
         const uint32_t np = e->round_end - e->round_first;
-        const size_t nt = std::max<size_t>((size_t)np * h.kstride, 1);
+        const size_t nt = (size_t)np * h.kstride + kRowTail;
         uint32_t* kt = nullptr;
         HIPCHK(hipMalloc(&kt, nt * sizeof(uint32_t)));
+        HIPCHK(hipMemset(kt, 0xff, nt * sizeof(uint32_t)));   // kKillPad
         e->allocs.push_back(kt);
         e->device_bytes += nt * sizeof(uint32_t);
         HIPCHK(launch_kill_table(ctx->d_primes, e->round_first, np, d.family,
@@ -979,6 +980,15 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         ra.round = r;
         ra.p_begin = P0;
         ra.p_end = P1;
+        // residue groups per row held in registers: three when every
+        // engine's kill sets fit twelve residues (both rows of a pass then
+        // load together), else five
+        uint32_t kg = 1;
+        for (size_t k = 0; k < b->engines.size(); k++)
+            if (used[k])
+                kg = std::max(kg, (kill_max(b->engines[k]->host.family,
+                                            b->engines[k]->host.n) + 3u) / 4u);
+        ra.kgroups = kg <= 3 ? 3u : 5u;
         const uint32_t np = P1 - P0;
         // Many primes and enough candidates: lane = prime (256 primes per
         // workgroup column, the queue cut into slices of about 256

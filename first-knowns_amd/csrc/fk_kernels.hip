@@ -504,17 +504,20 @@ __device__ __forceinline__ uint64_t match4(uint32_t r, uint4 v)
 // ---- lane = candidate (many candidates, few primes): the prime is
 // wave-uniform, its row is read with scalar loads into SGPRs.
 
-// One round prime's row as scalar values: the header and the first 20
-// killing residues, 32 words in all.
+// One round prime's row as scalar values: the header and the first 4 KG
+// killing residues (12 + 4 KG words; a table carries kRowTail words past its
+// last row, so reading a fixed size never leaves it).
+template <int KG>
 struct RowView {
     double qd, qi, c16, c32, c48;
     uint32_t q, cnt;
-    uint4 k[5];
+    uint4 k[KG];
 };
 
-__device__ __forceinline__ RowView load_row(const FK_CONST uint32_t* row)
+template <int KG>
+__device__ __forceinline__ RowView<KG> load_row(const FK_CONST uint32_t* row)
 {
-    RowView v;
+    RowView<KG> v;
     v.qd = load_f64(row + 0);
     v.qi = load_f64(row + 2);
     v.c16 = load_f64(row + 4);
@@ -523,15 +526,15 @@ __device__ __forceinline__ RowView load_row(const FK_CONST uint32_t* row)
     v.q = row[10];
     v.cnt = row[11];
 #pragma unroll
-    for (int i = 0; i < 5; i++) v.k[i] = load4(row + kRowHead + 4 * i);
+    for (int i = 0; i < KG; i++) v.k[i] = load4(row + kRowHead + 4 * i);
     return v;
 }
 
 // The lane's candidate residue mod the row's prime from its 16-bit limbs;
 // limbs past the fourth take 2^64..2^112 mod q from the prime table.
-template <int NL>
+template <int NL, int KG>
 __device__ __forceinline__ uint32_t row_residue(const double (&a)[8],
-                                                const RowView& v,
+                                                const RowView<KG>& v,
                                                 const FK_CONST PrimeD* P)
 {
     double s = fma(a[3], v.c48, fma(a[2], v.c32, fma(a[1], v.c16, a[0])));
@@ -543,27 +546,27 @@ __device__ __forceinline__ uint32_t row_residue(const double (&a)[8],
 }
 
 // Lanes whose residue matches the row's kill set: only the four-residue
-// groups the row's count reaches (uniform branches), the first 20 residues
-// from registers, longer rows in a loop.  Folded rows (Dickson) take
-// min(r, q - r).
-__device__ __forceinline__ uint64_t row_match(uint32_t r, const RowView& v,
+// groups the row's count reaches (uniform branches), the first 4 KG
+// residues from registers, longer rows in a loop.  Folded rows (Dickson)
+// take min(r, q - r).
+template <int KG>
+__device__ __forceinline__ uint64_t row_match(uint32_t r, const RowView<KG>& v,
                                               const FK_CONST uint32_t* row,
                                               bool fold)
 {
     if (fold) r = min(r, v.q - r);
     uint64_t hit = match4(r, v.k[0]);
-    if (v.cnt > 4) hit |= match4(r, v.k[1]);
-    if (v.cnt > 8) hit |= match4(r, v.k[2]);
-    if (v.cnt > 12) hit |= match4(r, v.k[3]);
-    if (v.cnt > 16) hit |= match4(r, v.k[4]);
-    if (v.cnt > 20) {
-        for (uint32_t c = 20; c < v.cnt; c += 4)
+#pragma unroll
+    for (int g = 1; g < KG; g++)
+        if (v.cnt > 4u * g) hit |= match4(r, v.k[g]);
+    if (v.cnt > 4u * KG) {
+        for (uint32_t c = 4u * KG; c < v.cnt; c += 4)
             hit |= match4(r, load4(row + kRowHead + c));
     }
     return hit;
 }
 
-template <int NL>
+template <int NL, int KG>
 __device__ __forceinline__ uint64_t round_scan(const double (&a)[8],
                                                const FK_CONST PrimeD* pd,
                                                const FK_CONST uint32_t* kt,
@@ -577,15 +580,26 @@ __device__ __forceinline__ uint64_t round_scan(const double (&a)[8],
     for (; p + 1 < hi; p += 2) {
         const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * stride;
         const FK_CONST uint32_t* r1 = r0 + stride;
-        const RowView v0 = load_row(r0);
-        const RowView v1 = load_row(r1);
+        const RowView<KG> v0 = load_row<KG>(r0);
+        const RowView<KG> v1 = load_row<KG>(r1);
+        if (KG == 3) {
+            // both rows in scalar registers before any use: one wait per
+            // pass instead of one per residue group (the compiler would
+            // otherwise sink each group's load into its count branch)
+            asm volatile("" ::"s"(v0.k[1].x), "s"(v0.k[1].y), "s"(v0.k[1].z),
+                         "s"(v0.k[1].w), "s"(v0.k[2].x), "s"(v0.k[2].y),
+                         "s"(v0.k[2].z), "s"(v0.k[2].w), "s"(v1.q), "s"(v1.cnt),
+                         "s"(v1.k[1].x), "s"(v1.k[1].y), "s"(v1.k[1].z),
+                         "s"(v1.k[1].w), "s"(v1.k[2].x), "s"(v1.k[2].y),
+                         "s"(v1.k[2].z), "s"(v1.k[2].w));
+        }
         const uint32_t x0 = row_residue<NL>(a, v0, pd + p);
         const uint32_t x1 = row_residue<NL>(a, v1, pd + p + 1);
         hit |= row_match(x0, v0, r0, fold) | row_match(x1, v1, r1, fold);
     }
     if (p < hi) {
         const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * stride;
-        const RowView v0 = load_row(r0);
+        const RowView<KG> v0 = load_row<KG>(r0);
         hit |= row_match(row_residue<NL>(a, v0, pd + p), v0, r0, fold);
     }
     return hit;
@@ -595,6 +609,7 @@ __device__ __forceinline__ uint64_t round_scan(const double (&a)[8],
 // blocks of 256 candidates (grid-stride), blockIdx.y takes a block of
 // pblock primes.  The engine is uniform per block, so the prime-side data
 // is too.  A killed candidate's flag is set.
+template <int KG>
 __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
 {
     const uint32_t pb0 = a.p_begin + blockIdx.y * a.pblock;
@@ -647,8 +662,8 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
         const uint32_t rf = E->round_first;
         const bool fold = kill_folded(E->family);
         const uint64_t m =
-            wide ? round_scan<8>(l, pd, kt, E->kstride, rf, lo, hi, fold)
-                 : round_scan<4>(l, pd, kt, E->kstride, rf, lo, hi, fold);
+            wide ? round_scan<8, KG>(l, pd, kt, E->kstride, rf, lo, hi, fold)
+                 : round_scan<4, KG>(l, pd, kt, E->kstride, rf, lo, hi, fold);
         if (live && ((m >> __lane_id()) & 1)) a.flags[i] = 1;
         touch ^= tv;
     }
@@ -672,7 +687,7 @@ struct LaneRow {
 };
 
 __device__ __forceinline__ void load_lane_row(LaneRow& L, const uint32_t* row,
-                                              const PrimeD* pd)
+                                              const PrimeD* pd, uint32_t stride)
 {
     RoundRow h;
     const uint4* r4 = reinterpret_cast<const uint4*>(row);
@@ -690,7 +705,10 @@ __device__ __forceinline__ void load_lane_row(LaneRow& L, const uint32_t* row,
     L.row = row;
 #pragma unroll
     for (int i = 0; i < kRegResidues / 4; i++) {
-        const uint4 v = (4 * i + 4 <= 20) ? r4[3 + i] : make_uint4(kKillPad, kKillPad, kKillPad, kKillPad);
+        // residue groups inside the row only (rows are 16..32 words here)
+        const uint4 v = (4 * i + 4 <= 20 && kRowHead + 4u * i + 4u <= stride)
+                            ? r4[3 + i]
+                            : make_uint4(kKillPad, kKillPad, kKillPad, kKillPad);
         L.k[4 * i] = v.x;
         L.k[4 * i + 1] = v.y;
         L.k[4 * i + 2] = v.z;
@@ -775,7 +793,7 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
         const bool fold = kill_folded(engs[e].family);
         if (mine) {
             load_lane_row(L, engs[e].kt + (size_t)(p - engs[e].round_first) * stride,
-                          a.primed + p);
+                          a.primed + p, stride);
             if (ncmp > 20) load_lane_row_wide(L, stride - kRowHead);
         }
         const uint32_t base = segs[e].x;
@@ -1125,7 +1143,12 @@ hipError_t launch_round_test(const RoundArgs& a, bool lane_prime, uint32_t gx,
         // every prime block must be launched
         if (a.pblock == 0 || (uint64_t)gy * a.pblock < np)
             return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_round_c, dim3(gx, gy), dim3(256), 0, stream, a);
+        if (a.kgroups == 3)
+            hipLaunchKernelGGL(k_round_c<3>, dim3(gx, gy), dim3(256), 0, stream, a);
+        else if (a.kgroups == 5)
+            hipLaunchKernelGGL(k_round_c<5>, dim3(gx, gy), dim3(256), 0, stream, a);
+        else
+            return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -135,13 +135,19 @@ struct RoundRow {
 constexpr uint32_t kRowHead = 12;           // words of RoundRow
 static_assert(sizeof(RoundRow) == 4 * kRowHead, "RoundRow layout");
 
-// Row stride in words: the header plus the largest kill set, rounded up to
-// whole 64-byte loads, at least 32 words (20 residues).
+// Row stride in words: the header plus the largest kill set in whole groups
+// of four residues (the round kernels compare four at a time), and whole
+// 64-byte lines past 32 words.  Dickson n <= 12 rows are 24 words, so the
+// two rows of a lane = candidate pass fit the scalar registers together.
 FK_HD uint32_t round_row_stride(int family, int n)
 {
-    const uint32_t w = kRowHead + kill_max(family, n);
-    return w <= 32 ? 32u : (w + 15u) & ~15u;
+    const uint32_t w = kRowHead + ((kill_max(family, n) + 3u) & ~3u);
+    return w <= 32 ? w : (w + 15u) & ~15u;
 }
+
+// Words a round table carries past its last row: the kernels read rows in
+// a fixed size (up to 32 words) and ignore words past the row's count.
+constexpr uint32_t kRowTail = 32;
 
 FK_HD void round_row(int family, int n, const uint32_t d[4], const PrimeK& p,
                      uint32_t* row, uint32_t stride)

@@ -47,6 +47,9 @@ struct RoundArgs {
     uint32_t p_begin, p_end;  // global prime indices: the union of the
                               // engines' round-`round` primes
     uint32_t pblock;          // lane = candidate: primes per blockIdx.y
+    uint32_t kgroups;         // lane = candidate: groups of four killing
+                              // residues held in registers per row (3 or 5;
+                              // rows with more are finished in a loop)
 };
 
 struct CompactArgs {
