@@ -216,64 +216,66 @@ FK_HD uint32_t neg_inv32(uint32_t a)
     return 0u - x;
 }
 
-// ------------------------------------------ radix-2^26 Montgomery residues
-// A residue modulo an odd m of bit length k is held in N = ceil((k + 2) / 26)
-// limbs of 26 bits (uint32_t each, little-endian), so R = 2^(26 N) > 4 m.
-// Every limb product is below 2^53, so one column of the schoolbook product
-// plus the reduction terms (at most 2 N - 1 < 20 of each) and the carry in
-// stay far below 2^64: a column is summed in one 64-bit accumulator with one
-// v_mad_u64_u32 per product and no carry handling, and the carry out is one
+// ------------------------------------------ radix-2^29 Montgomery residues
+// A residue modulo an odd m of bit length k is held in N = ceil((k + 2) / 29)
+// limbs of 29 bits (uint32_t each, little-endian), so R = 2^(29 N) > 4 m;
+// N <= 9 covers 256-bit values.  Every limb product is below 2^59 (2^58
+// for two limbs, 2^59 against a doubled limb in the square), and a column
+// holds at most N products and N - 1 reduction terms: below 2^62.2 with the
+// carry in.  So a column is summed in 64-bit accumulators with one
+// v_mad_u64_u32 per term and no carry handling, and the carry out is one
 // shift.  Residues are kept lazily, below 2 m rather than m, inside the
-// exponentiation; reduce26 makes them exact where they are compared.
+// exponentiation; reduce_w makes them exact where they are compared.
 
-constexpr int kRadixBits = 26;
-constexpr uint32_t kRadixMask = (1u << kRadixBits) - 1;
-constexpr int kMaxRadixLimbs = 10;        // 26 * 10 >= 256 + 2
+constexpr int kLimbBits = 29;
+constexpr uint32_t kLimbMask = (1u << kLimbBits) - 1;
+constexpr int kMaxLimbs = 9;                 // 29 * 9 >= 256 + 2
 
 template <int N>
-struct Mod26 {
+struct ModW {
     uint32_t m[N];
-    uint32_t minv;       // -m^-1 mod 2^26
-    double inv_top;      // 1 / top26<N>(m)
+    uint32_t minv;       // -m^-1 mod 2^29
+    double inv_top;      // 1 / top_w<N>(m)
 };
 
 // The top min(N, 4) limbs of x as a double (the same scale for every x of
-// N limbs).  m's top four limbs hold at least 76 of its bits, so ratios of
+// N limbs).  m's top four limbs hold at least 85 of its bits, so ratios of
 // these approximate ratios of the full values to about 2^-50.
 template <int N>
-FK_HD double top26(const uint32_t* x)
+FK_HD double top_w(const uint32_t* x)
 {
     constexpr int T = N < 4 ? N : 4;
     double r = 0.0;
 #pragma unroll
-    for (int i = N - 1; i >= N - T; i--) r = r * 67108864.0 + (double)x[i];
+    for (int i = N - 1; i >= N - T; i--)
+        r = r * (double)(1u << kLimbBits) + (double)x[i];
     return r;
 }
 
-// 26-bit limbs of the value in eight 32-bit words (bits beyond word 7 are 0)
+// 29-bit limbs of the value in eight 32-bit words (bits past word 7 are 0)
 template <int N>
-FK_HD void to26(uint32_t* r, const uint32_t* v8)
+FK_HD void to_w(uint32_t* r, const uint32_t* v8)
 {
 #pragma unroll
     for (int i = 0; i < N; i++) {
-        const int bit = kRadixBits * i, w = bit >> 5, sh = bit & 31;
+        const int bit = kLimbBits * i, w = bit >> 5, sh = bit & 31;
         const uint64_t lo = w < 8 ? v8[w] : 0u;
         const uint64_t hi = w + 1 < 8 ? v8[w + 1] : 0u;
-        r[i] = (uint32_t)((lo | (hi << 32)) >> sh) & kRadixMask;
+        r[i] = (uint32_t)((lo | (hi << 32)) >> sh) & kLimbMask;
     }
 }
 
 // x = x - m when x >= m (x < 2m on entry, exact residue on exit)
 template <int N>
-FK_HD void reduce26(uint32_t* x, const uint32_t* m)
+FK_HD void reduce_w(uint32_t* x, const uint32_t* m)
 {
     uint32_t d[N];
     int32_t borrow = 0;
 #pragma unroll
     for (int i = 0; i < N; i++) {
         const int32_t t = (int32_t)x[i] - (int32_t)m[i] + borrow;
-        d[i] = (uint32_t)t & kRadixMask;
-        borrow = t >> kRadixBits;            // 0 or -1
+        d[i] = (uint32_t)t & kLimbMask;
+        borrow = t >> kLimbBits;             // 0 or -1
     }
     const bool ge = borrow == 0;
 #pragma unroll
@@ -281,7 +283,7 @@ FK_HD void reduce26(uint32_t* x, const uint32_t* m)
 }
 
 template <int N>
-FK_HD bool eq26(const uint32_t* a, const uint32_t* b)
+FK_HD bool eq_w(const uint32_t* a, const uint32_t* b)
 {
     uint32_t o = 0;
 #pragma unroll
@@ -289,16 +291,15 @@ FK_HD bool eq26(const uint32_t* a, const uint32_t* b)
     return o == 0;
 }
 
-// r = x y R^-1 mod m, up to one m (r < 2m), for x, y < 2m (limbs < 2^26).
+// r = x y R^-1 mod m, up to one m (r < 2m), for x, y < 2m (limbs < 2^29).
 // Product scanning with the reduction interleaved column by column: u_i
 // clears column i, and the top N columns are the result (R > 4m bounds it
-// by (4 m^2 + R m) / R < 2 m).
+// by (4 m^2 + R m) / R < 2 m).  Two accumulators per column, the products
+// and the reduction terms, so the two multiply-add chains run side by side.
 template <int N>
-FK_HD void mont_mul26(uint32_t* r, const uint32_t* x, const uint32_t* y,
-                      const Mod26<N>& M)
+FK_HD void mont_mul_w(uint32_t* r, const uint32_t* x, const uint32_t* y,
+                      const ModW<N>& M)
 {
-    // two accumulators per column, the products and the reduction terms,
-    // so the two multiply-add chains run side by side
     uint32_t u[N], o[N];
     uint64_t acc = 0;
 #pragma unroll
@@ -309,9 +310,9 @@ FK_HD void mont_mul26(uint32_t* r, const uint32_t* x, const uint32_t* y,
 #pragma unroll
         for (int j = 0; j < i; j++) red += (uint64_t)u[j] * M.m[i - j];
         acc += red;
-        u[i] = ((uint32_t)acc * M.minv) & kRadixMask;
+        u[i] = ((uint32_t)acc * M.minv) & kLimbMask;
         acc += (uint64_t)u[i] * M.m[0];
-        acc >>= kRadixBits;
+        acc >>= kLimbBits;
     }
 #pragma unroll
     for (int i = N; i < 2 * N - 1; i++) {
@@ -322,18 +323,18 @@ FK_HD void mont_mul26(uint32_t* r, const uint32_t* x, const uint32_t* y,
             red += (uint64_t)u[j] * M.m[i - j];
         }
         acc += red;
-        o[i - N] = (uint32_t)acc & kRadixMask;
-        acc >>= kRadixBits;
+        o[i - N] = (uint32_t)acc & kLimbMask;
+        acc >>= kLimbBits;
     }
     o[N - 1] = (uint32_t)acc;
 #pragma unroll
     for (int i = 0; i < N; i++) r[i] = o[i];
 }
 
-// r = x^2 R^-1 mod m, up to one m, for x < 2m: mont_mul26 with each cross
+// r = x^2 R^-1 mod m, up to one m, for x < 2m: mont_mul_w with each cross
 // product x_j x_k (j < k) formed once against 2 x_j.
 template <int N>
-FK_HD void mont_sqr26(uint32_t* r, const uint32_t* x, const Mod26<N>& M)
+FK_HD void mont_sqr_w(uint32_t* r, const uint32_t* x, const ModW<N>& M)
 {
     uint32_t u[N], o[N], x2[N];
 #pragma unroll
@@ -350,16 +351,16 @@ FK_HD void mont_sqr26(uint32_t* r, const uint32_t* x, const Mod26<N>& M)
 #pragma unroll
             for (int j = 0; j < i; j++) red += (uint64_t)u[j] * M.m[i - j];
             acc += red;
-            u[i] = ((uint32_t)acc * M.minv) & kRadixMask;
+            u[i] = ((uint32_t)acc * M.minv) & kLimbMask;
             acc += (uint64_t)u[i] * M.m[0];
         } else {
 #pragma unroll
             for (int j = i - N + 1; j < N; j++)
                 red += (uint64_t)u[j] * M.m[i - j];
             acc += red;
-            o[i - N] = (uint32_t)acc & kRadixMask;
+            o[i - N] = (uint32_t)acc & kLimbMask;
         }
-        acc >>= kRadixBits;
+        acc >>= kLimbBits;
     }
     o[N - 1] = (uint32_t)acc;
 #pragma unroll
@@ -369,20 +370,21 @@ FK_HD void mont_sqr26(uint32_t* r, const uint32_t* x, const Mod26<N>& M)
 // r = a x mod m, up to one m (r < 2m), for x < 2m and a < 2^31.  The
 // quotient q of a x by m is below 2^32; the double estimate, scaled down by
 // 1 - 2^-40, is within 2^-8 + 2^-17 below a x / m, so its floor is q or
-// q - 1 and a x - floor * m lies in [0, 2m).  One signed pass per limb.
+// q - 1 and a x - floor * m lies in [0, 2m).  One signed pass per limb
+// (every term below 2^61 in magnitude).
 template <int N>
-FK_HD void mul_small26(uint32_t* r, const uint32_t* x, uint32_t a,
-                       const Mod26<N>& M)
+FK_HD void mul_small_w(uint32_t* r, const uint32_t* x, uint32_t a,
+                       const ModW<N>& M)
 {
-    const double qf = (double)a * top26<N>(x) * M.inv_top * (1.0 - 0x1p-40);
+    const double qf = (double)a * top_w<N>(x) * M.inv_top * (1.0 - 0x1p-40);
     const uint32_t q = (uint32_t)qf;
     int64_t c = 0;
 #pragma unroll
     for (int j = 0; j < N; j++) {
         const int64_t v = (int64_t)((uint64_t)a * x[j])
                         - (int64_t)((uint64_t)q * M.m[j]) + c;
-        r[j] = (uint32_t)v & kRadixMask;
-        c = v >> kRadixBits;
+        r[j] = (uint32_t)v & kLimbMask;
+        c = v >> kLimbBits;
     }
 }
 
@@ -409,20 +411,20 @@ FK_HD void mw_shr(uint32_t* a, int bits)
 }
 
 // Strong probable-prime test of odd m > 37 with no factor <= 37, of bit
-// length k <= 26 N - 2, to the bases first..first+nbases-1 of the seven.
+// length k <= 29 N - 2, to the bases first..first+nbases-1 of the seven.
 // huntlib/primes.py:31-47 (the loop over bases) and :61-80 (base 2 alone)
-// are the two uses.  In the Montgomery domain (R = 2^(26 N)) the ladder
+// are the two uses.  In the Montgomery domain (R = 2^(29 N)) the ladder
 // starts from one = R mod m and, per bit of d from the top, squares and
 // then multiplies by the bit's factor (the base or 1) as a plain scalar:
 // one instruction stream for every base, with no table and no branch.
 template <int N>
-FK_HD bool sprp26(const uint32_t* v8, int first, int nbases)
+FK_HD bool sprp_w(const uint32_t* v8, int first, int nbases)
 {
-    Mod26<N> M;
-    to26<N>(M.m, v8);
-    M.minv = neg_inv32(v8[0]) & kRadixMask;
-    M.inv_top = 1.0 / top26<N>(M.m);
-    // one = 2^(26 N) mod m: 2^(k-1) < m, times 2^(26 N - k + 1) in factors
+    ModW<N> M;
+    to_w<N>(M.m, v8);
+    M.minv = neg_inv32(v8[0]) & kLimbMask;
+    M.inv_top = 1.0 / top_w<N>(M.m);
+    // one = 2^(29 N) mod m: 2^(k-1) < m, times 2^(29 N - k + 1) in factors
     // of at most 2^28 (one factor unless N was raised to its minimum of 2)
     int k = 0;
 #pragma unroll
@@ -432,19 +434,19 @@ FK_HD bool sprp26(const uint32_t* v8, int first, int nbases)
     uint32_t one[N], mone[N];
 #pragma unroll
     for (int i = 0; i < N; i++)
-        one[i] = (i == tb / kRadixBits) ? 1u << (tb % kRadixBits) : 0u;
-    for (int sh = kRadixBits * N - tb; sh > 0; sh -= 28)
-        mul_small26<N>(one, one, 1u << (sh < 28 ? sh : 28), M);
-    reduce26<N>(one, M.m);
+        one[i] = (i == tb / kLimbBits) ? 1u << (tb % kLimbBits) : 0u;
+    for (int sh = kLimbBits * N - tb; sh > 0; sh -= 28)
+        mul_small_w<N>(one, one, 1u << (sh < 28 ? sh : 28), M);
+    reduce_w<N>(one, M.m);
     int32_t borrow = 0;                   // mone = m - one = (m - 1) R mod m
 #pragma unroll
     for (int i = 0; i < N; i++) {
         const int32_t t = (int32_t)M.m[i] - (int32_t)one[i] + borrow;
-        mone[i] = (uint32_t)t & kRadixMask;
-        borrow = t >> kRadixBits;
+        mone[i] = (uint32_t)t & kLimbMask;
+        borrow = t >> kLimbBits;
     }
     // m - 1 = d 2^s
-    constexpr int W = (kRadixBits * N + 31) / 32 < 8 ? (kRadixBits * N + 31) / 32 : 8;
+    constexpr int W = (kLimbBits * N + 31) / 32 < 8 ? (kLimbBits * N + 31) / 32 : 8;
     uint32_t d[W];
 #pragma unroll
     for (int i = 0; i < W; i++) d[i] = v8[i];
@@ -478,20 +480,20 @@ FK_HD bool sprp26(const uint32_t* v8, int first, int nbases)
             const uint32_t word = d[w];
             const int nb = w == dtop ? 32 - __builtin_clz(word) : 32;
             for (int bit = nb - 1; bit >= 0; bit--) {
-                mont_sqr26<N>(x, x, M);
-                mul_small26<N>(x, x, ((word >> bit) & 1u) ? a : 1u, M);
+                mont_sqr_w<N>(x, x, M);
+                mul_small_w<N>(x, x, ((word >> bit) & 1u) ? a : 1u, M);
             }
         }
-        reduce26<N>(x, M.m);
-        if (eq26<N>(x, one) || eq26<N>(x, mone)) continue;
+        reduce_w<N>(x, M.m);
+        if (eq_w<N>(x, one) || eq_w<N>(x, mone)) continue;
         bool ok = false;
         for (int r = 1; r < s; r++) {
-            mont_sqr26<N>(x, x, M);
+            mont_sqr_w<N>(x, x, M);
             uint32_t e[N];
 #pragma unroll
             for (int i = 0; i < N; i++) e[i] = x[i];
-            reduce26<N>(e, M.m);
-            if (eq26<N>(e, mone)) { ok = true; break; }
+            reduce_w<N>(e, M.m);
+            if (eq_w<N>(e, mone)) { ok = true; break; }
         }
         if (!ok) return false;
     }
@@ -501,7 +503,7 @@ FK_HD bool sprp26(const uint32_t* v8, int first, int nbases)
 // huntlib/primes.py:23-47 (bases 0..6) and :50-80 (base 0 alone) for a
 // value of up to 8 limbs: the small-prime pre-check answers m == sp, then
 // the strong test to bases first..first+nbases-1 runs at the narrowest
-// radix-2^26 limb count that leaves R > 4 m.  The seven-base verdict is the
+// radix-2^29 limb count that leaves R > 4 m.  The seven-base verdict is the
 // AND of the single-base verdicts, which is what lets the device test bases
 // in parallel.
 FK_HD bool is_sprp(const uint32_t* v8, int first, int nbases)
@@ -530,17 +532,16 @@ FK_HD bool is_sprp(const uint32_t* v8, int first, int nbases)
 #pragma unroll
     for (int i = 0; i < 8; i++)
         if (v8[i]) k = 32 * i + 32 - __builtin_clz(v8[i]);
-    const int n = (k + 2 + kRadixBits - 1) / kRadixBits;
+    const int n = (k + 2 + kLimbBits - 1) / kLimbBits;
     switch (n) {
-    case 0: case 1: case 2: return sprp26<2>(v8, first, nbases);
-    case 3: return sprp26<3>(v8, first, nbases);
-    case 4: return sprp26<4>(v8, first, nbases);
-    case 5: return sprp26<5>(v8, first, nbases);
-    case 6: return sprp26<6>(v8, first, nbases);
-    case 7: return sprp26<7>(v8, first, nbases);
-    case 8: return sprp26<8>(v8, first, nbases);
-    case 9: return sprp26<9>(v8, first, nbases);
-    default: return sprp26<10>(v8, first, nbases);
+    case 0: case 1: case 2: return sprp_w<2>(v8, first, nbases);
+    case 3: return sprp_w<3>(v8, first, nbases);
+    case 4: return sprp_w<4>(v8, first, nbases);
+    case 5: return sprp_w<5>(v8, first, nbases);
+    case 6: return sprp_w<6>(v8, first, nbases);
+    case 7: return sprp_w<7>(v8, first, nbases);
+    case 8: return sprp_w<8>(v8, first, nbases);
+    default: return sprp_w<9>(v8, first, nbases);
     }
 }
 

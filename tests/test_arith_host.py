@@ -1,8 +1,8 @@
 """The device arithmetic header (first-knowns_amd/csrc/fk_arith.h), compiled
 for the host with g++, against the oracle's Python-integer strong test and
 sympy's BPSW: random odd values of 10..256 bits, primes on both sides of
-every 32-bit word boundary and of every radix-2^26 limb-count boundary
-(bit lengths 26 N - 2 and 26 N - 1), strong pseudoprimes to each single
+every 32-bit word boundary and of every radix-2^29 limb-count boundary
+(bit lengths 29 N - 2 and 29 N - 1), strong pseudoprimes to each single
 base, and the small-prime edge cases."""
 import pathlib
 import random
@@ -66,7 +66,7 @@ def test_strong_test_agrees_with_bpsw(harness):
 def test_single_bases_and_limb_boundaries(harness):
     """Each base alone (the device runs one base per lane): composites that
     are strong pseudoprimes to that base, primes and random odd values at
-    the bit lengths where the radix-2^26 limb count changes."""
+    the bit lengths where the radix-2^29 limb count changes."""
     rng = random.Random(11)
     vals = []
     # composites with no factor <= 37 that pass some single base
@@ -79,8 +79,8 @@ def test_single_bases_and_limb_boundaries(harness):
             spsp.append(n)
     assert len(spsp) > 20
     vals += spsp
-    for nlimb in range(2, 11):
-        for bits in (26 * nlimb - 3, 26 * nlimb - 2, 26 * nlimb - 1):
+    for nlimb in range(2, 10):
+        for bits in (29 * nlimb - 3, 29 * nlimb - 2, 29 * nlimb - 1):
             if bits > 256:
                 continue
             vals += [rng.getrandbits(bits) | 1 | (1 << (bits - 1))
