@@ -154,6 +154,19 @@ FK_HD uint32_t resid_d(double s, double qd, double qi, uint32_t q)
     return u < dn ? u : dn;
 }
 
+// The same residue with one fix instead of two: the quotient estimate is
+// taken 1/8 low.  For s < 2^49 and q >= 2, s / q < 2^48; fma(s, qi, -1/8)
+// differs from s / q - 1/8 by at most 2^-5 (qi's relative error 2^-53
+// times s / q) plus 2^-5 (its own rounding at magnitude below 2^48), so its
+// floor is floor(s / q) or one less, and s - floor * q lies in [0, 2q).
+FK_HD uint32_t resid_d1(double s, double qd, double qi, uint32_t q)
+{
+    const double qt = floor(fma(s, qi, -0.125));
+    const uint32_t u = (uint32_t)fma(-qt, qd, s);    // exact, in [0, 2q)
+    const uint32_t dn = u - q;
+    return u < dn ? u : dn;
+}
+
 // ----------------------------------------------------- multiword values
 // value = sum limb[i] * 2^(32 i); L is the limb count.
 

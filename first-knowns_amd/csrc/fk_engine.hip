@@ -1008,7 +1008,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
                                     std::max(1.0, std::floor((1u << 22) / (double)gx)));
             gy = std::max(1u, std::min(gy, 65535u));
         } else {
-            gx = (uint32_t)std::min(1024.0, std::max(1.0, std::ceil(alive / 256.0)));
+            gx = (uint32_t)std::min(1024.0, std::max(1.0, std::ceil(alive / (256.0 * kRoundKC))));
             gy = std::min((np + 15) / 16, (kRoundWorkgroups + gx - 1) / gx);
             gy = std::max(1u, std::min(gy, 65535u));
             ra.pblock = (np + gy - 1) / gy;

@@ -542,19 +542,18 @@ __device__ __forceinline__ uint32_t row_residue(const double (&a)[8],
 #pragma unroll
         for (int k = 4; k < NL; k++) s = fma(a[k], P->c[k - 1], s);
     }
-    return resid_d(s, v.qd, v.qi, v.q);
+    return resid_d1(s, v.qd, v.qi, v.q);
 }
 
 // Lanes whose residue matches the row's kill set: only the four-residue
 // groups the row's count reaches (uniform branches), the first 4 KG
 // residues from registers, longer rows in a loop.  Folded rows (Dickson)
 // take min(r, q - r).
-template <int KG>
+template <int KG, bool FOLD>
 __device__ __forceinline__ uint64_t row_match(uint32_t r, const RowView<KG>& v,
-                                              const FK_CONST uint32_t* row,
-                                              bool fold)
+                                              const FK_CONST uint32_t* row)
 {
-    if (fold) r = min(r, v.q - r);
+    if (FOLD) r = min(r, v.q - r);
     uint64_t hit = match4(r, v.k[0]);
 #pragma unroll
     for (int g = 1; g < KG; g++)
@@ -566,16 +565,16 @@ __device__ __forceinline__ uint64_t row_match(uint32_t r, const RowView<KG>& v,
     return hit;
 }
 
-template <int NL, int KG>
-__device__ __forceinline__ uint64_t round_scan(const double (&a)[8],
-                                               const FK_CONST PrimeD* pd,
-                                               const FK_CONST uint32_t* kt,
-                                               uint32_t stride, uint32_t rf,
-                                               uint32_t lo, uint32_t hi,
-                                               bool fold)
+// KC candidates per lane against the primes [lo, hi), two primes per pass;
+// hit[c] collects the lanes whose c-th candidate some prime kills.
+template <int NL, int KG, bool FOLD, int KC>
+__device__ __forceinline__ void round_scan(const double (&a)[KC][8],
+                                           const FK_CONST PrimeD* pd,
+                                           const FK_CONST uint32_t* kt,
+                                           uint32_t stride, uint32_t rf,
+                                           uint32_t lo, uint32_t hi,
+                                           uint64_t (&hit)[KC])
 {
-    // two primes per pass, straight-line
-    uint64_t hit = 0;
     uint32_t p = lo;
     for (; p + 1 < hi; p += 2) {
         const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * stride;
@@ -593,25 +592,36 @@ __device__ __forceinline__ uint64_t round_scan(const double (&a)[8],
                          "s"(v1.k[1].w), "s"(v1.k[2].x), "s"(v1.k[2].y),
                          "s"(v1.k[2].z), "s"(v1.k[2].w));
         }
-        const uint32_t x0 = row_residue<NL>(a, v0, pd + p);
-        const uint32_t x1 = row_residue<NL>(a, v1, pd + p + 1);
-        hit |= row_match(x0, v0, r0, fold) | row_match(x1, v1, r1, fold);
+        uint32_t x0[KC], x1[KC];
+#pragma unroll
+        for (int c = 0; c < KC; c++) {
+            x0[c] = row_residue<NL>(a[c], v0, pd + p);
+            x1[c] = row_residue<NL>(a[c], v1, pd + p + 1);
+        }
+#pragma unroll
+        for (int c = 0; c < KC; c++)
+            hit[c] |= row_match<KG, FOLD>(x0[c], v0, r0) |
+                      row_match<KG, FOLD>(x1[c], v1, r1);
     }
     if (p < hi) {
         const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * stride;
         const RowView<KG> v0 = load_row<KG>(r0);
-        hit |= row_match(row_residue<NL>(a, v0, pd + p), v0, r0, fold);
+#pragma unroll
+        for (int c = 0; c < KC; c++)
+            hit[c] |= row_match<KG, FOLD>(row_residue<NL>(a[c], v0, pd + p), v0, r0);
     }
-    return hit;
 }
 
 // One round, lane = candidate: blockIdx.x walks the engines' segments in
-// blocks of 256 candidates (grid-stride), blockIdx.y takes a block of
-// pblock primes.  The engine is uniform per block, so the prime-side data
-// is too.  A killed candidate's flag is set.
+// blocks of 256 KC candidates (grid-stride; lane l takes entries l,
+// l + 256, ...), blockIdx.y takes a block of pblock primes.  The engine is
+// uniform per block, so the prime-side data is too, and each prime row read
+// serves KC candidates per lane.  A killed candidate's flag is set.
+
 template <int KG>
 __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
 {
+    constexpr int KC = kRoundKC;
     const uint32_t pb0 = a.p_begin + blockIdx.y * a.pblock;
     const uint32_t pb1 = min(pb0 + a.pblock, a.p_end);
     if (pb0 >= pb1) return;
@@ -620,7 +630,7 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
     uint32_t touch = 0;
     SegBlock sb;
     for (uint32_t blk = blockIdx.x;
-         seg_block(segs, counts, a.nengines, 256, blk, sb); blk += gridDim.x) {
+         seg_block(segs, counts, a.nengines, 256 * KC, blk, sb); blk += gridDim.x) {
         const FK_CONST DevEngine* E = uniform_view(a.engines) + sb.e;
         if (a.round >= E->nround) continue;
         const uint32_t lo = max(pb0, E->rb[a.round]);
@@ -637,34 +647,53 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
             const uint32_t nlines = ((hi - lo) * E->kstride + 15u) / 16u;
             if (threadIdx.x < nlines) tv = rows[16u * threadIdx.x];
         }
-        const uint32_t li = sb.lb * 256 + threadIdx.x;
-        const uint32_t i = segs[sb.e].x + li;
-        // the flag and the record are loaded together (one latency)
-        uint32_t t[4] = {0u, 0u, 0u, 0u};
-        uint32_t flag = 1;
-        if (li < sb.n) {
-            flag = a.flags[i];
-            const DevRec rec = a.recs[i];
-            for (int k = 0; k < 4; k++) t[k] = rec.t[k];
+        // the flags and the records are loaded together (one latency)
+        uint32_t t[KC][4], idx[KC];
+        bool live[KC];
+        bool any = false, wide = false;
+#pragma unroll
+        for (int c = 0; c < KC; c++) {
+            const uint32_t li = sb.lb * (256u * KC) + 256u * c + threadIdx.x;
+            idx[c] = segs[sb.e].x + li;
+            uint32_t flag = 1;
+            for (int k = 0; k < 4; k++) t[c][k] = 0u;
+            if (li < sb.n) {
+                flag = a.flags[idx[c]];
+                const DevRec rec = a.recs[idx[c]];
+                for (int k = 0; k < 4; k++) t[c][k] = rec.t[k];
+            }
+            live[c] = flag == 0;
+            any = any || live[c];
+            wide = wide || (live[c] && (t[c][2] | t[c][3]) != 0);
         }
-        const bool live = flag == 0;
-        if (!__ballot(live)) {
+        if (!__ballot(any)) {
             touch ^= tv;
             continue;
         }
-        double l[8];
+        double l[KC][8];
 #pragma unroll
-        for (int k = 0; k < 8; k++)
-            l[k] = (double)((t[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
-        const bool wide = __ballot(live && (t[2] | t[3]) != 0) != 0;
+        for (int c = 0; c < KC; c++)
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                l[c][k] = (double)((t[c][k >> 1] >> ((k & 1) * 16)) & 0xffffu);
+        const bool wwide = __ballot(wide) != 0;
         const FK_CONST PrimeD* pd = uniform_view(a.primed);
         const FK_CONST uint32_t* kt = uniform_view(E->kt);
         const uint32_t rf = E->round_first;
-        const bool fold = kill_folded(E->family);
-        const uint64_t m =
-            wide ? round_scan<8, KG>(l, pd, kt, E->kstride, rf, lo, hi, fold)
-                 : round_scan<4, KG>(l, pd, kt, E->kstride, rf, lo, hi, fold);
-        if (live && ((m >> __lane_id()) & 1)) a.flags[i] = 1;
+        const uint32_t st = E->kstride;
+        uint64_t m[KC];
+#pragma unroll
+        for (int c = 0; c < KC; c++) m[c] = 0;
+        if (kill_folded(E->family)) {
+            if (wwide) round_scan<8, KG, true, KC>(l, pd, kt, st, rf, lo, hi, m);
+            else round_scan<4, KG, true, KC>(l, pd, kt, st, rf, lo, hi, m);
+        } else {
+            if (wwide) round_scan<8, KG, false, KC>(l, pd, kt, st, rf, lo, hi, m);
+            else round_scan<4, KG, false, KC>(l, pd, kt, st, rf, lo, hi, m);
+        }
+#pragma unroll
+        for (int c = 0; c < KC; c++)
+            if (live[c] && ((m[c] >> __lane_id()) & 1)) a.flags[idx[c]] = 1;
         touch ^= tv;
     }
     // keeps the touch loads; round == ~0 never comes from the host

@@ -35,6 +35,12 @@ struct SieveArgs {
     uint32_t* counts;         // per engine: records emitted
 };
 
+// lane = candidate rounds: candidates per lane (a workgroup takes 256 of
+// them per lane slot).  Two per lane halve the scalar row loads per test but
+// spill scalar registers and halve the waves; on the Dickson n = 10 sweep
+// the rounds took 0.35 ms per window against 0.28 ms with one.
+constexpr int kRoundKC = 1;
+
 struct RoundArgs {
     const DevEngine* engines;
     uint32_t nengines;

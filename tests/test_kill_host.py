@@ -54,13 +54,18 @@ def test_residue_matches_python(harness):
              for _ in range(6)]
     cases, want = [], []
     for q in qs:
-        for v in vals:
+        # multiples of q and their neighbours: where a quotient estimate one
+        # off shows
+        near = [q * rng.getrandbits(b) + d for b in (20, 40, 60)
+                for d in (-1, 0, 1)]
+        for v in vals + [x for x in near if 0 <= x < 2**128]:
             nl = 4 if v < 2**64 else 8
             for use in {nl, 8}:
                 cases.append(f"R {q} {use} {limbs(v)}")
                 want.append(v % q)
-    got = [int(x) for x in call(harness, cases)]
-    assert got == want
+    got = [tuple(int(x) for x in line.split()) for line in call(harness, cases)]
+    assert [g[0] for g in got] == want          # resid_d (two fixes)
+    assert [g[1] for g in got] == want          # resid_d1 (one fix)
 
 
 def kill_brute(family, n, q):
