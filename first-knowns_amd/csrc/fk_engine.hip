@@ -731,8 +731,21 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         wp[w].B = B;
         wp[w].H = hi > B ? (uint64_t)(hi - B) : 0;
         const uint64_t npos = wp[w].H / e->D + 1;
-        uint32_t wpt = e0->wpt;
-        while (wpt > 1 && (uint64_t)64 * T * (wpt / 2) >= npos) wpt /= 2;
+        // the tile length (words per thread up to the engine's) with the
+        // least work per line: padded positions plus a per-tile charge for
+        // the tile's fixed stages (2048 positions' worth)
+        uint32_t wpt = 1;
+        uint64_t best = ~0ull;
+        for (uint32_t k = 1; k <= e0->wpt; k++) {
+            if (!sieve_shape_ok((int)T, (int)k)) continue;
+            const uint64_t S = (uint64_t)64 * T * k;
+            const uint64_t nt = (npos + S - 1) / S;
+            const uint64_t cost = nt * S + nt * 2048;
+            if (cost < best || (cost == best && k > wpt)) {
+                best = cost;
+                wpt = k;
+            }
+        }
         wp[w].wpt = wpt;
     }
     std::vector<uint32_t> order(nwin);

@@ -1076,9 +1076,12 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n)
 
 // --------------------------------------------------------------- launchers
 
+// Tile shapes (threads, words per thread): a tile is 64 * T * WPT line
+// positions.  The odd word counts let a window's tiles fit its lines more
+// tightly (the planner picks the shape with the fewest padded positions).
 #define FK_SIEVE_SHAPES(X)                                                 \
-    X(256, 1) X(256, 2) X(256, 4) X(256, 8) X(256, 16)                       \
-    X(512, 1) X(512, 2) X(512, 4) X(512, 8)                                  \
+    X(256, 1) X(256, 2) X(256, 3) X(256, 4) X(256, 6) X(256, 8) X(256, 16)   \
+    X(512, 1) X(512, 2) X(512, 3) X(512, 4) X(512, 8)                        \
     X(1024, 1) X(1024, 2) X(1024, 4)
 
 // Dynamic LDS above 64 KiB must be allowed per kernel (and device) before

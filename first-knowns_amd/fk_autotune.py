@@ -46,7 +46,7 @@ FAMILY = {"euler": N.FAMILY_EULER, "dickson": N.FAMILY_DICKSON,
 
 SPACE = {
     "threads": [256, 512, 1024],
-    "words_per_thread": [1, 2, 4, 8],
+    "words_per_thread": [1, 2, 3, 4, 6, 8],
     "q_pat": [64, 128, 192, 256, 320],
     "q_med": [512, 1024, 1536, 2048, 3072, 4096, 8192, 16384],
     "nwheel": [0, 3, 4, 5, 6, 7],
@@ -55,8 +55,8 @@ SPACE = {
     "q_bits": [0, 1, 256, 512, 1024, 2048],
 }
 
-VALID_SHAPES = {(256, 1), (256, 2), (256, 4), (256, 8), (256, 16),
-                (512, 1), (512, 2), (512, 4), (512, 8),
+VALID_SHAPES = {(256, 1), (256, 2), (256, 3), (256, 4), (256, 6), (256, 8),
+                (256, 16), (512, 1), (512, 2), (512, 3), (512, 4), (512, 8),
                 (1024, 1), (1024, 2), (1024, 4)}
 
 

@@ -221,6 +221,26 @@ def test_streams_do_not_depend_on_cuts_or_tiles():
         assert a + b == ref, cut
 
 
+def test_streams_do_not_depend_on_tile_fit():
+    """Windows whose lines are about 45k and 90k positions long, so the
+    planner fits them with three- and six-word tiles (256 threads) or
+    three-word tiles (512 threads), against the default shape."""
+    import ladder_gpu
+    n, q2 = 10, 1 << 16
+    say("tile fit")
+    for npos in (45_000, 90_000):
+        lo = 10**12 + 7
+        hi = lo + npos * 96_577
+        ref = ladder_gpu.GpuEngine(n, q2=q2).survivors_j(lo, hi).tolist()
+        assert len(ref) > 1000
+        for kw in ({"threads": 256, "words_per_thread": 8},
+                   {"threads": 256, "words_per_thread": 6},
+                   {"threads": 256, "words_per_thread": 3},
+                   {"threads": 512, "words_per_thread": 3}):
+            eng = ladder_gpu.GpuEngine(n, q2=q2, **kw)
+            assert eng.survivors_j(lo, hi).tolist() == ref, (npos, kw)
+
+
 # --------------------------------------------- both round kernels
 
 ROUND_CASES = [
