@@ -565,6 +565,8 @@ struct fk_batch {
     // graph mode: the captured launch sequence and the plan it was made from
     bool use_graph = false;
     bool graph_ran = false;           // the last submission ran as a graph
+    bool staged = false;              // the last submission recorded stage events
+    std::vector<Step> prev_plan;      // graph mode: the last sequence run eagerly
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<Step> graph_plan;
@@ -1128,10 +1130,16 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     }
     mark(5);
 
-    // ---- run: eagerly with stage events, or as a (cached) graph
+    // ---- run: eagerly with stage events; in graph mode, as a graph once a
+    // launch sequence repeats (captured on its second submission in a row,
+    // replayed while it stays the same), and eagerly without stage events
+    // while it has not repeated, so one-off sequences pay no capture
     b->graph_ran = false;
+    b->staged = false;
     if (b->use_graph) {
-        if (!b->graph_exec || !same_plan(plan, b->graph_plan)) {
+        const bool replay = b->graph_exec && same_plan(plan, b->graph_plan);
+        const bool repeat = !replay && same_plan(plan, b->prev_plan);
+        if (repeat) {
             if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
             if (b->graph) (void)hipGraphDestroy(b->graph);
             b->graph_exec = nullptr;
@@ -1152,12 +1160,18 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             b->graph_captures++;
         }
         HIPCHK(hipEventRecord(b->ev[0], s));
-        HIPCHK(hipGraphLaunch(b->graph_exec, s));
+        if (replay || repeat) {
+            HIPCHK(hipGraphLaunch(b->graph_exec, s));
+            b->graph_launches++;
+            b->graph_ran = true;
+        } else {
+            HIPCHK(emit_steps(plan, b->ev, false, s));
+        }
         HIPCHK(hipEventRecord(b->ev[5], s));
-        b->graph_launches++;
-        b->graph_ran = true;
+        b->prev_plan = plan;
     } else {
         HIPCHK(emit_steps(plan, b->ev, true, s));
+        b->staged = true;
     }
     b->final_slot = total_slot;
     b->final_q = cur;
@@ -1204,8 +1218,8 @@ static int batch_collect(fk_batch* b)
     };
     b->stage_counts.assign(1, stage_total(0));
     for (uint32_t st : b->round_slots) b->stage_counts.push_back(stage_total(st));
-    if (b->graph_ran) {
-        // a graph has no stage events: only the whole is timed
+    if (!b->staged) {
+        // graph mode records no stage events: only the whole is timed
         for (int i = 0; i < 4; i++) b->ms[i] = 0.0f;
     } else {
         HIPCHK(hipEventElapsedTime(&b->ms[0], b->ev[0], b->ev[1]));
@@ -1334,6 +1348,10 @@ int fk_engine_create(const fk_engine_desc* desc, fk_engine** out)
     e->desc = *desc;
     int rc = build_engine(e);
     if (rc == FK_OK) rc = create_batch(e->device, &e->own);
+    // sweeps of one window shape repeat one launch sequence: the engine's
+    // own batch replays it as a graph (fk_engine_batch + fk_batch_set_graph
+    // turn this off, for stage timings)
+    if (rc == FK_OK) e->own->use_graph = true;
     if (rc) {
         std::string keep = g_err;
         engine_free(e);

@@ -246,6 +246,14 @@ class Engine:
         check(lib().fk_engine_batch(self._h, ctypes.byref(b)))
         return batch_timing(b)
 
+    def stage_timing(self, on=True):
+        """Later sweeps run eagerly with HIP events between stages (on) or,
+        the default, replay a repeated launch sequence as a HIP graph (off:
+        last_timing then reports only the whole)."""
+        b = ctypes.c_void_p()
+        check(lib().fk_engine_batch(self._h, ctypes.byref(b)))
+        check(lib().fk_batch_set_graph(b, 0 if on else 1))
+
     def last_counts(self):
         """Queue sizes after the sieve and each round of the last sweep."""
         b = ctypes.c_void_p()

@@ -219,12 +219,15 @@ int fk_batch_timing(fk_batch* b, float* ms5, uint64_t* tiles,
 
 /*
  * Graph mode (enable != 0): the batch records each submission's launch
- * sequence and runs it as a HIP graph, captured the first time and
- * replayed for every later submission whose sequence is byte-identical
- * (same engines, window shapes and grids; the windows themselves may
- * differ).  fk_batch_timing then reports only the whole ([4]).  The
- * sequence holds no memset nodes (counters are cleared by a kernel).  No
- * reference counterpart.  FK_E_BUSY while work is in flight.
+ * sequence; a sequence that repeats on the next submission (same engines,
+ * window shapes and grids; the windows themselves may differ) is captured
+ * as a HIP graph then and replayed for every later submission while it
+ * stays the same.  A sequence not seen just before runs eagerly.  Either
+ * way fk_batch_timing then reports only the whole ([4]).  The sequence
+ * holds no memset nodes (counters are cleared by a kernel).  The engine's
+ * own batch (fk_engine_batch, used by fk_engine_sweep) starts in graph
+ * mode; turn it off there for per-stage timings.  No reference
+ * counterpart.  FK_E_BUSY while work is in flight.
  */
 int fk_batch_set_graph(fk_batch* b, int enable);
 

@@ -1,5 +1,6 @@
 """Graph mode of the batched engine: a submission whose launch sequence
-repeats is captured once into a HIP graph and replayed.  Every replay is
+repeats is captured once into a HIP graph (on its second submission in a
+row) and replayed.  Every replay is
 checked against the eager batch and the plain engine on the same windows,
 with eager device work and synchronisations between the replays, and the
 capture and launch counts show that the replays really were replays."""
@@ -42,7 +43,7 @@ def test_graph_replays_equal_eager():
         eager.submit([eng], [0], [(lo, hi)], [SPEC])
         want = records(eager)
         assert got == want, k
-        plain = eng.sweep(lo, hi, *SPEC)          # eager work in between
+        plain = eng.sweep(lo, hi, *SPEC)          # other work in between
         assert got[0] == N.record_values(plain)
         assert got[1] == plain["run"].tolist()
         total += len(got[0])
@@ -50,8 +51,9 @@ def test_graph_replays_equal_eager():
         assert ms[4] > 0
     assert total >= 20
     captures, launches = g.graph_stats()
-    assert launches == len(windows)
-    assert captures < launches                     # replayed, not recaptured
+    # the first submission runs eagerly, the second is captured and every
+    # later one replays
+    assert (captures, launches) == (1, len(windows) - 1)
 
 
 def test_graph_mixed_batch_repeats():
@@ -72,6 +74,6 @@ def test_graph_mixed_batch_repeats():
         say("mixed graph", k)
         g.submit(engs, [0, 1, 2], ws, specs)
         assert records(g) == want, k
-        engs[0].sweep(*ws[0], *SPEC)               # eager work in between
+        engs[0].sweep(*ws[0], *SPEC)               # other work in between
     captures, launches = g.graph_stats()
-    assert (captures, launches) == (1, 4)
+    assert (captures, launches) == (1, 3)
