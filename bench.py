@@ -75,33 +75,34 @@ def run_plain(steps, warmup):
     for s in range(warmup):
         lo, hi = c1_window(1000 + s)
         eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, C1_CAP)
-    surv, cand = 0, 0
-    deepest = (0, None)
-    whole = 0.0
+    out = []
     t0 = time.perf_counter()
     for s in range(steps):
         lo, hi = c1_window(s)
-        recs = eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, C1_CAP)
-        ms, tiles, c = eng._eng.last_timing()
-        whole += ms[4]
-        cand += c
-        surv += int(recs.size)
+        out.append(eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, C1_CAP))
+    wall = time.perf_counter() - t0
+    device_ms = eng._eng.last_timing()[0][4]      # the last timed sweep
+    surv = sum(int(r.size) for r in out)
+    deepest = (0, None)
+    for recs in out:
         if recs.size and int(recs["run"].max()) > deepest[0]:
             i = int(recs["run"].argmax())
             deepest = (int(recs["run"][i]), int(recs["v0"][i]))
-    wall = time.perf_counter() - t0
-    # per-stage split, eager with stage events (not timed)
+    # per-stage split and sieve candidates, eager with stage events (not
+    # timed), on the first windows of the timed loop
     eng._eng.stage_timing(True)
     stage = [0.0] * 5
-    nst = 3
+    nst = min(3, steps)
+    cand = 0
     for s in range(nst):
         lo, hi = c1_window(s)
         eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, C1_CAP)
-        ms, _, _ = eng._eng.last_timing()
+        ms, _, c = eng._eng.last_timing()
+        cand += c
         for i in range(5):
             stage[i] += ms[i] / nst
     eng._eng.stage_timing(False)
-    return eng, wall, surv, stage, cand, deepest, whole / steps
+    return eng, wall, surv, stage, cand / nst, deepest, device_ms
 
 
 def cpu_baseline(eng, sample_log2, threads):
@@ -296,7 +297,7 @@ def main():
     per_cand = 8.0 * info.n_pattern_primes / 64 + 8.0 * info.mark_density
     # one k_sieve launch per window: its duration is the eager sieve stage
     launch_s = stage[1] / 1e3
-    per_launch = cand / args.steps
+    per_launch = cand                    # wheel candidates per window
     achieved = per_launch * per_cand / launch_s if launch_s > 0 else 0.0
     traffic = rocprof_traffic()
     out = {
@@ -323,7 +324,7 @@ def main():
         "stages_note": "per-stage HIP events on three windows swept eagerly "
                        "after the timed loop; the timed sweeps replay the "
                        "engine's captured launch sequence (device_ms_per_step "
-                       "is their event-timed device time)",
+                       "is the last one's event-timed device time)",
         "roofline": {"bound": "lds", "kernel": "k_sieve",
                      "achieved": achieved / 1e9, "peak": LDS_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / LDS_PEAK,

@@ -212,8 +212,10 @@ class Engine:
         w = make_window(lo, hi)
         c = ClassifySpec(int(mode), int(cap), int(exact_from))
         n = ctypes.c_size_t(0)
-        cap_out = 1 << 12
-        out = np.empty(cap_out, dtype=RECORD_DTYPE)
+        out = getattr(self, "_out", None)        # reused between sweeps
+        if out is None:
+            out = self._out = np.empty(1 << 12, dtype=RECORD_DTYPE)
+        cap_out = out.size
         rc = lib().fk_engine_sweep(self._h, ctypes.byref(w), ctypes.byref(c),
                                    out.ctypes.data, cap_out, ctypes.byref(n))
         if rc == FK_E_OVERFLOW:
