@@ -700,8 +700,8 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
     if (a.round == 0xffffffffu && touch == 1u) a.flags[0] = touch;
 }
 
-// ---- lane = prime (many primes): each lane holds one prime's row in
-// VGPRs and the wave walks the queue.
+// ---- lane = prime (many primes, few candidates): each lane holds one
+// prime's row in VGPRs and the wave walks the candidates, staged in LDS.
 
 // One round prime's row in a lane's registers: the residue constants and
 // up to kRegResidues killing residues (kKillPad past the count).
@@ -757,20 +757,21 @@ __device__ __forceinline__ void load_lane_row_wide(LaneRow& L, uint32_t kmax)
         L.k[i] = ((uint32_t)i < kmax) ? res[i] : kKillPad;
 }
 
-// Does the lane's prime kill a candidate of residue-limbs a[]?  ncmp (wave
-// uniform, a multiple of 4) residues are compared from registers; residues
-// past 32 (kill sets larger than 32) from memory.
-template <int NL>
-__device__ __forceinline__ bool lane_kills(const LaneRow& L, const double (&a)[8],
-                                           uint32_t ncmp, bool fold)
+// Does the lane's prime kill a candidate of residue-limbs a[] (wave
+// uniform, read from LDS)?  ncmp (wave uniform, a multiple of 4) residues
+// are compared from registers; residues past 32 (kill sets larger than 32)
+// from memory.
+template <int NL, bool FOLD>
+__device__ __forceinline__ bool lane_kills(const LaneRow& L, const double* a,
+                                           uint32_t ncmp)
 {
     double s = fma(a[3], L.c48, fma(a[2], L.c32, fma(a[1], L.c16, a[0])));
     if (NL > 4) {
 #pragma unroll
         for (int k = 0; k < 4; k++) s = fma(a[4 + k], L.cw[k], s);
     }
-    uint32_t r = resid_d(s, L.qd, L.qi, L.q);
-    if (fold) r = min(r, L.q - r);
+    uint32_t r = resid_d1(s, L.qd, L.qi, L.q);
+    if (FOLD) r = min(r, L.q - r);
     bool hit = false;
 #pragma unroll
     for (int i = 0; i < kRegResidues; i += 4) {
@@ -787,25 +788,46 @@ __device__ __forceinline__ bool lane_kills(const LaneRow& L, const double (&a)[8
 
 // One round, lane = prime: blockIdx.x picks 256 primes (one per lane),
 // blockIdx.y walks slices of kRoundSlice candidates of the engines'
-// segments (grid-stride).  Each lane holds its prime's row (residue
-// constants and kill set) for the slice's engine in registers; the wave
-// walks the slice a candidate at a time, wave-uniform (scalar loads of the
-// records, four per load batch), and a candidate that some lane's prime
-// kills is flagged by the lowest such lane.  The arithmetic per test is the
-// exact double-precision residue (resid_d) and one compare per killing
-// residue; there is no memory access per (candidate, prime).
+// segments (grid-stride).  Per slice the workgroup stages every live
+// candidate's 16-bit limbs (as doubles) in LDS once; each wave then walks
+// them with broadcast LDS reads, every lane testing its own prime: the
+// exact double-precision residue (resid_d1) and one compare per killing
+// residue.  A candidate that some lane's prime kills is flagged by the
+// lowest such lane.  There is no global memory access per (candidate,
+// prime).
 constexpr uint32_t kRoundSlice = 256;
+
+template <bool FOLD>
+__device__ __forceinline__ void lane_prime_slice(const LaneRow& L, bool mine,
+                                                 uint32_t ncmp, bool wide,
+                                                 const double* limbs,
+                                                 const uint32_t* live_idx,
+                                                 uint32_t nlive, uint32_t* flags)
+{
+    const uint32_t lane = __lane_id();
+    for (uint32_t c = 0; c < nlive; c++) {
+        const double* a = limbs + 8 * c;
+        bool kill = false;
+        if (mine)
+            kill = wide ? lane_kills<8, FOLD>(L, a, ncmp)
+                        : lane_kills<4, FOLD>(L, a, ncmp);
+        const uint64_t m = __ballot(kill);
+        if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1))
+            flags[live_idx[c]] = 1;
+    }
+}
 
 __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
 {
+    __shared__ double sh_limbs[kRoundSlice * 8];
+    __shared__ uint32_t sh_idx[kRoundSlice];
+    __shared__ uint32_t sh_n, sh_wide;
     const uint32_t p = a.p_begin + blockIdx.x * 256 + threadIdx.x;
-    const uint32_t wave_p0 = a.p_begin + blockIdx.x * 256 + (threadIdx.x & ~63u);
-    if (wave_p0 >= a.p_end) return;              // whole wave past the round
+    const uint32_t blk_p0 = a.p_begin + blockIdx.x * 256;
+    if (blk_p0 >= a.p_end) return;                // whole workgroup past it
     const FK_CONST uint2* segs = uniform_view(a.segs);
     const FK_CONST uint32_t* counts = uniform_view(a.counts);
-    const FK_CONST DevRec* recs = uniform_view(a.recs);
     const FK_CONST DevEngine* engs = uniform_view(a.engines);
-    const uint32_t lane = __lane_id();
     LaneRow L;
     SegBlock sb;
     for (uint32_t blk = blockIdx.y;
@@ -814,44 +836,50 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
         const uint32_t e = sb.e;
         if (a.round >= engs[e].nround) continue;
         const uint32_t rlo = engs[e].rb[a.round], rhi = engs[e].rb[a.round + 1];
+        // workgroup-uniform: does any of its 256 primes belong to e's round?
+        if (blk_p0 + 256 <= rlo || blk_p0 >= rhi) continue;
         const bool mine = p >= rlo && p < rhi && p < a.p_end;
-        if (!__ballot(mine)) continue;
-        // compares per test: the family's largest kill set, in fours
-        const uint32_t ncmp = (kill_max(engs[e].family, engs[e].n) + 3u) & ~3u;
-        const uint32_t stride = engs[e].kstride;
-        const bool fold = kill_folded(engs[e].family);
-        if (mine) {
-            load_lane_row(L, engs[e].kt + (size_t)(p - engs[e].round_first) * stride,
-                          a.primed + p, stride);
-            if (ncmp > 20) load_lane_row_wide(L, stride - kRowHead);
-        }
-        const uint32_t base = segs[e].x;
-        const uint32_t c0 = sb.lb * kRoundSlice;
-        const uint32_t c1 = min(sb.n, c0 + kRoundSlice);
-        for (uint32_t i0 = c0; i0 < c1; i0 += 4) {
-            DevRec r[4];
+        // stage the slice's live candidates: limbs and queue index
+        if (threadIdx.x == 0) { sh_n = 0; sh_wide = 0; }
+        __syncthreads();
+        {
+            const uint32_t li = sb.lb * kRoundSlice + threadIdx.x;
+            if (li < sb.n) {
+                const uint32_t i = segs[e].x + li;
+                if (a.flags[i] == 0) {
+                    const DevRec rec = a.recs[i];
+                    const uint32_t slot = atomicAdd(&sh_n, 1u);
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const FK_CONST DevRec* src = recs + base + min(i0 + j, c1 - 1);
-                for (int k = 0; k < 4; k++) r[j].t[k] = src->t[k];
-            }
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                if (i0 + j >= c1) break;
-                double l[8];
-#pragma unroll
-                for (int k = 0; k < 8; k++)
-                    l[k] = (double)((r[j].t[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
-                const bool wide = (r[j].t[2] | r[j].t[3]) != 0;
-                bool kill = false;
-                if (mine)
-                    kill = wide ? lane_kills<8>(L, l, ncmp, fold)
-                                : lane_kills<4>(L, l, ncmp, fold);
-                const uint64_t m = __ballot(kill);
-                if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1))
-                    a.flags[base + i0 + j] = 1;
+                    for (int k = 0; k < 8; k++)
+                        sh_limbs[8 * slot + k] =
+                            (double)((rec.t[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
+                    sh_idx[slot] = i;
+                    if ((rec.t[2] | rec.t[3]) != 0) sh_wide = 1;
+                }
             }
         }
+        __syncthreads();
+        const uint32_t nlive = sh_n;
+        const bool wide = sh_wide != 0;
+        if (nlive != 0) {
+            // compares per test: the family's largest kill set, in fours
+            const uint32_t ncmp = (kill_max(engs[e].family, engs[e].n) + 3u) & ~3u;
+            const uint32_t stride = engs[e].kstride;
+            if (mine) {
+                load_lane_row(L, engs[e].kt + (size_t)(p - engs[e].round_first) * stride,
+                              a.primed + p, stride);
+                if (ncmp > 20) load_lane_row_wide(L, stride - kRowHead);
+            }
+            if (__ballot(mine)) {
+                if (kill_folded(engs[e].family))
+                    lane_prime_slice<true>(L, mine, ncmp, wide, sh_limbs, sh_idx,
+                                           nlive, a.flags);
+                else
+                    lane_prime_slice<false>(L, mine, ncmp, wide, sh_limbs, sh_idx,
+                                            nlive, a.flags);
+            }
+        }
+        __syncthreads();                           // before the next staging
     }
 }
 
