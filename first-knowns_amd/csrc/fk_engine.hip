@@ -462,6 +462,8 @@ struct Step {
     const DevPiece* p_pieces;
     uint32_t p_npieces, p_nlds;
     uint32_t* p_bq;
+    uint32_t* p_zero;                // kPrep: counters to clear
+    uint32_t p_nzero;
     SieveArgs sa;
     RoundArgs ra;
     CompactArgs ca;
@@ -498,7 +500,7 @@ static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
             break;
         case Step::kPrep:
             e = launch_prep(st.p_engines, st.p_pieces, st.p_npieces, st.p_nlds,
-                            st.p_bq, s);
+                            st.p_bq, st.p_zero, st.p_nzero, s);
             break;
         case Step::kSieve:
             e = launch_sieve(st.sa, st.threads, st.wpt, st.grid, st.lds, s);
@@ -910,19 +912,16 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     };
     mark(0);
     {
-        Step st = new_step(Step::kZero);
-        st.dst = b->d_counts;
-        st.n = ncounts;
-        plan.push_back(st);
-        st.dst = b->d_flags;
-        st.n = queue_cap;
-        plan.push_back(st);
+        // k_prep clears the counters; the sieve clears each emitted record's
+        // flag (the compactions leave the flags they read cleared)
         Step pr = new_step(Step::kPrep);
         pr.p_engines = b->d_engines;
         pr.p_pieces = b->d_pieces;
         pr.p_npieces = (uint32_t)pieces.size();
         pr.p_nlds = nlds_max;
         pr.p_bq = b->d_bq;
+        pr.p_zero = b->d_counts;
+        pr.p_nzero = (uint32_t)ncounts;
         plan.push_back(pr);
     }
     mark(1);
@@ -945,6 +944,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         a.segs = b->d_segs;
         a.out = b->d_q[0];
         a.counts = b->d_counts;               // stage 0
+        a.flags = b->d_flags;
         // layout of k_sieve's dynamic LDS: bits, s0, per-prime info,
         // pattern words, kill bitmaps
         const size_t lds = S / 8 + 16 * (size_t)((nlds_max + 3) / 4) +
@@ -1060,8 +1060,13 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     }
     alive = 0.0;
     for (double x : e_alive) alive += x;
-    // ---- pack the segments for the classification and the copy back
-    {
+    // ---- pack the segments for the classification and the copy back.  A
+    // single engine's segment already starts the queue and is dense after
+    // the last compaction: its last counter is the total, no copy needed.
+    uint32_t total_slot = (uint32_t)(nstage * ne);
+    if (ne == 1) {
+        total_slot = (uint32_t)(cstage * ne);
+    } else {
         Step gst = new_step(Step::kGather);
         GatherArgs& ga = gst.ga;
         ga.nengines = (uint32_t)ne;
@@ -1075,7 +1080,6 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         plan.push_back(gst);
         cur ^= 1;
     }
-    const uint32_t total_slot = (uint32_t)(nstage * ne);
     mark(3);
 
     // ---- classification

@@ -2,7 +2,8 @@
 //
 // Pipeline for a batch of window pieces (see fk_device.h for the line
 // model):
-//   k_prep          B mod q for every (piece, LDS prime)
+//   k_prep          B mod q for every (piece, LDS prime); clears the
+//                   batch's stage counters
 //   k_sieve         one LDS tile (S line positions of one wheel line) per
 //                   workgroup iteration: pattern words for the smallest
 //                   primes, optional ds_or stride marking, per-survivor
@@ -27,6 +28,7 @@
 // the shared line model and the on-device classification are this build's.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 #include <mutex>
 #include <vector>
@@ -51,8 +53,14 @@ __device__ __forceinline__ const FK_CONST T* uniform_view(const T* p)
 // ------------------------------------------------------------------ prep
 __global__ void k_prep(const DevEngine* __restrict__ engines,
                        const DevPiece* __restrict__ pieces, uint32_t npieces,
-                       uint32_t* __restrict__ bq)
+                       uint32_t* __restrict__ bq, uint32_t* __restrict__ zero,
+                       uint32_t nzero)
 {
+    // the batch's stage counters start at zero (the first row of blocks)
+    if (blockIdx.y == 0)
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nzero;
+             i += gridDim.x * blockDim.x)
+            zero[i] = 0;
     for (uint32_t pi = blockIdx.y; pi < npieces; pi += gridDim.y) {
         const DevPiece P = pieces[pi];
         const DevEngine& E = engines[P.engine];
@@ -135,6 +143,7 @@ __device__ __forceinline__ void emit_record(const SieveArgs& a,
     if (E.family == kDickson) x *= E.W;
     const uint2 seg = a.segs[P.engine];
     if (slot < seg.y) {
+        a.flags[seg.x + slot] = 0;        // live for the first round
         DevRec r;
         r.t[0] = (uint32_t)x;
         r.t[1] = (uint32_t)(x >> 32);
@@ -1179,14 +1188,13 @@ bool sieve_shape_ok(int threads, int wpt)
 
 hipError_t launch_prep(const DevEngine* engines, const DevPiece* pieces,
                        uint32_t npieces, uint32_t max_nlds, uint32_t* bq,
-                       hipStream_t stream)
+                       uint32_t* zero, uint32_t nzero, hipStream_t stream)
 {
-    if (npieces == 0 || max_nlds == 0) return hipSuccess;
-    uint32_t gx = (max_nlds + 255) / 256;
-    if (gx > 64) gx = 64;
-    const uint32_t gy = npieces < 32768 ? npieces : 32768;
+    uint32_t gx = (std::max(max_nlds, nzero) + 255) / 256;
+    gx = std::max(1u, std::min(gx, 64u));
+    const uint32_t gy = std::max(1u, std::min(npieces, 32768u));
     hipLaunchKernelGGL(k_prep, dim3(gx, gy), dim3(256), 0, stream,
-                       engines, pieces, npieces, bq);
+                       engines, pieces, npieces, bq, zero, nzero);
     return hipGetLastError();
 }
 

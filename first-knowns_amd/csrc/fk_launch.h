@@ -33,6 +33,8 @@ struct SieveArgs {
     const uint2* segs;        // per engine: queue segment (base, size)
     DevRec* out;
     uint32_t* counts;         // per engine: records emitted
+    uint32_t* flags;          // per queue entry: cleared for every record
+                              // emitted (the rounds' kill flags)
 };
 
 // lane = candidate rounds: candidates per lane (a workgroup takes 256 of
@@ -94,7 +96,7 @@ struct ClassifyArgs {
 
 hipError_t launch_prep(const DevEngine* engines, const DevPiece* pieces,
                        uint32_t npieces, uint32_t max_nlds, uint32_t* bq,
-                       hipStream_t stream);
+                       uint32_t* zero, uint32_t nzero, hipStream_t stream);
 hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,
                         uint32_t grid, size_t lds_bytes, hipStream_t stream);
 int sieve_occupancy(int threads, int wpt, size_t lds_bytes);
