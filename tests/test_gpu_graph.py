@@ -77,3 +77,26 @@ def test_graph_mixed_batch_repeats():
         engs[0].sweep(*ws[0], *SPEC)               # other work in between
     captures, launches = g.graph_stats()
     assert (captures, launches) == (1, 3)
+
+
+def test_engine_sweeps_replay_equal_eager():
+    """The engine's own sweeps replay a repeated launch sequence as a graph
+    (captured on the second sweep of one window shape); with stage timing
+    on they run eagerly.  Both give the same records on the same windows."""
+    span = 1 << 27
+    windows = [(10**13 + k * 3_000_000_019, 10**13 + k * 3_000_000_019 + span)
+               for k in range(4)]
+    graph = N.Engine(N.FAMILY_DICKSON, 10, 1 << 16, **F.tuning("dickson"))
+    eager = N.Engine(N.FAMILY_DICKSON, 10, 1 << 16, **F.tuning("dickson"))
+    eager.stage_timing(True)
+    total = 0
+    for k, (lo, hi) in enumerate(windows):
+        say("engine replay", k)
+        got = graph.sweep(lo, hi, *SPEC)
+        want = eager.sweep(lo, hi, *SPEC)
+        assert N.record_values(got) == N.record_values(want), k
+        assert got["run"].tolist() == want["run"].tolist(), k
+        ms, _, _ = eager.last_timing()
+        assert ms[1] > 0                           # stage events recorded
+        total += len(got)
+    assert total >= 10
