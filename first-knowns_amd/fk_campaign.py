@@ -12,6 +12,13 @@ host process pool classifies the previous one. Here the strong tests run on
 the device, so the overlap is different: two batches alternate, and the
 device sweeps segment i+1 while the host persists segment i.
 
+Operator output follows huntlib's formats: [STATUS] lines with the census
+of run lengths from a wall-clock heartbeat thread (fk_log), and an evidence
+directory holding one durable file per find at or above a chosen run
+length, with an idempotent ledger (fk_evidence).  Evidence for a segment is
+written before the cursor that covers it, so a redone segment rewrites its
+records rather than duplicating them.
+
 Durability follows huntlib/checkpoint.py (see fk_checkpoint):
 * A segment's records are appended to the ledger and fsynced before the
   cursor that covers them is saved.
@@ -27,6 +34,8 @@ import json
 import os
 
 import fk_checkpoint as C
+import fk_evidence as EV
+import fk_log as LG
 import fk_native as N
 
 FAMILY_NAME = {N.FAMILY_EULER: "euler", N.FAMILY_DICKSON: "dickson",
@@ -61,10 +70,16 @@ class Campaign:
     {"value", "run"}; default `path` + ".records.jsonl").
 
     `batches` is for tests: two objects with the fk_native.Batch interface.
+    `evidence_dir` with `evidence_min_run`: every record with run >= that
+    gets an evidence file and a ledger row there.  `log` (tag, message) is
+    where STATUS / DISCOVERY lines go (fk_log.log by default); `heartbeat`
+    is the [STATUS] interval in seconds (None: no heartbeat thread).
     """
 
     def __init__(self, engine, lo, hi, seg, path, ledger=None,
-                 classify=None, fresh=False, batches=None, warn=None):
+                 classify=None, fresh=False, batches=None, warn=None,
+                 evidence_dir=None, evidence_min_run=None, log=None,
+                 heartbeat=None):
         lo, hi, seg = int(lo), int(hi), int(seg)
         if not lo < hi:
             raise ValueError(f"empty range [{lo}, {hi})")
@@ -78,6 +93,11 @@ class Campaign:
                                                            engine.n))
         self.key = engine_key(engine, self.classify)
         self.warn = warn
+        self.evidence_dir = os.fspath(evidence_dir) if evidence_dir else None
+        self.evidence_min_run = evidence_min_run
+        self.log = log or LG.log
+        self.heartbeat = heartbeat
+        self._hb = None
         C.refuse_mismatch(self.path, self.key, fresh=fresh,
                           describe=lambda s: f"cursor at {s.get('next')}")
         state = None if fresh else C.load(self.path, self.key, warn=warn)
@@ -125,10 +145,43 @@ class Campaign:
     def cursor(self):
         return int(self.state["next"])
 
+    def _evidence(self, s0, s1, values, runs):
+        if not self.evidence_dir or self.evidence_min_run is None:
+            return
+        ledger = os.path.join(self.evidence_dir, "ledger.json")
+        for v, r in zip(values, runs):
+            if r < self.evidence_min_run:
+                continue
+            ev = {"family": self.key["family"], "n": self.key["n"],
+                  "value": int(v), "run": int(r), "segment": [str(s0), str(s1)],
+                  "classify": self.key["classify"]}
+            EV.record(ev, self.evidence_dir, f"hit_{int(v)}.json", ledger,
+                      key="value", label="DISCOVERY")
+            self.log("DISCOVERY", f"{self.key['family']} n={self.key['n']} "
+                                  f"value={int(v)} run={int(r)}")
+
+    def status_line(self):
+        """The [STATUS] text: cursor, swept, segments, survivors, best run,
+        rate (position units per second, from the heartbeat) and census."""
+        st = self.state
+        census = st["census"]
+        lo_r = min((int(k) for k in census), default=0)
+        hi_r = max((int(k) for k in census), default=-1)
+        msg = (f"{self.key['family']} n={self.key['n']} at {st['next']} "
+               f"swept {st['swept']} segments {st['segments']} "
+               f"survivors {st['survivors']} best {st['best_run']}")
+        if self._hb is not None:
+            msg += f" rate {self._hb.rate():.4g}/s"
+            stall = self._hb.stalled()
+            if stall:
+                msg += f" (busy {stall[0]} for {stall[1]:.0f} s)"
+        return msg + " " + LG.census_str(census, lo_r, hi_r)
+
     def _consume(self, s0, s1, batch, on_segment):
         recs, _ = batch.results()
         values = N.record_values(recs)
         runs = [int(r) for r in recs["run"].tolist()]
+        self._evidence(s0, s1, values, runs)
         size = self._append(values, runs)
         st = dict(self.state)
         census = dict(st["census"])
@@ -144,6 +197,9 @@ class Campaign:
                   ledger_bytes=size)
         C.save_json(self.path, st)
         self.state = st
+        if self._hb is not None:
+            self._hb.mark(s1 - self.lo)
+            self._hb.doing(f"segment at {s1}")
         if on_segment:
             on_segment(s0, s1, values, runs, st)
 
@@ -152,6 +208,18 @@ class Campaign:
         Returns the final cursor state. An interrupt leaves the cursor at
         the last completed segment boundary; so does any other error, and
         then nothing more is written."""
+        if self.heartbeat:
+            self._hb = LG.Heartbeat(self.heartbeat)
+            self._hb.mark(self.cursor - self.lo)
+            self._hb.start(self.status_line, self.log)
+        try:
+            return self._run(max_segments, on_segment)
+        finally:
+            if self._hb is not None:
+                self._hb.stop()
+                self._hb = None
+
+    def _run(self, max_segments, on_segment):
         eng = self.engine
         cur = self.cursor
         pending, slot, submitted = None, 0, 0
