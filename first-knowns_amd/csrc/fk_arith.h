@@ -167,6 +167,21 @@ FK_HD uint32_t resid_d1(double s, double qd, double qi, uint32_t q)
     return u < dn ? u : dn;
 }
 
+// The folded residue min(s mod q, q - s mod q) directly, for the rows that
+// hold folded kill sets.  The quotient is the nearest integer to s * qi;
+// with s < 2^49 and q >= 2 the estimate is within 2^-4 of s / q (as for
+// resid_d1), so it is the nearest integer to s / q or one off, and
+// t = s - qt * q (exact) has |t| < 3q/2.  For u = |t| in [0, q/2] the
+// folded residue is u, for u in (q/2, 3q/2) it is |u - q|: min(u, |u - q|)
+// gives both.
+FK_HD uint32_t resid_fold_d(double s, double qd, double qi, uint32_t q)
+{
+    const double qt = rint(s * qi);
+    const uint32_t u = (uint32_t)fabs(fma(-qt, qd, s));   // exact, < 3q/2
+    const uint32_t d = u > q ? u - q : q - u;
+    return u < d ? u : d;
+}
+
 // ----------------------------------------------------- multiword values
 // value = sum limb[i] * 2^(32 i); L is the limb count.
 

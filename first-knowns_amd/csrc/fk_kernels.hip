@@ -541,7 +541,7 @@ __device__ __forceinline__ RowView<KG> load_row(const FK_CONST uint32_t* row)
 
 // The lane's candidate residue mod the row's prime from its 16-bit limbs;
 // limbs past the fourth take 2^64..2^112 mod q from the prime table.
-template <int NL, int KG>
+template <int NL, bool FOLD, int KG>
 __device__ __forceinline__ uint32_t row_residue(const double (&a)[8],
                                                 const RowView<KG>& v,
                                                 const FK_CONST PrimeD* P)
@@ -551,18 +551,18 @@ __device__ __forceinline__ uint32_t row_residue(const double (&a)[8],
 #pragma unroll
         for (int k = 4; k < NL; k++) s = fma(a[k], P->c[k - 1], s);
     }
-    return resid_d1(s, v.qd, v.qi, v.q);
+    // folded rows (Dickson) compare min(r, q - r), formed directly
+    return FOLD ? resid_fold_d(s, v.qd, v.qi, v.q) : resid_d1(s, v.qd, v.qi, v.q);
 }
 
 // Lanes whose residue matches the row's kill set: only the four-residue
 // groups the row's count reaches (uniform branches), the first 4 KG
-// residues from registers, longer rows in a loop.  Folded rows (Dickson)
-// take min(r, q - r).
-template <int KG, bool FOLD>
+// residues from registers, longer rows in a loop.  (Folded rows get a
+// folded residue from row_residue.)
+template <int KG>
 __device__ __forceinline__ uint64_t row_match(uint32_t r, const RowView<KG>& v,
                                               const FK_CONST uint32_t* row)
 {
-    if (FOLD) r = min(r, v.q - r);
     uint64_t hit = match4(r, v.k[0]);
 #pragma unroll
     for (int g = 1; g < KG; g++)
@@ -604,20 +604,20 @@ __device__ __forceinline__ void round_scan(const double (&a)[KC][8],
         uint32_t x0[KC], x1[KC];
 #pragma unroll
         for (int c = 0; c < KC; c++) {
-            x0[c] = row_residue<NL>(a[c], v0, pd + p);
-            x1[c] = row_residue<NL>(a[c], v1, pd + p + 1);
+            x0[c] = row_residue<NL, FOLD>(a[c], v0, pd + p);
+            x1[c] = row_residue<NL, FOLD>(a[c], v1, pd + p + 1);
         }
 #pragma unroll
         for (int c = 0; c < KC; c++)
-            hit[c] |= row_match<KG, FOLD>(x0[c], v0, r0) |
-                      row_match<KG, FOLD>(x1[c], v1, r1);
+            hit[c] |= row_match<KG>(x0[c], v0, r0) |
+                      row_match<KG>(x1[c], v1, r1);
     }
     if (p < hi) {
         const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * stride;
         const RowView<KG> v0 = load_row<KG>(r0);
 #pragma unroll
         for (int c = 0; c < KC; c++)
-            hit[c] |= row_match<KG, FOLD>(row_residue<NL>(a[c], v0, pd + p), v0, r0);
+            hit[c] |= row_match<KG>(row_residue<NL, FOLD>(a[c], v0, pd + p), v0, r0);
     }
 }
 
@@ -779,8 +779,8 @@ __device__ __forceinline__ bool lane_kills(const LaneRow& L, const double* a,
 #pragma unroll
         for (int k = 0; k < 4; k++) s = fma(a[4 + k], L.cw[k], s);
     }
-    uint32_t r = resid_d1(s, L.qd, L.qi, L.q);
-    if (FOLD) r = min(r, L.q - r);
+    const uint32_t r = FOLD ? resid_fold_d(s, L.qd, L.qi, L.q)
+                            : resid_d1(s, L.qd, L.qi, L.q);
     bool hit = false;
 #pragma unroll
     for (int i = 0; i < kRegResidues; i += 4) {

@@ -5,7 +5,7 @@
 //   R q nl t0 t1 t2 t3            -> the residue of the value t (four hex
 //                                    32-bit limbs) by the 16-bit-limb
 //                                    double-precision path with nl limbs,
-//                                    two-fix and one-fix forms
+//                                    two-fix and one-fix forms, and folded
 #include "../first-knowns_amd/csrc/fk_kill.h"
 #include <cstdio>
 #include <cstring>
@@ -55,8 +55,9 @@ int main()
                 a[k] = (double)((t[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
             double s = a[0];
             for (int k = 1; k < nl; k++) s = fma(a[k], P.c[k - 1], s);
-            printf("%u %u\n", fk::resid_d(s, P.qd, P.qi, P.q),
-                   fk::resid_d1(s, P.qd, P.qi, P.q));
+            printf("%u %u %u\n", fk::resid_d(s, P.qd, P.qi, P.q),
+                   fk::resid_d1(s, P.qd, P.qi, P.q),
+                   fk::resid_fold_d(s, P.qd, P.qi, P.q));
         }
     }
 }

@@ -52,20 +52,26 @@ def test_residue_matches_python(harness):
             2**96 + 12345, 2**128 - 1]
     vals += [rng.getrandbits(b) for b in (20, 40, 63, 64, 80, 100, 127, 128)
              for _ in range(6)]
-    cases, want = [], []
+    cases, want, wq = [], [], []
     for q in qs:
         # multiples of q and their neighbours: where a quotient estimate one
         # off shows
         near = [q * rng.getrandbits(b) + d for b in (20, 40, 60)
                 for d in (-1, 0, 1)]
+        # and around the half-multiples, where a nearest quotient is one off
+        near += [q * rng.getrandbits(b) + q // 2 + d for b in (20, 40, 60)
+                 for d in (-1, 0, 1, 2)]
         for v in vals + [x for x in near if 0 <= x < 2**128]:
             nl = 4 if v < 2**64 else 8
             for use in {nl, 8}:
                 cases.append(f"R {q} {use} {limbs(v)}")
                 want.append(v % q)
+                wq.append(q)
     got = [tuple(int(x) for x in line.split()) for line in call(harness, cases)]
     assert [g[0] for g in got] == want          # resid_d (two fixes)
     assert [g[1] for g in got] == want          # resid_d1 (one fix)
+    fold = [min(w, q - w) for w, q in zip(want, wq)]
+    assert [g[2] for g in got] == fold          # resid_fold_d
 
 
 def kill_brute(family, n, q):
