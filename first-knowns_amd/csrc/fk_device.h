@@ -58,6 +58,9 @@ struct DevEngine {
     const uint32_t* bm_off;   // [nlds - nmark] first bitmap word per prime
     const uint32_t* bm;       // [bm_words] line-domain kill bitmaps, each
                               // q bits then its first 64 bits again
+    uint32_t sbm_words;       // kill bitmaps of every LDS prime, laid out
+    const uint32_t* sbm_off;  // as bm's: [nlds] first words, [sbm_words]
+    const uint32_t* sbm;      // bitmaps (the short-line sieve's tables)
     uint32_t ntr;             // in-tile rounds over [nmark, nlds)
     const uint32_t* tr_end;   // [ntr] LDS-prime index where round r ends
     const uint32_t* kt;       // [round_end - round_first][kstride] round
@@ -78,7 +81,13 @@ struct DevPiece {
     uint32_t engine;          // index into the engine array
     uint32_t window;          // caller's window index
     uint32_t bq_off;          // offset of this piece's (B mod q) block
+    uint32_t npw;             // short-line pieces: 64-bit words per line,
+                              // tiles of 256 (line, chunk) items; 0: one
+                              // tile per S positions of a line
 };
+
+// Words of one line a short-line sieve lane covers.
+constexpr uint32_t kShortChunk = 8;
 
 // A candidate in flight: the value the kill tests and the strong test need
 // (Euler p, Dickson k = W j, AP p) and its window.

@@ -146,6 +146,7 @@ struct fk_engine {
     u128 dval = 0;
     uint32_t q_pat = 0, q_med = 0, threads = 256, wpt = 8;
     uint32_t round_lane_prime = 8192, growth = 2;
+    uint32_t short_words = 1024;     // 0: no short-line sieve
     uint32_t q_mark = 0;             // [q_pat, q_mark) stride-marked
     uint32_t q_bits = 0;             // [q_mark, q_bits) ORed in from bitmaps
     uint32_t npat = 0;
@@ -258,6 +259,9 @@ static int build_engine(fk_engine* e)
     if (e->q_med > d.q2) e->q_med = d.q2;
     if (e->q_pat > e->q_med) e->q_pat = e->q_med;
     e->round_lane_prime = d.round_lane_prime ? d.round_lane_prime : 8192;
+    if (d.short_words > 4096)
+        return fail(FK_E_ARG, "short_words = %u > 4096", d.short_words);
+    e->short_words = d.short_words ? d.short_words : 1024;
     e->q_mark = d.q_mark ? d.q_mark : e->q_pat;
     if (e->q_mark < e->q_pat) e->q_mark = e->q_pat;
     if (e->q_mark > e->q_med) e->q_mark = e->q_med;
@@ -270,6 +274,7 @@ static int build_engine(fk_engine* e)
     // pattern primes, the stride-marked primes, the bitmap primes (ORed in
     // as 64-bit windows below q_bits, tested per surviving position above)
     std::vector<uint32_t> lq, lm32, ldinv, pat_off, pairs, bm_off, bm;
+    std::vector<uint32_t> sbm_off, sbm;     // every prime's bitmap
     std::vector<uint64_t> lm64, pat;
     uint32_t nmark = 0, nbits = 0;
     double surv = 1.0;
@@ -287,6 +292,7 @@ static int build_engine(fk_engine* e)
         ldinv.push_back(dinv);
         e->kill_w.push_back((uint32_t)kl.size());
         surv *= 1.0 - (double)kl.size() / q;
+        append_kill_bitmap(kl, q, sbm_off, sbm);
         if (q < e->q_pat) {
             pat_off.push_back((uint32_t)pat.size());
             std::vector<uint8_t> dead(q, 0);
@@ -303,16 +309,7 @@ static int build_engine(fk_engine* e)
             e->mark_density += (double)kl.size() / q;
             nmark = idx + 1;
         } else {
-            // periodic kill bitmap: q bits, then the first 64 again, so a
-            // 64-bit window starting at any x < q lies inside it; one word
-            // more for the window's third word
-            bm_off.push_back((uint32_t)bm.size());
-            const size_t base = bm.size();
-            bm.resize(base + (q - 1) / 32 + 3, 0u);
-            for (uint32_t r : kl) {
-                bm[base + (r >> 5)] |= 1u << (r & 31);
-                if (r < 64) bm[base + ((q + r) >> 5)] |= 1u << ((q + r) & 31);
-            }
+            append_kill_bitmap(kl, q, bm_off, bm);
             if (q < e->q_bits) nbits = idx + 1;
         }
     }
@@ -345,6 +342,8 @@ static int build_engine(fk_engine* e)
             return fail(FK_E_ARG, "tile needs %zu bytes of LDS (> 160 KiB): "
                         "lower q_med, q_pat or the tile", need);
     }
+    // the short-line sieve's LDS: per-prime arrays and every kill bitmap
+    if (32 * lq.size() + 4 * sbm.size() > 160 * 1024) e->short_words = 0;
     e->lds_q = lq;
     e->pat_words = (uint32_t)pat.size();
     e->surv_lds = surv;
@@ -392,6 +391,7 @@ static int build_engine(fk_engine* e)
     h.npairs = (uint32_t)pairs.size();
     h.pat_words = e->pat_words;
     h.bm_words = (uint32_t)bm.size();
+    h.sbm_words = (uint32_t)sbm.size();
     h.ntr = (uint32_t)tr_end.size();
     h.round_first = e->round_first;
     h.round_end = e->round_end;
@@ -415,6 +415,8 @@ static int build_engine(fk_engine* e)
     if ((rc = eng_upload(e, pairs, &h.pairs))) return rc;
     if ((rc = eng_upload(e, bm_off, &h.bm_off))) return rc;
     if ((rc = eng_upload(e, bm, &h.bm))) return rc;
+    if ((rc = eng_upload(e, sbm_off, &h.sbm_off))) return rc;
+    if ((rc = eng_upload(e, sbm, &h.sbm))) return rc;
     if ((rc = eng_upload(e, tr_end, &h.tr_end))) return rc;
     {
         // kill sets of the round primes, built on the device
@@ -503,7 +505,9 @@ static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
                             st.p_bq, st.p_zero, st.p_nzero, s);
             break;
         case Step::kSieve:
-            e = launch_sieve(st.sa, st.threads, st.wpt, st.grid, st.lds, s);
+            e = st.wpt == 0 ? launch_sieve_short(st.sa, st.grid, st.lds, s)
+                            : launch_sieve(st.sa, st.threads, st.wpt, st.grid,
+                                           st.lds, s);
             break;
         case Step::kRound:
             e = launch_round_test(st.ra, st.lane_prime != 0, st.grid, st.gy, s);
@@ -735,6 +739,14 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         wp[w].B = B;
         wp[w].H = hi > B ? (uint64_t)(hi - B) : 0;
         const uint64_t npos = wp[w].H / e->D + 1;
+        // lines of a few words: one lane per chunk of a line (k_sieve_short)
+        const uint64_t npw = (npos + 63) / 64;
+        if (npw <= e->short_words &&
+            (uint64_t)e->offs.size() * ((npw + kShortChunk - 1) / kShortChunk)
+                < (1ull << 32)) {
+            wp[w].wpt = 0;
+            continue;
+        }
         // the tile length (words per thread up to the engine's) with the
         // least work per line: padded positions plus a per-tile charge for
         // the tile's fixed stages (2048 positions' worth)
@@ -765,6 +777,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     std::vector<uint32_t> win_engine(nwin);
     uint64_t tiles = 0, bq_total = 0;
     uint32_t nlds_max = 0, pat_max = 0, bm_max = 0;
+    uint32_t s_nlds_max = 0, s_bm_max = 0;      // over short windows
     double expect = 0.0, cand = 0.0;
     std::vector<double> e_alive(b->engines.size(), 0.0);  // expected per engine
     for (fk_engine* e : b->engines) {
@@ -783,6 +796,37 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         const fk_engine* e = b->engines[b->engine_of[w]];
         const u128 lo = mk128(b->windows[w].lo), hi = mk128(b->windows[w].hi);
         if (hi <= lo) continue;
+        if (wp[w].wpt == 0) {
+            // short lines: one piece, tiles of 256 (line, chunk) items
+            if (groups.empty() || groups.back().wpt != 0)
+                groups.push_back({0u, tiles, tiles});
+            const uint64_t npos_all = wp[w].H / e->D + 1;
+            DevPiece P{};
+            for (int i = 0; i < 4; i++) P.base[i] = (uint32_t)(wp[w].B >> (32 * i));
+            P.lo_rel = lo > wp[w].B ? (uint64_t)(lo - wp[w].B) : 0;
+            P.hi_rel = (uint64_t)(hi - wp[w].B);
+            P.npos = (uint32_t)npos_all;
+            P.npw = (uint32_t)((npos_all + 63) / 64);
+            const uint64_t nch = (P.npw + kShortChunk - 1) / kShortChunk;
+            P.nseg = (uint32_t)(((uint64_t)e->offs.size() * nch + 255) / 256);
+            P.tile0 = (uint32_t)tiles;
+            P.engine = b->engine_of[w];
+            P.window = (uint32_t)w;
+            P.bq_off = (uint32_t)bq_total;
+            if (tiles + P.nseg >= (1ull << 32))
+                return fail(FK_E_ARG, "batch too large: more than 2^32 tiles");
+            tiles += P.nseg;
+            bq_total += e->host.nlds;
+            pieces.push_back(P);
+            groups.back().tile_end = tiles;
+            s_nlds_max = std::max(s_nlds_max, e->host.nlds);
+            s_bm_max = std::max(s_bm_max, e->host.sbm_words);
+            const double span = (double)(hi - lo);
+            expect += expected_lds(e, span);
+            e_alive[b->engine_of[w]] += expected_lds(e, span);
+            cand += span * e->line_density;
+            continue;
+        }
         const uint64_t S = (uint64_t)64 * T * wp[w].wpt;
         const uint64_t PIECE = std::max<uint64_t>(S, ((uint64_t)1 << 31) / S * S);
         if (groups.empty() || groups.back().wpt != wp[w].wpt)
@@ -932,6 +976,36 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         const uint64_t S = (uint64_t)64 * T * g.wpt;
         Step st = new_step(Step::kSieve);
         SieveArgs& a = st.sa;
+        if (g.wpt == 0) {
+            // short lines: two per-prime arrays and the kill bitmaps
+            a.engines = b->d_engines;
+            a.pieces = b->d_pieces;
+            a.npieces = (uint32_t)pieces.size();
+            a.tile_begin = (uint32_t)g.tile_begin;
+            a.ntiles = (uint32_t)g.tile_end;
+            a.nlds_max = s_nlds_max;
+            a.pat_max = 0;
+            a.bm_max = s_bm_max;
+            a.bq = b->d_bq;
+            a.segs = b->d_segs;
+            a.out = b->d_q[0];
+            a.counts = b->d_counts;
+            a.flags = b->d_flags;
+            const size_t lds = 32 * (size_t)s_nlds_max + 4 * (size_t)s_bm_max;
+            if (lds > 160 * 1024)
+                return fail(FK_E_ARG, "LDS need %zu bytes > 160 KiB (lower q_med"
+                            " or q_pat)", lds);
+            int occ = sieve_short_occupancy(lds);
+            if (occ < 1) occ = 1;
+            uint64_t grid = (uint64_t)occ * ctx->cus;
+            grid = std::min<uint64_t>(grid, g.tile_end - g.tile_begin);
+            st.threads = 256;
+            st.wpt = 0;
+            st.grid = (uint32_t)grid;
+            st.lds = lds;
+            plan.push_back(st);
+            continue;
+        }
         a.engines = b->d_engines;
         a.pieces = b->d_pieces;
         a.npieces = (uint32_t)pieces.size();
@@ -1403,6 +1477,7 @@ int fk_engine_get_info(const fk_engine* e, fk_engine_info* info)
     info->n_pattern_primes = e->npat;
     info->n_round_primes = e->round_end - e->round_first;
     info->n_rounds = e->nrounds;
+    info->short_words = e->short_words;
     info->device_bytes = e->device_bytes;
     info->survival_lds = e->surv_lds;
     info->mark_density = e->mark_density;

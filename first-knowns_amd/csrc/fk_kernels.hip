@@ -447,6 +447,138 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
     }
 }
 
+// ---------------------------------------------------- sieve, short lines
+
+// Windows whose lines hold only a few 64-position words (a short window
+// over a large wheel): one lane per chunk of kShortChunk consecutive words
+// of one line instead of one workgroup per line.  A tile here is 256
+// consecutive (line, chunk) items of one piece, lines major.  For every LDS
+// prime the lane forms its first word's position in the prime's kill
+// coordinates (the work the line kernel does once per tile), then walks its
+// words in steps of 64 mod q, ORing in the 64-bit window of the prime's
+// periodic kill bitmap.  Bitmaps for every prime (no 64-bit pattern
+// tables, which take 64 times the LDS) keep the workgroup's LDS to a few
+// KB, so that many workgroups share a CU.  Surviving bits become records
+// with one atomic per wave.  No survivor lists and no barrier but the
+// engine-table load.
+__global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
+{
+    extern __shared__ uint64_t lds64[];
+    uint4* pinfo = reinterpret_cast<uint4*>(lds64);  // {q, m32, bitmap, 64 mod q}
+    uint4* pk = pinfo + a.nlds_max;                  // {m64 lo, m64 hi, D^-1, 0}
+    uint32_t* bm = reinterpret_cast<uint32_t*>(pk + a.nlds_max);   // bm_max
+    constexpr uint32_t C = kShortChunk;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    uint32_t cur_engine = 0xffffffffu;
+    const uint64_t tiles_per =
+        ((uint64_t)(a.ntiles - a.tile_begin) + gridDim.x - 1) / gridDim.x;
+    const uint64_t tile_lo = a.tile_begin + (uint64_t)blockIdx.x * tiles_per;
+    const uint64_t tile_hi = min((uint64_t)a.ntiles, tile_lo + tiles_per);
+    for (uint32_t tile = (uint32_t)tile_lo; tile < tile_hi; tile++) {
+        const uint32_t pi = find_piece(uniform_view(a.pieces), a.npieces, tile);
+        const FK_CONST DevPiece& P = uniform_view(a.pieces)[pi];
+        const FK_CONST DevEngine& E = uniform_view(a.engines)[P.engine];
+        if (P.engine != cur_engine) {
+            __syncthreads();
+            for (uint32_t w = tid; w < E.sbm_words; w += 256) bm[w] = E.sbm[w];
+            for (uint32_t p = tid; p < E.nlds; p += 256) {
+                const uint32_t q = E.lds_q[p];
+                const uint64_t m64 = E.lds_m64[p];
+                pinfo[p] = make_uint4(q, E.lds_m32[p], E.sbm_off[p], 64u % q);
+                pk[p] = make_uint4((uint32_t)m64, (uint32_t)(m64 >> 32),
+                                   E.lds_dinv[p], 0u);
+            }
+            cur_engine = P.engine;
+            __syncthreads();
+        }
+        const uint32_t nch = (P.npw + C - 1) / C;
+        const uint32_t item = (tile - P.tile0) * 256u + tid;
+        const uint32_t line = item / nch;
+        const uint64_t v0 = 64ull * C * (item - line * nch);
+        uint64_t acc[C];
+#pragma unroll
+        for (uint32_t k = 0; k < C; k++) acc[k] = ~0ull;
+        uint64_t off = 0;
+        uint32_t nk = 0;            // words of this chunk before the line's end
+        if (line < E.nlines && v0 < P.npos) {
+            off = E.offs[line];
+            const uint64_t D = E.D;
+            const uint64_t vlo =
+                (off >= P.lo_rel) ? 0 : ceil_div_u64(P.lo_rel - off, D);
+            const uint64_t vhi =
+                (P.hi_rel > off) ? ceil_div_u64(P.hi_rel - off, D) : 0;
+            const uint64_t vend = (vhi < (uint64_t)P.npos) ? vhi : (uint64_t)P.npos;
+            if (vlo < vend && v0 < vend && v0 + 64 * C > vlo) {
+                nk = (uint32_t)min((uint64_t)C, (vend - v0 + 63) / 64);
+#pragma unroll
+                for (uint32_t k = 0; k < C; k++)
+                    acc[k] = edge_mask(v0 + 64 * k, vlo, vend);
+            }
+        }
+        if (nk) {
+            // the chunk's first word in prime p's coordinates:
+            // x = (v0 + (B + off) D^-1) mod q
+            for (uint32_t p = 0; p < E.nlds; p++) {
+                const uint4 pq = pinfo[p];
+                const uint4 kk = pk[p];
+                const uint32_t q = pq.x;
+                const uint64_t m64 = ((uint64_t)kk.y << 32) | kk.x;
+                uint32_t t = a.bq[P.bq_off + p] + red64(off, q, m64);
+                if (t >= q) t -= q;
+                t = mulmod32(t, kk.z, q, m64);
+                uint32_t x = red32((uint32_t)v0 + t, q, pq.y);
+                const uint32_t* b = bm + pq.z;
+#pragma unroll
+                for (uint32_t k = 0; k < C; k++) {
+                    if (k < nk) {
+                        const uint32_t i = x >> 5, o = x & 31u;
+                        const uint32_t w0 = b[i], w1 = b[i + 1], w2 = b[i + 2];
+                        acc[k] |= ((uint64_t)__builtin_amdgcn_alignbit(w2, w1, o) << 32) |
+                                  __builtin_amdgcn_alignbit(w1, w0, o);
+                    }
+                    x += pq.w;
+                    if (x >= q) x -= q;
+                }
+            }
+        }
+        // records: a wave-wide prefix of the per-lane counts, one atomic
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < C; k++) {
+            acc[k] = (k < nk) ? ~acc[k] : 0ull;        // now: live bits
+            c += __popcll(acc[k]);
+        }
+        uint32_t incl = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(incl, d);
+            if (lane >= (uint32_t)d) incl += o;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        uint32_t base = 0;
+        if (total) {
+            if (lane == 63) base = atomicAdd(a.counts + P.engine, total);
+            base = __shfl(base, 63);
+        }
+        if (c) {
+            const unsigned __int128 B =
+                ((unsigned __int128)(((uint64_t)P.base[3] << 32) | P.base[2]) << 64)
+                | (((uint64_t)P.base[1] << 32) | P.base[0]);
+            uint32_t slot = base + incl - c;
+#pragma unroll
+            for (uint32_t k = 0; k < C; k++) {
+                uint64_t m = acc[k];
+                while (m) {
+                    const int bit = __builtin_ctzll(m);
+                    m &= m - 1;
+                    emit_record(a, P, E, B, off, v0 + 64 * k + bit, slot++);
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- rounds
 
 // Round table of each engine, built once at engine creation: one thread
@@ -1145,6 +1277,24 @@ static void allow_lds(const void* fn, size_t lds_bytes)
     if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds_bytes) == hipSuccess)
         done.push_back({fn, dev, lds_bytes});
+}
+
+hipError_t launch_sieve_short(const SieveArgs& a, uint32_t grid,
+                              size_t lds_bytes, hipStream_t stream)
+{
+    allow_lds((const void*)k_sieve_short, lds_bytes);
+    hipLaunchKernelGGL(k_sieve_short, dim3(grid), dim3(256), lds_bytes, stream, a);
+    return hipGetLastError();
+}
+
+int sieve_short_occupancy(size_t lds_bytes)
+{
+    int blocks = 0;
+    allow_lds((const void*)k_sieve_short, lds_bytes);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &blocks, k_sieve_short, 256, lds_bytes) != hipSuccess)
+        blocks = 0;
+    return blocks;
 }
 
 hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,

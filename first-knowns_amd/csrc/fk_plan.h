@@ -31,6 +31,14 @@ uint32_t first_kill_prime(int family, int n);
 //   AP      {-i P(n) mod q : i < n}               (ap_search.py:102-104)
 std::vector<uint32_t> kill_set(int family, int n, uint32_t q);
 
+// Appends prime q's periodic kill bitmap to `words` and its first word's
+// index to `offs`: bit b is set iff b mod q is in `kills`, for b < q + 64,
+// so that the 64-bit window at any x < q lies inside it, in
+// (q - 1) / 32 + 3 words (one more for the window's third word).
+void append_kill_bitmap(const std::vector<uint32_t>& kills, uint32_t q,
+                        std::vector<uint32_t>& offs,
+                        std::vector<uint32_t>& words);
+
 // Admissible residues mod prod(wheel) (CRT lifting, sorted) and that
 // product.  Empty result with *ok = false if the product overflows.
 std::vector<uint64_t> wheel_offsets(int family, int n,

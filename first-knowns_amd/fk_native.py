@@ -40,7 +40,8 @@ class EngineDesc(ctypes.Structure):
                 ("nwheel", i32), ("wheel_primes", u32 * 12),
                 ("q_pat", u32), ("q_med", u32), ("threads", u32),
                 ("words_per_thread", u32), ("round_lane_prime", u32),
-                ("round_growth", u32), ("q_mark", u32), ("q_bits", u32)]
+                ("round_growth", u32), ("q_mark", u32), ("q_bits", u32),
+                ("short_words", u32)]
 
 
 class EngineInfo(ctypes.Structure):
@@ -51,7 +52,7 @@ class EngineInfo(ctypes.Structure):
                 ("threads", u32), ("words_per_thread", u32),
                 ("n_lds_primes", u32),
                 ("n_pattern_primes", u32), ("n_round_primes", u32),
-                ("n_rounds", u32), ("device_bytes", u64),
+                ("n_rounds", u32), ("short_words", u32), ("device_bytes", u64),
                 ("survival_lds", ctypes.c_double),
                 ("mark_density", ctypes.c_double)]
 
@@ -168,7 +169,8 @@ class Engine:
 
     def __init__(self, family, n, q2, device=0, nwheel=0, wheel_primes=None,
                  q_pat=0, q_med=0, threads=0, words_per_thread=0,
-                 round_lane_prime=0, round_growth=0, q_mark=0, q_bits=0):
+                 round_lane_prime=0, round_growth=0, q_mark=0, q_bits=0,
+                 short_words=0):
         d = EngineDesc()
         d.family, d.n, d.q2, d.device = int(family), int(n), int(q2), device
         d.nwheel = int(nwheel)
@@ -180,6 +182,7 @@ class Engine:
         d.round_growth = int(round_growth)
         d.q_mark = int(q_mark)
         d.q_bits = int(q_bits)
+        d.short_words = int(short_words)
         h = ctypes.c_void_p()
         check(lib().fk_engine_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h

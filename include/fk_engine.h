@@ -84,6 +84,11 @@ typedef struct fk_engine_desc {
                                 64-bit windows of their periodic kill bitmaps;
                                 [q_bits, q_med) are tested per surviving
                                 position (0 = q_mark: no bitmap windows) */
+    uint32_t short_words;    /* windows whose lines hold at most this many
+                                64-position words are sieved one lane per
+                                eight words of a line rather than one
+                                workgroup per line (0 = default 1024; at
+                                most 4096) */
 } fk_engine_desc;
 
 typedef struct fk_engine_info {
@@ -97,6 +102,7 @@ typedef struct fk_engine_info {
     uint64_t d[4];           /* AP difference P(n) (256-bit LE); 0 otherwise */
     uint32_t q_pat, q_mark, q_med, threads, words_per_thread;
     uint32_t n_lds_primes, n_pattern_primes, n_round_primes, n_rounds;
+    uint32_t short_words;    /* 0 when the short-line tables do not fit LDS */
     uint64_t device_bytes;   /* device memory the engine holds */
     double survival_lds;     /* expected fraction of wheel candidates that
                                 survive the LDS stage */

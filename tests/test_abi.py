@@ -36,11 +36,11 @@ def test_every_declared_symbol_is_exported():
 
 def test_struct_sizes_match_header():
     # sizes implied by the header's field lists (LP64)
-    assert ctypes.sizeof(N.EngineDesc) == 4 * 5 + 4 * 12 + 4 * 8
+    assert ctypes.sizeof(N.EngineDesc) == 4 * 5 + 4 * 12 + 4 * 9
     assert ctypes.sizeof(N.Window) == 32
     assert ctypes.sizeof(N.ClassifySpec) == 12
     assert N.RECORD_DTYPE.itemsize == 24
-    assert ctypes.sizeof(N.EngineInfo) == 16 + 64 + 8 * 7 + 36 + 4 + 8 + 8 + 8
+    assert ctypes.sizeof(N.EngineInfo) == 16 + 64 + 8 * 7 + 40 + 8 + 8 + 8
 
 
 def test_status_codes_match_header():

@@ -193,12 +193,16 @@ def test_ap_depths_match_oracle(golden):
 # ------------------------------------------------------- slicing
 
 def test_streams_do_not_depend_on_cuts_or_tiles():
+    """Lines of 311 positions: the reference stream from the short-line
+    sieve, every variant on the line sieve (short_words=1)."""
     import ladder_gpu
     n, q2, lo, hi = 10, 4096, 10**12 + 7, 10**12 + 7 + 30_000_017
     say("slicing")
     ref = ladder_gpu.GpuEngine(n, q2=q2).survivors_j(lo, hi).tolist()
     assert len(ref) > 100
-    for kw in ({"threads": 512, "words_per_thread": 4},
+    assert ref == O.dickson_survivors(n, lo, hi, q2=q2)
+    for kw in ({},
+               {"threads": 512, "words_per_thread": 4},
                {"threads": 1024, "words_per_thread": 1},
                {"threads": 256, "words_per_thread": 16},
                {"q_med": 1024, "q_pat": 64},
@@ -211,8 +215,9 @@ def test_streams_do_not_depend_on_cuts_or_tiles():
                {"round_lane_prime": 1, "q_med": 256},
                {"q_mark": 1024}, {"q_mark": 4096},
                {"q_mark": 256, "threads": 1024, "words_per_thread": 1},
-               {"q_mark": 512, "q_med": 3072, "q_pat": 64}):
-        eng = ladder_gpu.GpuEngine(n, q2=q2, **kw)
+               {"q_mark": 512, "q_med": 3072, "q_pat": 64},
+               {"q_pat": 16, "q_bits": 1024, "q_med": 1024}):
+        eng = ladder_gpu.GpuEngine(n, q2=q2, short_words=1, **kw)
         assert eng.survivors_j(lo, hi).tolist() == ref, kw
     eng = ladder_gpu.GpuEngine(n, q2=q2)
     for cut in (lo + 1, lo + 12_345_678, hi - 1):
@@ -224,7 +229,8 @@ def test_streams_do_not_depend_on_cuts_or_tiles():
 def test_streams_do_not_depend_on_tile_fit():
     """Windows whose lines are about 45k and 90k positions long, so the
     planner fits them with three- and six-word tiles (256 threads) or
-    three-word tiles (512 threads), against the default shape."""
+    three-word tiles (512 threads) on the line sieve, against the default
+    engine."""
     import ladder_gpu
     n, q2 = 10, 1 << 16
     say("tile fit")
@@ -237,7 +243,7 @@ def test_streams_do_not_depend_on_tile_fit():
                    {"threads": 256, "words_per_thread": 6},
                    {"threads": 256, "words_per_thread": 3},
                    {"threads": 512, "words_per_thread": 3}):
-            eng = ladder_gpu.GpuEngine(n, q2=q2, **kw)
+            eng = ladder_gpu.GpuEngine(n, q2=q2, short_words=1, **kw)
             assert eng.survivors_j(lo, hi).tolist() == ref, (npos, kw)
 
 
@@ -281,3 +287,44 @@ def test_round_kernels_match_oracle(case):
         got = N.record_values(eng.sweep(lo, hi))
         eng.close()
         assert got == want, (label, extra)
+
+
+# ------------------------------------------------ short lines
+
+SHORT_CASES = [
+    # (label, family, n, q2, lo, engine kwargs, words per line): line steps
+    # 96577, 2*...*17 = 510510 and 510510; the oracle finds about 6, 40 and
+    # 30 survivors per word of line; the Euler engine stride-marks [64, 1024)
+    ("dickson", N.FAMILY_DICKSON, 10, 1 << 14, 10**12 + 12_345, {},
+     (1, 2, 8, 9, 16, 17, 64, 65)),
+    ("ap", N.FAMILY_AP, 12, 1 << 14, 4 * 10**13 - 10, {"nwheel": 7},
+     (1, 8, 9, 65)),
+    ("euler-marked", N.FAMILY_EULER, 7, 1 << 14, 10**12,
+     {"wheel_primes": [2, 3, 5, 7, 11, 13, 17], "q_pat": 64, "q_mark": 1024,
+      "q_med": 1024}, (1, 8, 9, 64, 65)),
+]
+
+
+@pytest.mark.parametrize("case", SHORT_CASES, ids=[c[0] for c in SHORT_CASES])
+def test_short_lines_match_oracle(case):
+    """Windows whose lines hold 1 to 65 words of 64 positions, with ragged
+    ends, against the oracle: with short_words=64 lines of up to 64 words
+    are sieved one lane per eight words, with short_words=1 every line of
+    more than one word goes to the line sieve."""
+    label, fam, n, q2, lo0, kw, words_list = case
+    short = N.Engine(fam, n, q2, short_words=64, **kw)
+    line = N.Engine(fam, n, q2, short_words=1, **kw)
+    try:
+        assert short.info.short_words == 64 and line.info.short_words == 1
+        D = short.line_step
+        for words in words_list:
+            lo = lo0 + 3 * words
+            hi = lo + words * 64 * D - 37 * words
+            say("short lines", label, words, lo, hi)
+            want = round_case_oracle(fam, n, q2, lo, hi, kw)
+            assert len(want) >= 3 * words, (label, words, len(want))
+            assert N.record_values(short.sweep(lo, hi)) == want, (label, words)
+            assert N.record_values(line.sweep(lo, hi)) == want, (label, words)
+    finally:
+        short.close()
+        line.close()

@@ -1,0 +1,68 @@
+"""The engine's periodic kill bitmaps (first-knowns_amd/csrc/fk_plan.cpp,
+built here with g++): the 64-bit window the sieve kernels funnel-shift out
+of three consecutive words at any start x < q holds exactly the kill bits
+of positions x .. x + 63 taken mod q, for primes below and above 64 (below
+64 the window wraps the period more than once), and bitmaps appended to one
+table keep their own offsets."""
+import pathlib
+import random
+import subprocess
+
+import pytest
+
+HERE = pathlib.Path(__file__).resolve().parent
+CSRC = HERE.parent / "first-knowns_amd" / "csrc"
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    exe = tmp_path_factory.mktemp("plan") / "plan_harness"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", str(exe),
+                    str(HERE / "plan_harness.cpp"), str(CSRC / "fk_plan.cpp")],
+                   check=True)
+    return exe
+
+
+def run(harness, parts):
+    line = "B " + " ; ".join(" ".join(str(v) for v in [q] + ks)
+                             for q, ks in parts) + "\n"
+    out = subprocess.run([str(harness)], input=line, capture_output=True,
+                         text=True, check=True).stdout.splitlines()
+    offs = [int(v) for v in out[0].split()[1:]]
+    words = [int(v) for v in out[1].split()]
+    return offs, words
+
+
+def window(words, base, x):
+    """The kernels' read: words i, i+1, i+2 from base, i = x >> 5, shifted
+    right by x & 31 (two 32-bit funnel shifts)."""
+    i, o = x >> 5, x & 31
+    w0, w1, w2 = (words[base + i + k] for k in range(3))
+    lo = ((w1 << 32 | w0) >> o) & 0xffffffff
+    hi = ((w2 << 32 | w1) >> o) & 0xffffffff
+    return hi << 32 | lo
+
+
+@pytest.mark.parametrize("q", [2, 3, 5, 17, 31, 37, 61, 67, 127, 131, 1021])
+def test_window_is_the_periodic_kill_set(harness, q):
+    rng = random.Random(q)
+    kills = sorted(rng.sample(range(q), min(q - 1, rng.randint(1, 12))))
+    q2 = 29
+    kills2 = [0, 5, 28]
+    offs, words = run(harness, [(q, kills), (q2, kills2)])
+    assert offs == [0, (q - 1) // 32 + 3]
+    assert len(words) == offs[1] + (q2 - 1) // 32 + 3
+    for base, p, ks in ((offs[0], q, kills), (offs[1], q2, kills2)):
+        kset = set(ks)
+        for x in range(p):
+            want = sum(1 << b for b in range(64) if (x + b) % p in kset)
+            assert window(words, base, x) == want, (p, x)
+
+
+def test_window_past_one_repeat_of_the_period(harness):
+    # q = 17, kill residue 3: the window at x = 16 must hold the kills at
+    # 20, 37, 54 and 71; a tail holding one repeat of the period (bits
+    # q + r) would have only the first of them
+    offs, words = run(harness, [(17, [3])])
+    w = window(words, 0, 16)
+    assert (w >> (37 - 16)) & 1 and (w >> (54 - 16)) & 1 and (w >> (71 - 16)) & 1
