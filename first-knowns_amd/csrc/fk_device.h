@@ -86,9 +86,6 @@ struct DevPiece {
                               // tile per S positions of a line
 };
 
-// Words of one line a short-line sieve lane covers.
-constexpr uint32_t kShortChunk = 8;
-
 // A candidate in flight: the value the kill tests and the strong test need
 // (Euler p, Dickson k = W j, AP p) and its window.
 struct DevRec {

@@ -505,7 +505,8 @@ static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
                             st.p_bq, st.p_zero, st.p_nzero, s);
             break;
         case Step::kSieve:
-            e = st.wpt == 0 ? launch_sieve_short(st.sa, st.grid, st.lds, s)
+            e = st.wpt == 0 ? launch_sieve_short(st.sa, st.threads, st.grid,
+                                                 st.lds, s)
                             : launch_sieve(st.sa, st.threads, st.wpt, st.grid,
                                            st.lds, s);
             break;
@@ -696,6 +697,12 @@ static double round_survival(const fk_engine* e, double lo, double hi)
 // one pass
 static constexpr uint32_t kSingleLimit = 8192;
 
+// short-line sieve: chunk sizes 1, 2, 4 .. kShortChunkMax words per lane,
+// chosen per window by instructions per lane: about kShortStartCost to form
+// a chunk's start in a prime's coordinates, kShortWordCost per word
+static constexpr uint32_t kShortChunkMax = 16;
+static constexpr uint64_t kShortStartCost = 25, kShortWordCost = 11;
+
 // lane-per-candidate rounds: workgroups to aim for (candidate blocks times
 // prime blocks)
 static constexpr uint32_t kRoundWorkgroups = 4096;
@@ -730,7 +737,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     // holds a line), and each group of windows with one shape is one sieve
     // launch over a contiguous range of tiles
     const uint32_t T = e0->threads;
-    struct WinPlan { u128 B; uint64_t H; uint32_t wpt; };
+    struct WinPlan { u128 B; uint64_t H; uint32_t wpt, chunk; };
     std::vector<WinPlan> wp(nwin);
     for (size_t w = 0; w < nwin; w++) {
         const fk_engine* e = b->engines[b->engine_of[w]];
@@ -739,11 +746,23 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         wp[w].B = B;
         wp[w].H = hi > B ? (uint64_t)(hi - B) : 0;
         const uint64_t npos = wp[w].H / e->D + 1;
-        // lines of a few words: one lane per chunk of a line (k_sieve_short)
+        // lines of a few words: one lane per chunk of a line (k_sieve_short),
+        // the chunk with the least work per line: a start per prime and
+        // chunk, a bitmap window per prime and word (padded words included)
         const uint64_t npw = (npos + 63) / 64;
-        if (npw <= e->short_words &&
-            (uint64_t)e->offs.size() * ((npw + kShortChunk - 1) / kShortChunk)
-                < (1ull << 32)) {
+        wp[w].chunk = 0;
+        if (npw <= e->short_words) {
+            uint64_t best = ~0ull;
+            for (uint32_t c = 1; c <= kShortChunkMax; c *= 2) {
+                const uint64_t nch = (npw + c - 1) / c;
+                const uint64_t cost = nch * (kShortStartCost + c * kShortWordCost);
+                if (cost < best && (uint64_t)e->offs.size() * nch < (1ull << 32)) {
+                    best = cost;
+                    wp[w].chunk = c;
+                }
+            }
+        }
+        if (wp[w].chunk) {
             wp[w].wpt = 0;
             continue;
         }
@@ -767,9 +786,10 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     std::vector<uint32_t> order(nwin);
     for (size_t w = 0; w < nwin; w++) order[w] = (uint32_t)w;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-        return wp[x].wpt > wp[y].wpt;
+        return wp[x].wpt != wp[y].wpt ? wp[x].wpt > wp[y].wpt
+                                      : wp[x].chunk > wp[y].chunk;
     });
-    struct Group { uint32_t wpt; uint64_t tile_begin, tile_end; };
+    struct Group { uint32_t wpt, chunk; uint64_t tile_begin, tile_end; };
     std::vector<Group> groups;
 
     std::vector<DevPiece> pieces;
@@ -798,8 +818,9 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         if (hi <= lo) continue;
         if (wp[w].wpt == 0) {
             // short lines: one piece, tiles of 256 (line, chunk) items
-            if (groups.empty() || groups.back().wpt != 0)
-                groups.push_back({0u, tiles, tiles});
+            if (groups.empty() || groups.back().wpt != 0 ||
+                groups.back().chunk != wp[w].chunk)
+                groups.push_back({0u, wp[w].chunk, tiles, tiles});
             const uint64_t npos_all = wp[w].H / e->D + 1;
             DevPiece P{};
             for (int i = 0; i < 4; i++) P.base[i] = (uint32_t)(wp[w].B >> (32 * i));
@@ -807,7 +828,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             P.hi_rel = (uint64_t)(hi - wp[w].B);
             P.npos = (uint32_t)npos_all;
             P.npw = (uint32_t)((npos_all + 63) / 64);
-            const uint64_t nch = (P.npw + kShortChunk - 1) / kShortChunk;
+            const uint64_t nch = (P.npw + wp[w].chunk - 1) / wp[w].chunk;
             P.nseg = (uint32_t)(((uint64_t)e->offs.size() * nch + 255) / 256);
             P.tile0 = (uint32_t)tiles;
             P.engine = b->engine_of[w];
@@ -830,7 +851,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         const uint64_t S = (uint64_t)64 * T * wp[w].wpt;
         const uint64_t PIECE = std::max<uint64_t>(S, ((uint64_t)1 << 31) / S * S);
         if (groups.empty() || groups.back().wpt != wp[w].wpt)
-            groups.push_back({wp[w].wpt, tiles, tiles});
+            groups.push_back({wp[w].wpt, 0u, tiles, tiles});
         const uint64_t npos_all = wp[w].H / e->D + 1;
         for (uint64_t p0 = 0; p0 < npos_all; p0 += PIECE) {
             DevPiece P{};
@@ -995,11 +1016,11 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             if (lds > 160 * 1024)
                 return fail(FK_E_ARG, "LDS need %zu bytes > 160 KiB (lower q_med"
                             " or q_pat)", lds);
-            int occ = sieve_short_occupancy(lds);
+            int occ = sieve_short_occupancy((int)g.chunk, lds);
             if (occ < 1) occ = 1;
             uint64_t grid = (uint64_t)occ * ctx->cus;
             grid = std::min<uint64_t>(grid, g.tile_end - g.tile_begin);
-            st.threads = 256;
+            st.threads = (int)g.chunk;      // short lines: the chunk
             st.wpt = 0;
             st.grid = (uint32_t)grid;
             st.lds = lds;

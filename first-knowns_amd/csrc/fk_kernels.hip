@@ -450,27 +450,61 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
 // ---------------------------------------------------- sieve, short lines
 
 // Windows whose lines hold only a few 64-position words (a short window
-// over a large wheel): one lane per chunk of kShortChunk consecutive words
-// of one line instead of one workgroup per line.  A tile here is 256
-// consecutive (line, chunk) items of one piece, lines major.  For every LDS
-// prime the lane forms its first word's position in the prime's kill
-// coordinates (the work the line kernel does once per tile), then walks its
-// words in steps of 64 mod q, ORing in the 64-bit window of the prime's
-// periodic kill bitmap.  Bitmaps for every prime (no 64-bit pattern
-// tables, which take 64 times the LDS) keep the workgroup's LDS to a few
-// KB, so that many workgroups share a CU.  Surviving bits become records
-// with one atomic per wave.  No survivor lists and no barrier but the
-// engine-table load.
+// over a large wheel): one lane per chunk of C consecutive words of one
+// line instead of one workgroup per line.  A tile here is 256 consecutive
+// (line, chunk) items of one piece, lines major.  For every LDS prime the
+// lane forms its first word's position in the prime's kill coordinates
+// (the work the line kernel does once per tile), then walks its C words in
+// steps of 64 mod q, ORing in the 64-bit window of the prime's periodic
+// kill bitmap.  Bitmaps for every prime (no 64-bit pattern tables, which
+// take 64 times the LDS) keep the workgroup's LDS to a few KB, so that many
+// workgroups share a CU.  Surviving bits become records with one atomic
+// per wave.  No survivor lists; barriers only where the tile's piece
+// changes.
+
+// The chunk's C words ORed with every LDS prime's bitmap window.  x0 is
+// (v0 + (B + off) D^-1) mod q: with D < 2^32, off mod q comes from a 32-bit
+// reduction, and every product below stays under 2^32 because the LDS
+// primes are below 2^16.
+template <int C, bool D32>
+__device__ __forceinline__ void short_words(const uint4* pinfo, const uint4* pk,
+                                            const uint32_t* bm, uint32_t nlds,
+                                            uint64_t off, uint32_t v0,
+                                            uint64_t (&acc)[C])
+{
+    for (uint32_t p = 0; p < nlds; p++) {
+        const uint4 pq = pinfo[p];
+        const uint4 kk = pk[p];
+        const uint32_t q = pq.x, m32 = pq.y;
+        const uint32_t r = D32 ? red32((uint32_t)off, q, m32)
+                               : red64(off, q, ((uint64_t)kk.y << 32) | kk.x);
+        uint32_t t = kk.w + r;
+        if (t >= q) t -= q;
+        t = red32(t * kk.z, q, m32);
+        uint32_t x = red32(v0 + t, q, m32);
+        const uint32_t* b = bm + pq.z;
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+            const uint32_t i = x >> 5, o = x & 31u;
+            const uint32_t w0 = b[i], w1 = b[i + 1], w2 = b[i + 2];
+            acc[k] |= ((uint64_t)__builtin_amdgcn_alignbit(w2, w1, o) << 32) |
+                      __builtin_amdgcn_alignbit(w1, w0, o);
+            x += pq.w;
+            if (x >= q) x -= q;
+        }
+    }
+}
+
+template <int C>
 __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
 {
     extern __shared__ uint64_t lds64[];
     uint4* pinfo = reinterpret_cast<uint4*>(lds64);  // {q, m32, bitmap, 64 mod q}
-    uint4* pk = pinfo + a.nlds_max;                  // {m64 lo, m64 hi, D^-1, 0}
+    uint4* pk = pinfo + a.nlds_max;       // {m64 lo, m64 hi, D^-1, B mod q}
     uint32_t* bm = reinterpret_cast<uint32_t*>(pk + a.nlds_max);   // bm_max
-    constexpr uint32_t C = kShortChunk;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
-    uint32_t cur_engine = 0xffffffffu;
+    uint32_t cur_engine = 0xffffffffu, cur_piece = 0xffffffffu;
     const uint64_t tiles_per =
         ((uint64_t)(a.ntiles - a.tile_begin) + gridDim.x - 1) / gridDim.x;
     const uint64_t tile_lo = a.tile_begin + (uint64_t)blockIdx.x * tiles_per;
@@ -479,28 +513,38 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
         const uint32_t pi = find_piece(uniform_view(a.pieces), a.npieces, tile);
         const FK_CONST DevPiece& P = uniform_view(a.pieces)[pi];
         const FK_CONST DevEngine& E = uniform_view(a.engines)[P.engine];
-        if (P.engine != cur_engine) {
+        if (pi != cur_piece) {
+            // the engine's tables when the engine changes, the piece's
+            // B mod q always
             __syncthreads();
-            for (uint32_t w = tid; w < E.sbm_words; w += 256) bm[w] = E.sbm[w];
+            const bool tables = P.engine != cur_engine;
+            if (tables)
+                for (uint32_t w = tid; w < E.sbm_words; w += 256) bm[w] = E.sbm[w];
             for (uint32_t p = tid; p < E.nlds; p += 256) {
-                const uint32_t q = E.lds_q[p];
-                const uint64_t m64 = E.lds_m64[p];
-                pinfo[p] = make_uint4(q, E.lds_m32[p], E.sbm_off[p], 64u % q);
-                pk[p] = make_uint4((uint32_t)m64, (uint32_t)(m64 >> 32),
-                                   E.lds_dinv[p], 0u);
+                const uint32_t bq = a.bq[P.bq_off + p];
+                if (tables) {
+                    const uint32_t q = E.lds_q[p];
+                    const uint64_t m64 = E.lds_m64[p];
+                    pinfo[p] = make_uint4(q, E.lds_m32[p], E.sbm_off[p], 64u % q);
+                    pk[p] = make_uint4((uint32_t)m64, (uint32_t)(m64 >> 32),
+                                       E.lds_dinv[p], bq);
+                } else {
+                    pk[p].w = bq;
+                }
             }
             cur_engine = P.engine;
+            cur_piece = pi;
             __syncthreads();
         }
         const uint32_t nch = (P.npw + C - 1) / C;
         const uint32_t item = (tile - P.tile0) * 256u + tid;
         const uint32_t line = item / nch;
-        const uint64_t v0 = 64ull * C * (item - line * nch);
+        const uint32_t v0 = 64u * C * (item - line * nch);
         uint64_t acc[C];
 #pragma unroll
-        for (uint32_t k = 0; k < C; k++) acc[k] = ~0ull;
+        for (int k = 0; k < C; k++) acc[k] = ~0ull;
         uint64_t off = 0;
-        uint32_t nk = 0;            // words of this chunk before the line's end
+        bool here = false;
         if (line < E.nlines && v0 < P.npos) {
             off = E.offs[line];
             const uint64_t D = E.D;
@@ -510,43 +554,24 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
                 (P.hi_rel > off) ? ceil_div_u64(P.hi_rel - off, D) : 0;
             const uint64_t vend = (vhi < (uint64_t)P.npos) ? vhi : (uint64_t)P.npos;
             if (vlo < vend && v0 < vend && v0 + 64 * C > vlo) {
-                nk = (uint32_t)min((uint64_t)C, (vend - v0 + 63) / 64);
+                here = true;
+                // words at or past vend come out all dead
 #pragma unroll
-                for (uint32_t k = 0; k < C; k++)
-                    acc[k] = edge_mask(v0 + 64 * k, vlo, vend);
+                for (int k = 0; k < C; k++)
+                    acc[k] = edge_mask(v0 + 64u * k, vlo, vend);
             }
         }
-        if (nk) {
-            // the chunk's first word in prime p's coordinates:
-            // x = (v0 + (B + off) D^-1) mod q
-            for (uint32_t p = 0; p < E.nlds; p++) {
-                const uint4 pq = pinfo[p];
-                const uint4 kk = pk[p];
-                const uint32_t q = pq.x;
-                const uint64_t m64 = ((uint64_t)kk.y << 32) | kk.x;
-                uint32_t t = a.bq[P.bq_off + p] + red64(off, q, m64);
-                if (t >= q) t -= q;
-                t = mulmod32(t, kk.z, q, m64);
-                uint32_t x = red32((uint32_t)v0 + t, q, pq.y);
-                const uint32_t* b = bm + pq.z;
-#pragma unroll
-                for (uint32_t k = 0; k < C; k++) {
-                    if (k < nk) {
-                        const uint32_t i = x >> 5, o = x & 31u;
-                        const uint32_t w0 = b[i], w1 = b[i + 1], w2 = b[i + 2];
-                        acc[k] |= ((uint64_t)__builtin_amdgcn_alignbit(w2, w1, o) << 32) |
-                                  __builtin_amdgcn_alignbit(w1, w0, o);
-                    }
-                    x += pq.w;
-                    if (x >= q) x -= q;
-                }
-            }
+        if (here) {
+            if ((E.D >> 32) == 0)
+                short_words<C, true>(pinfo, pk, bm, E.nlds, off, v0, acc);
+            else
+                short_words<C, false>(pinfo, pk, bm, E.nlds, off, v0, acc);
         }
         // records: a wave-wide prefix of the per-lane counts, one atomic
         uint32_t c = 0;
 #pragma unroll
-        for (uint32_t k = 0; k < C; k++) {
-            acc[k] = (k < nk) ? ~acc[k] : 0ull;        // now: live bits
+        for (int k = 0; k < C; k++) {
+            acc[k] = ~acc[k];                           // now: live bits
             c += __popcll(acc[k]);
         }
         uint32_t incl = c;
@@ -567,12 +592,13 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
                 | (((uint64_t)P.base[1] << 32) | P.base[0]);
             uint32_t slot = base + incl - c;
 #pragma unroll
-            for (uint32_t k = 0; k < C; k++) {
+            for (int k = 0; k < C; k++) {
                 uint64_t m = acc[k];
                 while (m) {
                     const int bit = __builtin_ctzll(m);
                     m &= m - 1;
-                    emit_record(a, P, E, B, off, v0 + 64 * k + bit, slot++);
+                    emit_record(a, P, E, B, off, (uint64_t)v0 + 64u * k + bit,
+                                slot++);
                 }
             }
         }
@@ -1279,20 +1305,47 @@ static void allow_lds(const void* fn, size_t lds_bytes)
         done.push_back({fn, dev, lds_bytes});
 }
 
-hipError_t launch_sieve_short(const SieveArgs& a, uint32_t grid,
+static const void* sieve_short_fn(int chunk)
+{
+    switch (chunk) {
+    case 1: return (const void*)k_sieve_short<1>;
+    case 2: return (const void*)k_sieve_short<2>;
+    case 4: return (const void*)k_sieve_short<4>;
+    case 8: return (const void*)k_sieve_short<8>;
+    case 16: return (const void*)k_sieve_short<16>;
+    default: return nullptr;
+    }
+}
+
+bool sieve_short_chunk_ok(int chunk) { return sieve_short_fn(chunk) != nullptr; }
+
+hipError_t launch_sieve_short(const SieveArgs& a, int chunk, uint32_t grid,
                               size_t lds_bytes, hipStream_t stream)
 {
-    allow_lds((const void*)k_sieve_short, lds_bytes);
-    hipLaunchKernelGGL(k_sieve_short, dim3(grid), dim3(256), lds_bytes, stream, a);
+    const void* fn = sieve_short_fn(chunk);
+    if (!fn) return hipErrorInvalidValue;
+    allow_lds(fn, lds_bytes);
+#define FK_SHORT_CASE(CC)                                                  \
+    case CC:                                                               \
+        hipLaunchKernelGGL(k_sieve_short<CC>, dim3(grid), dim3(256),        \
+                           lds_bytes, stream, a);                          \
+        break;
+    switch (chunk) {
+        FK_SHORT_CASE(1) FK_SHORT_CASE(2) FK_SHORT_CASE(4) FK_SHORT_CASE(8)
+        FK_SHORT_CASE(16)
+    }
+#undef FK_SHORT_CASE
     return hipGetLastError();
 }
 
-int sieve_short_occupancy(size_t lds_bytes)
+int sieve_short_occupancy(int chunk, size_t lds_bytes)
 {
+    const void* fn = sieve_short_fn(chunk);
+    if (!fn) return 0;
     int blocks = 0;
-    allow_lds((const void*)k_sieve_short, lds_bytes);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, k_sieve_short, 256, lds_bytes) != hipSuccess)
+    allow_lds(fn, lds_bytes);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, 256,
+                                                     lds_bytes) != hipSuccess)
         blocks = 0;
     return blocks;
 }

@@ -100,10 +100,12 @@ hipError_t launch_prep(const DevEngine* engines, const DevPiece* pieces,
 hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,
                         uint32_t grid, size_t lds_bytes, hipStream_t stream);
 int sieve_occupancy(int threads, int wpt, size_t lds_bytes);
-// windows with short lines: one lane per (line, word), 256-item tiles
-hipError_t launch_sieve_short(const SieveArgs& a, uint32_t grid,
+// windows with short lines: one lane per chunk of `chunk` words of a line
+// (1, 2, 4, 8 or 16), 256-item tiles
+hipError_t launch_sieve_short(const SieveArgs& a, int chunk, uint32_t grid,
                               size_t lds_bytes, hipStream_t stream);
-int sieve_short_occupancy(size_t lds_bytes);
+int sieve_short_occupancy(int chunk, size_t lds_bytes);
+bool sieve_short_chunk_ok(int chunk);
 bool sieve_shape_ok(int threads, int wpt);
 hipError_t launch_round_test(const RoundArgs& a, bool lane_prime, uint32_t gx,
                              uint32_t gy, hipStream_t stream);

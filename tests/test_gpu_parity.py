@@ -328,3 +328,25 @@ def test_short_lines_match_oracle(case):
     finally:
         short.close()
         line.close()
+
+
+def test_short_lines_wheel_period_past_32_bits():
+    """A wheel of the ten primes 2..29 (period 6469693230 > 2^32, 7.6M
+    lines), eight positions per line: the short-line sieve's 64-bit start
+    path and the line sieve against the oracle."""
+    wp = [2, 3, 5, 7, 11, 13, 17, 19, 23, 29]
+    n, q2, lo = 7, 1 << 14, 10**12 + 5
+    short = N.Engine(N.FAMILY_EULER, n, q2, wheel_primes=wp)
+    line = N.Engine(N.FAMILY_EULER, n, q2, wheel_primes=wp, short_words=1)
+    try:
+        D = short.line_step
+        assert D > 1 << 32
+        hi = lo + 8 * D - 37
+        say("short lines, wide wheel", lo, hi)
+        want = O.euler_survivors(n, lo, hi, wheel_primes=wp, q2=q2)
+        assert len(want) > 10_000
+        assert N.record_values(short.sweep(lo, hi)) == want
+        assert N.record_values(line.sweep(lo, hi)) == want
+    finally:
+        short.close()
+        line.close()
