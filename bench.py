@@ -9,8 +9,9 @@ second over the timed steps, inputs resident on the device (the window is
 two integers; nothing is copied in per step).
 
 Beside it, in the same JSON line:
-  roofline      the LDS tile-sieve kernel (k_sieve), timed with HIP events
-                on its own stream inside the timed region
+  roofline      the sieve kernel the window runs (its lines of 695 words
+                take the short-line sieve, k_sieve_short), timed with HIP
+                events on the engine's stream
   cpu_baseline  the oracle's restatement of the reference CPU sieve
                 (dickson-ladders/ladder_search.py:92-112) on a bounded sample
                 of the same window on this host's cores, plus the
@@ -53,14 +54,14 @@ def splitmix64(x):
     return z ^ (z >> 31)
 
 
-# ------------------------------------------------------------- config 1
+# ------------------------------------ headline: the Dickson ladder sweep
 
-C1_N, C1_Q2, C1_SPAN, C1_CAP = 10, 1 << 22, 1 << 32, 18
+SWEEP_N, SWEEP_Q2, SWEEP_SPAN, SWEEP_CAP = 10, 1 << 22, 1 << 32, 18
 
 
-def c1_window(seed):
+def sweep_window(seed):
     j_lo = 432_900_432_900 + splitmix64(seed) % 10**11
-    return j_lo, j_lo + C1_SPAN
+    return j_lo, j_lo + SWEEP_SPAN
 
 
 def run_plain(steps, warmup):
@@ -71,15 +72,15 @@ def run_plain(steps, warmup):
     between the stages, outside the timed loop."""
     import fk_native as N
     import ladder_gpu
-    eng = ladder_gpu.GpuEngine(C1_N, q2=C1_Q2)
+    eng = ladder_gpu.GpuEngine(SWEEP_N, q2=SWEEP_Q2)
     for s in range(warmup):
-        lo, hi = c1_window(1000 + s)
-        eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, C1_CAP)
+        lo, hi = sweep_window(1000 + s)
+        eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, SWEEP_CAP)
     out = []
     t0 = time.perf_counter()
     for s in range(steps):
-        lo, hi = c1_window(s)
-        out.append(eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, C1_CAP))
+        lo, hi = sweep_window(s)
+        out.append(eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, SWEEP_CAP))
     wall = time.perf_counter() - t0
     device_ms = eng._eng.last_timing()[0][4]      # the last timed sweep
     surv = sum(int(r.size) for r in out)
@@ -95,8 +96,8 @@ def run_plain(steps, warmup):
     nst = min(3, steps)
     cand = 0
     for s in range(nst):
-        lo, hi = c1_window(s)
-        eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, C1_CAP)
+        lo, hi = sweep_window(s)
+        eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, SWEEP_CAP)
         ms, _, c = eng._eng.last_timing()
         cand += c
         for i in range(5):
@@ -109,30 +110,30 @@ def cpu_baseline(eng, sample_log2, threads):
     import fk_native as N
     sys.path.insert(0, str(ROOT / "oracle"))
     import fk_oracle as O
-    lo, _ = c1_window(0)
+    lo, _ = sweep_window(0)
     hi = lo + (1 << sample_log2)
-    sieve = O.DicksonSieve(C1_N, C1_Q2)            # tables outside the clock
+    sieve = O.DicksonSieve(SWEEP_N, SWEEP_Q2)            # tables outside the clock
     t0 = time.perf_counter()
     js = sieve.survivors(lo, hi, nthreads=threads)
-    runs = [O.dickson_run(j * eng.W, cap=C1_CAP) for j in js]
+    runs = [O.dickson_run(j * eng.W, cap=SWEEP_CAP) for j in js]
     wall = time.perf_counter() - t0
-    recs = eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, C1_CAP)
+    recs = eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, SWEEP_CAP)
     same = (N.record_values(recs) == js and
             [int(r) for r in recs["run"].tolist()] == runs)
     return {"value": (hi - lo) / wall, "unit": "j-candidates/s",
             "cores": threads, "kind": "port",
-            "sample": f"2^{sample_log2} j from the step-0 window, n={C1_N}, "
-                      f"q2=2^22, survivors classified (cap {C1_CAP}); "
+            "sample": f"2^{sample_log2} j from the step-0 window, n={SWEEP_N}, "
+                      f"q2=2^22, survivors classified (cap {SWEEP_CAP}); "
                       f"{len(js)} survivors, {wall:.1f} s",
             "sample_stream_equal": bool(same)}
 
 
-# ------------------------------------------------------------- config 3
+# ------------------------------------ 256 mixed windows, one batch
 
 FAMILY_Q2 = {"euler": 1 << 16, "dickson": 1 << 22, "ap": 1 << 24}
 
 
-def c3_windows(seed=3, count=256):
+def batch_windows(seed=3, count=256):
     """256 windows: family uniform, length 2^(20 + 10 U) on the family's
     line, n uniform in 8..14, heights spread so the strong tests run on
     64-, 128- and 192-bit values."""
@@ -160,7 +161,7 @@ def c3_windows(seed=3, count=256):
     return out
 
 
-def c3_spec(fam, n):
+def batch_spec(fam, n):
     import fk_native as N
     if fam == "euler":
         return (N.CLASSIFY_FULL, n + 3, 0)
@@ -172,7 +173,7 @@ def c3_spec(fam, n):
 def run_batched():
     import fk_native as N
     import fk_families as F
-    wins = c3_windows()
+    wins = batch_windows()
     engines, index = [], {}
     fam_id = {"euler": N.FAMILY_EULER, "dickson": N.FAMILY_DICKSON,
               "ap": N.FAMILY_AP}
@@ -187,7 +188,7 @@ def run_batched():
                                     **t))
     eo = [index[(f, n)] for f, n, _, _ in wins]
     ws = [(lo, hi) for _, _, lo, hi in wins]
-    specs = [c3_spec(f, n) for f, n, _, _ in wins]
+    specs = [batch_spec(f, n) for f, n, _, _ in wins]
     cand = sum(hi - lo for lo, hi in ws)
     # plain: one engine call per window
     for i in range(3):
@@ -218,7 +219,7 @@ def run_batched():
             "speedup": t_plain / t_batch, "streams_equal": same}
 
 
-# ------------------------------------------------------------- config 4
+# ------------------------------------ serving: Poisson arrivals
 
 def run_serving(count, rate, seed=4):
     """Poisson arrivals of range-sweep requests of all three families,
@@ -239,15 +240,34 @@ def run_serving(count, rate, seed=4):
     return st
 
 
-def rocprof_traffic():
-    """HBM bytes per k_sieve dispatch from the committed PMC summary."""
-    p = ROOT / "profiles" / "k_sieve_dickson_c1_traffic.json"
-    if p.exists():
+TRAFFIC_JSON = ROOT / "profiles" / "k_sieve_short_dickson_sweep_traffic.json"
+
+
+def rocprof_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed PMC summary
+    (None when the summary covers another kernel)."""
+    if TRAFFIC_JSON.exists():
         try:
-            return json.loads(p.read_text()).get("bytes_per_dispatch")
+            js = json.loads(TRAFFIC_JSON.read_text())
         except (OSError, ValueError):
             return None
+        if kernel in js.get("kernel", ""):
+            return js.get("bytes_per_dispatch")
     return None
+
+
+def sieve_kernel(info, span):
+    """The sieve kernel a window of `span` line values runs on this engine
+    and its algorithmic LDS bytes per wheel candidate.  Lines of at most
+    short_words 64-position words take k_sieve_short: one 64-bit window of
+    a kill bitmap (8 B) per word and LDS prime.  Longer lines take k_sieve:
+    8 B per pattern word read per pattern prime, 8 B (read + write) per
+    stride mark."""
+    words = (span // info.line_step + 1 + 63) // 64
+    if words <= info.short_words:
+        return "k_sieve_short", 8.0 * info.n_lds_primes / 64
+    return "k_sieve", (8.0 * info.n_pattern_primes / 64
+                       + 8.0 * info.mark_density)
 
 
 def copy_probe(mib):
@@ -289,17 +309,15 @@ def main():
 
     eng, wall, surv, stage, cand, deepest, device_ms = run_plain(args.steps,
                                                                 args.warmup)
-    j_total = args.steps * C1_SPAN
+    j_total = args.steps * SWEEP_SPAN
     value = j_total / wall
     info = eng._eng.info
-    # algorithmic LDS bytes of k_sieve: 8 B per pattern word read per
-    # pattern prime, 8 B (read + write) per stride mark
-    per_cand = 8.0 * info.n_pattern_primes / 64 + 8.0 * info.mark_density
-    # one k_sieve launch per window: its duration is the eager sieve stage
+    kernel, per_cand = sieve_kernel(info, SWEEP_SPAN)
+    # one sieve launch per window: its duration is the eager sieve stage
     launch_s = stage[1] / 1e3
     per_launch = cand                    # wheel candidates per window
     achieved = per_launch * per_cand / launch_s if launch_s > 0 else 0.0
-    traffic = rocprof_traffic()
+    traffic = rocprof_traffic(kernel)
     out = {
         "metric": METRIC, "value": value, "unit": "j-candidates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -310,8 +328,8 @@ def main():
                                "2^32 consecutive j near k=1e15 per step, "
                                "n=10, primes < 2^22 sieved, full-bases "
                                "ladder depth of every survivor",
-                   "n": C1_N, "q2": C1_Q2, "window_j": C1_SPAN,
-                   "classify_cap": C1_CAP,
+                   "n": SWEEP_N, "q2": SWEEP_Q2, "window_j": SWEEP_SPAN,
+                   "classify_cap": SWEEP_CAP,
                    "wheel_lines": int(info.nlines),
                    "wheel_period": int(info.line_step),
                    "q_pat": int(info.q_pat), "q_med": int(info.q_med),
@@ -325,7 +343,7 @@ def main():
                        "after the timed loop; the timed sweeps replay the "
                        "engine's captured launch sequence (device_ms_per_step "
                        "is the last one's event-timed device time)",
-        "roofline": {"bound": "lds", "kernel": "k_sieve",
+        "roofline": {"bound": "lds", "kernel": kernel,
                      "achieved": achieved / 1e9, "peak": LDS_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / LDS_PEAK,
                      "traffic": traffic,

@@ -58,9 +58,9 @@ struct DevEngine {
     const uint32_t* bm_off;   // [nlds - nmark] first bitmap word per prime
     const uint32_t* bm;       // [bm_words] line-domain kill bitmaps, each
                               // q bits then its first 64 bits again
-    uint32_t sbm_words;       // kill bitmaps of every LDS prime, laid out
-    const uint32_t* sbm_off;  // as bm's: [nlds] first words, [sbm_words]
-    const uint32_t* sbm;      // bitmaps (the short-line sieve's tables)
+    uint32_t sbm_words;       // kill bitmaps of every LDS prime with a
+    const uint32_t* sbm_off;  // tail of kShortTailBits: [nlds] first words,
+    const uint32_t* sbm;      // [sbm_words] bitmaps (short-line sieve)
     uint32_t ntr;             // in-tile rounds over [nmark, nlds)
     const uint32_t* tr_end;   // [ntr] LDS-prime index where round r ends
     const uint32_t* kt;       // [round_end - round_first][kstride] round
@@ -85,6 +85,12 @@ struct DevPiece {
                               // tiles of 256 (line, chunk) items; 0: one
                               // tile per S positions of a line
 };
+
+// Short-line sieve: at most this many words per lane, and the bits its
+// bitmaps repeat past q, so that a lane's words are the windows at x,
+// x + 64, ... with no reduction mod q.
+constexpr uint32_t kShortChunkMax = 16;
+constexpr uint32_t kShortTailBits = 64 * kShortChunkMax;
 
 // A candidate in flight: the value the kill tests and the strong test need
 // (Euler p, Dickson k = W j, AP p) and its window.

@@ -292,7 +292,7 @@ static int build_engine(fk_engine* e)
         ldinv.push_back(dinv);
         e->kill_w.push_back((uint32_t)kl.size());
         surv *= 1.0 - (double)kl.size() / q;
-        append_kill_bitmap(kl, q, sbm_off, sbm);
+        append_kill_bitmap(kl, q, kShortTailBits, sbm_off, sbm);
         if (q < e->q_pat) {
             pat_off.push_back((uint32_t)pat.size());
             std::vector<uint8_t> dead(q, 0);
@@ -309,7 +309,7 @@ static int build_engine(fk_engine* e)
             e->mark_density += (double)kl.size() / q;
             nmark = idx + 1;
         } else {
-            append_kill_bitmap(kl, q, bm_off, bm);
+            append_kill_bitmap(kl, q, 64, bm_off, bm);
             if (q < e->q_bits) nbits = idx + 1;
         }
     }
@@ -700,8 +700,7 @@ static constexpr uint32_t kSingleLimit = 8192;
 // short-line sieve: chunk sizes 1, 2, 4 .. kShortChunkMax words per lane,
 // chosen per window by instructions per lane: about kShortStartCost to form
 // a chunk's start in a prime's coordinates, kShortWordCost per word
-static constexpr uint32_t kShortChunkMax = 16;
-static constexpr uint64_t kShortStartCost = 25, kShortWordCost = 11;
+static constexpr uint64_t kShortStartCost = 28, kShortWordCost = 5;
 
 // lane-per-candidate rounds: workgroups to aim for (candidate blocks times
 // prime blocks)

@@ -453,16 +453,18 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
 // over a large wheel): one lane per chunk of C consecutive words of one
 // line instead of one workgroup per line.  A tile here is 256 consecutive
 // (line, chunk) items of one piece, lines major.  For every LDS prime the
-// lane forms its first word's position in the prime's kill coordinates
-// (the work the line kernel does once per tile), then walks its C words in
-// steps of 64 mod q, ORing in the 64-bit window of the prime's periodic
-// kill bitmap.  Bitmaps for every prime (no 64-bit pattern tables, which
-// take 64 times the LDS) keep the workgroup's LDS to a few KB, so that many
-// workgroups share a CU.  Surviving bits become records with one atomic
-// per wave.  No survivor lists; barriers only where the tile's piece
-// changes.
+// lane forms its first word's position x in the prime's kill coordinates
+// (the work the line kernel does once per tile); its C words are then the
+// 64-bit windows at x, x + 64, ... of the prime's periodic kill bitmap,
+// which repeats its period through kShortTailBits bits past q: 2C + 1
+// consecutive LDS words and two funnel shifts per word, with no reduction
+// mod q.  Bitmaps for every prime (no 64-bit pattern tables, which take 64
+// times the LDS) keep the workgroup's LDS to a few tens of KB, so that
+// many workgroups share a CU.  Surviving bits become records with one
+// atomic per wave.  No survivor lists; barriers only where the tile's
+// piece changes.
 
-// The chunk's C words ORed with every LDS prime's bitmap window.  x0 is
+// The chunk's C words ORed with every LDS prime's bitmap windows.  x is
 // (v0 + (B + off) D^-1) mod q: with D < 2^32, off mod q comes from a 32-bit
 // reduction, and every product below stays under 2^32 because the LDS
 // primes are below 2^16.
@@ -472,6 +474,7 @@ __device__ __forceinline__ void short_words(const uint4* pinfo, const uint4* pk,
                                             uint64_t off, uint32_t v0,
                                             uint64_t (&acc)[C])
 {
+    static_assert(C <= (int)kShortChunkMax, "chunk past the bitmap tail");
     for (uint32_t p = 0; p < nlds; p++) {
         const uint4 pq = pinfo[p];
         const uint4 kk = pk[p];
@@ -481,17 +484,17 @@ __device__ __forceinline__ void short_words(const uint4* pinfo, const uint4* pk,
         uint32_t t = kk.w + r;
         if (t >= q) t -= q;
         t = red32(t * kk.z, q, m32);
-        uint32_t x = red32(v0 + t, q, m32);
-        const uint32_t* b = bm + pq.z;
+        const uint32_t x = red32(v0 + t, q, m32);
+        const uint32_t* b = bm + pq.z + (x >> 5);
+        const uint32_t o = x & 31u;
+        uint32_t w[2 * C + 1];
 #pragma unroll
-        for (int k = 0; k < C; k++) {
-            const uint32_t i = x >> 5, o = x & 31u;
-            const uint32_t w0 = b[i], w1 = b[i + 1], w2 = b[i + 2];
-            acc[k] |= ((uint64_t)__builtin_amdgcn_alignbit(w2, w1, o) << 32) |
-                      __builtin_amdgcn_alignbit(w1, w0, o);
-            x += pq.w;
-            if (x >= q) x -= q;
-        }
+        for (int j = 0; j < 2 * C + 1; j++) w[j] = b[j];
+#pragma unroll
+        for (int k = 0; k < C; k++)
+            acc[k] |= ((uint64_t)__builtin_amdgcn_alignbit(w[2 * k + 2], w[2 * k + 1], o)
+                       << 32) |
+                      __builtin_amdgcn_alignbit(w[2 * k + 1], w[2 * k], o);
     }
 }
 
@@ -499,7 +502,7 @@ template <int C>
 __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
 {
     extern __shared__ uint64_t lds64[];
-    uint4* pinfo = reinterpret_cast<uint4*>(lds64);  // {q, m32, bitmap, 64 mod q}
+    uint4* pinfo = reinterpret_cast<uint4*>(lds64);  // {q, m32, bitmap, 0}
     uint4* pk = pinfo + a.nlds_max;       // {m64 lo, m64 hi, D^-1, B mod q}
     uint32_t* bm = reinterpret_cast<uint32_t*>(pk + a.nlds_max);   // bm_max
     const uint32_t tid = threadIdx.x;
@@ -525,7 +528,7 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
                 if (tables) {
                     const uint32_t q = E.lds_q[p];
                     const uint64_t m64 = E.lds_m64[p];
-                    pinfo[p] = make_uint4(q, E.lds_m32[p], E.sbm_off[p], 64u % q);
+                    pinfo[p] = make_uint4(q, E.lds_m32[p], E.sbm_off[p], 0u);
                     pk[p] = make_uint4((uint32_t)m64, (uint32_t)(m64 >> 32),
                                        E.lds_dinv[p], bq);
                 } else {

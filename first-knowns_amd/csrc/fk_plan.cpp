@@ -8,14 +8,14 @@
 namespace fk {
 
 void append_kill_bitmap(const std::vector<uint32_t>& kills, uint32_t q,
-                        std::vector<uint32_t>& offs,
+                        uint32_t tail, std::vector<uint32_t>& offs,
                         std::vector<uint32_t>& words)
 {
     offs.push_back((uint32_t)words.size());
     const size_t base = words.size();
-    words.resize(base + (q - 1) / 32 + 3, 0u);
+    words.resize(base + ((uint64_t)q - 1 + tail) / 32 + 1, 0u);
     for (uint32_t r : kills)
-        for (uint64_t b = r; b < (uint64_t)q + 64; b += q)
+        for (uint64_t b = r; b < (uint64_t)q + tail; b += q)
             words[base + (b >> 5)] |= 1u << (b & 31);
 }
 

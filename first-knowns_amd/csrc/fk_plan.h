@@ -32,11 +32,12 @@ uint32_t first_kill_prime(int family, int n);
 std::vector<uint32_t> kill_set(int family, int n, uint32_t q);
 
 // Appends prime q's periodic kill bitmap to `words` and its first word's
-// index to `offs`: bit b is set iff b mod q is in `kills`, for b < q + 64,
-// so that the 64-bit window at any x < q lies inside it, in
-// (q - 1) / 32 + 3 words (one more for the window's third word).
+// index to `offs`: bit b is set iff b mod q is in `kills`, for
+// b < q + tail (tail a multiple of 64), so that the 64-bit windows at
+// x, x + 64, ... below q + tail - 64 lie inside it, for any x < q, in
+// (q - 1 + tail) / 32 + 1 words (one more for a window's third word).
 void append_kill_bitmap(const std::vector<uint32_t>& kills, uint32_t q,
-                        std::vector<uint32_t>& offs,
+                        uint32_t tail, std::vector<uint32_t>& offs,
                         std::vector<uint32_t>& words);
 
 // Admissible residues mod prod(wheel) (CRT lifting, sorted) and that

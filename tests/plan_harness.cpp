@@ -1,7 +1,8 @@
 // Host build of the engine's table builders (first-knowns_amd/csrc/
 // fk_plan.cpp).  Reads lines
-//   B q r1 r2 ... ; q2 s1 s2 ...  -> two kill bitmaps appended to one
-//                                    table: "offsets o1 o2" then the words
+//   B tail q r1 r2 ... ; q2 s1 s2 ...
+//        -> kill bitmaps with that tail appended to one table:
+//           "offsets o1 o2 ..." then the words
 #include "../first-knowns_amd/csrc/fk_plan.h"
 #include <cstdio>
 #include <sstream>
@@ -16,6 +17,8 @@ int main()
         if (line.empty() || line[0] != 'B') continue;
         std::vector<uint32_t> offs, words;
         std::stringstream all(line.substr(1));
+        uint32_t tail = 0;
+        all >> tail;
         std::string part;
         while (std::getline(all, part, ';')) {
             std::stringstream ss(part);
@@ -23,7 +26,7 @@ int main()
             if (!(ss >> q)) continue;
             std::vector<uint32_t> kills;
             while (ss >> r) kills.push_back(r);
-            fk::append_kill_bitmap(kills, q, offs, words);
+            fk::append_kill_bitmap(kills, q, tail, offs, words);
         }
         printf("offsets");
         for (uint32_t o : offs) printf(" %u", o);

@@ -1,9 +1,11 @@
 """The engine's periodic kill bitmaps (first-knowns_amd/csrc/fk_plan.cpp,
 built here with g++): the 64-bit window the sieve kernels funnel-shift out
-of three consecutive words at any start x < q holds exactly the kill bits
-of positions x .. x + 63 taken mod q, for primes below and above 64 (below
-64 the window wraps the period more than once), and bitmaps appended to one
-table keep their own offsets."""
+of three consecutive words at x + 64 k, for any start x < q and every k
+below tail / 64, holds exactly the kill bits of positions x + 64 k ..
+x + 64 k + 63 taken mod q, for primes below and above 64 (below 64 a window
+wraps the period more than once), with the line sieve's tail of 64 bits
+and the short-line sieve's of 1024; bitmaps appended to one table keep
+their own offsets."""
 import pathlib
 import random
 import subprocess
@@ -23,9 +25,9 @@ def harness(tmp_path_factory):
     return exe
 
 
-def run(harness, parts):
-    line = "B " + " ; ".join(" ".join(str(v) for v in [q] + ks)
-                             for q, ks in parts) + "\n"
+def run(harness, parts, tail=64):
+    line = f"B {tail} " + " ; ".join(" ".join(str(v) for v in [q] + ks)
+                                     for q, ks in parts) + "\n"
     out = subprocess.run([str(harness)], input=line, capture_output=True,
                          text=True, check=True).stdout.splitlines()
     offs = [int(v) for v in out[0].split()[1:]]
@@ -43,20 +45,23 @@ def window(words, base, x):
     return hi << 32 | lo
 
 
+@pytest.mark.parametrize("tail", [64, 1024])
 @pytest.mark.parametrize("q", [2, 3, 5, 17, 31, 37, 61, 67, 127, 131, 1021])
-def test_window_is_the_periodic_kill_set(harness, q):
+def test_window_is_the_periodic_kill_set(harness, q, tail):
     rng = random.Random(q)
     kills = sorted(rng.sample(range(q), min(q - 1, rng.randint(1, 12))))
     q2 = 29
     kills2 = [0, 5, 28]
-    offs, words = run(harness, [(q, kills), (q2, kills2)])
-    assert offs == [0, (q - 1) // 32 + 3]
-    assert len(words) == offs[1] + (q2 - 1) // 32 + 3
+    offs, words = run(harness, [(q, kills), (q2, kills2)], tail)
+    assert offs == [0, (q - 1 + tail) // 32 + 1]
+    assert len(words) == offs[1] + (q2 - 1 + tail) // 32 + 1
     for base, p, ks in ((offs[0], q, kills), (offs[1], q2, kills2)):
         kset = set(ks)
         for x in range(p):
-            want = sum(1 << b for b in range(64) if (x + b) % p in kset)
-            assert window(words, base, x) == want, (p, x)
+            for k in range(tail // 64):
+                want = sum(1 << b for b in range(64)
+                           if (x + 64 * k + b) % p in kset)
+                assert window(words, base, x + 64 * k) == want, (p, x, k)
 
 
 def test_window_past_one_repeat_of_the_period(harness):
