@@ -16,6 +16,11 @@ tuner is free to pick whatever is fastest:
   nwheel                      kill primes enumerated by the wheel
   round_growth                prime-range growth factor between rounds
 
+A window whose lines hold at most short_words words (the engine's default,
+1024) takes the short-line sieve instead of k_sieve: there every LDS prime
+is a bitmap window and only q_med, nwheel and round_growth matter, so the
+tuning window's length decides which parameters the tuner can see.
+
 Coordinate descent from the current tuning (the saved one, else the family
 defaults): each pass tries every value of one parameter with the others
 held, keeps the fastest, and moves on; the result replaces the starting
@@ -48,7 +53,8 @@ SPACE = {
     "threads": [256, 512, 1024],
     "words_per_thread": [1, 2, 3, 4, 6, 8],
     "q_pat": [64, 128, 192, 256, 320],
-    "q_med": [512, 1024, 1536, 2048, 3072, 4096, 8192, 16384],
+    "q_med": [128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096,
+              8192, 16384],
     "nwheel": [0, 3, 4, 5, 6, 7],
     "round_growth": [2, 3, 4],
     "q_mark": [0, 192, 256, 512, 1024],
