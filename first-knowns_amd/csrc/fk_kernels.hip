@@ -975,7 +975,33 @@ __device__ __forceinline__ void lane_prime_slice(const LaneRow& L, bool mine,
                                                  uint32_t nlive, uint32_t* flags)
 {
     const uint32_t lane = __lane_id();
-    for (uint32_t c = 0; c < nlive; c++) {
+    // four candidates per step: four independent residue chains in flight
+    // per lane instead of one
+    constexpr uint32_t U = 4;
+    uint32_t c = 0;
+    for (; c + U <= nlive; c += U) {
+        bool kill[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) kill[u] = false;
+        if (mine) {
+            if (wide) {
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++)
+                    kill[u] = lane_kills<8, FOLD>(L, limbs + 8 * (c + u), ncmp);
+            } else {
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++)
+                    kill[u] = lane_kills<4, FOLD>(L, limbs + 8 * (c + u), ncmp);
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint64_t m = __ballot(kill[u]);
+            if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1))
+                flags[live_idx[c + u]] = 1;
+        }
+    }
+    for (; c < nlive; c++) {
         const double* a = limbs + 8 * c;
         bool kill = false;
         if (mine)
@@ -1074,17 +1100,26 @@ __global__ __launch_bounds__(256) void k_round_compact(CompactArgs a)
          blk += gridDim.x) {
         const uint32_t base = segs[sb.e].x;
         const uint32_t l0 = sb.lb * (256 * CP_PER);
+        // the records are loaded with their flags (one memory latency; the
+        // stores below then wait only for the atomic)
+        uint32_t flag[CP_PER];
+        DevRec rec[CP_PER];
+#pragma unroll
+        for (uint32_t k = 0; k < CP_PER; k++) {
+            const uint32_t li = l0 + k * 256 + threadIdx.x;
+            flag[k] = 1;
+            if (li < sb.n) {
+                flag[k] = a.flags[base + li];
+                rec[k] = a.in[base + li];
+            }
+        }
         uint64_t m[CP_PER];
         uint32_t tot = 0;
 #pragma unroll
         for (uint32_t k = 0; k < CP_PER; k++) {
             const uint32_t li = l0 + k * 256 + threadIdx.x;
-            bool alive = false;
-            if (li < sb.n) {
-                alive = a.flags[base + li] == 0;
-                a.flags[base + li] = 0;
-            }
-            m[k] = __ballot(alive);
+            if (li < sb.n) a.flags[base + li] = 0;
+            m[k] = __ballot(flag[k] == 0);
             tot += (uint32_t)__popcll((unsigned long long)m[k]);
         }
         if (lane == 0) wsum[wave] = tot;
@@ -1106,7 +1141,7 @@ __global__ __launch_bounds__(256) void k_round_compact(CompactArgs a)
                 const uint32_t below = __builtin_amdgcn_mbcnt_hi(
                     (uint32_t)(m[k] >> 32),
                     __builtin_amdgcn_mbcnt_lo((uint32_t)m[k], 0u));
-                a.out[off + below] = a.in[base + l0 + k * 256 + threadIdx.x];
+                a.out[off + below] = rec[k];
             }
             off += (uint32_t)__popcll((unsigned long long)m[k]);
         }
