@@ -37,9 +37,16 @@ import fk_checkpoint as C
 import fk_evidence as EV
 import fk_log as LG
 import fk_native as N
+import fk_verify as VF
 
 FAMILY_NAME = {N.FAMILY_EULER: "euler", N.FAMILY_DICKSON: "dickson",
                N.FAMILY_AP: "ap"}
+
+
+class CorruptEngineError(RuntimeError):
+    """A record the verification legs do not confirm.  The campaign stops
+    before the segment's cursor is written, as the reference's launchers
+    stop on an [ALARM] (dickson-ladders/launch.py:218, 923-925)."""
 
 
 def default_classify(family, n):
@@ -71,15 +78,21 @@ class Campaign:
 
     `batches` is for tests: two objects with the fk_native.Batch interface.
     `evidence_dir` with `evidence_min_run`: every record with run >= that
-    gets an evidence file and a ledger row there.  `log` (tag, message) is
-    where STATUS / DISCOVERY lines go (fk_log.log by default); `heartbeat`
-    is the [STATUS] interval in seconds (None: no heartbeat thread).
+    gets an evidence file and a ledger row there.  With `verify`, each such
+    record first passes the verification legs with certificates and a
+    breaker witness (fk_verify), whose output goes into the evidence, and
+    each record with near_min_run <= run < evidence_min_run passes the
+    three legs alone and is logged as [NEAR]; a record the legs do not
+    confirm raises CorruptEngineError.  `log` (tag, message) is where
+    STATUS / DISCOVERY / NEAR / ALARM lines go (fk_log.log by default);
+    `heartbeat` is the [STATUS] interval in seconds (None: no heartbeat
+    thread).
     """
 
     def __init__(self, engine, lo, hi, seg, path, ledger=None,
                  classify=None, fresh=False, batches=None, warn=None,
                  evidence_dir=None, evidence_min_run=None, log=None,
-                 heartbeat=None):
+                 heartbeat=None, verify=False, near_min_run=None):
         lo, hi, seg = int(lo), int(hi), int(seg)
         if not lo < hi:
             raise ValueError(f"empty range [{lo}, {hi})")
@@ -95,6 +108,8 @@ class Campaign:
         self.warn = warn
         self.evidence_dir = os.fspath(evidence_dir) if evidence_dir else None
         self.evidence_min_run = evidence_min_run
+        self.verify = bool(verify)
+        self.near_min_run = near_min_run
         self.log = log or LG.log
         self.heartbeat = heartbeat
         self._hb = None
@@ -145,20 +160,46 @@ class Campaign:
     def cursor(self):
         return int(self.state["next"])
 
+    def _legs(self, v, r, full):
+        """The verification legs on one record (certificates and witness
+        when `full`); raises CorruptEngineError when they do not confirm."""
+        if self._hb is not None:
+            self._hb.doing(f"verifying run {r} at {v}")
+        ok, res = VF.verify_record(self.engine.family, self.key["n"], v, r,
+                                   self.key["q2"], certify=full, witness=full)
+        if not ok:
+            why = res if isinstance(res, str) else res.get(
+                "error", f"legs {res.get('legs')}")
+            msg = f"verify failed at value={v} run={r}: {why}"
+            self.log("ALARM", msg)
+            raise CorruptEngineError(msg)
+        return res
+
     def _evidence(self, s0, s1, values, runs):
-        if not self.evidence_dir or self.evidence_min_run is None:
+        ev_min = (self.evidence_min_run if self.evidence_dir else None)
+        near = self.near_min_run if self.verify else None
+        if ev_min is None and near is None:
             return
-        ledger = os.path.join(self.evidence_dir, "ledger.json")
+        ledger = (os.path.join(self.evidence_dir, "ledger.json")
+                  if self.evidence_dir else None)
         for v, r in zip(values, runs):
-            if r < self.evidence_min_run:
-                continue
-            ev = {"family": self.key["family"], "n": self.key["n"],
-                  "value": int(v), "run": int(r), "segment": [str(s0), str(s1)],
-                  "classify": self.key["classify"]}
-            EV.record(ev, self.evidence_dir, f"hit_{int(v)}.json", ledger,
-                      key="value", label="DISCOVERY")
-            self.log("DISCOVERY", f"{self.key['family']} n={self.key['n']} "
-                                  f"value={int(v)} run={int(r)}")
+            if ev_min is not None and r >= ev_min:
+                ev = {"family": self.key["family"], "n": self.key["n"],
+                      "value": int(v), "run": int(r),
+                      "segment": [str(s0), str(s1)],
+                      "classify": self.key["classify"]}
+                if self.verify:
+                    ev["verification"] = self._legs(int(v), int(r), True)
+                EV.record(ev, self.evidence_dir, f"hit_{int(v)}.json", ledger,
+                          key="value", label="DISCOVERY")
+                self.log("DISCOVERY", f"{self.key['family']} n={self.key['n']} "
+                                      f"value={int(v)} run={int(r)}"
+                                      + (" (verified)" if self.verify else ""))
+            elif near is not None and r >= near:
+                self._legs(int(v), int(r), False)
+                self.log("NEAR", f"{self.key['family']} n={self.key['n']} "
+                                 f"value={int(v)} run={int(r)} (three legs "
+                                 f"agree)")
 
     def status_line(self):
         """The [STATUS] text: cursor, swept, segments, survivors, best run,

@@ -187,6 +187,19 @@ def main():
                    .read_text())
     ev["ap"] = [{"p": str(r["p"]), "n": r["n"], "depth": r["depth"],
                  "difference": str(r["difference"])} for r in a]
+    # the verification outputs recorded with each discovery: the values and
+    # breakers, Dickson's factorizations of k and BLS75 witnesses, AP's three
+    # legs and per-value proofs
+    ev["euler_full"] = [_jsonable({k: r[k] for k in ("p", "run", "values_prime_x",
+                                                 "breaker_x", "breaker",
+                                                 "breaker_factor")})
+                        for r in e]
+    ev["dickson_full"] = [_jsonable({k: r[k] for k in (
+        "k", "run", "values_prime_m", "k_factorization", "breaker_m",
+        "breaker_factor", "certificates_bls75")}) for r in d]
+    ev["ap_full"] = [_jsonable({k: r[k] for k in (
+        "p", "n", "depth", "difference", "values", "legs", "agree", "proofs")
+        if k in r}) for r in a]
     ev["_at"] = {"euler": "euler-prime-runs/evidence/euler_discoveries.json",
                  "dickson": "dickson-ladders/evidence/ladder_discoveries.json",
                  "ap": "primorial-ap/evidence/ap_discoveries.json"}

@@ -119,3 +119,76 @@ def test_campaign_writes_evidence_before_its_cursor(tmp_path, monkeypatch):
     assert any(t == "DISCOVERY" for t, _ in logs)
     line = c3.status_line()
     assert "census" in line and f"at {HI}" in line
+
+
+# ------------------------------------------- verification in the loop
+
+with open(os.path.join(os.path.dirname(__file__), "golden",
+                       "reference_vectors.json")) as _f:
+    GOLD = json.load(_f)
+D10 = GOLD["evidence"]["dickson_full"][0]           # run 10, n = 10
+
+
+class OneRecordBatch:
+    """Returns one record, (j, run), for the window that holds j."""
+
+    def __init__(self, j, run):
+        self.j, self.run = j, run
+
+    def submit(self, engines, engine_of, windows, specs):
+        (lo, hi), = windows
+        self.emit = lo <= self.j < hi
+
+    def results(self):
+        import numpy as np
+        import fk_native as N
+        recs = np.zeros(1 if self.emit else 0, dtype=N.RECORD_DTYPE)
+        if self.emit:
+            recs["v0"] = self.j
+            recs["run"] = self.run
+        return recs, np.array([recs.size], dtype=np.uint64)
+
+
+def _verified_campaign(tmp_path, run, **kw):
+    import fk_verify as VF
+    j = int(D10["k"]) // VF.dickson_wheel(10)
+    logs = []
+    c = FC.Campaign(StubEngine(), j, j + 10, 5, tmp_path / "cur.json",
+                    batches=[OneRecordBatch(j, run), OneRecordBatch(j, run)],
+                    log=lambda t, m: logs.append((t, m)),
+                    verify=True, **kw)
+    return c, j, logs
+
+
+def test_campaign_verifies_before_writing_evidence(tmp_path):
+    c, j, logs = _verified_campaign(tmp_path, D10["run"],
+                                    evidence_dir=tmp_path / "ev",
+                                    evidence_min_run=10)
+    c.run()
+    rows = EV.load_ledger(tmp_path / "ev" / "ledger.json")
+    assert [r["value"] for r in rows] == [j]
+    v = rows[0]["verification"]
+    assert str(v["k"]) == D10["k"] and v["run"] == D10["run"]
+    assert v["certificates_bls75"] == D10["certificates_bls75"]
+    assert v["breaker_factor"] == D10["breaker_factor"]
+    assert any(t == "DISCOVERY" and "verified" in m for t, m in logs)
+
+
+def test_campaign_alarm_stops_before_the_cursor(tmp_path):
+    c, j, logs = _verified_campaign(tmp_path, D10["run"] + 1,
+                                    evidence_dir=tmp_path / "ev",
+                                    evidence_min_run=10)
+    with pytest.raises(FC.CorruptEngineError):
+        c.run()
+    assert c.cursor == j                              # nothing committed
+    assert any(t == "ALARM" for t, _ in logs)
+    assert EV.load_ledger(tmp_path / "ev" / "ledger.json") == []
+
+
+def test_campaign_near_is_verified_not_evidenced(tmp_path):
+    c, j, logs = _verified_campaign(tmp_path, D10["run"],
+                                    evidence_dir=tmp_path / "ev",
+                                    evidence_min_run=11, near_min_run=9)
+    c.run()
+    assert any(t == "NEAR" and f"value={j}" in m for t, m in logs)
+    assert EV.load_ledger(tmp_path / "ev" / "ledger.json") == []
