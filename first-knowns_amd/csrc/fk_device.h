@@ -80,7 +80,6 @@ struct DevPiece {
     uint32_t tile0;           // first global tile index
     uint32_t engine;          // index into the engine array
     uint32_t window;          // caller's window index
-    uint32_t bq_off;          // offset of this piece's (B mod q) block
     uint32_t npw;             // short-line pieces: 64-bit words per line,
                               // tiles of 256 (line, chunk) items; 0: one
                               // tile per S positions of a line

@@ -449,7 +449,7 @@ static int build_engine(fk_engine* e)
 // device-side inputs uploaded before it differ).  Steps are zero-filled
 // before they are set, so that byte comparison is exact.
 struct Step {
-    enum Kind : int32_t { kMark, kZero, kPrep, kSieve, kRound, kCompact,
+    enum Kind : int32_t { kMark, kZero, kSieve, kRound, kCompact,
                           kGather, kClassify, kD2H };
     int32_t kind;
     int32_t ev;                      // kMark: event index
@@ -460,12 +460,6 @@ struct Step {
     void* dst;                       // kZero / kD2H
     const void* src;                 // kD2H
     size_t n;                        // kZero: words; kD2H: bytes
-    const DevEngine* p_engines;      // kPrep
-    const DevPiece* p_pieces;
-    uint32_t p_npieces, p_nlds;
-    uint32_t* p_bq;
-    uint32_t* p_zero;                // kPrep: counters to clear
-    uint32_t p_nzero;
     SieveArgs sa;
     RoundArgs ra;
     CompactArgs ca;
@@ -499,10 +493,6 @@ static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
             break;
         case Step::kZero:
             e = launch_zero((uint32_t*)st.dst, (uint32_t)st.n, s);
-            break;
-        case Step::kPrep:
-            e = launch_prep(st.p_engines, st.p_pieces, st.p_npieces, st.p_nlds,
-                            st.p_bq, st.p_zero, st.p_nzero, s);
             break;
         case Step::kSieve:
             e = st.wpt == 0 ? launch_sieve_short(st.sa, st.threads, st.grid,
@@ -544,16 +534,16 @@ struct fk_batch {
     uint8_t* h_up = nullptr;
     size_t up_bytes = 0;
     size_t o_engines = 0, o_pieces = 0, o_win_engine = 0, o_wins = 0, o_segs = 0;
+    size_t o_counts = 0;
     DevEngine* d_engines = nullptr;   size_t cap_engines = 0;
     DevPiece* d_pieces = nullptr;     size_t cap_pieces = 0;
     uint32_t* d_win_engine = nullptr; size_t cap_win = 0;
     DevWinInfo* d_wins = nullptr;
-    uint32_t* d_bq = nullptr;         size_t cap_bq = 0;
     DevRec* d_q[2] = {nullptr, nullptr}; size_t cap_q = 0;
     uint32_t* d_flags = nullptr;
     uint64_t* d_fail2 = nullptr;      // per record: base-2 failures
     uint64_t* d_failx = nullptr;      // per record: other-base failures
-    uint32_t* d_counts = nullptr;     size_t cap_counts = 0;
+    uint32_t* d_counts = nullptr;     size_t cap_counts = 0;  // in d_up
     // pinned host mirrors
     uint32_t* h_counts = nullptr;
     DevRec* h_recs = nullptr;         size_t cap_hrecs = 0;
@@ -592,8 +582,8 @@ static int batch_free(fk_batch* b)
     b->graph_exec = nullptr;
     b->graph = nullptr;
     (void)hipSetDevice(b->device);
-    void* ds[] = {b->d_up, b->d_bq, b->d_q[0], b->d_q[1], b->d_flags,
-                  b->d_fail2, b->d_failx, b->d_counts};
+    void* ds[] = {b->d_up, b->d_q[0], b->d_q[1], b->d_flags,
+                  b->d_fail2, b->d_failx};
     for (void* p : ds) if (p) (void)hipFree(p);
     if (b->h_up) (void)hipHostFree(b->h_up);
     if (b->h_counts) (void)hipHostFree(b->h_counts);
@@ -620,16 +610,17 @@ static int grow(T** p, size_t* cap, size_t need)
 // pointers (and a captured graph's kernel arguments) stay put while the
 // submissions fit.
 static int grow_uploads(fk_batch* b, size_t neng, size_t npieces,
-                        size_t nwin, size_t nseg)
+                        size_t nwin, size_t nseg, size_t ncount)
 {
     auto need = [](size_t cap, size_t n) {
         return n > cap ? std::max<size_t>(std::max<size_t>(n, cap * 3 / 2), 16)
                        : cap;
     };
     const size_t ce = need(b->cap_engines, neng), cp = need(b->cap_pieces, npieces),
-                 cw = need(b->cap_win, nwin), cs = need(b->cap_segs, nseg);
+                 cw = need(b->cap_win, nwin), cs = need(b->cap_segs, nseg),
+                 cc = need(b->cap_counts, ncount);
     if (b->d_up && ce == b->cap_engines && cp == b->cap_pieces &&
-        cw == b->cap_win && cs == b->cap_segs)
+        cw == b->cap_win && cs == b->cap_segs && cc == b->cap_counts)
         return FK_OK;
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t off = 0;
@@ -638,22 +629,28 @@ static int grow_uploads(fk_batch* b, size_t neng, size_t npieces,
     b->o_win_engine = off; off = align(off + cw * sizeof(uint32_t));
     b->o_wins = off;       off = align(off + cw * sizeof(DevWinInfo));
     b->o_segs = off;       off = align(off + cs * sizeof(uint2));
+    b->o_counts = off;     off = align(off + cc * sizeof(uint32_t));
     if (b->d_up) (void)hipFree(b->d_up);
     if (b->h_up) (void)hipHostFree(b->h_up);
+    if (b->h_counts) (void)hipHostFree(b->h_counts);
+    b->h_counts = nullptr;
     b->d_up = nullptr;
     b->h_up = nullptr;
     HIPCHK(hipMalloc(&b->d_up, off));
     HIPCHK(hipHostMalloc(&b->h_up, off));
+    HIPCHK(hipHostMalloc(&b->h_counts, cc * sizeof(uint32_t)));
     b->up_bytes = off;
     b->d_engines = reinterpret_cast<DevEngine*>(b->d_up + b->o_engines);
     b->d_pieces = reinterpret_cast<DevPiece*>(b->d_up + b->o_pieces);
     b->d_win_engine = reinterpret_cast<uint32_t*>(b->d_up + b->o_win_engine);
     b->d_wins = reinterpret_cast<DevWinInfo*>(b->d_up + b->o_wins);
     b->d_segs = reinterpret_cast<uint2*>(b->d_up + b->o_segs);
+    b->d_counts = reinterpret_cast<uint32_t*>(b->d_up + b->o_counts);
     b->cap_engines = ce;
     b->cap_pieces = cp;
     b->cap_win = cw;
     b->cap_segs = cs;
+    b->cap_counts = cc;
     return FK_OK;
 }
 
@@ -794,7 +791,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     std::vector<DevPiece> pieces;
     std::vector<DevWinInfo> wins(nwin);
     std::vector<uint32_t> win_engine(nwin);
-    uint64_t tiles = 0, bq_total = 0;
+    uint64_t tiles = 0;
     uint32_t nlds_max = 0, pat_max = 0, bm_max = 0;
     uint32_t s_nlds_max = 0, s_bm_max = 0;      // over short windows
     double expect = 0.0, cand = 0.0;
@@ -832,11 +829,9 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             P.tile0 = (uint32_t)tiles;
             P.engine = b->engine_of[w];
             P.window = (uint32_t)w;
-            P.bq_off = (uint32_t)bq_total;
             if (tiles + P.nseg >= (1ull << 32))
                 return fail(FK_E_ARG, "batch too large: more than 2^32 tiles");
             tiles += P.nseg;
-            bq_total += e->host.nlds;
             pieces.push_back(P);
             groups.back().tile_end = tiles;
             s_nlds_max = std::max(s_nlds_max, e->host.nlds);
@@ -863,12 +858,10 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             P.tile0 = (uint32_t)tiles;
             P.engine = b->engine_of[w];
             P.window = (uint32_t)w;
-            P.bq_off = (uint32_t)bq_total;
             const uint64_t t = (uint64_t)P.nseg * e->offs.size();
             if (tiles + t >= (1ull << 32))
                 return fail(FK_E_ARG, "batch too large: more than 2^32 tiles");
             tiles += t;
-            bq_total += e->host.nlds;
             pieces.push_back(P);
         }
         groups.back().tile_end = tiles;
@@ -918,9 +911,8 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
 
     // ---- buffers
     if ((rc = grow_uploads(b, b->engines.size(), std::max<size_t>(pieces.size(), 1),
-                           nwin, ne)))
+                           nwin, ne, ncounts)))
         return rc;
-    if ((rc = grow(&b->d_bq, &b->cap_bq, std::max<uint64_t>(bq_total, 1)))) return rc;
     if (queue_cap > b->cap_q || !b->d_q[0]) {
         size_t c = 0;
         for (int i = 0; i < 2; i++) {
@@ -943,13 +935,6 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         }
         b->cap_q = c;
     }
-    if (ncounts > b->cap_counts || !b->d_counts) {
-        size_t c = 0;
-        if ((rc = grow(&b->d_counts, &c, ncounts))) return rc;
-        if (b->h_counts) (void)hipHostFree(b->h_counts);
-        HIPCHK(hipHostMalloc(&b->h_counts, c * sizeof(uint32_t)));
-        b->cap_counts = c;
-    }
 
     // ---- uploads (small): staged in pinned memory, one asynchronous copy.
     // The staging area is rewritten only by the next submission, which
@@ -964,7 +949,9 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         std::memcpy(h + b->o_win_engine, win_engine.data(), nwin * sizeof(uint32_t));
         std::memcpy(h + b->o_wins, wins.data(), nwin * sizeof(DevWinInfo));
         std::memcpy(h + b->o_segs, segs.data(), ne * sizeof(uint2));
-        const size_t used = b->o_segs + ne * sizeof(uint2);
+        // the stage counters start at zero: the copy clears them
+        std::memset(h + b->o_counts, 0, ncounts * sizeof(uint32_t));
+        const size_t used = b->o_counts + ncounts * sizeof(uint32_t);
         HIPCHK(hipMemcpyAsync(b->d_up, h, used, hipMemcpyHostToDevice, s));
     }
 
@@ -975,19 +962,8 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         plan.push_back(st);
     };
     mark(0);
-    {
-        // k_prep clears the counters; the sieve clears each emitted record's
-        // flag (the compactions leave the flags they read cleared)
-        Step pr = new_step(Step::kPrep);
-        pr.p_engines = b->d_engines;
-        pr.p_pieces = b->d_pieces;
-        pr.p_npieces = (uint32_t)pieces.size();
-        pr.p_nlds = nlds_max;
-        pr.p_bq = b->d_bq;
-        pr.p_zero = b->d_counts;
-        pr.p_nzero = (uint32_t)ncounts;
-        plan.push_back(pr);
-    }
+    // the upload zeroed the counters; the sieve clears each emitted
+    // record's flag (the compactions leave the flags they read cleared)
     mark(1);
 
     // ---- sieve: one launch per tile shape
@@ -1006,7 +982,6 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             a.nlds_max = s_nlds_max;
             a.pat_max = 0;
             a.bm_max = s_bm_max;
-            a.bq = b->d_bq;
             a.segs = b->d_segs;
             a.out = b->d_q[0];
             a.counts = b->d_counts;
@@ -1034,7 +1009,6 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         a.nlds_max = nlds_max;
         a.pat_max = pat_max;
         a.bm_max = bm_max;
-        a.bq = b->d_bq;
         a.segs = b->d_segs;
         a.out = b->d_q[0];
         a.counts = b->d_counts;               // stage 0

@@ -2,18 +2,19 @@
 //
 // Pipeline for a batch of window pieces (see fk_device.h for the line
 // model):
-//   k_prep          B mod q for every (piece, LDS prime); clears the
-//                   batch's stage counters
 //   k_sieve         one LDS tile (S line positions of one wheel line) per
 //                   workgroup iteration: pattern words for the smallest
 //                   primes, optional ds_or stride marking, per-survivor
 //                   tests against LDS kill bitmaps for the medium primes,
 //                   then compaction of the surviving bits into the queue
-//   k_round         per-candidate kill tests for one range of large primes:
-//                   lane = candidate, the prime uniform across the wave (its
-//                   constants and kill set in scalar registers), the residue
-//                   formed exactly in double precision; killed candidates
-//                   flagged
+//   k_sieve_short   windows whose lines hold a few words: one lane per
+//                   chunk of 1..16 words of a line, every LDS prime a
+//                   64-bit window of its kill bitmap
+//   k_round_c/_p    per-candidate kill tests for one range of large primes:
+//                   lane = candidate (the prime uniform across the wave, its
+//                   constants and kill set in scalar registers) or lane =
+//                   prime, the residue formed exactly in double precision;
+//                   killed candidates flagged
 //   k_kill_table    (engine creation) the value-domain kill set of every
 //                   round prime
 //   k_round_compact wave-ballot compaction of the unflagged candidates
@@ -51,32 +52,6 @@ __device__ __forceinline__ const FK_CONST T* uniform_view(const T* p)
 }
 
 // ------------------------------------------------------------------ prep
-__global__ void k_prep(const DevEngine* __restrict__ engines,
-                       const DevPiece* __restrict__ pieces, uint32_t npieces,
-                       uint32_t* __restrict__ bq, uint32_t* __restrict__ zero,
-                       uint32_t nzero)
-{
-    // the batch's stage counters start at zero (the first row of blocks)
-    if (blockIdx.y == 0)
-        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nzero;
-             i += gridDim.x * blockDim.x)
-            zero[i] = 0;
-    for (uint32_t pi = blockIdx.y; pi < npieces; pi += gridDim.y) {
-        const DevPiece P = pieces[pi];
-        const DevEngine& E = engines[P.engine];
-        for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < E.nlds;
-             p += gridDim.x * blockDim.x) {
-            const uint32_t q = E.lds_q[p];
-            const uint64_t m64 = E.lds_m64[p];
-            uint32_t r = red64(P.base[3], q, m64);
-            r = red64(((uint64_t)r << 32) | P.base[2], q, m64);
-            r = red64(((uint64_t)r << 32) | P.base[1], q, m64);
-            r = red64(((uint64_t)r << 32) | P.base[0], q, m64);
-            bq[P.bq_off + p] = r;
-        }
-    }
-}
-
 // ----------------------------------------------------------------- sieve
 
 __device__ __forceinline__ uint32_t find_piece(const FK_CONST DevPiece* pieces,
@@ -89,6 +64,16 @@ __device__ __forceinline__ uint32_t find_piece(const FK_CONST DevPiece* pieces,
         if (pieces[mid].tile0 <= tile) lo = mid; else hi = mid;
     }
     return lo;
+}
+
+// B mod q for a piece's base B (four 32-bit limbs, most significant first)
+__device__ __forceinline__ uint32_t base_mod(const FK_CONST DevPiece& P,
+                                             uint32_t q, uint64_t m64)
+{
+    uint32_t r = red64(P.base[3], q, m64);
+    r = red64(((uint64_t)r << 32) | P.base[2], q, m64);
+    r = red64(((uint64_t)r << 32) | P.base[1], q, m64);
+    return red64(((uint64_t)r << 32) | P.base[0], q, m64);
 }
 
 __device__ __forceinline__ uint64_t ceil_div_u64(uint64_t a, uint64_t b)
@@ -234,7 +219,7 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
         for (uint32_t p = tid; p < E.nlds; p += T) {
             const uint32_t q = E.lds_q[p];
             const uint64_t m64 = E.lds_m64[p];
-            uint32_t t = a.bq[P.bq_off + p] + red64(off, q, m64);
+            uint32_t t = base_mod(P, q, m64) + red64(off, q, m64);
             if (t >= q) t -= q;
             t = mulmod32(t, E.lds_dinv[p], q, m64);
             s0[p] = red64(v0 + t, q, m64);
@@ -524,7 +509,7 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
             if (tables)
                 for (uint32_t w = tid; w < E.sbm_words; w += 256) bm[w] = E.sbm[w];
             for (uint32_t p = tid; p < E.nlds; p += 256) {
-                const uint32_t bq = a.bq[P.bq_off + p];
+                const uint32_t bq = base_mod(P, E.lds_q[p], E.lds_m64[p]);
                 if (tables) {
                     const uint32_t q = E.lds_q[p];
                     const uint64_t m64 = E.lds_m64[p];
@@ -1100,17 +1085,21 @@ __global__ __launch_bounds__(256) void k_round_compact(CompactArgs a)
          blk += gridDim.x) {
         const uint32_t base = segs[sb.e].x;
         const uint32_t l0 = sb.lb * (256 * CP_PER);
-        // the records are loaded with their flags (one memory latency; the
-        // stores below then wait only for the atomic)
+        // the records (24 B: three 8-byte words) are loaded with their
+        // flags, one memory latency; the stores below then wait only for the
+        // atomic
         uint32_t flag[CP_PER];
-        DevRec rec[CP_PER];
+        uint2 rec[CP_PER][3];
 #pragma unroll
         for (uint32_t k = 0; k < CP_PER; k++) {
             const uint32_t li = l0 + k * 256 + threadIdx.x;
             flag[k] = 1;
             if (li < sb.n) {
                 flag[k] = a.flags[base + li];
-                rec[k] = a.in[base + li];
+                const uint2* src = reinterpret_cast<const uint2*>(a.in + base + li);
+                rec[k][0] = src[0];
+                rec[k][1] = src[1];
+                rec[k][2] = src[2];
             }
         }
         uint64_t m[CP_PER];
@@ -1141,7 +1130,10 @@ __global__ __launch_bounds__(256) void k_round_compact(CompactArgs a)
                 const uint32_t below = __builtin_amdgcn_mbcnt_hi(
                     (uint32_t)(m[k] >> 32),
                     __builtin_amdgcn_mbcnt_lo((uint32_t)m[k], 0u));
-                a.out[off + below] = rec[k];
+                uint2* dst = reinterpret_cast<uint2*>(a.out + off + below);
+                dst[0] = rec[k][0];
+                dst[1] = rec[k][1];
+                dst[2] = rec[k][2];
             }
             off += (uint32_t)__popcll((unsigned long long)m[k]);
         }
@@ -1425,18 +1417,6 @@ bool sieve_shape_ok(int threads, int wpt)
     FK_SIEVE_SHAPES(FK_OK_CASE)
 #undef FK_OK_CASE
     return false;
-}
-
-hipError_t launch_prep(const DevEngine* engines, const DevPiece* pieces,
-                       uint32_t npieces, uint32_t max_nlds, uint32_t* bq,
-                       uint32_t* zero, uint32_t nzero, hipStream_t stream)
-{
-    uint32_t gx = (std::max(max_nlds, nzero) + 255) / 256;
-    gx = std::max(1u, std::min(gx, 64u));
-    const uint32_t gy = std::max(1u, std::min(npieces, 32768u));
-    hipLaunchKernelGGL(k_prep, dim3(gx, gy), dim3(256), 0, stream,
-                       engines, pieces, npieces, bq, zero, nzero);
-    return hipGetLastError();
 }
 
 hipError_t launch_round_test(const RoundArgs& a, bool lane_prime, uint32_t gx,

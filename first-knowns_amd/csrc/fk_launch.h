@@ -29,7 +29,6 @@ struct SieveArgs {
     uint32_t nlds_max;        // LDS room for per-prime arrays (max over engines)
     uint32_t pat_max;         // LDS room for pattern words (max over engines)
     uint32_t bm_max;          // LDS room for kill-bitmap words (max over engines)
-    const uint32_t* bq;       // per piece, per LDS prime: base mod q
     const uint2* segs;        // per engine: queue segment (base, size)
     DevRec* out;
     uint32_t* counts;         // per engine: records emitted
@@ -94,9 +93,6 @@ struct ClassifyArgs {
                               // bases run in one pass
 };
 
-hipError_t launch_prep(const DevEngine* engines, const DevPiece* pieces,
-                       uint32_t npieces, uint32_t max_nlds, uint32_t* bq,
-                       uint32_t* zero, uint32_t nzero, hipStream_t stream);
 hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,
                         uint32_t grid, size_t lds_bytes, hipStream_t stream);
 int sieve_occupancy(int threads, int wpt, size_t lds_bytes);
