@@ -449,22 +449,20 @@ static int build_engine(fk_engine* e)
 // device-side inputs uploaded before it differ).  Steps are zero-filled
 // before they are set, so that byte comparison is exact.
 struct Step {
-    enum Kind : int32_t { kMark, kZero, kSieve, kRound, kCompact,
-                          kGather, kClassify, kD2H };
+    enum Kind : int32_t { kMark, kReadback, kSieve, kRound, kCompact,
+                          kGather, kClassify };
     int32_t kind;
     int32_t ev;                      // kMark: event index
     uint32_t grid, gy;
     int32_t threads, wpt;
     size_t lds;
     int32_t lane_prime;
-    void* dst;                       // kZero / kD2H
-    const void* src;                 // kD2H
-    size_t n;                        // kZero: words; kD2H: bytes
     SieveArgs sa;
     RoundArgs ra;
     CompactArgs ca;
     GatherArgs ga;
     ClassifyArgs cla;
+    ReadbackArgs rb;
 };
 
 static Step new_step(int32_t kind)
@@ -491,9 +489,6 @@ static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
         case Step::kMark:
             if (events) e = hipEventRecord(ev[st.ev], s);
             break;
-        case Step::kZero:
-            e = launch_zero((uint32_t*)st.dst, (uint32_t)st.n, s);
-            break;
         case Step::kSieve:
             e = st.wpt == 0 ? launch_sieve_short(st.sa, st.threads, st.grid,
                                                  st.lds, s)
@@ -512,8 +507,8 @@ static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
         case Step::kClassify:
             e = launch_classify(st.cla, st.grid, s);
             break;
-        case Step::kD2H:
-            e = hipMemcpyAsync(st.dst, st.src, st.n, hipMemcpyDeviceToHost, s);
+        case Step::kReadback:
+            e = launch_readback(st.rb, s);
             break;
         }
         if (e != hipSuccess) return e;
@@ -638,7 +633,10 @@ static int grow_uploads(fk_batch* b, size_t neng, size_t npieces,
     b->h_up = nullptr;
     HIPCHK(hipMalloc(&b->d_up, off));
     HIPCHK(hipHostMalloc(&b->h_up, off));
-    HIPCHK(hipHostMalloc(&b->h_counts, cc * sizeof(uint32_t)));
+    // coherent: the readback kernel's stores are seen by the host once the
+    // stream is synchronized, without passing through the GPU's caches
+    HIPCHK(hipHostMalloc(&b->h_counts, cc * sizeof(uint32_t),
+                         hipHostMallocCoherent));
     b->up_bytes = off;
     b->d_engines = reinterpret_cast<DevEngine*>(b->d_up + b->o_engines);
     b->d_pieces = reinterpret_cast<DevPiece*>(b->d_up + b->o_pieces);
@@ -1186,18 +1184,22 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         if (b->h_recs) (void)hipHostFree(b->h_recs);
         b->h_recs = nullptr;
         size_t c = std::max<size_t>(copy, b->cap_hrecs * 3 / 2);
-        HIPCHK(hipHostMalloc(&b->h_recs, c * sizeof(DevRec)));
+        HIPCHK(hipHostMalloc(&b->h_recs, c * sizeof(DevRec),
+                             hipHostMallocCoherent));
         b->cap_hrecs = c;
     }
     {
-        Step st = new_step(Step::kD2H);
-        st.dst = b->h_counts;
-        st.src = b->d_counts;
-        st.n = ncounts * sizeof(uint32_t);
-        plan.push_back(st);
-        st.dst = b->h_recs;
-        st.src = b->d_q[cur];
-        st.n = (size_t)copy * sizeof(DevRec);
+        // one kernel writes both into the pinned buffers: the counters and
+        // as many of the `copy` leading records as the queue holds
+        Step st = new_step(Step::kReadback);
+        ReadbackArgs& r = st.rb;
+        r.counts = b->d_counts;
+        r.ncounts = (uint32_t)ncounts;
+        r.counts_host = b->h_counts;
+        r.recs = b->d_q[cur];
+        r.count = b->d_counts + total_slot;
+        r.cap = copy;
+        r.recs_host = b->h_recs;
         plan.push_back(st);
     }
     mark(5);

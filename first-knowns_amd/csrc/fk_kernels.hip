@@ -1292,12 +1292,6 @@ __global__ __launch_bounds__(256) void k_cls_finish(ClassifyArgs a)
     }
 }
 
-__global__ void k_zero_u32(uint32_t* p, uint32_t n)
-{
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += gridDim.x * blockDim.x)
-        p[i] = 0;
-}
 
 // --------------------------------------------------------------- launchers
 
@@ -1479,6 +1473,27 @@ hipError_t launch_classify(const ClassifyArgs& a, uint32_t grid,
     return hipGetLastError();
 }
 
+// The readback kernel: one launch in place of two copies; only the records
+// the queue holds cross the bus (8-byte words, grid-stride).
+__global__ __launch_bounds__(256) void k_readback(ReadbackArgs a)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t step = gridDim.x * blockDim.x;
+    for (uint32_t i = t; i < a.ncounts; i += step) a.counts_host[i] = a.counts[i];
+    const uint32_t n = min(*a.count, a.cap);
+    const uint2* src = reinterpret_cast<const uint2*>(a.recs);
+    uint2* dst = reinterpret_cast<uint2*>(a.recs_host);
+    for (uint32_t i = t; i < 3u * n; i += step) dst[i] = src[i];
+}
+
+hipError_t launch_readback(const ReadbackArgs& a, hipStream_t stream)
+{
+    uint32_t grid = (3u * a.cap + 255) / 256;
+    grid = grid < 1 ? 1 : (grid > 256 ? 256 : grid);
+    hipLaunchKernelGGL(k_readback, dim3(grid), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_gather(const GatherArgs& a, uint32_t grid,
                          hipStream_t stream)
 {
@@ -1486,13 +1501,5 @@ hipError_t launch_gather(const GatherArgs& a, uint32_t grid,
     return hipGetLastError();
 }
 
-hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t stream)
-{
-    if (n == 0) return hipSuccess;
-    uint32_t grid = (n + 255) / 256;
-    if (grid > 1024) grid = 1024;
-    hipLaunchKernelGGL(k_zero_u32, dim3(grid), dim3(256), 0, stream, p, n);
-    return hipGetLastError();
-}
 
 }  // namespace fk

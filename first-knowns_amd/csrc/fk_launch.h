@@ -80,6 +80,18 @@ struct GatherArgs {
     uint32_t* total;
 };
 
+// The readback: the stage counters, then the first min(*count, cap)
+// records of the final queue, written straight into pinned host memory.
+struct ReadbackArgs {
+    const uint32_t* counts;
+    uint32_t ncounts;
+    uint32_t* counts_host;
+    const DevRec* recs;
+    const uint32_t* count;    // the final record count
+    uint32_t cap;             // records the host buffer holds
+    DevRec* recs_host;
+};
+
 struct ClassifyArgs {
     const DevEngine* engines;
     const DevWinInfo* wins;
@@ -110,10 +122,10 @@ hipError_t launch_kill_table(const PrimeK* primes, uint32_t p0, uint32_t np,
                              uint32_t stride, uint32_t* kt, hipStream_t stream);
 hipError_t launch_round_compact(const CompactArgs& a, uint32_t grid,
                                 hipStream_t stream);
+hipError_t launch_readback(const ReadbackArgs& a, hipStream_t stream);
 hipError_t launch_gather(const GatherArgs& a, uint32_t grid,
                          hipStream_t stream);
 hipError_t launch_classify(const ClassifyArgs& a, uint32_t grid,
                            hipStream_t stream);
-hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t stream);
 
 }  // namespace fk
