@@ -155,10 +155,12 @@ FK_HD uint32_t resid_d(double s, double qd, double qi, uint32_t q)
 }
 
 // The same residue with one fix instead of two: the quotient estimate is
-// taken 1/8 low.  For s < 2^49 and q >= 2, s / q < 2^48; fma(s, qi, -1/8)
-// differs from s / q - 1/8 by at most 2^-5 (qi's relative error 2^-53
-// times s / q) plus 2^-5 (its own rounding at magnitude below 2^48), so its
-// floor is floor(s / q) or one less, and s - floor * q lies in [0, 2q).
+// taken 1/8 low.  For an integer s held exactly (s < 2^53) with
+// s / q < 2^48 -- s < 2^49 and q >= 2, or s < 2^53 and q >= 32 --
+// fma(s, qi, -1/8) differs from s / q - 1/8 by at most 2^-5 (qi's relative
+// error 2^-53 times s / q) plus 2^-5 (its own rounding at magnitude below
+// 2^48), so its floor is floor(s / q) or one less, and s - floor * q, below
+// 2q and formed by one fma, is exact and lies in [0, 2q).
 FK_HD uint32_t resid_d1(double s, double qd, double qi, uint32_t q)
 {
     const double qt = floor(fma(s, qi, -0.125));
@@ -169,8 +171,8 @@ FK_HD uint32_t resid_d1(double s, double qd, double qi, uint32_t q)
 
 // The folded residue min(s mod q, q - s mod q) directly, for the rows that
 // hold folded kill sets.  The quotient is the nearest integer to s * qi;
-// with s < 2^49 and q >= 2 the estimate is within 2^-4 of s / q (as for
-// resid_d1), so it is the nearest integer to s / q or one off, and
+// under resid_d1's condition (s exact, s / q < 2^48) the estimate is within
+// 2^-4 of s / q, so it is the nearest integer to s / q or one off, and
 // t = s - qt * q (exact) has |t| < 3q/2.  For u = |t| in [0, q/2] the
 // folded residue is u, for u in (q/2, 3q/2) it is |u - q|: min(u, |u - q|)
 // gives both.

@@ -1061,6 +1061,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         ra.round = r;
         ra.p_begin = P0;
         ra.p_end = P1;
+        ra.q_min = ctx->primes[P0];
         // residue groups per row held in registers: three when every
         // engine's kill sets fit twelve residues (both rows of a pass then
         // load together), else five

@@ -686,7 +686,10 @@ __device__ __forceinline__ RowView<KG> load_row(const FK_CONST uint32_t* row)
 }
 
 // The lane's candidate residue mod the row's prime from its 16-bit limbs;
-// limbs past the fourth take 2^64..2^112 mod q from the prime table.
+// limbs past the fourth take 2^64..2^112 mod q from the prime table.  (This
+// kernel waits on its scalar row loads, not on VALU: feeding a value below
+// 2^53 to the residue directly, as the lane = prime kernel does, measured
+// no faster here.)
 template <int NL, bool FOLD, int KG>
 __device__ __forceinline__ uint32_t row_residue(const double (&a)[8],
                                                 const RowView<KG>& v,
@@ -920,7 +923,9 @@ template <int NL, bool FOLD>
 __device__ __forceinline__ bool lane_kills(const LaneRow& L, const double* a,
                                            uint32_t ncmp)
 {
-    double s = fma(a[3], L.c48, fma(a[2], L.c32, fma(a[1], L.c16, a[0])));
+    // NL == 1: a[0] is the value itself (below 2^53, primes at least 32)
+    double s = a[0];
+    if (NL > 1) s = fma(a[3], L.c48, fma(a[2], L.c32, fma(a[1], L.c16, a[0])));
     if (NL > 4) {
 #pragma unroll
         for (int k = 0; k < 4; k++) s = fma(a[4 + k], L.cw[k], s);
@@ -952,12 +957,14 @@ __device__ __forceinline__ bool lane_kills(const LaneRow& L, const double* a,
 // prime).
 constexpr uint32_t kRoundSlice = 256;
 
-template <bool FOLD>
-__device__ __forceinline__ void lane_prime_slice(const LaneRow& L, bool mine,
-                                                 uint32_t ncmp, bool wide,
-                                                 const double* limbs,
-                                                 const uint32_t* live_idx,
-                                                 uint32_t nlive, uint32_t* flags)
+// One slice's candidates against the lane's prime; candidate c's input is
+// a + step * c (NL doubles: 16-bit limbs, or with NL == 1 the value).
+template <int NL, bool FOLD>
+__device__ __forceinline__ void lane_prime_walk(const LaneRow& L, bool mine,
+                                                uint32_t ncmp, const double* a,
+                                                uint32_t step,
+                                                const uint32_t* live_idx,
+                                                uint32_t nlive, uint32_t* flags)
 {
     const uint32_t lane = __lane_id();
     // four candidates per step: four independent residue chains in flight
@@ -967,18 +974,8 @@ __device__ __forceinline__ void lane_prime_slice(const LaneRow& L, bool mine,
     for (; c + U <= nlive; c += U) {
         bool kill[U];
 #pragma unroll
-        for (uint32_t u = 0; u < U; u++) kill[u] = false;
-        if (mine) {
-            if (wide) {
-#pragma unroll
-                for (uint32_t u = 0; u < U; u++)
-                    kill[u] = lane_kills<8, FOLD>(L, limbs + 8 * (c + u), ncmp);
-            } else {
-#pragma unroll
-                for (uint32_t u = 0; u < U; u++)
-                    kill[u] = lane_kills<4, FOLD>(L, limbs + 8 * (c + u), ncmp);
-            }
-        }
+        for (uint32_t u = 0; u < U; u++)
+            kill[u] = mine && lane_kills<NL, FOLD>(L, a + step * (c + u), ncmp);
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             const uint64_t m = __ballot(kill[u]);
@@ -987,22 +984,36 @@ __device__ __forceinline__ void lane_prime_slice(const LaneRow& L, bool mine,
         }
     }
     for (; c < nlive; c++) {
-        const double* a = limbs + 8 * c;
-        bool kill = false;
-        if (mine)
-            kill = wide ? lane_kills<8, FOLD>(L, a, ncmp)
-                        : lane_kills<4, FOLD>(L, a, ncmp);
+        const bool kill = mine && lane_kills<NL, FOLD>(L, a + step * c, ncmp);
         const uint64_t m = __ballot(kill);
         if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1))
             flags[live_idx[c]] = 1;
     }
 }
 
+template <bool FOLD>
+__device__ __forceinline__ void lane_prime_slice(const LaneRow& L, bool mine,
+                                                 uint32_t ncmp, bool wide,
+                                                 bool narrow,
+                                                 const double* limbs,
+                                                 const double* vals,
+                                                 const uint32_t* live_idx,
+                                                 uint32_t nlive, uint32_t* flags)
+{
+    if (wide)
+        lane_prime_walk<8, FOLD>(L, mine, ncmp, limbs, 8, live_idx, nlive, flags);
+    else if (narrow)
+        lane_prime_walk<1, FOLD>(L, mine, ncmp, vals, 1, live_idx, nlive, flags);
+    else
+        lane_prime_walk<4, FOLD>(L, mine, ncmp, limbs, 8, live_idx, nlive, flags);
+}
+
 __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
 {
     __shared__ double sh_limbs[kRoundSlice * 8];
+    __shared__ double sh_val[kRoundSlice];      // the value, when below 2^53
     __shared__ uint32_t sh_idx[kRoundSlice];
-    __shared__ uint32_t sh_n, sh_wide;
+    __shared__ uint32_t sh_n, sh_wide, sh_wide53;
     const uint32_t p = a.p_begin + blockIdx.x * 256 + threadIdx.x;
     const uint32_t blk_p0 = a.p_begin + blockIdx.x * 256;
     if (blk_p0 >= a.p_end) return;                // whole workgroup past it
@@ -1021,7 +1032,7 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
         if (blk_p0 + 256 <= rlo || blk_p0 >= rhi) continue;
         const bool mine = p >= rlo && p < rhi && p < a.p_end;
         // stage the slice's live candidates: limbs and queue index
-        if (threadIdx.x == 0) { sh_n = 0; sh_wide = 0; }
+        if (threadIdx.x == 0) { sh_n = 0; sh_wide = 0; sh_wide53 = 0; }
         __syncthreads();
         {
             const uint32_t li = sb.lb * kRoundSlice + threadIdx.x;
@@ -1034,14 +1045,19 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
                     for (int k = 0; k < 8; k++)
                         sh_limbs[8 * slot + k] =
                             (double)((rec.t[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
+                    sh_val[slot] =
+                        (double)(((uint64_t)rec.t[1] << 32) | rec.t[0]);
                     sh_idx[slot] = i;
                     if ((rec.t[2] | rec.t[3]) != 0) sh_wide = 1;
+                    if ((rec.t[2] | rec.t[3]) != 0 || rec.t[1] >= (1u << 21))
+                        sh_wide53 = 1;
                 }
             }
         }
         __syncthreads();
         const uint32_t nlive = sh_n;
         const bool wide = sh_wide != 0;
+        const bool below53 = sh_wide53 == 0;
         if (nlive != 0) {
             // compares per test: the family's largest kill set, in fours
             const uint32_t ncmp = (kill_max(engs[e].family, engs[e].n) + 3u) & ~3u;
@@ -1051,13 +1067,16 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
                               a.primed + p, stride);
                 if (ncmp > 20) load_lane_row_wide(L, stride - kRowHead);
             }
+            // the one-value form needs every value below 2^53 and every
+            // prime of the round at least 32
+            const bool narrow = below53 && a.q_min >= 32u;
             if (__ballot(mine)) {
                 if (kill_folded(engs[e].family))
-                    lane_prime_slice<true>(L, mine, ncmp, wide, sh_limbs, sh_idx,
-                                           nlive, a.flags);
+                    lane_prime_slice<true>(L, mine, ncmp, wide, narrow, sh_limbs,
+                                           sh_val, sh_idx, nlive, a.flags);
                 else
-                    lane_prime_slice<false>(L, mine, ncmp, wide, sh_limbs, sh_idx,
-                                            nlive, a.flags);
+                    lane_prime_slice<false>(L, mine, ncmp, wide, narrow, sh_limbs,
+                                            sh_val, sh_idx, nlive, a.flags);
             }
         }
         __syncthreads();                           // before the next staging

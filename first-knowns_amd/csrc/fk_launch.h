@@ -57,6 +57,8 @@ struct RoundArgs {
     uint32_t kgroups;         // lane = candidate: groups of four killing
                               // residues held in registers per row (3 or 5;
                               // rows with more are finished in a loop)
+    uint32_t q_min;           // the round's smallest prime (the one-value
+                              // residue form needs q >= 32)
 };
 
 struct CompactArgs {

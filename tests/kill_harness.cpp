@@ -6,6 +6,8 @@
 //                                    32-bit limbs) by the 16-bit-limb
 //                                    double-precision path with nl limbs,
 //                                    two-fix and one-fix forms, and folded
+//   N q vlo vhi                   -> one-fix and folded residues of the value
+//                                    (vhi:vlo < 2^53) held as one double
 #include "../first-knowns_amd/csrc/fk_kill.h"
 #include <cstdio>
 #include <cstring>
@@ -43,6 +45,14 @@ int main()
                    h.c16, h.c32, h.c48, h.q, h.cnt);
             for (uint32_t i = fk::kRowHead; i < stride; i++) printf(" %u", row[i]);
             printf("\n");
+        } else if (line[0] == 'N') {
+            unsigned q;
+            uint32_t lo, hi;
+            if (sscanf(line + 1, "%u %x %x", &q, &lo, &hi) != 3) continue;
+            const fk::PrimeD P = fk::make_primed(q);
+            const double s = (double)(((uint64_t)hi << 32) | lo);
+            printf("%u %u\n", fk::resid_d1(s, P.qd, P.qi, P.q),
+                   fk::resid_fold_d(s, P.qd, P.qi, P.q));
         } else if (line[0] == 'R') {
             unsigned q;
             int nl;

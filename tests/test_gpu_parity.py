@@ -350,3 +350,40 @@ def test_short_lines_wheel_period_past_32_bits():
     finally:
         short.close()
         line.close()
+
+
+# ------------------------------------------- values either side of 2^53
+
+@pytest.mark.parametrize("fam", ["dickson", "ap"])
+def test_rounds_either_side_of_2_53(fam):
+    """Windows whose values cross 2^53 (Dickson k = 2310 j, AP p): below it
+    the round kernels feed the value itself to the residue, above it the
+    16-bit limbs; waves hold either or both.  Both round kernels (forced
+    by round_lane_prime), and round primes below 32 (a one-prime wheel
+    with a small q_med), which must not take the one-value form."""
+    if fam == "dickson":
+        n, q2 = 10, 1 << 14
+        j0 = (1 << 53) // 2310
+        lo, hi = j0 - 3 * 10**7, j0 + 3 * 10**7
+        want = O.dickson_survivors(n, lo, hi, q2=q2)
+        family = N.FAMILY_DICKSON
+    else:
+        n, q2 = 12, 1 << 14
+        lo = (1 << 53) - 6 * 10**8
+        lo -= lo % 30
+        hi = lo + 12 * 10**8
+        want = [lo + o for o in O.ap_survivors(n, lo, hi - lo, q2=q2)]
+        family = N.FAMILY_AP
+    say("2^53", fam, lo, hi, len(want))
+    assert len(want) >= 40
+    assert min(want) * (2310 if fam == "dickson" else 1) < 1 << 53
+    assert max(want) * (2310 if fam == "dickson" else 1) >= 1 << 53
+    small = {"nwheel": 1, "q_med": 24 if fam == "dickson" else 16,
+             "q_pat": 16}
+    for kw in ({"round_lane_prime": 1 << 30}, {"round_lane_prime": 1},
+               small, dict(small, round_lane_prime=1)):
+        eng = N.Engine(family, n, q2, **kw)
+        try:
+            assert N.record_values(eng.sweep(lo, hi)) == want, (fam, kw)
+        finally:
+            eng.close()

@@ -74,6 +74,29 @@ def test_residue_matches_python(harness):
     assert [g[2] for g in got] == fold          # resid_fold_d
 
 
+def test_residue_of_a_53_bit_value_in_one_double(harness):
+    """The round kernels' one-limb form: a value below 2^53 as one double,
+    for primes q >= 32 (then s / q < 2^48), at multiples of q, their
+    neighbours and the half-multiples, up to 2^53 - 1."""
+    rng = random.Random(53)
+    qs = [32 + 5, 37, 65521, 2**22 - 3, 2**30 - 35]
+    qs += [random_prime(rng, 32, 2**30) for _ in range(30)]
+    cases, want, wq = [], [], []
+    for q in qs:
+        top = (2**53 - 1) // q
+        vals = [2**53 - 1, 2**53 - 2, 0, 1, q - 1, q, q + 1]
+        for m in [top, top - 1] + [rng.randrange(1, top) for _ in range(6)]:
+            vals += [m * q + d for d in (-1, 0, 1, q // 2, q // 2 + 1)]
+        for v in vals:
+            if 0 <= v < 2**53:
+                cases.append(f"N {q} {v & 0xFFFFFFFF:x} {v >> 32:x}")
+                want.append(v % q)
+                wq.append(q)
+    got = [tuple(int(x) for x in line.split()) for line in call(harness, cases)]
+    assert [g[0] for g in got] == want
+    assert [g[1] for g in got] == [min(w, q - w) for w, q in zip(want, wq)]
+
+
 def kill_brute(family, n, q):
     d = O.primorial(n) if family == 3 else 0
     out = set()
