@@ -932,18 +932,20 @@ __device__ __forceinline__ bool lane_kills(const LaneRow& L, const double* a,
     }
     const uint32_t r = FOLD ? resid_fold_d(s, L.qd, L.qi, L.q)
                             : resid_d1(s, L.qd, L.qi, L.q);
-    bool hit = false;
+    // zero iff r is one of the residues: a minimum of r ^ k in vector
+    // registers (padding 0xffffffff never matches a residue below 2^30)
+    uint32_t d = 0xffffffffu;
 #pragma unroll
     for (int i = 0; i < kRegResidues; i += 4) {
         if ((uint32_t)i < ncmp)
-            hit = hit | (r == L.k[i]) | (r == L.k[i + 1]) | (r == L.k[i + 2]) |
-                  (r == L.k[i + 3]);
+            d = min(d, min(min(r ^ L.k[i], r ^ L.k[i + 1]),
+                           min(r ^ L.k[i + 2], r ^ L.k[i + 3])));
     }
     if (ncmp > (uint32_t)kRegResidues) {
         for (uint32_t c = kRegResidues; c < L.cnt; c++)
-            hit = hit || r == L.row[kRowHead + c];
+            d = min(d, r ^ L.row[kRowHead + c]);
     }
-    return hit;
+    return d == 0;
 }
 
 // One round, lane = prime: blockIdx.x picks 256 primes (one per lane),
