@@ -40,6 +40,8 @@ struct DevEngine {
     uint32_t round_first;     // global prime-table index of the first round prime
     uint32_t round_end;       // one past the last prime below q2
     uint32_t kstride;         // round-table row stride in words
+    uint32_t kidx;            // 1: the round rows are AP index rows (no
+                              // round prime divides d), 0: residue rows
     uint32_t nround;          // compaction rounds of this engine
     uint32_t rb[33];          // [nround + 1] prime-table index where each
                               // round starts (rb[0] = round_first, the last
@@ -66,7 +68,7 @@ struct DevEngine {
     const uint32_t* kt;       // [round_end - round_first][kstride] round
                               // table: per round prime a RoundRow header
                               // (fk_kill.h), then the killing residues of
-                              // the value, padded
+                              // the value, padded; or an IndexRow
 };
 
 // One piece of one window: a contiguous range of line positions of every

@@ -395,7 +395,14 @@ static int build_engine(fk_engine* e)
     h.ntr = (uint32_t)tr_end.size();
     h.round_first = e->round_first;
     h.round_end = e->round_end;
-    h.kstride = round_row_stride(d.family, d.n);
+    // AP rounds test one index per prime when no round prime divides d:
+    // the primes dividing d are the first n, so the first round prime
+    // decides
+    h.kidx = 0;
+    if (kill_indexable(d.family))
+        h.kidx = (e->round_first >= e->round_end ||
+                  e->dval % ctx->primes[e->round_first] != 0) ? 1u : 0u;
+    h.kstride = h.kidx ? kIndexStride : round_row_stride(d.family, d.n);
     h.nround = e->nrounds;
     for (size_t i = 0; i < 33; i++)
         h.rb[i] = i < rb.size() ? rb[i] : e->round_end;
@@ -428,7 +435,7 @@ static int build_engine(fk_engine* e)
         e->allocs.push_back(kt);
         e->device_bytes += nt * sizeof(uint32_t);
         HIPCHK(launch_kill_table(ctx->d_primes, e->round_first, np, d.family,
-                                 d.n, h.d, h.kstride, kt, nullptr));
+                                 d.n, h.d, h.kidx != 0, h.kstride, kt, nullptr));
         HIPCHK(hipDeviceSynchronize());
         h.kt = kt;
     }
@@ -1038,68 +1045,79 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     b->round_slots.clear();
     double alive = expect;
     for (uint32_t r = 0; r < nround_max; r++) {
-        // the primes of round r of every engine that has one
-        uint32_t P0 = 0xffffffffu, P1 = 0;
-        alive = 0.0;
-        for (size_t k = 0; k < b->engines.size(); k++) {
-            const DevEngine& h = b->engines[k]->host;
-            if (!used[k] || r >= h.nround || h.rb[r + 1] <= h.rb[r]) continue;
-            P0 = std::min(P0, h.rb[r]);
-            P1 = std::max(P1, h.rb[r + 1]);
-            alive += e_alive[k];
+        // one test launch per row kind (residue rows, AP index rows) among
+        // the engines with a round r: each kernel then compiles only its
+        // own path, at its own register count
+        bool any_round = false;
+        double alive_r = 0.0;
+        for (uint32_t kind : {kRowsResidue, kRowsIndex}) {
+            // the primes of round r of every engine of this kind
+            uint32_t P0 = 0xffffffffu, P1 = 0;
+            uint32_t kg = 1;
+            alive = 0.0;
+            for (size_t k = 0; k < b->engines.size(); k++) {
+                const DevEngine& h = b->engines[k]->host;
+                if (!used[k] || r >= h.nround || h.rb[r + 1] <= h.rb[r]) continue;
+                if ((h.kidx ? kRowsIndex : kRowsResidue) != kind) continue;
+                P0 = std::min(P0, h.rb[r]);
+                P1 = std::max(P1, h.rb[r + 1]);
+                alive += e_alive[k];
+                // residue groups per row held in registers: three when
+                // every engine's kill sets fit twelve residues (both rows
+                // of a pass then load together), else five
+                if (!h.kidx) kg = std::max(kg, (kill_max(h.family, h.n) + 3u) / 4u);
+            }
+            if (P0 >= P1) continue;
+            any_round = true;
+            alive_r += alive;
+            Step rst = new_step(Step::kRound);
+            RoundArgs& ra = rst.ra;
+            ra.engines = b->d_engines;
+            ra.nengines = (uint32_t)ne;
+            ra.segs = b->d_segs;
+            ra.primed = ctx->d_primed;
+            ra.recs = b->d_q[cur];
+            ra.counts = b->d_counts + cstage * ne;
+            ra.flags = b->d_flags;
+            ra.round = r;
+            ra.p_begin = P0;
+            ra.p_end = P1;
+            ra.q_min = ctx->primes[P0];
+            ra.kgroups = kg <= 3 ? 3u : 5u;
+            ra.rows = kind;
+            const uint32_t np = P1 - P0;
+            // Many primes and enough candidates: lane = prime (256 primes
+            // per workgroup column, the queue cut into slices of about 256
+            // candidates, so that no wave walks a long chain of
+            // candidates).  Otherwise lane = candidate (candidate blocks
+            // times prime blocks of at least 16, about kRoundWorkgroups
+            // workgroups: a workgroup pulls its block's rows toward the
+            // caches once, then its waves walk them with a cache latency
+            // per pair).  Both read the true count on the device; grids
+            // sized from an estimate that is off still cover every
+            // candidate.
+            const bool lane_prime = np >= e0->round_lane_prime &&
+                                    alive >= e0->round_lane_prime / 8.0;
+            uint32_t gx, gy;
+            if (lane_prime) {
+                gx = (np + 255) / 256;
+                gy = (uint32_t)std::min(std::ceil(alive / 256.0),
+                                        std::max(1.0, std::floor((1u << 22) / (double)gx)));
+                gy = std::max(1u, std::min(gy, 65535u));
+            } else {
+                gx = (uint32_t)std::min(1024.0, std::max(1.0, std::ceil(alive / (256.0 * kRoundKC))));
+                gy = std::min((np + 15) / 16, (kRoundWorkgroups + gx - 1) / gx);
+                gy = std::max(1u, std::min(gy, 65535u));
+                ra.pblock = (np + gy - 1) / gy;
+                gy = (np + ra.pblock - 1) / ra.pblock;
+            }
+            rst.lane_prime = lane_prime ? 1 : 0;
+            rst.grid = gx;
+            rst.gy = gy;
+            plan.push_back(rst);
         }
-        if (P0 >= P1) continue;
-        Step rst = new_step(Step::kRound);
-        RoundArgs& ra = rst.ra;
-        ra.engines = b->d_engines;
-        ra.nengines = (uint32_t)ne;
-        ra.segs = b->d_segs;
-        ra.primed = ctx->d_primed;
-        ra.recs = b->d_q[cur];
-        ra.counts = b->d_counts + cstage * ne;
-        ra.flags = b->d_flags;
-        ra.round = r;
-        ra.p_begin = P0;
-        ra.p_end = P1;
-        ra.q_min = ctx->primes[P0];
-        // residue groups per row held in registers: three when every
-        // engine's kill sets fit twelve residues (both rows of a pass then
-        // load together), else five
-        uint32_t kg = 1;
-        for (size_t k = 0; k < b->engines.size(); k++)
-            if (used[k])
-                kg = std::max(kg, (kill_max(b->engines[k]->host.family,
-                                            b->engines[k]->host.n) + 3u) / 4u);
-        ra.kgroups = kg <= 3 ? 3u : 5u;
-        const uint32_t np = P1 - P0;
-        // Many primes and enough candidates: lane = prime (256 primes per
-        // workgroup column, the queue cut into slices of about 256
-        // candidates, so that no wave walks a long chain of candidates).
-        // Otherwise lane = candidate (candidate blocks times
-        // prime blocks of at least 16, about kRoundWorkgroups workgroups:
-        // a workgroup pulls its block's rows toward the caches once, then
-        // its waves walk them with a cache latency per pair).  Both
-        // read the true count on the device; grids sized from an estimate
-        // that is off still cover every candidate.
-        const bool lane_prime = np >= e0->round_lane_prime &&
-                                alive >= e0->round_lane_prime / 8.0;
-        uint32_t gx, gy;
-        if (lane_prime) {
-            gx = (np + 255) / 256;
-            gy = (uint32_t)std::min(std::ceil(alive / 256.0),
-                                    std::max(1.0, std::floor((1u << 22) / (double)gx)));
-            gy = std::max(1u, std::min(gy, 65535u));
-        } else {
-            gx = (uint32_t)std::min(1024.0, std::max(1.0, std::ceil(alive / (256.0 * kRoundKC))));
-            gy = std::min((np + 15) / 16, (kRoundWorkgroups + gx - 1) / gx);
-            gy = std::max(1u, std::min(gy, 65535u));
-            ra.pblock = (np + gy - 1) / gy;
-            gy = (np + ra.pblock - 1) / ra.pblock;
-        }
-        rst.lane_prime = lane_prime ? 1 : 0;
-        rst.grid = gx;
-        rst.gy = gy;
-        plan.push_back(rst);
+        if (!any_round) continue;
+        alive = alive_r;
         Step cst = new_step(Step::kCompact);
         CompactArgs& ca = cst.ca;
         ca.nengines = (uint32_t)ne;

@@ -596,15 +596,19 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
 // ---------------------------------------------------------------- rounds
 
 // Round table of each engine, built once at engine creation: one thread
-// per round prime writes its RoundRow and kill set.
+// per round prime writes its RoundRow and kill set, or its IndexRow.
 __global__ void k_kill_table(const PrimeK* __restrict__ primes, uint32_t p0,
                              uint32_t np, int family, int n, uint4 d,
-                             uint32_t stride, uint32_t* __restrict__ kt)
+                             uint32_t index, uint32_t stride,
+                             uint32_t* __restrict__ kt)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const uint32_t dl[4] = {d.x, d.y, d.z, d.w};
-    round_row(family, n, dl, primes[p0 + i], kt + (size_t)i * stride, stride);
+    if (index)
+        index_row(n, dl, primes[p0 + i], kt + (size_t)i * stride);
+    else
+        round_row(family, n, dl, primes[p0 + i], kt + (size_t)i * stride, stride);
 }
 
 
@@ -770,13 +774,79 @@ __device__ __forceinline__ void round_scan(const double (&a)[KC][8],
     }
 }
 
+// ---- index rows (AP): the lane's t = -p / d mod q from the row's linear
+// form in p's 16-bit limbs (fk_kill.h IndexRow); q kills the lanes with
+// t < n.  One compare per (candidate, prime) instead of one per residue.
+struct IdxView {
+    double qd, qi, u0, u1, u2, u3;
+    uint32_t q, n;
+};
+
+__device__ __forceinline__ IdxView load_idx(const FK_CONST uint32_t* row)
+{
+    IdxView v;
+    v.qd = load_f64(row + 0);
+    v.qi = load_f64(row + 2);
+    v.u0 = load_f64(row + 4);
+    v.u1 = load_f64(row + 6);
+    v.u2 = load_f64(row + 8);
+    v.u3 = load_f64(row + 10);
+    v.q = row[12];
+    v.n = row[13];
+    return v;
+}
+
+// Each product a_k u_k is below 2^46 (q < 2^30) and the sum of eight below
+// 2^49, all exact in a double; s / q < 2^19 keeps resid_d1 exact.
+template <int NL>
+__device__ __forceinline__ bool idx_kills(const double (&a)[8], const IdxView& v,
+                                          const FK_CONST uint32_t* row)
+{
+    double s = fma(a[3], v.u3, fma(a[2], v.u2, fma(a[1], v.u1, a[0] * v.u0)));
+    if (NL > 4) {
+#pragma unroll
+        for (int k = 4; k < NL; k++)
+            s = fma(a[k], load_f64(row + kIndexHead + 2 * (k - 4)), s);
+    }
+    return resid_d1(s, v.qd, v.qi, v.q) < v.n;
+}
+
+// KC candidates per lane against the index rows [lo, hi): R rows per pass
+// (their scalar loads in flight together), four for values below 2^64.
+template <int NL, int KC>
+__device__ __forceinline__ void index_scan(const double (&a)[KC][8],
+                                           const FK_CONST uint32_t* kt,
+                                           uint32_t rf, uint32_t lo, uint32_t hi,
+                                           uint64_t (&hit)[KC])
+{
+    constexpr uint32_t R = NL > 4 ? 2 : 4;
+    uint32_t p = lo;
+    for (; p + R <= hi; p += R) {
+        const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * kIndexStride;
+        IdxView v[R];
+#pragma unroll
+        for (uint32_t i = 0; i < R; i++) v[i] = load_idx(r0 + i * kIndexStride);
+#pragma unroll
+        for (uint32_t i = 0; i < R; i++)
+#pragma unroll
+            for (int c = 0; c < KC; c++)
+                hit[c] |= __ballot(idx_kills<NL>(a[c], v[i], r0 + i * kIndexStride));
+    }
+    for (; p < hi; p++) {
+        const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * kIndexStride;
+        const IdxView v = load_idx(r0);
+#pragma unroll
+        for (int c = 0; c < KC; c++) hit[c] |= __ballot(idx_kills<NL>(a[c], v, r0));
+    }
+}
+
 // One round, lane = candidate: blockIdx.x walks the engines' segments in
 // blocks of 256 KC candidates (grid-stride; lane l takes entries l,
 // l + 256, ...), blockIdx.y takes a block of pblock primes.  The engine is
 // uniform per block, so the prime-side data is too, and each prime row read
 // serves KC candidates per lane.  A killed candidate's flag is set.
 
-template <int KG>
+template <int KG, uint32_t ROWS>
 __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
 {
     constexpr int KC = kRoundKC;
@@ -791,6 +861,8 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
          seg_block(segs, counts, a.nengines, 256 * KC, blk, sb); blk += gridDim.x) {
         const FK_CONST DevEngine* E = uniform_view(a.engines) + sb.e;
         if (a.round >= E->nround) continue;
+        // engines with the other kind of row have a launch of their own
+        if ((E->kidx ? kRowsIndex : kRowsResidue) != ROWS) continue;
         const uint32_t lo = max(pb0, E->rb[a.round]);
         const uint32_t hi = min(pb1, E->rb[a.round + 1]);
         if (lo >= hi) continue;
@@ -842,7 +914,10 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
         uint64_t m[KC];
 #pragma unroll
         for (int c = 0; c < KC; c++) m[c] = 0;
-        if (kill_folded(E->family)) {
+        if (ROWS == kRowsIndex) {
+            if (wwide) index_scan<8, KC>(l, kt, rf, lo, hi, m);
+            else index_scan<4, KC>(l, kt, rf, lo, hi, m);
+        } else if (kill_folded(E->family)) {
             if (wwide) round_scan<8, KG, true, KC>(l, pd, kt, st, rf, lo, hi, m);
             else round_scan<4, KG, true, KC>(l, pd, kt, st, rf, lo, hi, m);
         } else {
@@ -960,11 +1035,11 @@ __device__ __forceinline__ bool lane_kills(const LaneRow& L, const double* a,
 constexpr uint32_t kRoundSlice = 256;
 
 // One slice's candidates against the lane's prime; candidate c's input is
-// a + step * c (NL doubles: 16-bit limbs, or with NL == 1 the value).
-template <int NL, bool FOLD>
-__device__ __forceinline__ void lane_prime_walk(const LaneRow& L, bool mine,
-                                                uint32_t ncmp, const double* a,
-                                                uint32_t step,
+// a + step * c, and kills(x) says whether the lane's prime kills the
+// candidate of input x.
+template <typename Kills>
+__device__ __forceinline__ void lane_prime_walk(Kills kills, bool mine,
+                                                const double* a, uint32_t step,
                                                 const uint32_t* live_idx,
                                                 uint32_t nlive, uint32_t* flags)
 {
@@ -977,7 +1052,7 @@ __device__ __forceinline__ void lane_prime_walk(const LaneRow& L, bool mine,
         bool kill[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++)
-            kill[u] = mine && lane_kills<NL, FOLD>(L, a + step * (c + u), ncmp);
+            kill[u] = mine && kills(a + step * (c + u));
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             const uint64_t m = __ballot(kill[u]);
@@ -986,13 +1061,15 @@ __device__ __forceinline__ void lane_prime_walk(const LaneRow& L, bool mine,
         }
     }
     for (; c < nlive; c++) {
-        const bool kill = mine && lane_kills<NL, FOLD>(L, a + step * c, ncmp);
+        const bool kill = mine && kills(a + step * c);
         const uint64_t m = __ballot(kill);
         if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1))
             flags[live_idx[c]] = 1;
     }
 }
 
+// Residue rows: 16-bit limbs (8 past 2^64, else 4), or with every value
+// below 2^53 the value itself (NL == 1).
 template <bool FOLD>
 __device__ __forceinline__ void lane_prime_slice(const LaneRow& L, bool mine,
                                                  uint32_t ncmp, bool wide,
@@ -1003,13 +1080,58 @@ __device__ __forceinline__ void lane_prime_slice(const LaneRow& L, bool mine,
                                                  uint32_t nlive, uint32_t* flags)
 {
     if (wide)
-        lane_prime_walk<8, FOLD>(L, mine, ncmp, limbs, 8, live_idx, nlive, flags);
+        lane_prime_walk([&](const double* x) { return lane_kills<8, FOLD>(L, x, ncmp); },
+                        mine, limbs, 8, live_idx, nlive, flags);
     else if (narrow)
-        lane_prime_walk<1, FOLD>(L, mine, ncmp, vals, 1, live_idx, nlive, flags);
+        lane_prime_walk([&](const double* x) { return lane_kills<1, FOLD>(L, x, ncmp); },
+                        mine, vals, 1, live_idx, nlive, flags);
     else
-        lane_prime_walk<4, FOLD>(L, mine, ncmp, limbs, 8, live_idx, nlive, flags);
+        lane_prime_walk([&](const double* x) { return lane_kills<4, FOLD>(L, x, ncmp); },
+                        mine, limbs, 8, live_idx, nlive, flags);
 }
 
+// An index row (AP) in a lane's registers.
+struct LaneIdx {
+    double qd, qi;
+    double u[8];                  // u 2^(16k) mod q
+    uint32_t q, n;
+};
+
+__device__ __forceinline__ void load_lane_idx(LaneIdx& I, const uint32_t* row)
+{
+    IndexRow h;
+    const uint4* r4 = reinterpret_cast<const uint4*>(row);
+    uint32_t hw[kIndexStride];
+#pragma unroll
+    for (int i = 0; i < (int)kIndexStride / 4; i++) {
+        const uint4 v = r4[i];
+        hw[4 * i] = v.x;
+        hw[4 * i + 1] = v.y;
+        hw[4 * i + 2] = v.z;
+        hw[4 * i + 3] = v.w;
+    }
+    __builtin_memcpy(&h, hw, sizeof h);
+    I.qd = h.qd;
+    I.qi = h.qi;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        I.u[k] = h.u[k];
+        I.u[4 + k] = h.uw[k];
+    }
+    I.q = h.q;
+    I.n = h.n;
+}
+
+template <int NL>
+__device__ __forceinline__ bool lane_idx_kills(const LaneIdx& I, const double* a)
+{
+    double s = a[0] * I.u[0];
+#pragma unroll
+    for (int k = 1; k < NL; k++) s = fma(a[k], I.u[k], s);
+    return resid_d1(s, I.qd, I.qi, I.q) < I.n;
+}
+
+template <uint32_t ROWS>
 __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
 {
     __shared__ double sh_limbs[kRoundSlice * 8];
@@ -1029,6 +1151,8 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
          blk += gridDim.y) {
         const uint32_t e = sb.e;
         if (a.round >= engs[e].nround) continue;
+        // engines with the other kind of row have a launch of their own
+        if ((engs[e].kidx ? kRowsIndex : kRowsResidue) != ROWS) continue;
         const uint32_t rlo = engs[e].rb[a.round], rhi = engs[e].rb[a.round + 1];
         // workgroup-uniform: does any of its 256 primes belong to e's round?
         if (blk_p0 + 256 <= rlo || blk_p0 >= rhi) continue;
@@ -1064,7 +1188,7 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
             // compares per test: the family's largest kill set, in fours
             const uint32_t ncmp = (kill_max(engs[e].family, engs[e].n) + 3u) & ~3u;
             const uint32_t stride = engs[e].kstride;
-            if (mine) {
+            if (ROWS == kRowsResidue && mine) {
                 load_lane_row(L, engs[e].kt + (size_t)(p - engs[e].round_first) * stride,
                               a.primed + p, stride);
                 if (ncmp > 20) load_lane_row_wide(L, stride - kRowHead);
@@ -1072,7 +1196,20 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
             // the one-value form needs every value below 2^53 and every
             // prime of the round at least 32
             const bool narrow = below53 && a.q_min >= 32u;
-            if (__ballot(mine)) {
+            if (ROWS == kRowsIndex) {
+                LaneIdx I;
+                if (mine)
+                    load_lane_idx(I, engs[e].kt + (size_t)(p - engs[e].round_first) *
+                                                      kIndexStride);
+                if (__ballot(mine)) {
+                    if (wide)
+                        lane_prime_walk([&](const double* x) { return lane_idx_kills<8>(I, x); },
+                                        mine, sh_limbs, 8, sh_idx, nlive, a.flags);
+                    else
+                        lane_prime_walk([&](const double* x) { return lane_idx_kills<4>(I, x); },
+                                        mine, sh_limbs, 8, sh_idx, nlive, a.flags);
+                }
+            } else if (__ballot(mine)) {
                 if (kill_folded(engs[e].family))
                     lane_prime_slice<true>(L, mine, ncmp, wide, narrow, sh_limbs,
                                            sh_val, sh_idx, nlive, a.flags);
@@ -1442,15 +1579,27 @@ hipError_t launch_round_test(const RoundArgs& a, bool lane_prime, uint32_t gx,
     if (lane_prime) {
         // every prime must have its lane: 256 primes per blockIdx.x
         if ((uint64_t)gx * 256 < np) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_round_p, dim3(gx, gy), dim3(256), 0, stream, a);
+        if (a.rows == kRowsResidue)
+            hipLaunchKernelGGL(k_round_p<kRowsResidue>, dim3(gx, gy), dim3(256), 0,
+                               stream, a);
+        else if (a.rows == kRowsIndex)
+            hipLaunchKernelGGL(k_round_p<kRowsIndex>, dim3(gx, gy), dim3(256), 0,
+                               stream, a);
+        else
+            return hipErrorInvalidValue;
     } else {
         // every prime block must be launched
         if (a.pblock == 0 || (uint64_t)gy * a.pblock < np)
             return hipErrorInvalidValue;
-        if (a.kgroups == 3)
-            hipLaunchKernelGGL(k_round_c<3>, dim3(gx, gy), dim3(256), 0, stream, a);
-        else if (a.kgroups == 5)
-            hipLaunchKernelGGL(k_round_c<5>, dim3(gx, gy), dim3(256), 0, stream, a);
+        if (a.rows == kRowsIndex)
+            hipLaunchKernelGGL((k_round_c<3, kRowsIndex>), dim3(gx, gy), dim3(256),
+                               0, stream, a);
+        else if (a.rows == kRowsResidue && a.kgroups == 3)
+            hipLaunchKernelGGL((k_round_c<3, kRowsResidue>), dim3(gx, gy), dim3(256),
+                               0, stream, a);
+        else if (a.rows == kRowsResidue && a.kgroups == 5)
+            hipLaunchKernelGGL((k_round_c<5, kRowsResidue>), dim3(gx, gy), dim3(256),
+                               0, stream, a);
         else
             return hipErrorInvalidValue;
     }
@@ -1458,14 +1607,17 @@ hipError_t launch_round_test(const RoundArgs& a, bool lane_prime, uint32_t gx,
 }
 
 hipError_t launch_kill_table(const PrimeK* primes, uint32_t p0, uint32_t np,
-                             int family, int n, const uint32_t d[4],
+                             int family, int n, const uint32_t d[4], bool index,
                              uint32_t stride, uint32_t* kt, hipStream_t stream)
 {
     if (np == 0) return hipSuccess;
-    if (stride != round_row_stride(family, n)) return hipErrorInvalidValue;
+    if (index ? (!kill_indexable(family) || stride != kIndexStride)
+              : stride != round_row_stride(family, n))
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_kill_table, dim3((np + 255) / 256), dim3(256), 0,
                        stream, primes, p0, np, family, n,
-                       make_uint4(d[0], d[1], d[2], d[3]), stride, kt);
+                       make_uint4(d[0], d[1], d[2], d[3]), index ? 1u : 0u,
+                       stride, kt);
     return hipGetLastError();
 }
 

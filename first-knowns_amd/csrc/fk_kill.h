@@ -166,4 +166,51 @@ FK_HD void round_row(int family, int n, const uint32_t d[4], const PrimeK& p,
     for (uint32_t i = 0; i < kRowHead; i++) row[i] = hw[i];
 }
 
+// ---- index rows (AP): the kill set of a prime q that does not divide d
+// is {-j d mod q : 0 <= j < n}, n distinct residues, so q kills p exactly
+// when t = -p / d mod q is below n.  t is a linear form in p's 16-bit
+// limbs: t = sum_k a_k (u 2^(16k) mod q) mod q with u = -1/d mod q, the
+// same fused-multiply-add sum and exact reduction as a residue, so one
+// compare replaces n.  (A prime dividing d kills p iff q | p; an engine
+// with such a round prime keeps the residue rows.)
+struct IndexRow {
+    double qd;        // q
+    double qi;        // nearest double to 1/q
+    double u[4];      // u 2^(16k) mod q, k = 0..3
+    uint32_t q;
+    uint32_t n;       // t < n kills
+    uint32_t pad[2];
+    double uw[4];     // u 2^(16k) mod q, k = 4..7 (values past 2^64)
+};
+constexpr uint32_t kIndexStride = 24;       // words of IndexRow
+constexpr uint32_t kIndexHead = 16;         // words read for values < 2^64
+static_assert(sizeof(IndexRow) == 4 * kIndexStride, "IndexRow layout");
+
+// Families whose rows may be index rows.
+FK_HD bool kill_indexable(int family) { return family == 3; }
+
+// The index row of q (q must not divide d).
+FK_HD void index_row(int n, const uint32_t d[4], const PrimeK& p, uint32_t* row)
+{
+    const PrimeD pd = make_primed(p.q);
+    const uint32_t dq = red128(d, p);                   // d mod q
+    const uint32_t di = pow_mod(dq, p.q - 2, p);          // 1/d, q prime
+    uint32_t c = di ? p.q - di : 0u;                      // u = -1/d
+    const uint32_t two16 = (uint32_t)((1ull << 16) % p.q);
+    IndexRow h;
+    h.qd = pd.qd;
+    h.qi = pd.qi;
+    for (int k = 0; k < 8; k++) {
+        if (k < 4) h.u[k] = (double)c;
+        else h.uw[k - 4] = (double)c;
+        c = mulmod32(c, two16, p.q, p.m64);
+    }
+    h.q = p.q;
+    h.n = (uint32_t)n;
+    h.pad[0] = h.pad[1] = 0;
+    uint32_t hw[kIndexStride];
+    __builtin_memcpy(hw, &h, sizeof h);          // no type punning
+    for (uint32_t i = 0; i < kIndexStride; i++) row[i] = hw[i];
+}
+
 }  // namespace fk

@@ -59,7 +59,11 @@ struct RoundArgs {
                               // rows with more are finished in a loop)
     uint32_t q_min;           // the round's smallest prime (the one-value
                               // residue form needs q >= 32)
+    uint32_t rows;            // the launch tests the engines whose round
+                              // rows are of this kind (kRowsResidue or
+                              // kRowsIndex) and skips the others
 };
+constexpr uint32_t kRowsResidue = 1, kRowsIndex = 2;
 
 struct CompactArgs {
     uint32_t nengines;
@@ -120,7 +124,7 @@ bool sieve_shape_ok(int threads, int wpt);
 hipError_t launch_round_test(const RoundArgs& a, bool lane_prime, uint32_t gx,
                              uint32_t gy, hipStream_t stream);
 hipError_t launch_kill_table(const PrimeK* primes, uint32_t p0, uint32_t np,
-                             int family, int n, const uint32_t d[4],
+                             int family, int n, const uint32_t d[4], bool index,
                              uint32_t stride, uint32_t* kt, hipStream_t stream);
 hipError_t launch_round_compact(const CompactArgs& a, uint32_t grid,
                                 hipStream_t stream);

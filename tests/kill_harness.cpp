@@ -8,6 +8,9 @@
 //                                    two-fix and one-fix forms, and folded
 //   N q vlo vhi                   -> one-fix and folded residues of the value
 //                                    (vhi:vlo < 2^53) held as one double
+//   I n q nl d0 d1 d2 d3 t0 t1 t2 t3 -> the AP index -t/d mod q of the value t
+//                                    from q's index row (nl limbs), and the
+//                                    row's words
 #include "../first-knowns_amd/csrc/fk_kill.h"
 #include <cstdio>
 #include <cstring>
@@ -53,6 +56,26 @@ int main()
             const double s = (double)(((uint64_t)hi << 32) | lo);
             printf("%u %u\n", fk::resid_d1(s, P.qd, P.qi, P.q),
                    fk::resid_fold_d(s, P.qd, P.qi, P.q));
+        } else if (line[0] == 'I') {
+            int n, nl;
+            unsigned q;
+            uint32_t d[4], t[4];
+            if (sscanf(line + 1, "%d %u %d %x %x %x %x %x %x %x %x", &n, &q, &nl,
+                       &d[0], &d[1], &d[2], &d[3], &t[0], &t[1], &t[2],
+                       &t[3]) != 11) continue;
+            const fk::PrimeK pk = fk::make_primek(q);
+            uint32_t row[fk::kIndexStride];
+            fk::index_row(n, d, pk, row);
+            fk::IndexRow h;
+            memcpy(&h, row, sizeof h);
+            double s = 0;
+            for (int k = 0; k < nl; k++) {
+                const double a = (double)((t[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
+                s = fma(a, k < 4 ? h.u[k] : h.uw[k - 4], s);
+            }
+            printf("%u %u %u", fk::resid_d1(s, h.qd, h.qi, h.q), h.q, h.n);
+            for (uint32_t i = 0; i < fk::kIndexStride; i++) printf(" %u", row[i]);
+            printf("\n");
         } else if (line[0] == 'R') {
             unsigned q;
             int nl;

@@ -259,6 +259,9 @@ ROUND_CASES = [
      {"wheel_primes": [2, 3, 5, 7, 11, 13]}, 50),
     ("ap", N.FAMILY_AP, 12, 1 << 16, 4 * 10**13 - 10,
      4 * 10**13 - 10 + 3 * 10**8, {}, 80),
+    # p past 2^64: the index rows' eight-limb form
+    ("ap-wide", N.FAMILY_AP, 12, 1 << 16, 10**20 - 10,
+     10**20 - 10 + 3 * 10**8, {}, 60),
 ]
 
 

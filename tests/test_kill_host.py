@@ -191,3 +191,35 @@ def test_round_rows(harness, family, n):
         assert len(res) == stride - 12
         assert len(set(res[:cnt])) == cnt
         assert all(x == 0xFFFFFFFF for x in res[cnt:])
+
+
+def test_ap_index_rows(harness):
+    """AP index rows: for a prime q not dividing d = P(n), the row's linear
+    form gives t = -v / d mod q, and t < n exactly when q kills v (brute
+    force over every residue for small q; members of the kill set, their
+    neighbours and random values up to 2^128 for large q)."""
+    rng = random.Random(31)
+    cases, want, kill = [], [], []
+    for n in (8, 12, 17):
+        d = O.primorial(n)
+        pn = [q for q in range(2, 200) if is_prime(q)][n - 1]
+        small = [q for q in range(pn + 1, 400) if is_prime(q)]
+        # round primes are below 2^30 (engines take q2 <= 2^30)
+        large = [random_prime(rng, 2**16, 2**30) for _ in range(12)]
+        large += [2**30 - 35, 2**29 - 3]
+        for q in small + large:
+            ks = {(-j * d) % q for j in range(n)}
+            vals = list(range(q)) if q < 400 else (
+                [k + q * rng.getrandbits(60) for k in ks] +
+                [k + 1 + q * rng.getrandbits(100) for k in ks] +
+                [rng.getrandbits(b) for b in (40, 64, 90, 128) for _ in range(8)])
+            for v in vals:
+                if v >= 2**128:
+                    continue
+                nl = 4 if v < 2**64 else 8
+                cases.append(f"I {n} {q} {nl} {limbs(d)} {limbs(v)}")
+                want.append((-v * pow(d, -1, q)) % q)
+                kill.append(v % q in ks)
+    got = [[int(x) for x in line.split()] for line in call(harness, cases)]
+    assert [g[0] for g in got] == want
+    assert [g[0] < g[2] for g in got] == kill
