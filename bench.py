@@ -19,7 +19,8 @@ Beside it, in the same JSON line:
                 stream checked equal to the GPU's
   batched       256 mixed windows of the three families (lengths 2^20..2^30,
                 n 8..14), one persistent batched launch vs the plain engine
-                window by window, with the two streams checked equal
+                window by window (two in flight), with the two streams
+                checked equal
   serving       Poisson arrivals of requests of the three families (spans
                 2^18..2^32, n 8..14, Dickson k to 1e22), continuous batching
                 over 4 slots: p50/p99 completion latency and candidates/s
@@ -65,24 +66,26 @@ def sweep_window(seed):
 
 
 def run_plain(steps, warmup):
-    """The timed loop: `steps` windows swept one after another on the plain
-    engine (each sweep synchronous, as the drop-in's callers make it; the
-    engine replays its repeated launch sequence as a HIP graph).  The stage
-    split comes from three more windows swept eagerly with HIP events
-    between the stages, outside the timed loop."""
+    """The timed loop: `steps` windows swept in order on the plain engine,
+    two in flight (Engine.sweep_many: each window one engine call on its
+    own stream and captured graph, the next enqueued before the last is
+    read back, as the reference's launch loop does).  device_ms is one
+    window's device time alone: two more windows swept one at a time
+    (graph replays) after the timed loop.  The stage split comes from
+    three more windows swept eagerly with HIP events between the stages."""
     import fk_native as N
     import ladder_gpu
     eng = ladder_gpu.GpuEngine(SWEEP_N, q2=SWEEP_Q2)
-    for s in range(warmup):
-        lo, hi = sweep_window(1000 + s)
-        eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, SWEEP_CAP)
-    out = []
+    for _ in eng._eng.sweep_many([sweep_window(1000 + s) for s in range(warmup)],
+                                 N.CLASSIFY_FULL, SWEEP_CAP):
+        pass
+    wins = [sweep_window(s) for s in range(steps)]
     t0 = time.perf_counter()
-    for s in range(steps):
-        lo, hi = sweep_window(s)
-        out.append(eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, SWEEP_CAP))
+    out = list(eng._eng.sweep_many(wins, N.CLASSIFY_FULL, SWEEP_CAP))
     wall = time.perf_counter() - t0
-    device_ms = eng._eng.last_timing()[0][4]      # the last timed sweep
+    for s in range(3):                            # alone, graph replays
+        eng._eng.sweep(*sweep_window(2000 + s), N.CLASSIFY_FULL, SWEEP_CAP)
+    device_ms = eng._eng.last_timing()[0][4]
     surv = sum(int(r.size) for r in out)
     deepest = (0, None)
     for recs in out:
@@ -190,12 +193,16 @@ def run_batched():
     ws = [(lo, hi) for _, _, lo, hi in wins]
     specs = [batch_spec(f, n) for f, n, _, _ in wins]
     cand = sum(hi - lo for lo, hi in ws)
-    # plain: one engine call per window
-    for i in range(3):
-        engines[eo[i]].sweep(*ws[i], *specs[i])
+    # plain: one engine call per window, two in flight (the plain engine's
+    # own pipelining, as in the headline loop)
+    pipe = N.Pipeline()
+    jobs = [(engines[eo[i]], *ws[i], specs[i]) for i in range(len(ws))]
+    for _ in pipe.run(jobs[:3]):
+        pass
     t0 = time.perf_counter()
-    plain = [engines[eo[i]].sweep(*ws[i], *specs[i]) for i in range(len(ws))]
+    plain = [r for _, r in pipe.run(jobs)]
     t_plain = time.perf_counter() - t0
+    pipe.close()
     # batched: one submission
     b = N.Batch()
     b.submit(engines, eo, ws, specs)
@@ -327,7 +334,8 @@ def main():
         "config": {"workload": "dickson-ladder sweep, plain GPU engine: "
                                "2^32 consecutive j near k=1e15 per step, "
                                "n=10, primes < 2^22 sieved, full-bases "
-                               "ladder depth of every survivor",
+                               "ladder depth of every survivor; windows "
+                               "in order, two in flight",
                    "n": SWEEP_N, "q2": SWEEP_Q2, "window_j": SWEEP_SPAN,
                    "classify_cap": SWEEP_CAP,
                    "wheel_lines": int(info.nlines),
@@ -336,13 +344,14 @@ def main():
                    "tile_bits": None},
         "survivors": surv,
         "deepest_ladder": {"run": deepest[0], "j": deepest[1]},
-        "device_ms_per_step": device_ms,
+        "device_ms_per_window_alone": device_ms,
         "stages_ms_per_step": dict(zip(("prep", "sieve", "rounds",
                                         "classify", "whole"), stage)),
         "stages_note": "per-stage HIP events on three windows swept eagerly "
-                       "after the timed loop; the timed sweeps replay the "
-                       "engine's captured launch sequence (device_ms_per_step "
-                       "is the last one's event-timed device time)",
+                       "after the timed loop; the timed sweeps keep two "
+                       "windows in flight, each replaying its slot's captured "
+                       "launch sequence; device_ms_per_window_alone is one "
+                       "window swept by itself (graph replay, event-timed)",
         "roofline": {"bound": "lds", "kernel": kernel,
                      "achieved": achieved / 1e9, "peak": LDS_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / LDS_PEAK,

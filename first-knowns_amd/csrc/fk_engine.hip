@@ -563,6 +563,7 @@ struct fk_batch {
     uint2* d_segs = nullptr;          size_t cap_segs = 0;
     // graph mode: the captured launch sequence and the plan it was made from
     bool use_graph = false;
+    bool graph_at_once = false;       // capture a new sequence at once
     bool graph_ran = false;           // the last submission ran as a graph
     bool staged = false;              // the last submission recorded stage events
     std::vector<Step> prev_plan;      // graph mode: the last sequence run eagerly
@@ -1231,7 +1232,8 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     b->staged = false;
     if (b->use_graph) {
         const bool replay = b->graph_exec && same_plan(plan, b->graph_plan);
-        const bool repeat = !replay && same_plan(plan, b->prev_plan);
+        const bool repeat = !replay && (b->graph_at_once ||
+                                        same_plan(plan, b->prev_plan));
         if (repeat) {
             if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
             if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -1657,6 +1659,7 @@ int fk_batch_set_graph(fk_batch* b, int enable)
     if (!b) return fail(FK_E_ARG, "null argument");
     if (b->pending) return fail(FK_E_BUSY, "batch has work in flight");
     b->use_graph = enable != 0;
+    b->graph_at_once = enable == 2;
     return FK_OK;
 }
 

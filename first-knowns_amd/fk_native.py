@@ -191,6 +191,7 @@ class Engine:
         check(lib().fk_engine_get_info(h, ctypes.byref(info)))
         self.info = info
         self.family, self.n, self.q2 = int(family), int(n), int(q2)
+        self.device = int(device)
         self.line_step = int(info.line_step)
         self.nlines = int(info.nlines)
         self.wheel = [int(info.wheel_primes[i]) for i in range(info.nwheel)]
@@ -200,6 +201,9 @@ class Engine:
         return self._h
 
     def close(self):
+        if getattr(self, "_pipe", None) is not None:
+            self._pipe.close()
+            self._pipe = None
         if getattr(self, "_h", None):
             lib().fk_engine_destroy(self._h)
             self._h = None
@@ -229,6 +233,21 @@ class Engine:
                                         ctypes.byref(n), None)
         check(rc)
         return out[:n.value].copy()
+
+    def sweep_many(self, windows, mode=CLASSIFY_NONE, cap=0, exact_from=0):
+        """Yield the sorted records of each window [lo, hi) of `windows` in
+        order, two windows in flight on two streams (Pipeline): the same
+        records as sweep() window by window."""
+        ws = [(int(lo), int(hi)) for lo, hi in windows]
+        if getattr(self, "_pipe", None) is None:
+            self._pipe = Pipeline(self.device)
+        # windows of one length repeat their launch sequence: capture it
+        # at once
+        graph = 2 if len({hi - lo for lo, hi in ws}) <= 1 else 1
+        spec = (mode, cap, exact_from)
+        for _, recs in self._pipe.run([(self, lo, hi, spec) for lo, hi in ws],
+                                      graph=graph):
+            yield recs
 
     def classify(self, values, mode=CLASSIFY_FULL, cap=64, exact_from=0):
         """Chain lengths of line values (Python ints), computed on device."""
@@ -293,7 +312,8 @@ class Batch:
         self._h = h
         self._nwin = 0
         if graph:
-            check(lib().fk_batch_set_graph(h, 1))
+            # 1 (True): capture a launch sequence when it repeats; 2: at once
+            check(lib().fk_batch_set_graph(h, int(graph)))
 
     def graph_stats(self):
         """(graphs captured, graph launches) so far."""
@@ -351,3 +371,58 @@ class Batch:
 
     def counts(self):
         return batch_counts(self._h)
+
+
+class Pipeline:
+    """Windows swept one engine call each, `depth` (2) in flight: window i
+    is enqueued on its own batch (its own stream, queues and captured
+    graph) before window i - 1 is read back, so the device starts the next
+    window's sieve while the last one's rounds and classification (few
+    candidates, few busy CUs) finish, and never waits on the host.  The
+    launch loop of dickson-ladders/ladder_gpu.py:717-747 (two CUDA streams,
+    chain i + 1 enqueued before chain i is read back)."""
+
+    def __init__(self, device=0, depth=2):
+        self.device, self.depth = int(device), int(depth)
+        self._slots = []
+        self._graph = None
+
+    def _ready(self, graph):
+        if self._graph != graph:
+            self.close()
+            self._slots = [Batch(self.device, graph=graph)
+                           for _ in range(self.depth)]
+            self._graph = graph
+
+    def run(self, jobs, graph=1):
+        """jobs: (engine, lo, hi, (mode, cap, exact_from)) each; yields
+        (index, sorted records) in job order.  Windows still in flight when
+        the caller stops early are waited for and dropped."""
+        self._ready(graph)
+        pend = [None] * self.depth
+        try:
+            done = 0
+            for i, (eng, lo, hi, spec) in enumerate(jobs):
+                s = i % self.depth
+                if pend[s] is not None:
+                    recs = self._slots[s].results()[0]
+                    pend[s] = None
+                    yield i - self.depth, recs
+                    done += 1
+                self._slots[s].submit([eng], [0], [(lo, hi)], [spec])
+                pend[s] = i
+            for i in range(done, done + sum(p is not None for p in pend)):
+                s = i % self.depth
+                recs = self._slots[s].results()[0]
+                pend[s] = None
+                yield i, recs
+        finally:
+            for s, p in enumerate(pend):
+                if p is not None:
+                    self._slots[s].results()
+
+    def close(self):
+        for b in self._slots:
+            b.close()
+        self._slots = []
+        self._graph = None

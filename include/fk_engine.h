@@ -228,7 +228,10 @@ int fk_batch_timing(fk_batch* b, float* ms5, uint64_t* tiles,
  * sequence; a sequence that repeats on the next submission (same engines,
  * window shapes and grids; the windows themselves may differ) is captured
  * as a HIP graph then and replayed for every later submission while it
- * stays the same.  A sequence not seen just before runs eagerly.  Either
+ * stays the same.  A sequence not seen just before runs eagerly; with
+ * enable == 2 it is captured at once instead (for a batch whose
+ * submissions are known to repeat, such as one slot of a pipelined
+ * sweep of equal windows).  Either
  * way fk_batch_timing then reports only the whole ([4]).  The sequence
  * holds no memset nodes (counters are cleared by a kernel).  The engine's
  * own batch (fk_engine_batch, used by fk_engine_sweep) starts in graph

@@ -100,3 +100,55 @@ def test_engine_sweeps_replay_equal_eager():
         assert ms[1] > 0                           # stage events recorded
         total += len(got)
     assert total >= 10
+
+
+def test_pipelined_sweeps_equal_plain():
+    """Engine.sweep_many keeps two windows in flight on two batches: equal
+    windows (each slot captures its launch sequence at once and replays
+    it), windows of mixed lengths (eager until a sequence repeats), a
+    caller that stops early, and a mixed-engine Pipeline; every window's
+    records equal the plain engine's."""
+    eng = N.Engine(N.FAMILY_DICKSON, 10, 1 << 16, **F.tuning("dickson"))
+    ap = N.Engine(N.FAMILY_AP, 9, 1 << 16, **F.tuning("ap"))
+    span = 1 << 27
+    equal = [(10**12 + k * 3_000_000_019, 10**12 + k * 3_000_000_019 + span)
+             for k in range(7)]
+    mixed = [(lo, lo + (span >> (k % 3))) for k, (lo, _) in enumerate(equal)]
+    for name, wins in (("equal", equal), ("mixed", mixed)):
+        say("pipelined", name)
+        got = list(eng.sweep_many(wins, *SPEC))
+        assert len(got) == len(wins)
+        for (lo, hi), recs in zip(wins, got):
+            plain = eng.sweep(lo, hi, *SPEC)
+            assert N.record_values(recs) == N.record_values(plain), (name, lo)
+            assert recs["run"].tolist() == plain["run"].tolist()
+        if name == "equal":
+            # slot 0 swept windows 0, 2, 4, 6: captured at once, replayed
+            assert eng._pipe._slots[0].graph_stats() == (1, 4)
+    say("pipelined early stop")
+    it = eng.sweep_many(equal, *SPEC)
+    first = next(it)
+    it.close()                                    # one window still in flight
+    assert N.record_values(first) == N.record_values(eng.sweep(*equal[0], *SPEC))
+    again = list(eng.sweep_many(equal[:3], *SPEC))
+    assert [N.record_values(r) for r in again] == \
+        [N.record_values(eng.sweep(lo, hi, *SPEC)) for lo, hi in equal[:3]]
+    say("pipelined mixed engines")
+    pipe = N.Pipeline()
+    aps = (N.CLASSIFY_FULL, 12, 0)
+    jobs = []
+    for k in range(6):
+        if k % 2:
+            lo = 4 * 10**13 + k * 10**9
+            jobs.append((ap, lo, lo + (1 << 26), aps))
+        else:
+            jobs.append((eng, *equal[k], SPEC))
+    out = list(pipe.run(jobs))
+    assert [i for i, _ in out] == list(range(len(jobs)))
+    for (e, lo, hi, spec), (_, recs) in zip(jobs, out):
+        plain = e.sweep(lo, hi, *spec)
+        assert N.record_values(recs) == N.record_values(plain)
+        assert recs["run"].tolist() == plain["run"].tolist()
+    pipe.close()
+    eng.close()
+    ap.close()
