@@ -63,6 +63,9 @@ struct DevEngine {
     uint32_t sbm_words;       // kill bitmaps of every LDS prime with a
     const uint32_t* sbm_off;  // tail of kShortTailBits: [nlds] first words,
     const uint32_t* sbm;      // [sbm_words] bitmaps (short-line sieve)
+    const uint32_t* spk;      // [4 nlds] the short-line sieve's per-prime
+                              // constants q, m32, sbm_off, D^-1, read
+                              // with scalar loads
     uint32_t ntr;             // in-tile rounds over [nmark, nlds)
     const uint32_t* tr_end;   // [ntr] LDS-prime index where round r ends
     const uint32_t* kt;       // [round_end - round_first][kstride] round

@@ -343,7 +343,7 @@ static int build_engine(fk_engine* e)
                         "lower q_med, q_pat or the tile", need);
     }
     // the short-line sieve's LDS: per-prime arrays and every kill bitmap
-    if (32 * lq.size() + 4 * sbm.size() > 160 * 1024) e->short_words = 0;
+    if (short_lds(lq.size(), sbm.size()) > 160 * 1024) e->short_words = 0;
     e->lds_q = lq;
     e->pat_words = (uint32_t)pat.size();
     e->surv_lds = surv;
@@ -423,6 +423,16 @@ static int build_engine(fk_engine* e)
     if ((rc = eng_upload(e, bm_off, &h.bm_off))) return rc;
     if ((rc = eng_upload(e, bm, &h.bm))) return rc;
     if ((rc = eng_upload(e, sbm_off, &h.sbm_off))) return rc;
+    {
+        std::vector<uint32_t> spk(4 * lq.size());
+        for (size_t p = 0; p < lq.size(); p++) {
+            spk[4 * p] = lq[p];
+            spk[4 * p + 1] = lm32[p];
+            spk[4 * p + 2] = sbm_off[p];
+            spk[4 * p + 3] = ldinv[p];
+        }
+        if ((rc = eng_upload(e, spk, &h.spk))) return rc;
+    }
     if ((rc = eng_upload(e, sbm, &h.sbm))) return rc;
     if ((rc = eng_upload(e, tr_end, &h.tr_end))) return rc;
     {
@@ -992,7 +1002,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             a.out = b->d_q[0];
             a.counts = b->d_counts;
             a.flags = b->d_flags;
-            const size_t lds = 32 * (size_t)s_nlds_max + 4 * (size_t)s_bm_max;
+            const size_t lds = short_lds(s_nlds_max, s_bm_max);
             if (lds > 160 * 1024)
                 return fail(FK_E_ARG, "LDS need %zu bytes > 160 KiB (lower q_med"
                             " or q_pat)", lds);

@@ -51,6 +51,12 @@ __device__ __forceinline__ const FK_CONST T* uniform_view(const T* p)
     return (const FK_CONST T*)p;
 }
 
+// Four consecutive words through a uniform view (one scalar x4 load).
+__device__ __forceinline__ uint4 load4(const FK_CONST uint32_t* p)
+{
+    return make_uint4(p[0], p[1], p[2], p[3]);
+}
+
 // ------------------------------------------------------------------ prep
 // ----------------------------------------------------------------- sieve
 
@@ -452,23 +458,31 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
 // The chunk's C words ORed with every LDS prime's bitmap windows.  x is
 // (v0 + (B + off) D^-1) mod q: with D < 2^32, off mod q comes from a 32-bit
 // reduction, and every product below stays under 2^32 because the LDS
-// primes are below 2^16.
+// primes are below 2^16.  The prime's constants come from the engine's
+// table with scalar loads (the index is wave-uniform), B mod q from LDS
+// (one broadcast word): LDS bandwidth goes to the bitmap windows.
 template <int C, bool D32>
-__device__ __forceinline__ void short_words(const uint4* pinfo, const uint4* pk,
+__device__ __forceinline__ void short_words(const FK_CONST uint32_t* sp,
+                                            const FK_CONST uint64_t* m64,
+                                            const uint32_t* bqs,
                                             const uint32_t* bm, uint32_t nlds,
                                             uint64_t off, uint32_t v0,
                                             uint64_t (&acc)[C])
 {
     static_assert(C <= (int)kShortChunkMax, "chunk past the bitmap tail");
+    // A wave walks the primes as one chain of dependent latencies (scalar
+    // load, reductions, LDS reads); unrolled, several primes' chains are in
+    // flight at once.  The workgroups' LDS already caps a SIMD at a few
+    // waves, so the registers this takes are free.
+#pragma unroll 4
     for (uint32_t p = 0; p < nlds; p++) {
-        const uint4 pq = pinfo[p];
-        const uint4 kk = pk[p];
+        const uint4 pq = load4(sp + 4 * p);      // {q, m32, bitmap, D^-1}
         const uint32_t q = pq.x, m32 = pq.y;
         const uint32_t r = D32 ? red32((uint32_t)off, q, m32)
-                               : red64(off, q, ((uint64_t)kk.y << 32) | kk.x);
-        uint32_t t = kk.w + r;
+                               : red64(off, q, m64[p]);
+        uint32_t t = bqs[p] + r;
         if (t >= q) t -= q;
-        t = red32(t * kk.z, q, m32);
+        t = red32(t * pq.w, q, m32);
         const uint32_t x = red32(v0 + t, q, m32);
         const uint32_t* b = bm + pq.z + (x >> 5);
         const uint32_t o = x & 31u;
@@ -487,9 +501,8 @@ template <int C>
 __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
 {
     extern __shared__ uint64_t lds64[];
-    uint4* pinfo = reinterpret_cast<uint4*>(lds64);  // {q, m32, bitmap, 0}
-    uint4* pk = pinfo + a.nlds_max;       // {m64 lo, m64 hi, D^-1, B mod q}
-    uint32_t* bm = reinterpret_cast<uint32_t*>(pk + a.nlds_max);   // bm_max
+    uint32_t* bqs = reinterpret_cast<uint32_t*>(lds64);   // [nlds_max] B mod q
+    uint32_t* bm = bqs + ((a.nlds_max + 3) & ~3u);         // [bm_max] bitmaps
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     uint32_t cur_engine = 0xffffffffu, cur_piece = 0xffffffffu;
@@ -505,21 +518,20 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
             // the engine's tables when the engine changes, the piece's
             // B mod q always
             __syncthreads();
-            const bool tables = P.engine != cur_engine;
-            if (tables)
-                for (uint32_t w = tid; w < E.sbm_words; w += 256) bm[w] = E.sbm[w];
-            for (uint32_t p = tid; p < E.nlds; p += 256) {
-                const uint32_t bq = base_mod(P, E.lds_q[p], E.lds_m64[p]);
-                if (tables) {
-                    const uint32_t q = E.lds_q[p];
-                    const uint64_t m64 = E.lds_m64[p];
-                    pinfo[p] = make_uint4(q, E.lds_m32[p], E.sbm_off[p], 0u);
-                    pk[p] = make_uint4((uint32_t)m64, (uint32_t)(m64 >> 32),
-                                       E.lds_dinv[p], bq);
-                } else {
-                    pk[p].w = bq;
-                }
+            if (P.engine != cur_engine) {
+                // the bitmaps in 16-byte pieces, eight loads in flight per
+                // thread (one workgroup usually sieves one tile: this copy
+                // is its prologue)
+                const uint32_t n4 = E.sbm_words / 4;
+                const uint4* src = reinterpret_cast<const uint4*>(E.sbm);
+                uint4* dst = reinterpret_cast<uint4*>(bm);
+#pragma unroll 8
+                for (uint32_t w = tid; w < n4; w += 256) dst[w] = src[w];
+                for (uint32_t w = 4 * n4 + tid; w < E.sbm_words; w += 256)
+                    bm[w] = E.sbm[w];
             }
+            for (uint32_t p = tid; p < E.nlds; p += 256)
+                bqs[p] = base_mod(P, E.lds_q[p], E.lds_m64[p]);
             cur_engine = P.engine;
             cur_piece = pi;
             __syncthreads();
@@ -550,10 +562,12 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
             }
         }
         if (here) {
+            const FK_CONST uint32_t* sp = uniform_view(E.spk);
+            const FK_CONST uint64_t* m64 = uniform_view(E.lds_m64);
             if ((E.D >> 32) == 0)
-                short_words<C, true>(pinfo, pk, bm, E.nlds, off, v0, acc);
+                short_words<C, true>(sp, m64, bqs, bm, E.nlds, off, v0, acc);
             else
-                short_words<C, false>(pinfo, pk, bm, E.nlds, off, v0, acc);
+                short_words<C, false>(sp, m64, bqs, bm, E.nlds, off, v0, acc);
         }
         // records: a wave-wide prefix of the per-lane counts, one atomic
         uint32_t c = 0;
@@ -641,11 +655,6 @@ __device__ __forceinline__ bool seg_block(const FK_CONST uint2* segs,
         before += n;
     }
     return false;
-}
-
-__device__ __forceinline__ uint4 load4(const FK_CONST uint32_t* p)
-{
-    return make_uint4(p[0], p[1], p[2], p[3]);
 }
 
 __device__ __forceinline__ double load_f64(const FK_CONST uint32_t* p)
