@@ -117,6 +117,15 @@ class Campaign:
                           describe=lambda s: f"cursor at {s.get('next')}")
         state = None if fresh else C.load(self.path, self.key, warn=warn)
         if state is None:
+            # a ledger with records and no cursor covering them is a lost
+            # cursor, not a new campaign: only `fresh` may empty it
+            if (not fresh and os.path.exists(self.ledger) and
+                    os.path.getsize(self.ledger) > 0):
+                raise C.CheckpointCorrupt(
+                    f"{self.ledger} holds records but no cursor under this "
+                    f"key covers them ({self.path}); nothing was changed. "
+                    f"Restore the cursor, or pass fresh=True to discard "
+                    f"the records on purpose")
             state = {"key": self.key, "next": str(lo), "segments": 0,
                      "survivors": 0, "swept": "0", "census": {},
                      "best_run": -1, "best_value": None, "ledger_bytes": 0}

@@ -1,36 +1,45 @@
-"""Primality proofs for recorded discoveries: the N-1 certificates of
-Brillhart, Lehmer and Selfridge (Math. Comp. 29 (1975)), with the routes,
-bounds and JSON shape of the reference's certificate module
-(huntlib/certificate.py:1-421), so that a proof written by either side is
-checked by the other.
+"""Primality proofs for recorded discoveries: N-1 certificates after
+Brillhart, Lehmer and Selfridge (Math. Comp. 29 (1975)).
 
-The mathematics (the reference module states the same two theorems):
+A proof is a JSON object that `verify` re-checks from nothing.  Its keys
+("proof", "N", "F", "R", "factors", "witnesses", "subproofs", "s", "r",
+"bound", "note") and its three route names are those of the reference's
+certificate module (huntlib/certificate.py), so that either side checks
+the other's proofs; the code here is this package's own.
 
-* Pocklington in BLS form.  If N - 1 = F R with F completely factored and
-  every prime p | F has a witness a with a^(N-1) = 1 (mod N) and
+The mathematics:
+
+* Pocklington in BLS form.  Let N - 1 = F R with F completely factored.
+  If every prime p | F has a base a with a^(N-1) = 1 (mod N) and
   gcd(a^((N-1)/p) - 1, N) = 1, then every prime factor of N is 1 mod F.
-  With F > sqrt(N) that proves N prime (Theorem 1).
-* Theorem 5.  With only F > N^(1/3), F even and gcd(F, R) = 1, write
-  R = 2 F s + r with 1 <= r < 2F: N is prime iff s = 0 or r^2 - 8 s is not
-  a perfect square.  (A composite N is then (1 + aF)(1 + bF), which forces
-  r = a + b and s = ab/2, and r^2 - 8s = (a - b)^2.)
+  Two such factors exceed F each, so F > sqrt(N) proves N prime
+  ("bls75-thm1").
+* BLS75 Theorem 5.  Under the same conditions with only F^3 > N, F even
+  and gcd(F, R) = 1, write R = 2 F s + r with 0 <= r < 2F.  A composite N
+  is then (1 + aF)(1 + bF) with a + b = r and ab = 2s, so r^2 - 8s =
+  (a - b)^2 is a square; N is prime when s = 0 or r^2 - 8s is not a square
+  ("bls75-thm5").
+* Below PSI_13 = 3,317,044,064,679,887,385,961,981, the least strong
+  pseudoprime to all of the first 13 prime bases (Sorenson and Webster,
+  Math. Comp. 86 (2017)), Miller-Rabin to the bases 2, 3, ..., 41 decides
+  primality ("deterministic-mr").
 
-Routes, in the order `prove` tries them:
-  "deterministic-mr"   N below PSI_13 = 3317044064679887385961981, the
-                       smallest strong pseudoprime to all of the first 13
-                       prime bases (Sorenson and Webster, Math. Comp. 86
-                       (2017)): Miller-Rabin to the bases 2, 3, ..., 41 is a
-                       proof there
-  "bls75-thm1"         N-1 factored past sqrt(N)
-  "bls75-thm5"         N-1 factored past N^(1/3)
+Searching for a proof:
 
-Every prime put inside F is itself proved: below PSI_13 by the
-deterministic test, above it by a subproof carried in the certificate (at
-most PROOF_DEPTH levels).  Factoring is bounded: trial division to TDIV_TO,
-a bounded Brent rho, a bounded number of ECM curves (sympy), then it stops
-and says what is left.  Witnesses are the primes in ascending order below
-CERT_BASE_CAP, the smallest that works for each prime of F, so a proof is
-reproducible from N and F alone.
+1. N - 1 is split as far as a fixed effort allows: trial division by the
+   primes below TDIV_TO, then Pollard p-1 (stage one to PM1_BOUND), Brent
+   rho and a few ECM curves on each composite piece, all charged to one
+   budget of modular multiplications.  A prime found anywhere is divided
+   out of every remaining piece, so the factored part F and the rest R are
+   coprime by construction.
+2. F holds only proved primes.  A piece that is prime past PSI_13 (by
+   BPSW, which is only the reason to try) joins F with a proof of its own,
+   one level down, carried under "subproofs"; the largest such pieces are
+   tried first, and only while F is still too small.
+3. The route is Theorem 1 when F > sqrt(N), else Theorem 5 when F^3 > N.
+   The base for each prime of F is the smallest prime (below
+   CERT_BASE_CAP) meeting both conditions, the rule the reference's
+   recorded proofs follow, so a proof depends on N and F alone.
 """
 
 import math
@@ -39,64 +48,77 @@ import random
 PSI_13 = 3317044064679887385961981
 MR_PROOF_BASES = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41)
 
-CERT_BASE_CAP = 10_000        # witness bases: the primes below this
-TDIV_TO = 1 << 17             # trial division bound in factor_partial
-RHO_ITERS = 60_000            # Brent rho: iterations per try
-RHO_TRIES = 3
-ECM_CURVES = 24               # bounded ECM after rho
-SPLIT_BUDGET = 8              # splits attempted per factor_partial call
-PROOF_DEPTH = 3               # subproof levels for large prime factors
+CERT_BASE_CAP = 10_000        # candidate bases: the primes below this
+TDIV_TO = 1 << 17             # trial division bound
+PM1_BOUND = 20_000            # Pollard p-1 stage-one bound
+RHO_ITERS = 60_000            # Brent rho iterations per seed
+RHO_TRIES = 3                 # Brent rho seeds per piece
+ECM_CURVES = 24               # ECM curves per piece (sympy), after rho
+EFFORT = 2_000_000            # modular multiplications per factor_partial
+PROOF_DEPTH = 3               # nesting of subproofs
 
-_SMALL_PRIMES = None
+_PRIMES = []                  # cached primes below _TOP
+_TOP = 0
 
 
 def small_primes(limit=TDIV_TO):
-    """Primes below `limit` (sieve of Eratosthenes, cached at TDIV_TO)."""
-    global _SMALL_PRIMES
-    if limit <= TDIV_TO and _SMALL_PRIMES is not None:
-        return [p for p in _SMALL_PRIMES if p < limit]
-    n = max(limit, TDIV_TO)
-    sieve = bytearray([1]) * n
-    sieve[0:2] = b"\0\0"
-    for i in range(2, math.isqrt(n - 1) + 1):
-        if sieve[i]:
-            sieve[i * i::i] = bytearray(len(range(i * i, n, i)))
-    ps = [i for i in range(n) if sieve[i]]
-    if n == TDIV_TO:
-        _SMALL_PRIMES = ps
-    return [p for p in ps if p < limit]
+    """The primes below `limit` (an odd-only sieve, extended on demand)."""
+    global _PRIMES, _TOP
+    if limit > _TOP:
+        top = max(limit, TDIV_TO)
+        half = bytearray([1]) * (top // 2)      # index i stands for 2i + 1
+        half[0] = 0
+        for i in range(1, (math.isqrt(top) + 1) // 2 + 1):
+            if half[i]:
+                step = 2 * i + 1
+                start = (step * step) // 2
+                half[start::step] = bytes(len(range(start, len(half), step)))
+        _PRIMES = [2] + [2 * i + 1 for i, alive in enumerate(half) if alive]
+        _TOP = top
+    return _PRIMES[:_bisect(_PRIMES, limit)]
+
+
+def _bisect(xs, v):
+    lo, hi = 0, len(xs)
+    while lo < hi:
+        mid = (lo + hi) // 2
+        if xs[mid] < v:
+            lo = mid + 1
+        else:
+            hi = mid
+    return lo
 
 
 def strong_probable_prime(n, a):
-    """Miller-Rabin round to base a (n odd > 2): False proves n composite."""
-    d, s = n - 1, 0
-    while not d & 1:
-        d >>= 1
-        s += 1
-    x = pow(a % n, d, n)
-    if x in (0, 1, n - 1):
+    """One Miller-Rabin round to base a for odd n > 2: False proves n
+    composite."""
+    e = n - 1
+    t = (e & -e).bit_length() - 1      # n - 1 = 2^t e'
+    y = pow(a % n, e >> t, n)
+    if y in (0, 1, n - 1):
         return True
-    for _ in range(s - 1):
-        x = x * x % n
-        if x == n - 1:
+    for _ in range(t - 1):
+        y = y * y % n
+        if y == n - 1:
             return True
+        if y == 1:
+            return False
     return False
 
 
 def proved_prime(n):
-    """True iff n is a prime this module may take as proved without a
-    certificate: n < PSI_13 and Miller-Rabin to the first 13 prime bases."""
-    if n < 2 or n >= PSI_13:
+    """True iff n < PSI_13 and n is prime by the 13-prime-base test: a
+    prime this module may use without a certificate."""
+    if not 1 < n < PSI_13:
         return False
-    for p in MR_PROOF_BASES:
-        if n % p == 0:
-            return n == p
-    return all(strong_probable_prime(n, a) for a in MR_PROOF_BASES)
+    for b in MR_PROOF_BASES:
+        if n % b == 0:
+            return n == b
+    return all(strong_probable_prime(n, b) for b in MR_PROOF_BASES)
 
 
-def _probable_prime(n):
-    """BPSW-strength check (sympy) for deciding where to spend a subproof;
-    never a proof by itself."""
+def _bpsw(n):
+    """sympy's BPSW: the reason to attempt a subproof, never a proof."""
     from sympy import isprime
     return isprime(n)
 
@@ -104,39 +126,66 @@ def _probable_prime(n):
 # ------------------------------------------------------------- witnesses
 
 def witnesses(N, primes_of_F, base_cap=CERT_BASE_CAP):
-    """{p: a} meeting both BLS conditions for every p in primes_of_F, the
-    smallest prime a for each; None if some p has none below base_cap or if
-    a base fails Fermat (then N is composite)."""
-    out = {}
-    bases = small_primes(base_cap)
-    for p in primes_of_F:
-        e = (N - 1) // int(p)
-        for a in bases:
-            if pow(a, N - 1, N) != 1:
-                return None
-            if math.gcd(pow(a, e, N) - 1, N) == 1:
-                out[int(p)] = a
-                break
-        else:
+    """{p: a}, for each p in primes_of_F the smallest prime a < base_cap
+    with a^(N-1) = 1 and gcd(a^((N-1)/p) - 1, N) = 1 (mod N); None when
+    some p has no such base or a base fails Fermat's condition (N is then
+    composite)."""
+    todo = sorted({int(p) for p in primes_of_F})
+    found = {}
+    if not todo:
+        return found
+    for a in small_primes(base_cap):
+        if pow(a, N - 1, N) != 1:
             return None
-    return out
+        for p in todo:
+            if p not in found and math.gcd(pow(a, (N - 1) // p, N) - 1, N) == 1:
+                found[p] = a
+        if len(found) == len(todo):
+            return found
+    return None
 
 
 # ------------------------------------------------------------- factoring
 
-def brent_rho(m, iters=RHO_ITERS, tries=RHO_TRIES):
-    """A nontrivial factor of composite m, or None: Brent's cycle search on
-    x -> x^2 + c, the differences multiplied in batches of 128 between
-    gcds.  Seeded from m, so a verification reproduces exactly."""
+class _Effort:
+    """A budget of modular multiplications shared by every method."""
+
+    def __init__(self, total):
+        self.left = int(total)
+
+    def take(self, k):
+        self.left -= k
+        return self.left >= 0
+
+
+def _pollard_pm1(m, effort, bound=PM1_BOUND):
+    """A factor of m when some prime factor p has p - 1 smooth to `bound`
+    (stage one: the base raised to every prime power below it), or None."""
+    x = 2
+    for q in small_primes(bound):
+        qk = q
+        while qk * q < bound:
+            qk *= q
+        if not effort.take(qk.bit_length()):
+            return None
+        x = pow(x, qk, m)
+    g = math.gcd(x - 1, m)
+    return g if 1 < g < m else None
+
+
+def brent_rho(m, iters=RHO_ITERS, tries=RHO_TRIES, effort=None):
+    """A nontrivial factor of composite m, or None: Brent's cycle search
+    on y -> y^2 + c, differences multiplied in batches of 128 between
+    gcds.  The seeds come from m, so a verification reproduces exactly."""
     if m % 2 == 0:
         return 2
     rng = random.Random(m & 0xFFFFFFFF)
     for _ in range(tries):
         y, c = rng.randrange(1, m), rng.randrange(1, m)
-        g, r, q = 1, 1, 1
+        g, r, acc = 1, 1, 1
         x = ys = y
-        steps = 0
-        while g == 1 and steps < iters:
+        spent = 0
+        while g == 1 and spent < iters:
             x = y
             for _ in range(r):
                 y = (y * y + c) % m
@@ -145,12 +194,14 @@ def brent_rho(m, iters=RHO_ITERS, tries=RHO_TRIES):
                 ys = y
                 for _ in range(min(128, r - k)):
                     y = (y * y + c) % m
-                    q = q * abs(x - y) % m
-                g = math.gcd(q, m)
+                    acc = acc * abs(x - y) % m
+                g = math.gcd(acc, m)
                 k += 128
-            steps += r
+            spent += r
+            if effort is not None and not effort.take(2 * r):
+                return None
             r *= 2
-        if g == m:                      # the batch overshot: step singly
+        if g == m:                       # a batch overshot: step singly
             g = 1
             while g == 1:
                 ys = (ys * ys + c) % m
@@ -160,212 +211,280 @@ def brent_rho(m, iters=RHO_ITERS, tries=RHO_TRIES):
     return None
 
 
-def _split(v, ecm_curves):
-    f = brent_rho(v)
-    if f is None and ecm_curves:
-        try:
-            from sympy.ntheory import ecm
-            found = ecm(v, max_curve=ecm_curves)
-            f = min(found) if found else None
-        except Exception:               # ECM declining is not an error
-            f = None
-    return f if (f is not None and 1 < f < v) else None
+def _ecm(m, curves):
+    try:
+        from sympy.ntheory import ecm
+        found = ecm(m, max_curve=curves)
+    except Exception:                    # ECM declining is not an error
+        return None
+    f = min(found) if found else None
+    return f if f is not None and 1 < f < m else None
 
 
-def factor_partial(m, ecm_curves=ECM_CURVES, budget=SPLIT_BUDGET):
-    """(fac, R) with m = F R, fac the factorization of F into proved
-    primes, gcd(F, R) = 1 and R what the bounded effort could not split
-    (R = 1: m factored completely)."""
-    fac, rest = {}, int(m)
-    for q in small_primes():
+def _one_split(m, effort, ecm_curves):
+    for method in (lambda: _pollard_pm1(m, effort),
+                   lambda: brent_rho(m, effort=effort),
+                   lambda: _ecm(m, ecm_curves) if ecm_curves else None):
+        if effort.left <= 0:
+            return None
+        f = method()
+        if f:
+            return f
+    return None
+
+
+def _factor(m, ecm_curves=ECM_CURVES, effort=EFFORT):
+    """(fac, stuck): m = prod p^e (fac, proved primes) times the product of
+    the pieces in `stuck`, which the bounded effort could not split; every
+    piece is coprime to every prime of fac."""
+    fac, pieces = {}, []
+    rest = int(m)
+    for q in small_primes(TDIV_TO):
         if q * q > rest:
             break
         if rest % q == 0:
-            e = 0
+            rest //= q
+            e = 1
             while rest % q == 0:
                 rest //= q
                 e += 1
             fac[q] = e
-    R = 1
-    pending = [rest] if rest > 1 else []
-    while pending:
-        v = pending.pop()
+    if rest > 1:
+        pieces.append(rest)
+    budget = _Effort(effort)
+    stuck = []
+    while pieces:
+        v = pieces.pop()
+        for p in list(fac):              # keep every piece coprime to F
+            while v % p == 0:
+                v //= p
+                fac[p] += 1
         if v == 1:
             continue
         if proved_prime(v):
             fac[v] = fac.get(v, 0) + 1
             continue
-        if budget <= 0:
-            R *= v
-            continue
-        budget -= 1
-        f = _split(v, ecm_curves)
+        f = _one_split(v, budget, ecm_curves) if budget.left > 0 else None
         if f is None:
-            R *= v
-            continue
-        pending += [f, v // f]
-    # a prime that landed on both sides goes to F, keeping gcd(F, R) = 1
-    for p in list(fac):
-        while R % p == 0:
-            R //= p
-            fac[p] += 1
-    return fac, R
+            stuck.append(v)
+        else:
+            g = math.gcd(f, v // f)      # a repeated factor: peel the gcd
+            pieces += [g, f // g, v // f // g, g] if g > 1 else [f, v // f]
+    # a prime found after a piece got stuck still leaves that piece
+    out = []
+    for v in stuck:
+        for p in fac:
+            while v % p == 0:
+                v //= p
+                fac[p] += 1
+        if v > 1:
+            out.append(v)
+    return fac, out
+
+
+def factor_partial(m, ecm_curves=ECM_CURVES, effort=EFFORT):
+    """(fac, R): m = F R with fac the factorization of F into proved
+    primes, gcd(F, R) = 1, and R the part the bounded effort left
+    unsplit (R = 1 when m factored completely)."""
+    fac, stuck = _factor(m, ecm_curves, effort)
+    return fac, math.prod(stuck)
 
 
 # ------------------------------------------------------------- proofs
 
 def theorem5_split(F, R):
     """(s, r) with R = 2 F s + r, 0 <= r < 2F."""
-    s, r = divmod(int(R), 2 * int(F))
-    return s, r
+    return divmod(int(R), 2 * int(F))
 
 
 def is_square(x):
-    if x < 0:
-        return False
-    t = math.isqrt(x)
-    return t * t == x
+    return x >= 0 and math.isqrt(x) ** 2 == x
+
+
+def _product(fac):
+    out = 1
+    for p, e in fac.items():
+        out *= p ** e
+    return out
+
+
+def _route(N, F, R):
+    """(route, extra) for the factored part F of N - 1, or (None, why)."""
+    if F > math.isqrt(N):
+        return "bls75-thm1", {}
+    if F ** 3 > N:
+        s, r = theorem5_split(F, R)
+        if s and is_square(r * r - 8 * s):
+            return None, "composite"     # the theorem's other direction
+        return "bls75-thm5", {"s": s, "r": r}
+    return None, "F too small"
 
 
 def prove(N, fac=None, base_cap=CERT_BASE_CAP, depth=PROOF_DEPTH, **kw):
-    """A proof dict that `verify` re-checks from scratch, or None when no
-    proof is found within the bounds (not a claim that N is composite).
-    `fac`, a known partial factorization of N-1, is checked, not trusted."""
+    """A proof that `verify` re-checks from nothing, or None when none was
+    found within the bounds (which says nothing about N).  `fac`, a known
+    factorization of part of N - 1, is checked and then used as it is."""
     N = int(N)
-    if N < 2:
-        return None
     if N < PSI_13:
         if not proved_prime(N):
             return None
         return {"proof": "deterministic-mr", "N": N, "bound": PSI_13,
-                "note": "Miller-Rabin to the 13 prime bases 2..41 is "
-                        "deterministic below the bound"}
+                "note": "Miller-Rabin to the 13 prime bases 2..41 decides "
+                        "primality below the bound"}
     if N % 2 == 0:
         return None
     if fac is None:
-        fac, R = factor_partial(N - 1, **kw)
+        fac, stuck = _factor(N - 1, **kw)
     else:
         fac = {int(p): int(e) for p, e in fac.items()}
-        F = 1
-        for p, e in fac.items():
-            F *= p ** e
-        if F < 1 or (N - 1) % F:
+        F0 = _product(fac)
+        if F0 < 1 or (N - 1) % F0:
             return None
-        R = (N - 1) // F
+        stuck = [(N - 1) // F0] if N - 1 > F0 else []
+    R = math.prod(stuck)
     subs = {}
-    # a large prime cofactor: prove it one level down and admit it
-    if R > 1 and depth > 0 and R >= PSI_13 and _probable_prime(R):
-        sub = prove(R, base_cap=base_cap, depth=depth - 1, **kw)
-        if sub is not None:
-            fac[R] = fac.get(R, 0) + 1
-            subs[str(R)] = sub
-            R = 1
-    F = 1
-    for p, e in fac.items():
-        if not (proved_prime(p) or str(p) in subs):
+    # unsplit pieces that are prime past PSI_13 join F with a subproof
+    # each, the largest first, while F is short of Theorem 5
+    if depth > 0:
+        for v in sorted(stuck, reverse=True):
+            if _product(fac) ** 3 > N:
+                break
+            if v < PSI_13 or v in fac or not _bpsw(v):
+                continue
+            sub = prove(v, base_cap=base_cap, depth=depth - 1, **kw)
+            if sub is None:
+                continue
+            subs[str(v)] = sub
+            fac[v] = 1
+            R //= v
+    for p in fac:
+        if not proved_prime(p) and str(p) not in subs:
             return None
-        F *= p ** e
+    F = _product(fac)
     if F % 2 or math.gcd(F, R) != 1:
         return None
-    w = witnesses(N, sorted(fac), base_cap)
+    route, extra = _route(N, F, R)
+    if route is None:
+        return None
+    w = witnesses(N, fac, base_cap)
     if w is None:
         return None
-    out = {"N": N, "F": F, "R": R,
+    out = {"proof": route, "N": N, "F": F, "R": R,
            "factors": {str(p): e for p, e in sorted(fac.items())},
            "witnesses": {str(p): a for p, a in sorted(w.items())}}
+    out.update(extra)
     if subs:
         out["subproofs"] = subs
-    if F > math.isqrt(N):
-        out["proof"] = "bls75-thm1"
-        return out
-    if F ** 3 > N:
-        s, r = theorem5_split(F, R)
-        if s != 0 and is_square(r * r - 8 * s):
-            return None                 # Theorem 5 is an iff: N composite
-        out["proof"] = "bls75-thm5"
-        out["s"], out["r"] = s, r
-        return out
-    return None
+    return out
+
+
+def _check_mr(proof, N):
+    if N >= PSI_13:
+        return False, (f"{N} is at or past {PSI_13}, where the 13-base test "
+                       f"proves nothing")
+    if not proved_prime(N):
+        return False, f"{N} fails the 13-base test"
+    return True, "deterministic: 13 prime bases, below PSI_13"
+
+
+def _check_nminus1(proof, N):
+    """The part both theorems share: F, its primes and their bases."""
+    raw = proof.get("factors") or {}
+    try:
+        fac = {int(p): int(e) for p, e in raw.items()}
+    except (TypeError, ValueError):
+        return None, "unreadable factors"
+    if not fac:
+        return None, "the proof factors nothing of N - 1"
+    if any(e < 1 for e in fac.values()):
+        return None, "an exponent below 1"
+    F = _product(fac)
+    if (N - 1) % F:
+        return None, f"F = {F} is not a divisor of N - 1"
+    R = (N - 1) // F
+    if F % 2:
+        return None, "F is odd"
+    if math.gcd(F, R) != 1:
+        return None, "F and R share a factor"
+    bases = proof.get("witnesses") or {}
+    for p in sorted(fac):
+        a = bases.get(str(p))
+        if a is None:
+            return None, f"prime {p} of F has no base"
+        a = int(a)
+        if pow(a, N - 1, N) != 1:
+            return None, f"base {a} (for {p}): a^(N-1) is not 1 mod N"
+        if math.gcd(pow(a, (N - 1) // p, N) - 1, N) != 1:
+            return None, f"base {a} (for {p}): a^((N-1)/p) - 1 shares a factor with N"
+    return (fac, F, R), None
 
 
 def verify(proof):
-    """(ok, message): every claim in the dict re-derived, nothing trusted
-    (the factorization, the primality of each factor, F | N-1, the
-    witnesses, the theorem's side conditions)."""
-    if not isinstance(proof, dict):
-        return False, "not a proof object"
-    kind = proof.get("proof")
-    N = int(proof.get("N", 0))
-    if N < 2:
-        return False, "no N"
-    if kind == "deterministic-mr":
-        if N >= PSI_13:
-            return False, (f"N = {N} is not below {PSI_13}: the deterministic "
-                           f"test is no proof there")
-        if not proved_prime(N):
-            return False, "N fails Miller-Rabin to the 13 prime bases"
-        return True, "Miller-Rabin to the 13 prime bases below 3.317e24"
-    if kind not in ("bls75-thm1", "bls75-thm5"):
-        return False, f"unknown proof kind {kind!r}"
-    fac = {int(p): int(e) for p, e in (proof.get("factors") or {}).items()}
-    if not fac:
-        return False, "no factorization of N-1"
-    subs = proof.get("subproofs") or {}
-    F = 1
-    for p, e in sorted(fac.items()):
-        if e < 1:
-            return False, f"bad exponent {e} for {p}"
-        if not proved_prime(p):
-            sub = subs.get(str(p))
-            if sub is None:
-                return False, (f"factor {p} is past the deterministic bound "
-                               f"and carries no subproof")
-            if int(sub.get("N", 0)) != p:
-                return False, f"subproof for {p} is about {sub.get('N')}"
-            ok, why = verify(sub)
+    """(ok, message): the proof re-derived with nothing taken on trust --
+    F divides N - 1, every prime of F is proved (below PSI_13 directly,
+    above it by its own subproof, checked the same way), every base meets
+    both conditions, and the route's bound and side condition hold."""
+    todo = [(proof, None)]               # (proof, the prime it must be about)
+    top = None
+    while todo:
+        pf, about = todo.pop()
+        if not isinstance(pf, dict):
+            return False, "a proof must be a JSON object"
+        try:
+            N = int(pf.get("N", 0))
+        except (TypeError, ValueError):
+            return False, "unreadable N"
+        if about is not None and N != about:
+            return False, f"the subproof for {about} proves {N}"
+        if N < 2:
+            return False, "N below 2"
+        kind = pf.get("proof")
+        if kind == "deterministic-mr":
+            ok, msg = _check_mr(pf, N)
             if not ok:
-                return False, f"subproof for {p} fails: {why}"
-        F *= p ** e
-    if (N - 1) % F:
-        return False, "F does not divide N-1"
-    R = (N - 1) // F
-    if F % 2:
-        return False, "F must be even"
-    if math.gcd(F, R) != 1:
-        return False, f"gcd(F, R) = {math.gcd(F, R)} != 1"
-    wit = proof.get("witnesses") or {}
-    for p in sorted(fac):
-        a = wit.get(str(p))
-        if a is None:
-            return False, f"no witness for {p}"
-        a = int(a)
-        if pow(a, N - 1, N) != 1:
-            return False, f"witness {a} fails Fermat at p = {p}"
-        if math.gcd(pow(a, (N - 1) // p, N) - 1, N) != 1:
-            return False, f"witness {a} fails the gcd condition at p = {p}"
-    if kind == "bls75-thm1":
-        if F <= math.isqrt(N):
-            return False, f"F = {F} does not exceed sqrt(N)"
-        return True, "BLS75 Theorem 1: F > sqrt(N)"
-    if F ** 3 <= N:
-        return False, f"F = {F} does not exceed N^(1/3)"
-    s, r = theorem5_split(F, R)
-    if s != int(proof.get("s", -1)) or r != int(proof.get("r", -1)):
-        return False, (f"stated (s, r) = ({proof.get('s')}, {proof.get('r')}) "
-                       f"but R = 2Fs + r gives ({s}, {r})")
-    if s != 0 and is_square(r * r - 8 * s):
-        return False, f"r^2 - 8s = {r * r - 8 * s} is a square: N is composite"
-    return True, "BLS75 Theorem 5: F > N^(1/3) and the square test passes"
+                return False, msg if about is None else f"subproof {about}: {msg}"
+            top = msg if top is None else top
+            continue
+        if kind not in ("bls75-thm1", "bls75-thm5"):
+            return False, f"no route called {kind!r}"
+        got, why = _check_nminus1(pf, N)
+        if got is None:
+            return False, why if about is None else f"subproof {about}: {why}"
+        fac, F, R = got
+        subs = pf.get("subproofs") or {}
+        for p in fac:
+            if not proved_prime(p):
+                if str(p) not in subs:
+                    return False, (f"prime {p} of F is past PSI_13 and has "
+                                   f"no subproof")
+                todo.append((subs[str(p)], p))
+        if kind == "bls75-thm1":
+            if F <= math.isqrt(N):
+                return False, f"Theorem 1 needs F > sqrt(N); F = {F}"
+            msg = "Theorem 1 (BLS75): F exceeds sqrt(N)"
+        else:
+            if F ** 3 <= N:
+                return False, f"Theorem 5 needs F^3 > N; F = {F}"
+            s, r = theorem5_split(F, R)
+            if (s, r) != (int(pf.get("s", -1)), int(pf.get("r", -1))):
+                return False, (f"R = 2Fs + r gives (s, r) = ({s}, {r}), the "
+                               f"proof says ({pf.get('s')}, {pf.get('r')})")
+            if s and is_square(r * r - 8 * s):
+                return False, (f"r^2 - 8s = {r * r - 8 * s} is a square: "
+                               f"Theorem 5 makes N composite")
+            msg = "Theorem 5 (BLS75): F^3 exceeds N and r^2 - 8s is no square"
+        top = msg if top is None else top
+    return True, top
 
 
 def theorem1(N, fac, base_cap=CERT_BASE_CAP):
-    """Witnesses for N when N-1 = prod p^e (fac) is completely factored
-    and every p is proved (a Dickson value m k^2 + 1), or None."""
-    return witnesses(N, sorted(int(p) for p in fac), base_cap)
+    """Bases for N when N - 1 = prod p^e (fac) is completely factored into
+    proved primes (a Dickson value m k^2 + 1), or None."""
+    return witnesses(N, fac, base_cap)
 
 
 def theorem1_verify(N, fac, wit):
-    """`verify` on the Theorem 1 certificate (N, fac, wit)."""
+    """`verify` of the Theorem 1 proof made of (N, fac, wit)."""
     return verify({"proof": "bls75-thm1", "N": int(N),
                    "factors": {str(int(p)): int(e) for p, e in fac.items()},
                    "witnesses": {str(int(p)): int(a) for p, a in wit.items()}})

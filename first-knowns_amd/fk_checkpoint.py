@@ -1,118 +1,162 @@
-"""Crash-durable, config-keyed JSON cursors for campaign segment loops.
+"""Crash-durable, config-keyed JSON cursor files for campaign segment loops.
 
-The rules are huntlib/checkpoint.py:1-27's, kept because a campaign's
-cursor files must mean the same thing on either engine:
+A cursor file is one JSON object with a `key` (the engine parameters its
+position is counted in) beside the position itself.  The file format and
+its rules are huntlib/checkpoint.py:1-27's, so that either engine reads
+the other's cursors: a write is all or nothing, the previous file stays
+behind as `<path>.bak`, an unreadable file is an error rather than an
+absent one, and a cursor under another key is refused, not reinterpreted.
 
-* a file is written to a temporary name, flushed and fsynced, and only
-  then renamed over the old one, so a crash leaves either the old or the
-  new file, never a torn one (a rename can reach the disk before the data,
-  which is why the fsync comes first);
-* the file that was on disk before the rename is kept as `<path>.bak`, and
-  `load` falls back to it when the main file cannot be read;
-* a file that exists but cannot be read is an error (CheckpointCorrupt),
-  never "no checkpoint": for a live frontier those need opposite answers;
-* every cursor carries a `key` naming the engine parameters its position
-  is counted in; `load` ignores a file whose key differs, and
-  `refuse_mismatch` refuses to start over one unless told `fresh`.
+How this module meets them:
+
+* `save_json` writes the new text to `<path>.tmp` and fsyncs it, makes the
+  current file the backup by a hard link (or a durable copy where the file
+  system has no links), and renames the temporary file over the main one,
+  then fsyncs the directory.  The main file therefore exists at every
+  instant from its first save on: a crash at any point leaves the old or
+  the new state under `<path>`, and never a missing cursor beside a full
+  records ledger.
+* `read_state` looks at the main file and then the backup and returns the
+  first that parses, with where it came from.  Files that exist but none
+  of which parse raise CheckpointCorrupt; no file at all is "no cursor".
 """
 
 import json
 import os
+import shutil
 
 
 class CheckpointCorrupt(RuntimeError):
-    """A cursor file exists and neither it nor its backup can be read."""
+    """A cursor file exists and no readable state stands behind it."""
 
 
 class CursorRefused(RuntimeError):
     """A cursor exists that this configuration must not reinterpret."""
 
 
-def save_json(path, obj):
-    """Write obj durably: temporary file, fsync, keep the old file as
-    .bak, then rename into place (huntlib/checkpoint.py:40-66)."""
-    path = os.fspath(path)
-    tmp = path + ".tmp"
-    with open(tmp, "w", newline="\n") as f:
-        json.dump(obj, f, indent=1)
+def _sync_dir(path):
+    """Make a rename in `path`'s directory durable (best effort where the
+    platform cannot open directories)."""
+    try:
+        fd = os.open(os.path.dirname(os.path.abspath(path)) or ".",
+                     os.O_RDONLY)
+    except OSError:
+        return
+    try:
+        os.fsync(fd)
+    except OSError:
+        pass
+    finally:
+        os.close(fd)
+
+
+def _write_synced(path, text):
+    with open(path, "w", newline="\n") as f:
+        f.write(text)
         f.flush()
         os.fsync(f.fileno())
+
+
+def _refresh_backup(path):
+    """<path>.bak becomes the current content of <path>, with <path>
+    itself left in place."""
+    bak = path + ".bak"
+    staged = bak + ".tmp"
+    try:
+        if os.path.exists(staged):
+            os.unlink(staged)
+        os.link(path, staged)
+    except OSError:
+        # no hard links here: a synced copy does the same job
+        shutil.copyfile(path, staged)
+        with open(staged, "r+b") as f:
+            os.fsync(f.fileno())
+    os.replace(staged, bak)
+
+
+def save_json(path, obj):
+    """Durably replace the JSON at `path` by `obj`, the old content kept as
+    `<path>.bak`."""
+    path = os.fspath(path)
+    fresh = path + ".tmp"
+    _write_synced(fresh, json.dumps(obj, indent=1) + "\n")
     if os.path.exists(path):
         try:
-            os.replace(path, path + ".bak")
+            _refresh_backup(path)
         except OSError:
-            pass                      # the backup is best effort
-    os.replace(tmp, path)
+            pass            # a missing backup costs a fallback, not the state
+    os.replace(fresh, path)
+    _sync_dir(path)
 
 
-def _read(path):
+def _parse(path):
     with open(path) as f:
         obj = json.load(f)
     if not isinstance(obj, dict):
-        raise ValueError("cursor file does not hold a JSON object")
+        raise ValueError("not a JSON object")
     return obj
 
 
-def load(path, expect_key, warn=None):
-    """The cursor dict, or None when there is none or its key differs
-    (huntlib/checkpoint.py:80-110).  Falls back to `.bak` when the file
-    cannot be read, and raises CheckpointCorrupt when neither can."""
+def read_state(path):
+    """(state, source, problems): the state from `path`, else from its
+    backup (source names the file read), or (None, None, []) when neither
+    file exists.  problems lists the files that exist but did not parse.
+    Raises CheckpointCorrupt when files exist and none of them parses."""
     path = os.fspath(path)
-    if not os.path.exists(path):
-        return None
-    try:
-        state = _read(path)
-    except Exception as err:
-        bak = path + ".bak"
-        if not os.path.exists(bak):
-            raise CheckpointCorrupt(
-                f"{path} cannot be read ({err}) and there is no {bak}; "
-                f"restart deliberately with fresh=True") from err
+    problems = []
+    for cand in (path, path + ".bak"):
+        if not os.path.exists(cand):
+            continue
         try:
-            state = _read(bak)
-        except Exception as err2:
-            raise CheckpointCorrupt(
-                f"{path} cannot be read ({err}), nor {bak} ({err2})") from err2
-        if warn:
-            warn(f"{path} cannot be read ({err}); resuming from {bak}, "
-                 f"one segment behind")
+            return _parse(cand), cand, problems
+        except (OSError, ValueError) as err:
+            problems.append(f"{cand}: {err}")
+    if problems:
+        raise CheckpointCorrupt(
+            "no readable cursor: " + "; ".join(problems) +
+            ".  Nothing was changed; start again with fresh=True only to "
+            "discard this frontier on purpose")
+    return None, None, []
+
+
+def load(path, expect_key, warn=None):
+    """The cursor under `expect_key`, or None when there is none or it was
+    counted under another key.  A state recovered from the backup is one
+    segment behind the lost one; `warn` is told so."""
+    path = os.fspath(path)
+    state, source, problems = read_state(path)
+    if state is None:
+        return None
+    if source != path and warn:
+        lost = "; ".join(problems) if problems else f"{path} is missing"
+        warn(f"resuming from {source} ({lost}): one segment is redone")
     if state.get("key") != expect_key:
         if warn:
-            warn(f"cursor key {state.get('key')!r} differs; ignoring it")
+            warn(f"cursor in {source} is keyed {state.get('key')!r}, not "
+                 f"{expect_key!r}; it is not used")
         return None
     return state
 
 
 def refuse_mismatch(path, expect_key, fresh=False, describe=None):
-    """Raise unless starting is safe (huntlib/checkpoint.py:117-165): an
-    existing cursor under another key is refused rather than ignored, since
-    ignoring it would restart the sweep at its floor and reinterpreting it
-    would misplace the frontier."""
+    """Return when starting under `expect_key` is safe: there is no cursor,
+    or its key is this one, or `fresh` discards it on purpose.  Raise
+    CursorRefused for a cursor under another key (resuming it would
+    misplace its frontier, restarting would abandon it) and
+    CheckpointCorrupt when no readable state exists."""
+    if fresh:
+        return
     path = os.fspath(path)
-    if fresh or not os.path.exists(path):
+    state, source, _ = read_state(path)
+    if state is None or state.get("key") == expect_key:
         return
-    try:
-        state = _read(path)
-    except Exception as err:
-        bak = path + ".bak"
-        if os.path.exists(bak):
-            try:
-                _read(bak)
-                return                # load() recovers it and says so
-            except Exception:
-                pass
-        raise CheckpointCorrupt(
-            f"{path} exists but cannot be read ({err}) and no usable backup "
-            f"stands behind it; restart deliberately with fresh=True") from err
-    if state.get("key") == expect_key:
-        return
-    extra = ""
-    if describe:
+    where = ""
+    if describe is not None:
         try:
-            extra = f"; stored: {describe(state)}"
-        except Exception:
-            extra = ""
+            where = f" ({describe(state)})"
+        except Exception:           # describing is a courtesy only
+            where = ""
     raise CursorRefused(
-        f"a cursor exists under another configuration: stored key "
-        f"{state.get('key')!r}, wanted {expect_key!r}{extra}.  Starting would "
-        f"abandon that frontier; pass fresh=True to discard it deliberately.")
+        f"{source} holds a cursor for {state.get('key')!r}{where}; this "
+        f"run is {expect_key!r}.  Use another cursor path, or fresh=True to "
+        f"give that frontier up")

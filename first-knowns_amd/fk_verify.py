@@ -212,45 +212,65 @@ def dickson_verify(k, claimed, n_filter, certify=True, witness=True):
     return ev, "ok"
 
 
+def _leading_primes(values, test):
+    """How many of `values`, from the first, pass `test`."""
+    return next((i for i, v in enumerate(values) if not test(v)), len(values))
+
+
+def _prove_all(values):
+    """(proofs, None) with a verified proof for every value, or (None, the
+    first value without one)."""
+    proofs = []
+    for v in values:
+        pr = C.prove(v)
+        if pr is None or not C.verify(pr)[0]:
+            return None, v
+        proofs.append(pr)
+    return proofs, None
+
+
 def ap_verify(p, n, claimed, q2=8192, certify=True, witness=True):
-    """The evidence dict of primorial-ap/launch.py:380-446.  A claim of n
-    means the chain reaches at least n (the walk goes VERIFY_OVERSHOOT past
-    it to find the true end); a claim below n is exact.  `agree` is False
-    when a leg disagrees or a proof is missing."""
+    """The legs behind an AP record of depth `claimed` at p, as an evidence
+    dict with the fields of primorial-ap/launch.py:380-446.
+
+    The classification walks with cap n, so a claim of n only says that
+    the first n values are prime (the chain may run on: a(17)'s runs 19
+    deep), while a claim below n is the exact place the chain breaks.  The
+    legs walk the chain that far, and VERIFY_OVERSHOOT values further for
+    a claim of n, so the true depth and the value that ends it are on
+    record.  `agree` is False when a leg disagrees or a proof is missing."""
     p, n, claimed = int(p), int(n), int(claimed)
     d = difference(n)
-    exact = claimed < n
-    cap = claimed + 1 if exact else claimed + 1 + VERIFY_OVERSHOOT
-    r_own = ap_depth(p, n, cap)
-    r_sym = ap_depth(p, n, cap, prime=_sym_prime)
-    depth = alt_q2(q2)
-    alt = _survives([p + j * d for j in range(n)], depth)
-    depth_ok = (r_own == claimed) if exact else (r_own >= claimed)
-    out = {"p": p, "n": n, "depth": r_own, "difference": d,
-           "values": [p + j * d for j in range(min(claimed, n))],
-           "legs": {"engine_mr": r_own, "sympy_bpsw": r_sym,
-                    "alt_sieve_survives": alt, "alt_sieve_depth": depth},
-           "agree": bool(r_own == r_sym and depth_ok and alt)}
-    if not out["agree"]:
-        return out
-    if r_own == cap:
-        out["depth_is_lower_bound"] = True
-    if certify and claimed >= n:
-        proofs = []
-        for v in out["values"]:
-            pr = C.prove(v)
-            if pr is None or not C.verify(pr)[0]:
-                out["agree"] = False
-                out["error"] = f"could not prove value {v} prime"
-                return out
-            proofs.append(pr)
-        out["proofs"] = proofs
-        out["proof_kinds"] = sorted({pr["proof"] for pr in proofs})
-    if witness and r_own < cap:
-        breaker = p + r_own * d
-        out["chain_breaker"] = breaker
-        out["chain_breaker_factor"] = factor_witness(breaker)
-    return out
+    at_cap = claimed >= n
+    horizon = claimed + 1 + (VERIFY_OVERSHOOT if at_cap else 0)
+    chain = [p + j * d for j in range(horizon)]
+    own = _leading_primes(chain, is_prime7)
+    bpsw = _leading_primes(chain, _sym_prime)
+    alt_depth = alt_q2(q2)
+    sieved = _survives([p + j * d for j in range(n)], alt_depth)
+    holds = own >= claimed if at_cap else own == claimed
+    ev = {"p": p, "n": n, "depth": own, "difference": d,
+          "values": chain[:min(claimed, n)],
+          "legs": {"engine_mr": own, "sympy_bpsw": bpsw,
+                   "alt_sieve_survives": sieved, "alt_sieve_depth": alt_depth},
+          "agree": own == bpsw and holds and sieved}
+    if not ev["agree"]:
+        return ev
+    if own == horizon:
+        ev["depth_is_lower_bound"] = True      # no composite within reach
+    if certify and at_cap:
+        proofs, missing = _prove_all(ev["values"])
+        if proofs is None:
+            ev["agree"] = False
+            ev["error"] = f"could not prove value {missing} prime"
+            return ev
+        ev["proofs"] = proofs
+        ev["proof_kinds"] = sorted({pr["proof"] for pr in proofs})
+    if witness and own < horizon:
+        # the first composite of the chain, which both legs stopped at
+        ev["chain_breaker"] = chain[own]
+        ev["chain_breaker_factor"] = factor_witness(chain[own])
+    return ev
 
 
 def verify_record(family, n, value, run, q2, n_filter=None, certify=True,

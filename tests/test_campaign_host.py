@@ -192,3 +192,65 @@ def test_segments_alternate_batches(tmp_path):
     campaign(tmp_path, batches=b).run()
     assert log[0] == (LO, LO + SEG) and log[-1][1] == HI
     assert all(a[1] == b_[0] for a, b_ in zip(log, log[1:]))
+
+
+def test_crash_inside_save_keeps_the_cursor(tmp_path, monkeypatch):
+    """A crash after the backup is refreshed and before the new state is
+    renamed into place leaves the old cursor readable under its own name:
+    the resume redoes one segment and loses no record."""
+    c = campaign(tmp_path)
+    c.run(max_segments=2)
+    real = os.replace
+
+    def dies_on_main_rename(src, dst):
+        if str(src).endswith("cur.json.tmp"):
+            raise OSError("power lost")
+        real(src, dst)
+
+    monkeypatch.setattr(os, "replace", dies_on_main_rename)
+    with pytest.raises(OSError):
+        campaign(tmp_path).run()
+    monkeypatch.setattr(os, "replace", real)
+    assert (tmp_path / "cur.json").exists()
+    c3 = campaign(tmp_path)
+    assert c3.cursor == LO + 2 * SEG
+    c3.run()
+    assert c3.records() == whole()
+
+
+def test_missing_cursor_resumes_from_backup(tmp_path):
+    c = campaign(tmp_path)
+    c.run(max_segments=3)
+    os.remove(c.path)
+    msgs = []
+    c2 = campaign(tmp_path, warn=msgs.append)
+    assert c2.cursor == LO + 2 * SEG      # the backup: one segment behind
+    assert msgs and "redone" in msgs[0]
+    c2.run()
+    assert c2.records() == whole()
+
+
+def test_records_without_cursor_are_not_discarded(tmp_path):
+    c = campaign(tmp_path)
+    c.run(max_segments=2)
+    size = os.path.getsize(c.ledger)
+    assert size > 0
+    os.remove(c.path)
+    os.remove(c.path + ".bak")
+    with pytest.raises(C.CheckpointCorrupt):
+        campaign(tmp_path)
+    assert os.path.getsize(c.ledger) == size
+    c2 = campaign(tmp_path, fresh=True)
+    assert c2.cursor == LO and c2.records() == []
+
+
+def test_backup_is_a_separate_file(tmp_path):
+    """Rewriting the main file in place never changes the backup."""
+    p = tmp_path / "c.json"
+    C.save_json(p, {"key": "k", "next": 1})
+    C.save_json(p, {"key": "k", "next": 2})
+    C.save_json(p, {"key": "k", "next": 3})
+    p.write_text("{")
+    assert C.load(p, "k")["next"] == 2
+    state, source, problems = C.read_state(p)
+    assert source.endswith(".bak") and len(problems) == 1

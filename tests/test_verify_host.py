@@ -114,7 +114,8 @@ def test_theorem1_route():
     assert n1 > C.PSI_13
     p1 = C.prove(n1, fac={2: 1, 3: 2, 5: 18, 7: 14})
     assert p1["proof"] == "bls75-thm1"
-    assert C.verify(p1) == (True, "BLS75 Theorem 1: F > sqrt(N)")
+    ok, msg = C.verify(p1)
+    assert ok and "Theorem 1" in msg
 
 
 def test_theorem5_route_and_composite_control():
@@ -157,3 +158,24 @@ def test_bounded_factoring():
     assert V.factor_witness(101 * 103) == 101
     assert V.factor_witness(1000003 * 1000033) == 1000003
     assert V.factor_witness(1000003) is None              # a prime
+
+
+@pytest.mark.parametrize("rec", EV["dickson_full"], ids=lambda r: f"run{r['run']}")
+def test_reference_certificates_verify_here(rec):
+    """The Theorem 1 certificates the reference recorded for each value of
+    its Dickson discoveries, put in the proof JSON shape with the
+    reference's own bases, pass this module's verify (a proof written on
+    the other side is checked here), and a wrong base does not."""
+    k = int(rec["k"])
+    for m_s, bases in rec["certificates_bls75"].items():
+        m = int(m_s)
+        N = m * k * k + 1
+        fac = V.dickson_factor_n_minus_1(m, k)
+        proof = {"proof": "bls75-thm1", "N": N,
+                 "factors": {str(p): e for p, e in sorted(fac.items())},
+                 "witnesses": dict(bases)}
+        ok, msg = C.verify(json.loads(json.dumps(proof)))
+        assert ok, (m, msg)
+        p2 = min(int(p) for p in bases)
+        bad = dict(proof, witnesses=dict(bases, **{str(p2): N - 1}))
+        assert not C.verify(bad)[0]
