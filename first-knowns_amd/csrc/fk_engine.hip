@@ -147,6 +147,7 @@ struct fk_engine {
     uint32_t q_pat = 0, q_med = 0, threads = 256, wpt = 8;
     uint32_t round_lane_prime = 8192, growth = 2;
     uint32_t short_words = 1024;     // 0: no short-line sieve
+    uint32_t segment_cap = 0;        // 0: queue segment from the expectation
     uint32_t q_mark = 0;             // [q_pat, q_mark) stride-marked
     uint32_t q_bits = 0;             // [q_mark, q_bits) ORed in from bitmaps
     uint32_t npat = 0;
@@ -189,6 +190,12 @@ static int eng_upload(fk_engine* e, const std::vector<T>& v, const T** out)
 // reaches this fraction (fewer rounds: fewer barriers, more tests on
 // positions an earlier cut would have dropped).
 static constexpr double kTileRoundSurvival = 0.25;
+
+// Largest survivor queue segment of one engine in one batch, in records
+// (24 B each, two queues): the sieve's per-engine counters are 32-bit, and
+// a window expected to need more is refused (split it) rather than risk a
+// counter that wraps.
+static constexpr uint32_t kSegmentMax = 1u << 30;
 
 static int build_engine(fk_engine* e)
 {
@@ -262,6 +269,9 @@ static int build_engine(fk_engine* e)
     if (d.short_words > 4096)
         return fail(FK_E_ARG, "short_words = %u > 4096", d.short_words);
     e->short_words = d.short_words ? d.short_words : 1024;
+    if (d.segment_cap > kSegmentMax)
+        return fail(FK_E_ARG, "segment_cap = %u > 2^30", d.segment_cap);
+    e->segment_cap = d.segment_cap;
     e->q_mark = d.q_mark ? d.q_mark : e->q_pat;
     if (e->q_mark < e->q_pat) e->q_mark = e->q_pat;
     if (e->q_mark > e->q_med) e->q_mark = e->q_med;
@@ -562,7 +572,12 @@ struct fk_batch {
     // the submitted job, kept for results() and re-runs
     std::vector<fk_engine*> engines;
     std::vector<uint32_t> engine_of;
-    std::vector<fk_window> windows;
+    std::vector<fk_window> windows;   // what this submission sweeps
+    // the job behind the submissions: each window's requested end and its
+    // cursor (the start of what is not yet swept; advanced when a
+    // submission's results are collected), and the sweep budget
+    std::vector<u128> job_hi, cursor;
+    u128 budget = 0;                  // line values per submission (0: all)
     std::vector<fk_classify_spec> cls;
     uint32_t final_slot = 0;          // which counter holds the final count
     std::vector<uint32_t> round_slots;  // counter of each executed round
@@ -581,6 +596,7 @@ struct fk_batch {
     hipGraphExec_t graph_exec = nullptr;
     std::vector<Step> graph_plan;
     uint64_t graph_captures = 0, graph_launches = 0;
+    uint64_t reruns = 0;              // batches rerun after a queue overflow
     uint32_t copied = 0;
     uint64_t tiles = 0, candidates = 0;
     std::vector<fk_record> out;       // sorted results
@@ -822,7 +838,8 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         wins[w].engine = b->engine_of[w];
         wins[w].mode = b->cls[w].mode;
         wins[w].cap = b->cls[w].cap;
-        wins[w].exact_from = b->cls[w].exact_from;
+        wins[w].exact_from = b->cls[w].exact_from - b->cls[w].first;
+        wins[w].first = b->cls[w].first;
     }
     for (uint32_t w : order) {
         const fk_engine* e = b->engines[b->engine_of[w]];
@@ -900,10 +917,16 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     for (size_t k = 0; k < ne; k++) {
         if (used[k]) {
             double c = 1.5 * e_alive[k] + 8.0 * std::sqrt(e_alive[k]) + 1024.0;
+            if (e_alive[k] > 0.9 * kSegmentMax)
+                return fail(FK_E_RANGE, "engine %zu's windows in this batch "
+                            "are expected to leave %.3g sieve survivors, "
+                            "more than one queue segment holds (%u): split "
+                            "them", k, e_alive[k], kSegmentMax);
+            if (b->engines[k]->segment_cap) c = b->engines[k]->segment_cap;
             if (caps_in && k < caps_in->size() && (*caps_in)[k])
                 c = (*caps_in)[k];
-            caps[k] = (uint32_t)std::min(c, 1.0e9);
-            caps[k] = (caps[k] + 255u) & ~255u;
+            caps[k] = (uint32_t)std::min(c, (double)kSegmentMax);
+            caps[k] = std::max(256u, (caps[k] + 255u) & ~255u);
         }
         segs[k] = make_uint2((uint32_t)qtot, caps[k]);
         qtot += caps[k];
@@ -1296,23 +1319,33 @@ static int batch_collect(fk_batch* b)
     HIPCHK(hipEventSynchronize(b->ev[5]));
     b->pending = false;
     const size_t ne = b->seg_caps.size();
-    bool over = false;
-    std::vector<uint32_t> caps = b->seg_caps;
-    for (size_t k = 0; k < ne; k++) {
-        const uint32_t c = b->h_counts[k];          // stage 0: the sieve
-        if (c > caps[k]) {
-            // the LDS stage found more survivors than the engine's segment
-            // holds: size it from the true count and run the batch again
-            if (c >= 0xf0000000u)
-                return fail(FK_E_OVERFLOW, "survivor queue overflow: %u", c);
-            caps[k] = (uint32_t)std::min<double>(1.25 * c + 65536.0, 4.0e9);
+    // A segment the sieve overfilled (its counter passed its size; nothing
+    // was written past it) is resized to the true count and the batch run
+    // again.  The counts are exact, so one rerun suffices; a second
+    // overflow means the counts cannot be trusted and is an error.
+    for (int attempt = 0;; attempt++) {
+        std::vector<uint32_t> caps = b->seg_caps;
+        bool over = false;
+        for (size_t k = 0; k < ne; k++) {
+            const uint32_t c = b->h_counts[k];      // stage 0: the sieve
+            if (c <= caps[k]) continue;
+            if (c >= kSegmentMax)
+                return fail(FK_E_RANGE, "engine %zu's windows left %u sieve "
+                            "survivors, more than one queue segment holds: "
+                            "split them", k, c);
+            caps[k] = (uint32_t)std::min<double>(1.25 * c + 65536.0,
+                                                 kSegmentMax);
             over = true;
         }
-    }
-    if (over) {
+        if (!over) break;
+        if (attempt >= 1)
+            return fail(FK_E_OVERFLOW, "survivor queue overflowed again "
+                        "after resizing to the measured counts");
+        b->reruns++;
         int rc = batch_enqueue(b, &caps);
         if (rc) return rc;
-        return batch_collect(b);
+        HIPCHK(hipEventSynchronize(b->ev[5]));
+        b->pending = false;
     }
     const uint32_t n = b->h_counts[b->final_slot];
     auto stage_total = [&](uint32_t st) {
@@ -1363,8 +1396,41 @@ static int batch_collect(fk_batch* b)
                   if (x.value[1] != y.value[1]) return x.value[1] < y.value[1];
                   return x.value[0] < y.value[0];
               });
+    for (size_t w = 0; w < nwin && w < b->cursor.size(); w++)
+        b->cursor[w] = mk128(b->windows[w].hi);
     b->have_results = true;
     return FK_OK;
+}
+
+// The next submission of the job: every window from its cursor to its end,
+// cut so that the submission sweeps at most `budget` line values, taken
+// window by window in order.
+static int batch_start(fk_batch* b)
+{
+    const size_t nwin = b->job_hi.size();
+    u128 left = b->budget ? b->budget : ~(u128)0;
+    u128 total = 0;
+    for (size_t w = 0; w < nwin; w++) {
+        const u128 lo = b->cursor[w], hi = b->job_hi[w];
+        const u128 take = hi > lo ? std::min(hi - lo, left) : 0;
+        left -= take;
+        total += take;
+        fk_window& win = b->windows[w];
+        win.lo[0] = (uint64_t)lo;
+        win.lo[1] = (uint64_t)(lo >> 64);
+        win.hi[0] = (uint64_t)(lo + take);
+        win.hi[1] = (uint64_t)((lo + take) >> 64);
+    }
+    if (total == 0) {
+        // nothing left to sweep: an empty result, nothing on the device
+        b->out.clear();
+        b->counts.assign(nwin, 0);
+        b->stage_counts.clear();
+        b->have_results = true;
+        b->pending = false;
+        return FK_OK;
+    }
+    return batch_enqueue(b, nullptr);
 }
 
 static int batch_submit(fk_batch* b, fk_engine* const* engines,
@@ -1400,8 +1466,10 @@ static int batch_submit(fk_batch* b, fk_engine* const* engines,
         if (e->family == FK_FAMILY_DICKSON) top = hi * e->W;
         if ((top >> 127) != 0)
             return fail(FK_E_RANGE, "window %zu values exceed 127 bits", w);
-        fk_classify_spec c{0, 0, 0};
+        fk_classify_spec c{0, 0, 0, 0};
         if (cls) c = cls[w];
+        if (c.mode && (c.first < 0 || c.first > (1 << 20)))
+            return fail(FK_E_ARG, "chain index %d outside [0, 2^20]", c.first);
         if (c.mode < 0 || c.mode > 2)
             return fail(FK_E_ARG, "classification mode %d", c.mode);
         if (c.mode && (c.cap < 0 || c.cap > 64))
@@ -1409,7 +1477,13 @@ static int batch_submit(fk_batch* b, fk_engine* const* engines,
                         c.cap);
         b->cls[w] = c;
     }
-    return batch_enqueue(b, nullptr);
+    b->job_hi.resize(nwin);
+    b->cursor.resize(nwin);
+    for (size_t w = 0; w < nwin; w++) {
+        b->cursor[w] = mk128(windows[w].lo);
+        b->job_hi[w] = std::max(mk128(windows[w].hi), b->cursor[w]);
+    }
+    return batch_start(b);
 }
 
 // ------------------------------------------------------------------ ABI
@@ -1532,23 +1606,18 @@ int fk_engine_sweep(fk_engine* eng, const fk_window* win,
     return FK_OK;
 }
 
-int fk_engine_classify(fk_engine* e, const uint64_t* values, size_t n,
-                       const fk_classify_spec* cls, int32_t* runs)
+// One classification pass on the device: chain values win.first ..
+// win.first + win.cap - 1 of each value, the run within the pass into runs.
+static int classify_pass(fk_engine* e, const std::vector<u128>& vals,
+                         const DevWinInfo& win, std::vector<int32_t>& runs)
 {
-    if (!e || (!values && n) || !cls || (!runs && n))
-        return fail(FK_E_ARG, "null argument");
-    if (n == 0) return FK_OK;
-    if (cls->mode < 1 || cls->mode > 2 || cls->cap < 0 || cls->cap > 64)
-        return fail(FK_E_ARG, "bad classification spec");
-    if (n > (1u << 26)) return fail(FK_E_ARG, "at most 2^26 values per call");
+    const size_t n = vals.size();
     std::vector<DevRec> recs(n);
     for (size_t i = 0; i < n; i++) {
-        const u128 v = ((u128)values[2 * i + 1] << 64) | values[2 * i];
-        for (int k = 0; k < 4; k++) recs[i].t[k] = (uint32_t)(v >> (32 * k));
+        for (int k = 0; k < 4; k++) recs[i].t[k] = (uint32_t)(vals[i] >> (32 * k));
         recs[i].window = 0;
         recs[i].run = -1;
     }
-    DevWinInfo win{0, cls->mode, cls->cap, cls->exact_from};
     const uint32_t cnt = (uint32_t)n;
     HIPCHK(hipSetDevice(e->device));
     hipStream_t s = e->own->stream;
@@ -1575,7 +1644,7 @@ int fk_engine_classify(fk_engine* e, const uint64_t* values, size_t n,
         ca.recs = d_recs;
         ca.count = d_cnt;
         ca.cap = cnt;
-        ca.cap_max = (uint32_t)std::max(1, cls->cap);
+        ca.cap_max = (uint32_t)std::max(1, win.cap);
         ca.fail2 = d_f;
         ca.failx = d_f + n;
         ca.single_limit = kSingleLimit;
@@ -1595,7 +1664,52 @@ int fk_engine_classify(fk_engine* e, const uint64_t* values, size_t n,
     if (err != hipSuccess)
         return fail(FK_E_DEVICE, "classification failed: %s",
                     hipGetErrorString(err));
+    runs.resize(n);
     for (size_t i = 0; i < n; i++) runs[i] = recs[i].run;
+    return FK_OK;
+}
+
+// Chain values one pass covers (the per-record failure masks are 64 bits).
+static constexpr int kPassValues = 64;
+
+int fk_engine_classify(fk_engine* e, const uint64_t* values, size_t n,
+                       const fk_classify_spec* cls, int32_t* runs)
+{
+    if (!e || (!values && n) || !cls || (!runs && n))
+        return fail(FK_E_ARG, "null argument");
+    if (n == 0) return FK_OK;
+    if (cls->mode < 1 || cls->mode > 2 || cls->cap < 0 ||
+        cls->cap > FK_CLASSIFY_CAP_MAX)
+        return fail(FK_E_ARG, "bad classification spec (mode %d, cap %d; "
+                    "cap at most %d)", cls->mode, cls->cap,
+                    FK_CLASSIFY_CAP_MAX);
+    if (n > (1u << 26)) return fail(FK_E_ARG, "at most 2^26 values per call");
+    // pass k looks at chain values 64k .. 64k + 63, for the values whose
+    // chain ran through the whole of pass k - 1
+    std::vector<size_t> live(n);
+    for (size_t i = 0; i < n; i++) {
+        live[i] = i;
+        runs[i] = 0;
+    }
+    if (cls->first < 0 || cls->first > (1 << 20))
+        return fail(FK_E_ARG, "chain index %d outside [0, 2^20]", cls->first);
+    for (int done = 0; done < cls->cap && !live.empty(); done += kPassValues) {
+        const int pcap = std::min(kPassValues, cls->cap - done);
+        const int first = cls->first + done;
+        DevWinInfo win{0, cls->mode, pcap, cls->exact_from - first, first};
+        std::vector<u128> vals(live.size());
+        for (size_t j = 0; j < live.size(); j++)
+            vals[j] = ((u128)values[2 * live[j] + 1] << 64) | values[2 * live[j]];
+        std::vector<int32_t> got;
+        int rc = classify_pass(e, vals, win, got);
+        if (rc) return rc;
+        std::vector<size_t> on;
+        for (size_t j = 0; j < live.size(); j++) {
+            runs[live[j]] = done + got[j];
+            if (got[j] == pcap) on.push_back(live[j]);
+        }
+        live.swap(on);
+    }
     return FK_OK;
 }
 
@@ -1692,6 +1806,55 @@ int fk_batch_counts(fk_batch* b, uint32_t* counts, size_t cap, size_t* n)
                ? fail(FK_E_OVERFLOW, "%zu stage counts, room for %zu",
                       b->stage_counts.size(), cap)
                : FK_OK;
+}
+
+int fk_batch_reruns(fk_batch* b, uint64_t* reruns)
+{
+    if (!b || !reruns) return fail(FK_E_ARG, "null argument");
+    *reruns = b->reruns;
+    return FK_OK;
+}
+
+int fk_batch_set_budget(fk_batch* b, const uint64_t values[2])
+{
+    if (!b) return fail(FK_E_ARG, "null argument");
+    if (b->pending) return fail(FK_E_BUSY, "batch has work in flight");
+    b->budget = values ? mk128(values) : 0;
+    return FK_OK;
+}
+
+int fk_batch_cursor_get(fk_batch* b, size_t window, uint64_t next[2])
+{
+    if (!b || !next) return fail(FK_E_ARG, "null argument");
+    if (b->pending) return fail(FK_E_BUSY, "batch has work in flight");
+    if (window >= b->cursor.size())
+        return fail(FK_E_ARG, "window %zu of a job of %zu", window,
+                    b->cursor.size());
+    next[0] = (uint64_t)b->cursor[window];
+    next[1] = (uint64_t)(b->cursor[window] >> 64);
+    return FK_OK;
+}
+
+int fk_batch_cursor_set(fk_batch* b, size_t window, const uint64_t next[2])
+{
+    if (!b || !next) return fail(FK_E_ARG, "null argument");
+    if (b->pending) return fail(FK_E_BUSY, "batch has work in flight");
+    if (window >= b->cursor.size())
+        return fail(FK_E_ARG, "window %zu of a job of %zu", window,
+                    b->cursor.size());
+    const u128 v = mk128(next);
+    if (v > b->job_hi[window])
+        return fail(FK_E_RANGE, "cursor past the end of window %zu", window);
+    b->cursor[window] = v;
+    return FK_OK;
+}
+
+int fk_batch_resume(fk_batch* b)
+{
+    if (!b) return fail(FK_E_ARG, "null argument");
+    if (b->pending) return fail(FK_E_BUSY, "batch has work in flight");
+    if (b->job_hi.empty()) return fail(FK_E_ARG, "nothing submitted");
+    return batch_start(b);
 }
 
 int fk_engine_batch(fk_engine* eng, fk_batch** out)

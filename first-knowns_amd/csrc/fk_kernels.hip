@@ -1401,7 +1401,7 @@ __global__ __launch_bounds__(128) void k_cls_base2(ClassifyArgs a)
         if (W.mode == 0 || i >= W.cap) continue;
         const DevEngine& E = a.engines[W.engine];
         uint32_t v[8];
-        rec_chain_value(E, rec, i, v);
+        rec_chain_value(E, rec, W.first + i, v);
         if (!is_sprp(v, 0, 1))
             atomicOr((unsigned long long*)&a.fail2[r], 1ull << i);
     }
@@ -1434,7 +1434,7 @@ __global__ __launch_bounds__(128) void k_cls_full(ClassifyArgs a)
         }
         const DevEngine& E = a.engines[W.engine];
         uint32_t v[8];
-        rec_chain_value(E, rec, i, v);
+        rec_chain_value(E, rec, W.first + i, v);
         if (!is_sprp(v, b, 1))
             atomicOr((unsigned long long*)(b == 0 ? &a.fail2[r] : &a.failx[r]),
                      1ull << i);

@@ -12,8 +12,9 @@ namespace fk {
 struct DevWinInfo {
     uint32_t engine;
     int32_t mode;
-    int32_t cap;
-    int32_t exact_from;
+    int32_t cap;              // chain values looked at (at most 64 per pass)
+    int32_t exact_from;       // relative to `first`
+    int32_t first;            // chain index of the pass's first value
 };
 
 // Queue segments: engine e owns entries [segs[e].x, segs[e].x + segs[e].y)

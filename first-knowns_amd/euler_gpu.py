@@ -18,6 +18,7 @@ import fk_native as N
 P_FLOOR = F.EULER_P_FLOOR
 P_CEIL = F.EULER_P_CEIL
 Q2 = F.EULER_Q2
+SWEEP_CAP = 64          # chain values a sweep classifies per survivor
 
 
 class GpuEngine:
@@ -59,23 +60,26 @@ class GpuEngine:
 
     def hunt(self, lo, hi, cap=100, periods_per_launch=None):
         """Ascending [(p, exact_run)] with run >= n in [lo, hi)
-        (euler_gpu.py:1458-1464); the chains run on the device."""
+        (euler_gpu.py:1458-1464); the chains run on the device, 64 values
+        in the sweep and further, to cap, for the chains that reach 64."""
         lo, hi = self._check(lo, hi)
         cap = int(cap)
-        look = min(max(cap, self.n), 64)
+        look = min(max(cap, self.n), SWEEP_CAP)
         out = []
         for a, b in self._pieces(lo, hi, periods_per_launch):
             if b <= a:
                 continue
             recs = self._eng.sweep(a, b, N.CLASSIFY_FULL, look)
-            for p, run in zip(N.record_values(recs), recs["run"].tolist()):
+            ps = N.record_values(recs)
+            runs = self._eng.extend_runs(ps, recs["run"].tolist(), look, cap)
+            for p, run in zip(ps, runs):
                 if run >= self.n:
                     out.append((p, min(int(run), cap)))
         return out
 
     def mr_run_length(self, p, cap=100):
-        """euler_search.py:258-262 on the device (cap <= 64)."""
-        return self._eng.classify([int(p)], N.CLASSIFY_FULL, min(cap, 64))[0]
+        """euler_search.py:258-262 on the device (cap up to 256)."""
+        return self._eng.classify([int(p)], N.CLASSIFY_FULL, int(cap))[0]
 
     def run_lengths(self, ps, cap=100):
-        return self._eng.classify(ps, N.CLASSIFY_FULL, min(cap, 64))
+        return self._eng.classify(ps, N.CLASSIFY_FULL, int(cap))

@@ -22,7 +22,7 @@ CLASSIFY_NONE, CLASSIFY_FULL, CLASSIFY_TWO_PASS = 0, 1, 2
 
 FK_OK, FK_E_ARG, FK_E_RANGE, FK_E_OVERFLOW = 0, 1, 2, 3
 FK_E_DEVICE, FK_E_OOM, FK_E_NODEV, FK_E_BUSY = 4, 5, 6, 7
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # The exported entry points, in the order include/fk_engine.h declares them.
 EXPORTS = ("fk_abi_version", "fk_last_error", "fk_device_count",
@@ -30,7 +30,9 @@ EXPORTS = ("fk_abi_version", "fk_last_error", "fk_device_count",
            "fk_engine_sweep", "fk_engine_classify", "fk_batch_create",
            "fk_batch_destroy", "fk_batch_submit", "fk_batch_query",
            "fk_batch_results", "fk_batch_timing", "fk_batch_set_graph",
-           "fk_batch_graph_stats", "fk_batch_counts", "fk_engine_batch")
+           "fk_batch_graph_stats", "fk_batch_counts", "fk_engine_batch",
+           "fk_batch_reruns", "fk_batch_set_budget", "fk_batch_cursor_get",
+           "fk_batch_cursor_set", "fk_batch_resume")
 
 u32, i32, u64 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
 
@@ -41,7 +43,7 @@ class EngineDesc(ctypes.Structure):
                 ("q_pat", u32), ("q_med", u32), ("threads", u32),
                 ("words_per_thread", u32), ("round_lane_prime", u32),
                 ("round_growth", u32), ("q_mark", u32), ("q_bits", u32),
-                ("short_words", u32)]
+                ("short_words", u32), ("segment_cap", u32)]
 
 
 class EngineInfo(ctypes.Structure):
@@ -62,7 +64,8 @@ class Window(ctypes.Structure):
 
 
 class ClassifySpec(ctypes.Structure):
-    _fields_ = [("mode", i32), ("cap", i32), ("exact_from", i32)]
+    _fields_ = [("mode", i32), ("cap", i32), ("exact_from", i32),
+                ("first", i32)]
 
 
 RECORD_DTYPE = np.dtype([("v0", "<u8"), ("v1", "<u8"), ("run", "<i4"),
@@ -110,6 +113,11 @@ def lib():
     L.fk_batch_set_graph.argtypes = [vp, ctypes.c_int]
     L.fk_batch_graph_stats.argtypes = [vp, P(u64), P(u64)]
     L.fk_engine_batch.argtypes = [vp, P(vp)]
+    L.fk_batch_reruns.argtypes = [vp, P(u64)]
+    L.fk_batch_set_budget.argtypes = [vp, vp]
+    L.fk_batch_cursor_get.argtypes = [vp, sz, vp]
+    L.fk_batch_cursor_set.argtypes = [vp, sz, vp]
+    L.fk_batch_resume.argtypes = [vp]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
     if L.fk_abi_version() != ABI_VERSION:
@@ -170,7 +178,7 @@ class Engine:
     def __init__(self, family, n, q2, device=0, nwheel=0, wheel_primes=None,
                  q_pat=0, q_med=0, threads=0, words_per_thread=0,
                  round_lane_prime=0, round_growth=0, q_mark=0, q_bits=0,
-                 short_words=0):
+                 short_words=0, segment_cap=0):
         d = EngineDesc()
         d.family, d.n, d.q2, d.device = int(family), int(n), int(q2), device
         d.nwheel = int(nwheel)
@@ -183,6 +191,7 @@ class Engine:
         d.q_mark = int(q_mark)
         d.q_bits = int(q_bits)
         d.short_words = int(short_words)
+        d.segment_cap = int(segment_cap)
         h = ctypes.c_void_p()
         check(lib().fk_engine_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
@@ -214,8 +223,28 @@ class Engine:
         except Exception:
             pass
 
+    # expected sieve survivors one device sweep takes at most; wider
+    # windows are swept in consecutive pieces (the library refuses a window
+    # whose survivors would not fit one queue segment, 2^30 records)
+    PIECE_SURVIVORS = 1 << 28
+
+    def max_span(self):
+        """Widest window one device sweep takes (line values)."""
+        per = self.nlines / self.line_step * float(self.info.survival_lds)
+        return max(self.line_step, int(self.PIECE_SURVIVORS / max(per, 1e-30)))
+
     def sweep(self, lo, hi, mode=CLASSIFY_NONE, cap=0, exact_from=0):
         """Sorted records (numpy RECORD_DTYPE) of the survivors in [lo, hi)."""
+        lo, hi = int(lo), int(hi)
+        step = self.max_span()
+        if hi - lo > step:
+            parts = [self._sweep_one(a, min(a + step, hi), mode, cap,
+                                     exact_from)
+                     for a in range(lo, hi, step)]
+            return np.concatenate(parts)
+        return self._sweep_one(lo, hi, mode, cap, exact_from)
+
+    def _sweep_one(self, lo, hi, mode, cap, exact_from):
         w = make_window(lo, hi)
         c = ClassifySpec(int(mode), int(cap), int(exact_from))
         n = ctypes.c_size_t(0)
@@ -241,16 +270,25 @@ class Engine:
         ws = [(int(lo), int(hi)) for lo, hi in windows]
         if getattr(self, "_pipe", None) is None:
             self._pipe = Pipeline(self.device)
+        # a second generator while one is paused gets a pipeline of its own
+        # (the engine's keeps the first one's windows in flight)
+        pipe = self._pipe if not self._pipe.active else Pipeline(self.device)
         # windows of one length repeat their launch sequence: capture it
         # at once
         graph = 2 if len({hi - lo for lo, hi in ws}) <= 1 else 1
         spec = (mode, cap, exact_from)
-        for _, recs in self._pipe.run([(self, lo, hi, spec) for lo, hi in ws],
-                                      graph=graph):
-            yield recs
+        try:
+            for _, recs in pipe.run([(self, lo, hi, spec) for lo, hi in ws],
+                                    graph=graph):
+                yield recs
+        finally:
+            if pipe is not self._pipe:
+                pipe.close()
 
-    def classify(self, values, mode=CLASSIFY_FULL, cap=64, exact_from=0):
-        """Chain lengths of line values (Python ints), computed on device."""
+    def classify(self, values, mode=CLASSIFY_FULL, cap=64, exact_from=0,
+                 first=0):
+        """Chain lengths of line values (Python ints), computed on device:
+        consecutive chain values from index `first` that pass, up to cap."""
         vals = list(values)
         if not vals:
             return []
@@ -258,10 +296,26 @@ class Engine:
         for i, v in enumerate(vals):
             arr[2 * i], arr[2 * i + 1] = split128(v)
         runs = np.empty(len(vals), dtype=np.int32)
-        c = ClassifySpec(int(mode), int(cap), int(exact_from))
+        c = ClassifySpec(int(mode), int(cap), int(exact_from), int(first))
         check(lib().fk_engine_classify(self._h, arr.ctypes.data, len(vals),
                                        ctypes.byref(c), runs.ctypes.data))
         return [int(r) for r in runs.tolist()]
+
+    def extend_runs(self, values, runs, swept_cap, cap, mode=CLASSIFY_FULL,
+                    exact_from=0):
+        """Runs of a sweep classified with swept_cap (at most 64 values per
+        chain), carried on to `cap`: chains that reached swept_cap are
+        classified again on the device up to cap."""
+        runs = [int(r) for r in runs]
+        if cap <= swept_cap:
+            return runs
+        again = [i for i, r in enumerate(runs) if r >= swept_cap]
+        if again:
+            more = self.classify([values[i] for i in again], mode, cap,
+                                 exact_from)
+            for i, r in zip(again, more):
+                runs[i] = r
+        return runs
 
     def last_timing(self):
         """(ms[prep, sieve, rounds, classify, whole], tiles, candidates) of
@@ -369,6 +423,32 @@ class Batch:
     def timing(self):
         return batch_timing(self._h)
 
+    def reruns(self):
+        """Submissions rerun after a survivor queue segment overflowed."""
+        n = u64(0)
+        check(lib().fk_batch_reruns(self._h, ctypes.byref(n)))
+        return int(n.value)
+
+    def set_budget(self, values):
+        """Later submissions sweep at most `values` line values in all,
+        window by window (0: no limit)."""
+        v = (u64 * 2)(*split128(values))
+        check(lib().fk_batch_set_budget(self._h, v))
+
+    def cursor(self, window):
+        """First line value of `window` not yet swept (its end when done)."""
+        v = (u64 * 2)()
+        check(lib().fk_batch_cursor_get(self._h, int(window), v))
+        return int(v[0]) | (int(v[1]) << 64)
+
+    def set_cursor(self, window, value):
+        v = (u64 * 2)(*split128(value))
+        check(lib().fk_batch_cursor_set(self._h, int(window), v))
+
+    def resume(self):
+        """Submit the job again, every window from its cursor."""
+        check(lib().fk_batch_resume(self._h))
+
     def counts(self):
         return batch_counts(self._h)
 
@@ -386,6 +466,7 @@ class Pipeline:
         self.device, self.depth = int(device), int(depth)
         self._slots = []
         self._graph = None
+        self.active = False           # a run() generator is not finished
 
     def _ready(self, graph):
         if self._graph != graph:
@@ -397,8 +478,13 @@ class Pipeline:
     def run(self, jobs, graph=1):
         """jobs: (engine, lo, hi, (mode, cap, exact_from)) each; yields
         (index, sorted records) in job order.  Windows still in flight when
-        the caller stops early are waited for and dropped."""
+        the caller stops early are waited for and dropped.  One run at a
+        time: a second while the first is paused raises RuntimeError."""
+        if self.active:
+            raise RuntimeError("this pipeline is already running windows; "
+                               "finish or close that generator first")
         self._ready(graph)
+        self.active = True
         pend = [None] * self.depth
         try:
             done = 0
@@ -420,6 +506,7 @@ class Pipeline:
             for s, p in enumerate(pend):
                 if p is not None:
                     self._slots[s].results()
+            self.active = False
 
     def close(self):
         for b in self._slots:

@@ -114,9 +114,13 @@ class GpuEngine:
         k_lo, k_hi = int(k_lo), int(k_hi)
         j_lo = max(1, -(-k_lo // self.W))
         j_hi = k_hi // self.W + 1
-        recs = self._sweep(j_lo, j_hi, N.CLASSIFY_FULL, min(cap, 64))
+        look = min(cap, 64)
+        recs = self._sweep(j_lo, j_hi, N.CLASSIFY_FULL, look)
+        js = N.record_values(recs)
+        runs = self._eng.extend_runs([j * self.W for j in js],
+                                     recs["run"].tolist(), look, cap)
         out = []
-        for j, r in zip(N.record_values(recs), recs["run"].tolist()):
+        for j, r in zip(js, runs):
             k = j * self.W
             if k < k_lo or k > k_hi:
                 continue

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FK_ABI_VERSION 1
+#define FK_ABI_VERSION 2
 
 enum fk_family {
     FK_FAMILY_EULER = 1,    /* x^2 + x + p prime for x = 0..n-1 */
@@ -89,6 +89,13 @@ typedef struct fk_engine_desc {
                                 eight words of a line rather than one
                                 workgroup per line (0 = default 1024; at
                                 most 4096) */
+    uint32_t segment_cap;    /* survivor queue entries this engine's windows
+                                get in a batch (0 = sized from the expected
+                                LDS-stage survivors, which is the normal
+                                use).  A segment the sieve overfills is
+                                resized to the true count and the batch run
+                                again, so a small value only costs a rerun;
+                                at most 2^30 */
 } fk_engine_desc;
 
 typedef struct fk_engine_info {
@@ -124,7 +131,10 @@ typedef struct fk_record {
 typedef struct fk_classify_spec {
     int32_t mode;            /* fk_classify */
     int32_t cap;             /* longest chain looked at */
-    int32_t exact_from;      /* TWO_PASS threshold */
+    int32_t exact_from;      /* TWO_PASS threshold (chain index) */
+    int32_t first;           /* chain index the run is counted from: 0 for
+                                the families' runs; run = consecutive
+                                values first, first + 1, ... that pass */
 } fk_classify_spec;
 
 /* ABI version compiled into the library (FK_ABI_VERSION). */
@@ -168,6 +178,9 @@ int fk_engine_sweep(fk_engine* eng, const fk_window* win,
                     const fk_classify_spec* cls, fk_record* out,
                     size_t out_cap, size_t* n_out);
 
+/* Longest chain fk_engine_classify looks at (sweeps: 64). */
+#define FK_CLASSIFY_CAP_MAX 256
+
 /*
  * Chain classification of caller-given values on the device: Euler p
  * (x^2 + x + p, x = 0..), Dickson k (m k^2 + 1, m = 1..), AP p (p + i P(n),
@@ -175,7 +188,9 @@ int fk_engine_sweep(fk_engine* eng, const fk_window* win,
  * dickson-ladders/ladder_gpu.py:813-817 (run_length),
  * dickson-ladders/launch.py:310-352 (sprp_run, _classify_chunk) and
  * primorial-ap/ap_search.py:160-194 (chain_depth).  values: 2 uint64_t per
- * value; runs: one int32 per value.
+ * value; runs: one int32 per value.  cap may reach FK_CLASSIFY_CAP_MAX
+ * (the reference's run cap is 100, euler_search.py:258-262): chains are
+ * looked at 64 values per device pass.
  */
 int fk_engine_classify(fk_engine* eng, const uint64_t* values, size_t n,
                        const fk_classify_spec* cls, int32_t* runs);
@@ -254,6 +269,36 @@ int fk_batch_counts(fk_batch* b, uint32_t* counts, size_t cap, size_t* n);
 
 /* The engine's own internal batch, for timing its last sweep. */
 int fk_engine_batch(fk_engine* eng, fk_batch** out);
+
+/*
+ * Batches rerun so far because the sieve overfilled a survivor queue
+ * segment (the rerun sizes the segment from the true count; see
+ * fk_engine_desc.segment_cap).  No reference counterpart: the reference
+ * raises on a full queue (euler-prime-runs/euler_gpu.py:1439-1450) where
+ * this library resizes and reruns, and this count shows that it did.
+ */
+int fk_batch_reruns(fk_batch* b, uint64_t* reruns);
+
+/*
+ * Checkpoint cursors of a batch job.  A job is what fk_batch_submit was
+ * last given; each submission sweeps every window from its cursor towards
+ * its end, at most `values` line values in all (taken window by window in
+ * order; NULL or 0 = no limit).  The cursor of a window is the first line
+ * value not yet swept: its records so far cover exactly [lo, cursor), and
+ * it moves when a submission's results are collected (the last fully
+ * completed boundary).  fk_batch_resume submits the job again from the
+ * cursors; fk_batch_cursor_set moves one (a cursor restored from a
+ * checkpoint file, or a window to redo).  Any cut gives the same records as
+ * one sweep (split == whole).  These stand in for the cursor attributes
+ * the reference's launchers count in: euler-prime-runs/launch.py:191-218
+ * (next_k), dickson-ladders/launch.py:491-497, 772-784 (next_j) and
+ * primorial-ap/launch.py:686-770 (next_u, _finalize), for many windows in
+ * one job.
+ */
+int fk_batch_set_budget(fk_batch* b, const uint64_t values[2]);
+int fk_batch_cursor_get(fk_batch* b, size_t window, uint64_t next[2]);
+int fk_batch_cursor_set(fk_batch* b, size_t window, const uint64_t next[2]);
+int fk_batch_resume(fk_batch* b);
 
 #ifdef __cplusplus
 }

@@ -344,6 +344,24 @@ def ap_forbidden(q, n):
     return sorted({(-j * d) % q for j in range(n)})
 
 
+def ap_survives(n, p, q2=1 << 16):
+    """ap_search.py:143-157 survives: p kept by the sieve (p is prime to
+    30 and p mod q avoids the kill residues of every prime 7 <= q < q2),
+    at any height, the engine floor included."""
+    p = int(p)
+    if p % 2 == 0 or p % 3 == 0 or p % 5 == 0:
+        return False
+    d = primorial(n)
+    for q in primes_below(q2):
+        if q < 7:
+            continue
+        # p is in {(-j d) mod q : j < n} iff some p + j d is 0 mod q
+        r, dq = p % q, d % q
+        if any((r + j * dq) % q == 0 for j in range(n if dq else 1)):
+            return False
+    return True
+
+
 def ap_survivors(n, base, span, q2=1 << 16, nthreads=None, block=1 << 19):
     """ap_search.py:109-141: sorted offsets o, base + o surviving."""
     base, span = int(base), int(span)

@@ -291,3 +291,40 @@ def test_planted_table_mistake_is_caught(monkeypatch):
                   for q in O.primes_below(q2) if q >= n + 2 and q != 13
                   for m in range(1, n + 1))]
     assert bad != want
+
+
+@pytest.mark.parametrize("n,base,span", [
+    (6, 10**16 - 10**16 % 30, 30 * (1 << 18)),
+    (8, 10**15 - 10**15 % 30, 30 * (1 << 21)),
+])
+def test_ap_direct_division_pins_deeper_streams(n, base, span):
+    """The direct-division helper the config-2 GPU test uses: the oracle's
+    stream at 2^16, minus the members some prime in [2^16, 2^20) kills,
+    is the oracle's stream at 2^20; with P(n - 1) in place of P(n) (a
+    plausible slip) it is not."""
+    import ap_direct
+    s16 = [base + o for o in O.ap_survivors(n, base, span, q2=1 << 16)]
+    s20 = [base + o for o in O.ap_survivors(n, base, span, q2=1 << 20)]
+    assert len(s20) >= 20 and len(s16) > len(s20)
+    dead = ap_direct.kills_between(s16, n, 1 << 16, 1 << 20)
+    assert [p for p, k in zip(s16, dead) if not k] == s20
+    slip = ap_direct.kills_between(s16, n - 1, 1 << 16, 1 << 20)
+    assert [p for p, k in zip(s16, slip) if not k] != s20
+
+
+def test_ap_survives_is_the_sieve_membership():
+    """The one-candidate membership test agrees with the windowed sieve on
+    every p of a populated window (survivors and not), and with the
+    residue definition below the engines' floor, where p may itself be a
+    sieving prime (which the sieve kills: 0 is always a kill residue)."""
+    n, q2 = 8, 4096
+    base = 4 * 10**13 - 10
+    span = 30 * 6_000
+    keep = {base + o for o in O.ap_survivors(n, base, span, q2=q2)}
+    assert len(keep) >= 10
+    for p in range(base, base + span):
+        assert O.ap_survives(n, p, q2) == (p in keep), p
+    assert not O.ap_survives(n, 4093, q2)          # a sieving prime itself
+    assert not O.ap_survives(n, 7, q2)
+    small = [p for p in range(5000, 9000) if O.ap_survives(n, p, q2)]
+    assert small and all(p % 2 and p % 3 and p % 5 for p in small)
