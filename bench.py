@@ -34,6 +34,7 @@ import math
 import os
 import pathlib
 import random
+import statistics
 import sys
 import time
 
@@ -58,6 +59,7 @@ def splitmix64(x):
 # ------------------------------------ headline: the Dickson ladder sweep
 
 SWEEP_N, SWEEP_Q2, SWEEP_SPAN, SWEEP_CAP = 10, 1 << 22, 1 << 32, 18
+N_STEP_SIEVE, N_STEP_ROUND_C, N_STEP_COMPACT = 1, 2, 4
 
 
 def sweep_window(seed):
@@ -87,6 +89,10 @@ def run_plain(steps, warmup):
         eng._eng.sweep(*sweep_window(2000 + s), N.CLASSIFY_FULL, SWEEP_CAP)
     device_ms = eng._eng.last_timing()[0][4]
     surv = sum(int(r.size) for r in out)
+    fp = 0
+    for recs in out:
+        for v in N.record_values(recs):
+            fp ^= int(v)
     deepest = (0, None)
     for recs in out:
         if recs.size and int(recs["run"].max()) > deepest[0]:
@@ -98,6 +104,8 @@ def run_plain(steps, warmup):
     stage = [0.0] * 5
     nst = min(3, steps)
     cand = 0
+    launches = []                  # per window: [(kind, ms)] of its launches
+    counts = []
     for s in range(nst):
         lo, hi = sweep_window(s)
         eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, SWEEP_CAP)
@@ -105,30 +113,49 @@ def run_plain(steps, warmup):
         cand += c
         for i in range(5):
             stage[i] += ms[i] / nst
+        launches.append(eng._eng.last_step_times())
+        counts.append(eng._eng.last_counts())
     eng._eng.stage_timing(False)
-    return eng, wall, surv, stage, cand / nst, deepest, device_ms
+    return {"eng": eng, "wall": wall, "surv": surv, "fp": fp, "stage": stage,
+            "cand": cand / nst, "deepest": deepest, "device_ms": device_ms,
+            "launches": launches, "counts": counts}
 
 
-def cpu_baseline(eng, sample_log2, threads):
+def cpu_baseline(eng, sample_log2, threads, one_core_log2):
+    """The oracle's C restatement of the reference CPU sieve plus the
+    Python-integer classification of its survivors, on `threads` threads
+    over 2^sample_log2 j of the step-0 window, and on one thread over
+    2^one_core_log2 j; each sample's stream is checked against the GPU's."""
     import fk_native as N
     sys.path.insert(0, str(ROOT / "oracle"))
     import fk_oracle as O
-    lo, _ = sweep_window(0)
-    hi = lo + (1 << sample_log2)
     sieve = O.DicksonSieve(SWEEP_N, SWEEP_Q2)            # tables outside the clock
-    t0 = time.perf_counter()
-    js = sieve.survivors(lo, hi, nthreads=threads)
-    runs = [O.dickson_run(j * eng.W, cap=SWEEP_CAP) for j in js]
-    wall = time.perf_counter() - t0
-    recs = eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, SWEEP_CAP)
-    same = (N.record_values(recs) == js and
-            [int(r) for r in recs["run"].tolist()] == runs)
-    return {"value": (hi - lo) / wall, "unit": "j-candidates/s",
+
+    def leg(log2, nthreads):
+        lo, _ = sweep_window(0)
+        hi = lo + (1 << log2)
+        t0 = time.perf_counter()
+        js = sieve.survivors(lo, hi, nthreads=nthreads)
+        runs = [O.dickson_run(j * eng.W, cap=SWEEP_CAP) for j in js]
+        wall = time.perf_counter() - t0
+        recs = eng._eng.sweep(lo, hi, N.CLASSIFY_FULL, SWEEP_CAP)
+        same = (N.record_values(recs) == js and
+                [int(r) for r in recs["run"].tolist()] == runs)
+        return (hi - lo) / wall, len(js), wall, same
+
+    rate, nsurv, wall, same = leg(sample_log2, threads)
+    rate1, nsurv1, wall1, same1 = leg(one_core_log2, 1)
+    return {"value": rate, "unit": "j-candidates/s",
             "cores": threads, "kind": "port",
+            "host_cpu_count": os.cpu_count(),
             "sample": f"2^{sample_log2} j from the step-0 window, n={SWEEP_N}, "
                       f"q2=2^22, survivors classified (cap {SWEEP_CAP}); "
-                      f"{len(js)} survivors, {wall:.1f} s",
-            "sample_stream_equal": bool(same)}
+                      f"{nsurv} survivors, {wall:.1f} s on {threads} threads",
+            "sample_stream_equal": bool(same),
+            "value_one_core": rate1,
+            "sample_one_core": f"2^{one_core_log2} j of the same window on one "
+                               f"thread; {nsurv1} survivors, {wall1:.1f} s",
+            "sample_one_core_stream_equal": bool(same1)}
 
 
 # ------------------------------------ 256 mixed windows, one batch
@@ -173,7 +200,7 @@ def batch_spec(fam, n):
     return (N.CLASSIFY_TWO_PASS, n, 6)
 
 
-def run_batched():
+def run_batched(repeats=3):
     import fk_native as N
     import fk_families as F
     wins = batch_windows()
@@ -194,23 +221,29 @@ def run_batched():
     specs = [batch_spec(f, n) for f, n, _, _ in wins]
     cand = sum(hi - lo for lo, hi in ws)
     # plain: one engine call per window, two in flight (the plain engine's
-    # own pipelining, as in the headline loop)
+    # own pipelining, as in the headline loop); median of three passes
     pipe = N.Pipeline()
     jobs = [(engines[eo[i]], *ws[i], specs[i]) for i in range(len(ws))]
     for _ in pipe.run(jobs[:3]):
         pass
-    t0 = time.perf_counter()
-    plain = [r for _, r in pipe.run(jobs)]
-    t_plain = time.perf_counter() - t0
+    t_plain = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        plain = [r for _, r in pipe.run(jobs)]
+        t_plain.append(time.perf_counter() - t0)
     pipe.close()
-    # batched: one submission
-    b = N.Batch()
+    # batched: one submission, its launch sequence captured as a HIP graph
+    # on the first submission and replayed after; median of three replays
+    b = N.Batch(graph=2)
     b.submit(engines, eo, ws, specs)
     b.results()
-    t0 = time.perf_counter()
-    b.submit(engines, eo, ws, specs)
-    recs, counts = b.results()
-    t_batch = time.perf_counter() - t0
+    t_batch = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        b.submit(engines, eo, ws, specs)
+        recs, counts = b.results()
+        t_batch.append(time.perf_counter() - t0)
+    captures, launches = b.graph_stats()
     same = True
     pos = 0
     for i, p in enumerate(plain):
@@ -219,48 +252,193 @@ def run_batched():
         if (N.record_values(mine) != N.record_values(p) or
                 mine["run"].tolist() != p["run"].tolist()):
             same = False
+    tp, tb = statistics.median(t_plain), statistics.median(t_batch)
     return {"windows": len(ws), "engines": len(engines),
             "candidates": cand, "survivors": int(recs.size),
-            "plain_s": t_plain, "batched_s": t_batch,
-            "plain_rate": cand / t_plain, "batched_rate": cand / t_batch,
-            "speedup": t_plain / t_batch, "streams_equal": same}
+            "plain_s": tp, "batched_s": tb,
+            "plain_s_runs": t_plain, "batched_s_runs": t_batch,
+            "plain_rate": cand / tp, "batched_rate": cand / tb,
+            "speedup": tp / tb, "streams_equal": same,
+            "batched_graph": {"captures": captures, "launches": launches},
+            "note": f"medians of {repeats} passes; the batch replays its "
+                    f"captured launch sequence"}
 
 
 # ------------------------------------ serving: Poisson arrivals
 
-def run_serving(count, rate, seed=4):
+def run_serving(count, rate, seed=4, repeats=3):
     """Poisson arrivals of range-sweep requests of all three families,
     spans log-uniform in [2^18, 2^32] line values, depths 8..14, Dickson k
-    up to about 1e22 (192-bit values); continuous batching over 4 slots."""
+    up to about 1e22 (192-bit values); continuous batching over 4 slots.
+    The figures are medians over `repeats` runs of the same arrivals."""
     import fk_serve as S
     reqs = S.poisson_requests(seed, count, rate)
     srv = S.Server(slots=4, budget=64 << 30)
     # engines are built (resident tables) before the clock starts
     for r in reqs:
         srv.engine(r.family, r.n, r.q2)
-    st = srv.run(reqs)
-    st["rate_per_s"] = rate
-    st["slots"] = 4
-    st["submissions"] = len(srv.submissions)
-    st["mean_requests_per_submission"] = (
-        sum(srv.submissions) / len(srv.submissions) if srv.submissions else 0)
-    return st
+    runs = []
+    for _ in range(repeats):
+        srv.submissions = []
+        st = srv.run(reqs)
+        st["submissions"] = len(srv.submissions)
+        st["mean_requests_per_submission"] = (
+            sum(srv.submissions) / len(srv.submissions)
+            if srv.submissions else 0)
+        runs.append(st)
+    out = dict(runs[0])
+    for k in ("p50_ms", "p99_ms", "wall_s", "candidates_per_s",
+              "mean_requests_per_submission"):
+        out[k] = statistics.median(r[k] for r in runs)
+    out["p99_ms_runs"] = [r["p99_ms"] for r in runs]
+    out["rate_per_s"] = rate
+    out["slots"] = 4
+    out["note"] = f"medians of {repeats} runs of the same arrivals"
+    return out
 
 
-TRAFFIC_JSON = ROOT / "profiles" / "k_sieve_short_dickson_sweep_traffic.json"
+# ------------------------------------ the Euler n = 17 shape, two wheels
+
+EULER_WIDE = (611 * 10**18, 2 * 10**16)     # euler-prime-runs/score.py:57-77
+
+
+def run_euler_wide(golden_xor=None):
+    """The Euler SCORE_WIDE shape (n = 17, p in [6.11e20, +2e16), every
+    prime below 2^16 sieved) on the 31# wheel (29,937,600 residues) and on
+    the 37# wheel (598,752,000, built factored on the device): p/s of each
+    sweep after one warm-up sweep, and each stream's count and XOR."""
+    import euler_gpu
+    lo, span = EULER_WIDE
+    out = {}
+    for name, wp in (("wheel_31", [2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31]),
+                     ("wheel_37", [2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31,
+                                   37])):
+        eng = euler_gpu.GpuEngine(17, wheel_primes=wp)
+        eng.survivors_pre_mr(lo - 10**15, lo)                # warm-up
+        t0 = time.perf_counter()
+        got = eng.survivors_pre_mr(lo, lo + span)
+        wall = time.perf_counter() - t0
+        x = 0
+        for p in got:
+            x ^= p
+        out[name] = {"value": span / wall, "unit": "p-candidates/s",
+                     "seconds": wall, "residues": eng.n_offs,
+                     "count": len(got), "xor": x}
+        eng._eng.close()
+    out["speedup_37_over_31"] = (out["wheel_37"]["value"] /
+                                 out["wheel_31"]["value"])
+    return out
+
+
+# ------------------------------------ config 2: the AP sweep
+
+AP_N, AP_Q2 = 17, 1 << 24
+AP_SPAN = 30 * ((1 << 32) // 30)
+
+
+def ap_window(seed):
+    base = 10**16 + splitmix64(seed) % 10**15
+    base -= base % 30
+    return base, base + AP_SPAN
+
+
+def run_ap(steps=6, warmup=2):
+    """The plain engine on config 2: n = 17, d = 17#, primes 61..2^24
+    sieved (7..59 kill residue 0), windows of 30 floor(2^32 / 30)
+    consecutive p near 1e16, survivors classified two-pass (base 2, every
+    base from depth 6, cap 17); windows in order, two in flight."""
+    import fk_native as N
+    import fk_families as F
+    eng = N.Engine(N.FAMILY_AP, AP_N, AP_Q2, **F.tuning("ap"))
+    spec = (N.CLASSIFY_TWO_PASS, AP_N, 6)
+    for _ in eng.sweep_many([ap_window(100 + s) for s in range(warmup)],
+                            *spec):
+        pass
+    wins = [ap_window(s) for s in range(steps)]
+    t0 = time.perf_counter()
+    out = list(eng.sweep_many(wins, *spec))
+    wall = time.perf_counter() - t0
+    eng.stage_timing(True)
+    eng.sweep(*wins[0], *spec)
+    ms = eng.last_timing()[0]
+    counts = eng.last_counts()
+    eng.stage_timing(False)
+    surv = sum(int(r.size) for r in out)
+    eng.close()
+    return {"value": steps * AP_SPAN / wall, "unit": "p-candidates/s",
+            "ms_per_window": 1e3 * wall / steps, "windows": steps,
+            "survivors": surv,
+            "stages_ms": dict(zip(("prep", "sieve", "rounds", "classify",
+                                   "whole"), ms)),
+            "queue_after_each_stage": counts,
+            "config": {"n": AP_N, "q2": AP_Q2, "window_p": AP_SPAN,
+                       "classify": "two-pass, exact from 6, cap 17"}}
+
+
+VALU_PEAK = 256 * 64 * 2.4e9           # lane-ops/s: 256 CUs x 64 lanes/clk
 
 
 def rocprof_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` from the committed PMC summary
-    (None when the summary covers another kernel)."""
-    if TRAFFIC_JSON.exists():
+    """HBM bytes per dispatch of `kernel` from its committed PMC summary,
+    profiles/<kernel>_dickson_sweep_traffic.json (None when there is none)."""
+    path = ROOT / "profiles" / f"{kernel}_dickson_sweep_traffic.json"
+    if path.exists():
         try:
-            js = json.loads(TRAFFIC_JSON.read_text())
+            js = json.loads(path.read_text())
         except (OSError, ValueError):
             return None
         if kernel in js.get("kernel", ""):
             return js.get("bytes_per_dispatch")
     return None
+
+
+def legendre_is_one(a, q):
+    """(a | q) == 1 for arrays of a and odd primes q (q < 2^26), by
+    a^((q-1)/2) mod q with numpy uint64 (products below 2^52)."""
+    import numpy as np
+    e = (q - 1) // 2
+    base = a % q
+    acc = np.ones_like(q)
+    while e.any():
+        odd = (e & 1).astype(bool)
+        acc = np.where(odd, acc * base % q, acc)
+        base = base * base % q
+        e >>= 1
+    return acc == 1
+
+
+def round_work(info, counts, growth):
+    """Algorithmic work of the Dickson sweep's compaction rounds, per
+    window: round r tests every candidate left after the stage before it
+    (counts[r]) against every prime of [q_med g^r, q_med g^(r+1)) (the last
+    round ends at q2).  A test is the exact residue of k mod q from its
+    16-bit limbs (3 FMAs), the reduction (FMA, floor, FMA, convert, two
+    integer fixes) and the fold: 9 VALU ops; then the compares against the
+    prime's folded kill set, four at a time up to its size: the size is
+    #{m <= n : (-m | q) = 1}.  Returns (tests, VALU lane-ops, per-round
+    [(first prime, primes, candidates)])."""
+    import numpy as np
+    import fk_families as F
+    ps = np.array([q for q in F.primes_below(int(info.q2))
+                   if q >= int(info.q_med)], dtype=np.uint64)
+    n = int(info.n)
+    cnt = np.zeros(ps.size, dtype=np.int64)
+    for m in range(1, n + 1):
+        cnt += legendre_is_one((ps - np.uint64(m)) % ps, ps)
+    ops = 9 + 4 * np.maximum(1, (cnt + 3) // 4)
+    edges = [int(info.q_med)]
+    while edges[-1] < int(info.q2):
+        edges.append(min(edges[-1] * growth, int(info.q2)))
+    tests = vops = 0
+    rounds = []
+    for r in range(len(edges) - 1):
+        lo = np.searchsorted(ps, edges[r])
+        hi = np.searchsorted(ps, edges[r + 1])
+        c = int(counts[r]) if r < len(counts) else 0
+        tests += c * int(hi - lo)
+        vops += c * int(ops[lo:hi].sum())
+        rounds.append((int(ps[lo]) if hi > lo else edges[r], int(hi - lo), c))
+    return int(tests), int(vops), rounds
 
 
 def sieve_kernel(info, span):
@@ -300,6 +478,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-ap", action="store_true")
+    ap.add_argument("--no-euler", action="store_true")
+    ap.add_argument("--cpu-one-core-log2", type=int, default=27)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -314,17 +495,30 @@ def main():
               f"--gpus {args.gpus} is not supported.", file=sys.stderr)
         return 2
 
-    eng, wall, surv, stage, cand, deepest, device_ms = run_plain(args.steps,
-                                                                args.warmup)
+    P = run_plain(args.steps, args.warmup)
+    eng, wall, stage = P["eng"], P["wall"], P["stage"]
     j_total = args.steps * SWEEP_SPAN
     value = j_total / wall
     info = eng._eng.info
+    # per-launch HIP-event durations of the eager windows
+    def launch_ms(kinds):
+        per = [sum(ms for k, ms in w if k in kinds) for w in P["launches"]]
+        n = [sum(1 for k, _ in w if k in kinds) for w in P["launches"]]
+        return sum(per) / len(per), sum(n) / len(n)
+    round_ms, round_launches = launch_ms({N_STEP_ROUND_C})
+    sieve_ms, _ = launch_ms({N_STEP_SIEVE})
+    compact_ms, _ = launch_ms({N_STEP_COMPACT})
+    # the rounds: tests and VALU lane-ops per window from the queue sizes
+    import fk_families as F
+    growth = int(F.tuning("dickson").get("round_growth") or 2)
+    tests, vops, rounds = round_work(info, P["counts"][0], growth)
+    r_achieved = vops / (round_ms / 1e3) if round_ms > 0 else 0.0
+    r_traffic = rocprof_traffic("k_round_c")
+    # the sieve: algorithmic LDS bytes of the window's wheel candidates
     kernel, per_cand = sieve_kernel(info, SWEEP_SPAN)
-    # one sieve launch per window: its duration is the eager sieve stage
-    launch_s = stage[1] / 1e3
-    per_launch = cand                    # wheel candidates per window
-    achieved = per_launch * per_cand / launch_s if launch_s > 0 else 0.0
-    traffic = rocprof_traffic(kernel)
+    s_launch = sieve_ms / 1e3
+    s_achieved = P["cand"] * per_cand / s_launch if s_launch > 0 else 0.0
+    s_traffic = rocprof_traffic(kernel)
     out = {
         "metric": METRIC, "value": value, "unit": "j-candidates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -342,38 +536,71 @@ def main():
                    "wheel_period": int(info.line_step),
                    "q_pat": int(info.q_pat), "q_med": int(info.q_med),
                    "tile_bits": None},
-        "survivors": surv,
-        "deepest_ladder": {"run": deepest[0], "j": deepest[1]},
-        "device_ms_per_window_alone": device_ms,
+        "survivors": P["surv"],
+        "fingerprint": {"count": P["surv"], "xor": P["fp"],
+                        "of": "the j of every survivor of the timed windows"},
+        "deepest_ladder": {"run": P["deepest"][0], "j": P["deepest"][1]},
+        "device_ms_per_window_alone": P["device_ms"],
         "stages_ms_per_step": dict(zip(("prep", "sieve", "rounds",
                                         "classify", "whole"), stage)),
-        "stages_note": "per-stage HIP events on three windows swept eagerly "
-                       "after the timed loop; the timed sweeps keep two "
-                       "windows in flight, each replaying its slot's captured "
-                       "launch sequence; device_ms_per_window_alone is one "
-                       "window swept by itself (graph replay, event-timed)",
-        "roofline": {"bound": "lds", "kernel": kernel,
-                     "achieved": achieved / 1e9, "peak": LDS_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / LDS_PEAK,
-                     "traffic": traffic,
-                     "algorithmic_bytes_per_candidate": per_cand,
-                     "candidates_per_launch": per_launch,
-                     "hbm_GBps": (traffic / launch_s / 1e9
-                                  if traffic and launch_s > 0 else None),
-                     "hbm_frac": (traffic / launch_s / HBM_PEAK
-                                  if traffic and launch_s > 0 else None)},
+        "launch_ms_per_step": {"k_round_c": round_ms,
+                               "k_round_c_launches": round_launches,
+                               "k_round_compact": compact_ms,
+                               "sieve": sieve_ms},
+        "stages_note": "per-stage and per-launch HIP events on three windows "
+                       "swept eagerly after the timed loop; the timed sweeps "
+                       "keep two windows in flight, each replaying its "
+                       "slot's captured launch sequence; "
+                       "device_ms_per_window_alone is one window swept by "
+                       "itself (graph replay, event-timed)",
+        "roofline": {"bound": "valu", "kernel": "k_round_c",
+                     "achieved": r_achieved / 1e12, "peak": VALU_PEAK / 1e12,
+                     "unit": "Tlane-op/s", "frac": r_achieved / VALU_PEAK,
+                     "traffic": r_traffic,
+                     "tests_per_window": tests,
+                     "valu_lane_ops_per_window": vops,
+                     "valu_ops_per_test": vops / tests if tests else None,
+                     "launches_per_window": round_launches,
+                     "rounds": [{"first_prime": a, "primes": b,
+                                 "candidates": c} for a, b, c in rounds],
+                     "hbm_GBps": (r_traffic * round_launches / (round_ms / 1e3)
+                                  / 1e9 if r_traffic and round_ms > 0
+                                  else None)},
+        "roofline_sieve": {"bound": "lds", "kernel": kernel,
+                           "achieved": s_achieved / 1e9,
+                           "peak": LDS_PEAK / 1e9, "unit": "GB/s",
+                           "frac": s_achieved / LDS_PEAK,
+                           "traffic": s_traffic,
+                           "algorithmic_bytes_per_candidate": per_cand,
+                           "candidates_per_launch": P["cand"],
+                           "hbm_GBps": (s_traffic / s_launch / 1e9
+                                        if s_traffic and s_launch > 0
+                                        else None)},
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(eng, args.cpu_sample_log2,
-                                           threads)
+                                           threads, args.cpu_one_core_log2)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        out["gpu_over_cpu_one_core"] = (value /
+                                        out["cpu_baseline"]["value_one_core"])
     if not args.no_batched:
         out["batched"] = run_batched()
     if not args.no_serving:
         out["serving"] = run_serving(args.serve_requests, args.serve_rate)
-    print(json.dumps(out), flush=True)
+    if not args.no_ap:
+        out["config2_ap"] = run_ap()
+    if not args.no_euler:
+        out["euler_score_wide"] = run_euler_wide()
+    print(json.dumps(out, default=_plain), flush=True)
     return 0
+
+
+def _plain(x):
+    """numpy scalars in the JSON line as plain numbers."""
+    if hasattr(x, "item"):
+        return x.item()
+    raise TypeError(f"{type(x).__name__} is not JSON serializable")
 
 
 if __name__ == "__main__":

@@ -141,7 +141,9 @@ struct fk_engine {
     uint32_t q2 = 0;
     std::vector<uint32_t> wheel;
     uint64_t D = 1;
-    std::vector<uint64_t> offs;
+    std::vector<uint64_t> offs;      // the wheel's residues (host-built wheels)
+    uint64_t nlines = 0;             // the wheel's residue count
+    bool factored = false;           // residues expanded on the device
     uint64_t W = 1;
     u128 dval = 0;
     uint32_t q_pat = 0, q_med = 0, threads = 256, wpt = 8;
@@ -197,6 +199,11 @@ static constexpr double kTileRoundSurvival = 0.25;
 // counter that wraps.
 static constexpr uint32_t kSegmentMax = 1u << 30;
 
+// Wheels of more residues than this are built factored (the last prime's
+// lifts expanded on the device): the Euler 37# wheel at n = 17 has
+// 598,752,000 (4.8 GB), the 31# one 29,937,600.
+static constexpr uint64_t kHostWheelResidues = 1ull << 25;
+
 static int build_engine(fk_engine* e)
 {
     const fk_engine_desc& d = e->desc;
@@ -244,15 +251,34 @@ static int build_engine(fk_engine* e)
     }
     if (wheel.empty()) return fail(FK_E_ARG, "no kill prime below q2");
     bool ok = false;
-    e->offs = wheel_offsets(d.family, d.n, wheel, &e->D, &ok);
+    e->nlines = wheel_count(d.family, d.n, wheel, &ok);
     if (!ok) return fail(FK_E_ARG, "wheel period exceeds 2^63");
-    if (e->offs.empty())
+    if (e->nlines == 0)
         return fail(FK_E_ARG, "the wheel admits no residue at n = %d", d.n);
-    if (e->offs.size() > (1u << 30))
-        return fail(FK_E_ARG, "wheel has %zu residues (> 2^30)",
-                    e->offs.size());
+    if (e->nlines > (1u << 30))
+        return fail(FK_E_ARG, "wheel has %llu residues (> 2^30)",
+                    (unsigned long long)e->nlines);
+    // a wheel of many residues is built factored: the residues of all but
+    // its last prime on the host, lifted over the last prime on the device
+    e->factored = e->nlines > kHostWheelResidues && wheel.size() > 1;
+    std::vector<uint64_t> fbase;
+    std::vector<uint32_t> flifts;
+    uint32_t fnj = 0;
+    uint64_t fMb = 0;
+    if (e->factored) {
+        std::vector<uint32_t> head(wheel.begin(), wheel.end() - 1);
+        fbase = wheel_offsets(d.family, d.n, head, &fMb, &ok);
+        if (!ok) return fail(FK_E_ARG, "wheel period exceeds 2^63");
+        flifts = wheel_lifts(d.family, d.n, wheel.back(), fMb, &fnj);
+        e->D = fMb * wheel.back();
+        if ((uint64_t)fbase.size() * fnj != e->nlines)
+            return fail(FK_E_ARG, "factored wheel count disagrees");
+    } else {
+        e->offs = wheel_offsets(d.family, d.n, wheel, &e->D, &ok);
+        if (!ok) return fail(FK_E_ARG, "wheel period exceeds 2^63");
+    }
     e->wheel = wheel;
-    e->line_density = (double)e->offs.size() / (double)e->D;
+    e->line_density = (double)e->nlines / (double)e->D;
 
     // ---- tuning
     e->threads = d.threads ? d.threads : 256;
@@ -393,7 +419,7 @@ static int build_engine(fk_engine* e)
     DevEngine& h = e->host;
     h.family = d.family;
     h.n = d.n;
-    h.nlines = (uint32_t)e->offs.size();
+    h.nlines = (uint32_t)e->nlines;
     h.nlds = (uint32_t)lq.size();
     h.npat = e->npat;
     h.nmark = nmark;
@@ -422,7 +448,35 @@ static int build_engine(fk_engine* e)
         h.d[i] = (uint32_t)(e->dval >> (32 * i));
         h.d[4 + i] = 0;
     }
-    if ((rc = eng_upload(e, e->offs, &h.offs))) return rc;
+    if (!e->factored) {
+        if ((rc = eng_upload(e, e->offs, &h.offs))) return rc;
+    } else {
+        // expand the factored wheel into the engine's residue array
+        uint64_t* offs = nullptr;
+        HIPCHK(hipMalloc(&offs, e->nlines * sizeof(uint64_t)));
+        e->allocs.push_back(offs);
+        e->device_bytes += e->nlines * sizeof(uint64_t);
+        uint64_t* db = nullptr;
+        uint32_t* dl = nullptr;
+        hipError_t err = hipMalloc(&db, fbase.size() * sizeof(uint64_t));
+        if (err == hipSuccess) err = hipMalloc(&dl, flifts.size() * sizeof(uint32_t));
+        if (err == hipSuccess)
+            err = hipMemcpy(db, fbase.data(), fbase.size() * sizeof(uint64_t),
+                            hipMemcpyHostToDevice);
+        if (err == hipSuccess)
+            err = hipMemcpy(dl, flifts.data(), flifts.size() * sizeof(uint32_t),
+                            hipMemcpyHostToDevice);
+        if (err == hipSuccess)
+            err = launch_wheel_expand(db, fbase.size(), dl, fnj, wheel.back(),
+                                      fMb, offs, nullptr);
+        if (err == hipSuccess) err = hipDeviceSynchronize();
+        if (db) (void)hipFree(db);
+        if (dl) (void)hipFree(dl);
+        if (err != hipSuccess)
+            return fail(err == hipErrorOutOfMemory ? FK_E_OOM : FK_E_DEVICE,
+                        "factored wheel: %s", hipGetErrorString(err));
+        h.offs = offs;
+    }
     if ((rc = eng_upload(e, lq, &h.lds_q))) return rc;
     if ((rc = eng_upload(e, lm32, &h.lds_m32))) return rc;
     if ((rc = eng_upload(e, lm64, &h.lds_m64))) return rc;
@@ -506,11 +560,32 @@ static bool same_plan(const std::vector<Step>& a, const std::vector<Step>& b)
            (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(Step)) == 0);
 }
 
-// Issue the recorded steps on `s`; stage events only when `events`.
+// The launch step's kind as fk_batch_step_times reports it.
+static int32_t step_kind(const Step& st)
+{
+    switch (st.kind) {
+    case Step::kSieve: return FK_STEP_SIEVE;
+    case Step::kRound:
+        return st.lane_prime ? FK_STEP_ROUND_PRIME : FK_STEP_ROUND_CANDIDATE;
+    case Step::kCompact: return FK_STEP_COMPACT;
+    case Step::kGather: return FK_STEP_GATHER;
+    case Step::kClassify: return FK_STEP_CLASSIFY;
+    case Step::kReadback: return FK_STEP_READBACK;
+    default: return 0;
+    }
+}
+
+// Issue the recorded steps on `s`; stage events only when `events`, and
+// with `step_ev` (one more event than the plan has launch steps) an event
+// before the first launch and after every launch.
 static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
-                             bool events, hipStream_t s)
+                             bool events, hipStream_t s,
+                             const std::vector<hipEvent_t>* step_ev = nullptr)
 {
     hipError_t e = hipSuccess;
+    size_t k = 0;
+    if (step_ev) e = hipEventRecord((*step_ev)[k++], s);
+    if (e != hipSuccess) return e;
     for (const Step& st : plan) {
         switch (st.kind) {
         case Step::kMark:
@@ -538,6 +613,8 @@ static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
             e = launch_readback(st.rb, s);
             break;
         }
+        if (e == hipSuccess && step_ev && st.kind != Step::kMark)
+            e = hipEventRecord((*step_ev)[k++], s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -547,6 +624,11 @@ struct fk_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
+    // eager submissions: an event around every launch (grow-only), the
+    // launches' kinds and, once collected, their durations
+    std::vector<hipEvent_t> step_ev;
+    std::vector<int32_t> step_kinds;
+    std::vector<float> step_ms;
     bool pending = false;
     bool have_results = false;
     // device buffers (grow-only)
@@ -618,6 +700,8 @@ static int batch_free(fk_batch* b)
     if (b->h_counts) (void)hipHostFree(b->h_counts);
     if (b->h_recs) (void)hipHostFree(b->h_recs);
     for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
+    for (auto& e : b->step_ev) if (e) (void)hipEventDestroy(e);
+    b->step_ev.clear();
     if (b->stream) (void)hipStreamDestroy(b->stream);
     return FK_OK;
 }
@@ -784,7 +868,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             for (uint32_t c = 1; c <= kShortChunkMax; c *= 2) {
                 const uint64_t nch = (npw + c - 1) / c;
                 const uint64_t cost = nch * (kShortStartCost + c * kShortWordCost);
-                if (cost < best && (uint64_t)e->offs.size() * nch < (1ull << 32)) {
+                if (cost < best && e->nlines * nch < (1ull << 32)) {
                     best = cost;
                     wp[w].chunk = c;
                 }
@@ -858,7 +942,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             P.npos = (uint32_t)npos_all;
             P.npw = (uint32_t)((npos_all + 63) / 64);
             const uint64_t nch = (P.npw + wp[w].chunk - 1) / wp[w].chunk;
-            P.nseg = (uint32_t)(((uint64_t)e->offs.size() * nch + 255) / 256);
+            P.nseg = (uint32_t)((e->nlines * nch + 255) / 256);
             P.tile0 = (uint32_t)tiles;
             P.engine = b->engine_of[w];
             P.window = (uint32_t)w;
@@ -891,7 +975,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             P.tile0 = (uint32_t)tiles;
             P.engine = b->engine_of[w];
             P.window = (uint32_t)w;
-            const uint64_t t = (uint64_t)P.nseg * e->offs.size();
+            const uint64_t t = (uint64_t)P.nseg * e->nlines;
             if (tiles + t >= (1ull << 32))
                 return fail(FK_E_ARG, "batch too large: more than 2^32 tiles");
             tiles += t;
@@ -1298,7 +1382,15 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         HIPCHK(hipEventRecord(b->ev[5], s));
         b->prev_plan = plan;
     } else {
-        HIPCHK(emit_steps(plan, b->ev, true, s));
+        b->step_kinds.clear();
+        for (const Step& st : plan)
+            if (st.kind != Step::kMark) b->step_kinds.push_back(step_kind(st));
+        while (b->step_ev.size() < b->step_kinds.size() + 1) {
+            hipEvent_t e = nullptr;
+            HIPCHK(hipEventCreate(&e));
+            b->step_ev.push_back(e);
+        }
+        HIPCHK(emit_steps(plan, b->ev, true, s, &b->step_ev));
         b->staged = true;
     }
     b->final_slot = total_slot;
@@ -1356,10 +1448,16 @@ static int batch_collect(fk_batch* b)
     };
     b->stage_counts.assign(1, stage_total(0));
     for (uint32_t st : b->round_slots) b->stage_counts.push_back(stage_total(st));
+    b->step_ms.clear();
     if (!b->staged) {
         // graph mode records no stage events: only the whole is timed
         for (int i = 0; i < 4; i++) b->ms[i] = 0.0f;
     } else {
+        for (size_t k = 0; k < b->step_kinds.size(); k++) {
+            float t = 0.0f;
+            HIPCHK(hipEventElapsedTime(&t, b->step_ev[k], b->step_ev[k + 1]));
+            b->step_ms.push_back(t);
+        }
         HIPCHK(hipEventElapsedTime(&b->ms[0], b->ev[0], b->ev[1]));
         HIPCHK(hipEventElapsedTime(&b->ms[1], b->ev[1], b->ev[2]));
         HIPCHK(hipEventElapsedTime(&b->ms[2], b->ev[2], b->ev[3]));
@@ -1565,7 +1663,7 @@ int fk_engine_get_info(const fk_engine* e, fk_engine_info* info)
     for (size_t i = 0; i < e->wheel.size() && i < 16; i++)
         info->wheel_primes[i] = e->wheel[i];
     info->line_step = e->D;
-    info->nlines = e->offs.size();
+    info->nlines = e->nlines;
     info->W = e->W;
     info->d[0] = (uint64_t)e->dval;
     info->d[1] = (uint64_t)(e->dval >> 64);
@@ -1606,10 +1704,23 @@ int fk_engine_sweep(fk_engine* eng, const fk_window* win,
     return FK_OK;
 }
 
-// One classification pass on the device: chain values win.first ..
-// win.first + win.cap - 1 of each value, the run within the pass into runs.
-static int classify_pass(fk_engine* e, const std::vector<u128>& vals,
-                         const DevWinInfo& win, std::vector<int32_t>& runs)
+}  // extern "C"
+
+// Device buffers of one classification pass: the records, the window
+// record, the count and the two failure masks per record.
+struct ClsBuffers {
+    DevRec* recs;
+    DevWinInfo* win;
+    uint32_t* cnt;
+    uint64_t* fail;              // 2 n words
+};
+
+// One classification pass on the device in the given buffers: chain values
+// win.first .. win.first + win.cap - 1 of each value, the run within the
+// pass into runs.  Allocates no device memory.
+static int classify_in(fk_engine* e, const std::vector<u128>& vals,
+                       const DevWinInfo& win, std::vector<int32_t>& runs,
+                       const ClsBuffers& B)
 {
     const size_t n = vals.size();
     std::vector<DevRec> recs(n);
@@ -1621,46 +1732,34 @@ static int classify_pass(fk_engine* e, const std::vector<u128>& vals,
     const uint32_t cnt = (uint32_t)n;
     HIPCHK(hipSetDevice(e->device));
     hipStream_t s = e->own->stream;
-    DevRec* d_recs = nullptr;
-    DevWinInfo* d_win = nullptr;
-    uint32_t* d_cnt = nullptr;
-    uint64_t* d_f = nullptr;
-    hipError_t err = hipMalloc(&d_recs, n * sizeof(DevRec));
-    if (err == hipSuccess) err = hipMalloc(&d_win, sizeof(DevWinInfo));
-    if (err == hipSuccess) err = hipMalloc(&d_cnt, sizeof(uint32_t));
-    if (err == hipSuccess) err = hipMalloc(&d_f, 2 * n * sizeof(uint64_t));
-    if (err == hipSuccess) err = hipMemset(d_f, 0, 2 * n * sizeof(uint64_t));
+    hipError_t err = launch_zero_u64(B.fail, 2 * n, s);
     if (err == hipSuccess)
-        err = hipMemcpy(d_recs, recs.data(), n * sizeof(DevRec),
-                        hipMemcpyHostToDevice);
+        err = hipMemcpyAsync(B.recs, recs.data(), n * sizeof(DevRec),
+                             hipMemcpyHostToDevice, s);
     if (err == hipSuccess)
-        err = hipMemcpy(d_win, &win, sizeof win, hipMemcpyHostToDevice);
+        err = hipMemcpyAsync(B.win, &win, sizeof win, hipMemcpyHostToDevice, s);
     if (err == hipSuccess)
-        err = hipMemcpy(d_cnt, &cnt, sizeof cnt, hipMemcpyHostToDevice);
+        err = hipMemcpyAsync(B.cnt, &cnt, sizeof cnt, hipMemcpyHostToDevice, s);
     if (err == hipSuccess) {
         ClassifyArgs ca{};
         ca.engines = e->d_engine;
-        ca.wins = d_win;
-        ca.recs = d_recs;
-        ca.count = d_cnt;
+        ca.wins = B.win;
+        ca.recs = B.recs;
+        ca.count = B.cnt;
         ca.cap = cnt;
         ca.cap_max = (uint32_t)std::max(1, win.cap);
-        ca.fail2 = d_f;
-        ca.failx = d_f + n;
+        ca.fail2 = B.fail;
+        ca.failx = B.fail + n;
         ca.single_limit = kSingleLimit;
         const uint64_t work = n * (uint64_t)ca.cap_max *
                               (n * (uint64_t)ca.cap_max <= kSingleLimit ? 7 : 1);
         uint32_t g = (uint32_t)std::min<uint64_t>(16384, (work + 127) / 128 + 4);
         err = launch_classify(ca, g, s);
     }
-    if (err == hipSuccess) err = hipStreamSynchronize(s);
     if (err == hipSuccess)
-        err = hipMemcpy(recs.data(), d_recs, n * sizeof(DevRec),
-                        hipMemcpyDeviceToHost);
-    if (d_recs) (void)hipFree(d_recs);
-    if (d_win) (void)hipFree(d_win);
-    if (d_cnt) (void)hipFree(d_cnt);
-    if (d_f) (void)hipFree(d_f);
+        err = hipMemcpyAsync(recs.data(), B.recs, n * sizeof(DevRec),
+                             hipMemcpyDeviceToHost, s);
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
     if (err != hipSuccess)
         return fail(FK_E_DEVICE, "classification failed: %s",
                     hipGetErrorString(err));
@@ -1669,13 +1768,41 @@ static int classify_pass(fk_engine* e, const std::vector<u128>& vals,
     return FK_OK;
 }
 
+// The same pass in device buffers allocated for it.
+static int classify_pass(fk_engine* e, const std::vector<u128>& vals,
+                         const DevWinInfo& win, std::vector<int32_t>& runs)
+{
+    const size_t n = vals.size();
+    HIPCHK(hipSetDevice(e->device));
+    ClsBuffers B{nullptr, nullptr, nullptr, nullptr};
+    hipError_t err = hipMalloc(&B.recs, n * sizeof(DevRec));
+    if (err == hipSuccess) err = hipMalloc(&B.win, sizeof(DevWinInfo));
+    if (err == hipSuccess) err = hipMalloc(&B.cnt, sizeof(uint32_t));
+    if (err == hipSuccess) err = hipMalloc(&B.fail, 2 * n * sizeof(uint64_t));
+    int rc = err == hipSuccess
+                 ? classify_in(e, vals, win, runs, B)
+                 : fail(err == hipErrorOutOfMemory ? FK_E_OOM : FK_E_DEVICE,
+                        "classification buffers: %s", hipGetErrorString(err));
+    if (B.recs) (void)hipFree(B.recs);
+    if (B.win) (void)hipFree(B.win);
+    if (B.cnt) (void)hipFree(B.cnt);
+    if (B.fail) (void)hipFree(B.fail);
+    return rc;
+}
+
 // Chain values one pass covers (the per-record failure masks are 64 bits).
 static constexpr int kPassValues = 64;
 
-int fk_engine_classify(fk_engine* e, const uint64_t* values, size_t n,
-                       const fk_classify_spec* cls, int32_t* runs)
+// The chain classification behind fk_engine_classify: pass k looks at
+// chain values first + 64k ..
+// first + 64k + 63, for the values whose chain ran through the whole of
+// pass k - 1.  `pass` runs one pass on the device.
+template <class Pass>
+static int classify_chains(const uint64_t* values, size_t n,
+                           const fk_classify_spec* cls, int32_t* runs,
+                           Pass pass)
 {
-    if (!e || (!values && n) || !cls || (!runs && n))
+    if ((!values && n) || !cls || (!runs && n))
         return fail(FK_E_ARG, "null argument");
     if (n == 0) return FK_OK;
     if (cls->mode < 1 || cls->mode > 2 || cls->cap < 0 ||
@@ -1683,16 +1810,13 @@ int fk_engine_classify(fk_engine* e, const uint64_t* values, size_t n,
         return fail(FK_E_ARG, "bad classification spec (mode %d, cap %d; "
                     "cap at most %d)", cls->mode, cls->cap,
                     FK_CLASSIFY_CAP_MAX);
-    if (n > (1u << 26)) return fail(FK_E_ARG, "at most 2^26 values per call");
-    // pass k looks at chain values 64k .. 64k + 63, for the values whose
-    // chain ran through the whole of pass k - 1
+    if (cls->first < 0 || cls->first > (1 << 20))
+        return fail(FK_E_ARG, "chain index %d outside [0, 2^20]", cls->first);
     std::vector<size_t> live(n);
     for (size_t i = 0; i < n; i++) {
         live[i] = i;
         runs[i] = 0;
     }
-    if (cls->first < 0 || cls->first > (1 << 20))
-        return fail(FK_E_ARG, "chain index %d outside [0, 2^20]", cls->first);
     for (int done = 0; done < cls->cap && !live.empty(); done += kPassValues) {
         const int pcap = std::min(kPassValues, cls->cap - done);
         const int first = cls->first + done;
@@ -1701,7 +1825,7 @@ int fk_engine_classify(fk_engine* e, const uint64_t* values, size_t n,
         for (size_t j = 0; j < live.size(); j++)
             vals[j] = ((u128)values[2 * live[j] + 1] << 64) | values[2 * live[j]];
         std::vector<int32_t> got;
-        int rc = classify_pass(e, vals, win, got);
+        int rc = pass(vals, win, got);
         if (rc) return rc;
         std::vector<size_t> on;
         for (size_t j = 0; j < live.size(); j++) {
@@ -1711,6 +1835,20 @@ int fk_engine_classify(fk_engine* e, const uint64_t* values, size_t n,
         live.swap(on);
     }
     return FK_OK;
+}
+
+extern "C" {
+
+int fk_engine_classify(fk_engine* e, const uint64_t* values, size_t n,
+                       const fk_classify_spec* cls, int32_t* runs)
+{
+    if (!e) return fail(FK_E_ARG, "null argument");
+    if (n > (1u << 26)) return fail(FK_E_ARG, "at most 2^26 values per call");
+    return classify_chains(values, n, cls, runs,
+                           [&](const std::vector<u128>& v, const DevWinInfo& w,
+                               std::vector<int32_t>& r) {
+                               return classify_pass(e, v, w, r);
+                           });
 }
 
 int fk_batch_create(int device, fk_batch** out)
@@ -1805,6 +1943,23 @@ int fk_batch_counts(fk_batch* b, uint32_t* counts, size_t cap, size_t* n)
     return cap < b->stage_counts.size() && counts
                ? fail(FK_E_OVERFLOW, "%zu stage counts, room for %zu",
                       b->stage_counts.size(), cap)
+               : FK_OK;
+}
+
+int fk_batch_step_times(fk_batch* b, int32_t* kinds, float* ms, size_t cap,
+                        size_t* n)
+{
+    if (!b || !n) return fail(FK_E_ARG, "null argument");
+    if (b->pending) return fail(FK_E_BUSY, "batch has work in flight");
+    *n = b->step_ms.size();
+    const size_t m = std::min(cap, b->step_ms.size());
+    for (size_t k = 0; k < m; k++) {
+        if (kinds) kinds[k] = b->step_kinds[k];
+        if (ms) ms[k] = b->step_ms[k];
+    }
+    return cap < b->step_ms.size() && (kinds || ms)
+               ? fail(FK_E_OVERFLOW, "%zu launches, room for %zu",
+                      b->step_ms.size(), cap)
                : FK_OK;
 }
 

@@ -1657,6 +1657,51 @@ hipError_t launch_classify(const ClassifyArgs& a, uint32_t grid,
 
 // The readback kernel: one launch in place of two copies; only the records
 // the queue holds cross the bus (8-byte words, grid-stride).
+// A factored wheel expanded into its residues: for each residue b of the
+// wheel without its last prime L (period Mb), the nj residues b + j Mb with
+// j from b's class (b mod L) of the lift table, written at b's index times
+// nj.  One thread per base residue, grid-stride.
+__global__ __launch_bounds__(256) void k_wheel_expand(
+    const uint64_t* __restrict__ base, uint64_t nbase,
+    const uint32_t* __restrict__ lifts, uint32_t nj, uint32_t L, uint64_t Mb,
+    uint64_t* __restrict__ out)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+         i < nbase; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t b = base[i];
+        const uint32_t* row = lifts + (size_t)(b % L) * nj;
+        uint64_t* dst = out + i * nj;
+        for (uint32_t t = 0; t < nj; t++) dst[t] = b + (uint64_t)row[t] * Mb;
+    }
+}
+
+hipError_t launch_wheel_expand(const uint64_t* base, uint64_t nbase,
+                               const uint32_t* lifts, uint32_t nj, uint32_t L,
+                               uint64_t Mb, uint64_t* out, hipStream_t stream)
+{
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((nbase + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_wheel_expand, dim3(std::max(grid, 1u)), dim3(256), 0,
+                       stream, base, nbase, lifts, nj, L, Mb, out);
+    return hipGetLastError();
+}
+
+// Zero n words (a fill kernel rather than a memset, so that a caller may
+// capture it in a graph).
+__global__ __launch_bounds__(256) void k_zero_u64(uint64_t* p, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+hipError_t launch_zero_u64(uint64_t* p, size_t n, hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
+    const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_zero_u64, dim3(grid), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void k_readback(ReadbackArgs a)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;

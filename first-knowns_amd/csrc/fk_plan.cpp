@@ -170,4 +170,36 @@ std::vector<uint64_t> wheel_offsets(int family, int n,
     return offs;
 }
 
+uint64_t wheel_count(int family, int n, const std::vector<uint32_t>& wheel,
+                     bool* ok)
+{
+    u128 m = 1, c = 1;
+    *ok = true;
+    for (uint32_t q : wheel) {
+        c *= q - kill_set(family, n, q).size();
+        m *= q;
+        if (m >> 63) {
+            *ok = false;
+            return 0;
+        }
+    }
+    return (uint64_t)c;
+}
+
+std::vector<uint32_t> wheel_lifts(int family, int n, uint32_t L, uint64_t Mb,
+                                  uint32_t* nj)
+{
+    std::vector<uint32_t> ks = kill_set(family, n, L);
+    std::vector<uint8_t> dead(L, 0);
+    for (uint32_t r : ks) dead[r] = 1;
+    *nj = L - (uint32_t)ks.size();
+    const uint64_t mbl = Mb % L;
+    std::vector<uint32_t> lifts;
+    lifts.reserve((size_t)L * *nj);
+    for (uint32_t c = 0; c < L; c++)
+        for (uint32_t j = 0; j < L; j++)
+            if (!dead[(c + (uint64_t)j * mbl) % L]) lifts.push_back(j);
+    return lifts;
+}
+
 }  // namespace fk

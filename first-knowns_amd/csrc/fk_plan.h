@@ -46,4 +46,18 @@ std::vector<uint64_t> wheel_offsets(int family, int n,
                                     const std::vector<uint32_t>& wheel,
                                     uint64_t* modulus, bool* ok);
 
+// Number of admissible residues mod prod(wheel), from the kill-set sizes
+// (the residues themselves are not formed); *ok = false if the product of
+// the wheel primes passes 2^63.
+uint64_t wheel_count(int family, int n, const std::vector<uint32_t>& wheel,
+                     bool* ok);
+
+// The wheel factored at its last prime L: a residue base + j Mb (base an
+// admissible residue of the wheel without L, whose period is Mb; j < L) is
+// admissible iff (base + j Mb) mod L avoids L's kill set.  For each class
+// c = base mod L the admissible j, ascending, are lifts[c * nj + t],
+// t < nj, and nj = L - |kill set| is the same for every class.
+std::vector<uint32_t> wheel_lifts(int family, int n, uint32_t L, uint64_t Mb,
+                                  uint32_t* nj);
+
 }  // namespace fk

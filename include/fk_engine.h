@@ -267,6 +267,26 @@ int fk_batch_graph_stats(fk_batch* b, uint64_t* captures, uint64_t* launches);
  */
 int fk_batch_counts(fk_batch* b, uint32_t* counts, size_t cap, size_t* n);
 
+/*
+ * Per-launch durations of the last collected submission that ran eagerly
+ * (graph mode off): one entry per kernel launch in order, its kind
+ * (fk_step) and its milliseconds between HIP events recorded on the
+ * batch's stream before and after it.  *n = the number of launches (0 after
+ * a graph replay); FK_E_OVERFLOW if cap is smaller (the first cap are
+ * written).  No reference counterpart: the roofline accounting reads it.
+ */
+enum fk_step {
+    FK_STEP_SIEVE = 1,            /* k_sieve / k_sieve_short */
+    FK_STEP_ROUND_CANDIDATE = 2,  /* k_round_c: one candidate per lane */
+    FK_STEP_ROUND_PRIME = 3,      /* k_round_p: one prime per lane */
+    FK_STEP_COMPACT = 4,          /* k_round_compact */
+    FK_STEP_GATHER = 5,           /* k_gather */
+    FK_STEP_CLASSIFY = 6,         /* k_cls_base2, k_cls_full, k_cls_finish */
+    FK_STEP_READBACK = 7          /* k_readback */
+};
+int fk_batch_step_times(fk_batch* b, int32_t* kinds, float* ms, size_t cap,
+                        size_t* n);
+
 /* The engine's own internal batch, for timing its last sweep. */
 int fk_engine_batch(fk_engine* eng, fk_batch** out);
 

@@ -1,7 +1,8 @@
 """The campaign segment loop over the real engine: a campaign stopped after
 some segments and resumed from its cursor writes exactly the records of one
-sweep of the whole range (values and classified runs), and a cursor under
-another configuration is refused."""
+sweep of the whole range (values and classified runs), which are the
+oracle's survivors and runs, and a cursor under another configuration is
+refused."""
 import sys
 
 import pytest
@@ -50,6 +51,13 @@ def test_dickson_campaign_resumes_to_one_sweep(tmp_path):
     assert c2.cursor == stop_at
     st = c2.run()
     assert c2.records() == want
+    # and both are the oracle's survivors and base-2-then-full runs
+    import fk_oracle as O
+    js = O.dickson_survivors(n, lo, hi, q2=q2)
+    W = O.wheel_modulus(n)
+    assert c2.records() == [(j, O.dickson_sprp_run(j * W, spec[1],
+                                                   exact_from=spec[2]))
+                            for j in js]
     assert int(st["next"]) == hi and st["survivors"] == len(want)
     assert st["best_run"] == max(r for _, r in want)
 
@@ -73,3 +81,6 @@ def test_ap_campaign_matches_one_sweep(tmp_path):
     c = FC.Campaign(eng, lo, hi, seg, tmp_path / "ap.json", classify=spec)
     c.run()
     assert c.records() == want
+    import fk_oracle as O
+    ps = [lo + o for o in O.ap_survivors(n, lo, hi - lo, q2=q2)]
+    assert c.records() == [(p, O.ap_chain_depth(p, n)) for p in ps]

@@ -61,3 +61,24 @@ def test_euler_fingerprint(golden, label):
     eng = euler_gpu.GpuEngine(s["n"])
     got = eng.survivors_pre_mr(lo, lo + span)
     assert (len(got), xor_all(got)) == (s["count"], big(s["xor"]))
+
+
+WHEEL_31 = [2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31]
+
+
+@pytest.mark.parametrize("label", ["SCORE", "SCORE_WIDE"])
+def test_euler_fingerprint_on_the_31_wheel(golden, label):
+    """The n = 17 fingerprints on the 31# wheel (29,937,600 residues built
+    on the host) as well as on the engine's default 37# wheel (598,752,000
+    residues: the 31# ones lifted over 37 on the device,
+    euler-prime-runs/euler_search.py:113-187): the same stream."""
+    import euler_gpu
+    s = next(x for x in golden["euler_fingerprints"] if x["label"] == label)
+    lo, span = big(s["lo"]), big(s["span"])
+    say("euler fingerprint, 31# wheel", label)
+    eng = euler_gpu.GpuEngine(s["n"], wheel_primes=WHEEL_31)
+    assert eng.n_offs == 29_937_600 and eng.M == 200560490130
+    got = eng.survivors_pre_mr(lo, lo + span)
+    assert (len(got), xor_all(got)) == (s["count"], big(s["xor"]))
+    default = euler_gpu.GpuEngine(s["n"])
+    assert default.n_offs == 598_752_000 and default.M == 7420738134810

@@ -71,3 +71,37 @@ def test_window_past_one_repeat_of_the_period(harness):
     offs, words = run(harness, [(17, [3])])
     w = window(words, 0, 16)
     assert (w >> (37 - 16)) & 1 and (w >> (54 - 16)) & 1 and (w >> (71 - 16)) & 1
+
+
+@pytest.mark.parametrize("family,n,wheel", [
+    (1, 8, [2, 3, 5, 7, 11, 13]),
+    (1, 17, [2, 3, 5, 7, 11, 13, 17]),
+    (2, 10, [13, 17, 19]),
+    (3, 12, [2, 3, 5, 37, 41]),
+])
+def test_factored_wheel_is_the_wheel(harness, family, n, wheel):
+    """The factored wheel (the residues of all but the last prime, lifted
+    over the last prime as the device expansion does) is the wheel: the
+    same residues, and as many as wheel_count says; the wheel itself is
+    every r < prod(wheel) that no wheel prime kills (brute force)."""
+    import math
+    import fk_oracle as O
+    line = f"W {family} {n} " + " ".join(map(str, wheel)) + "\n"
+    out = subprocess.run([str(harness)], input=line, capture_output=True,
+                         text=True, check=True).stdout.splitlines()
+    count = int(out[0].split()[1])
+    full = [int(v) for v in out[1].split()]
+    fact = [int(v) for v in out[2].split()]
+    assert count == len(full) == len(fact) > 0
+    assert sorted(fact) == full
+    M = math.prod(wheel)
+
+    def kills(q):
+        if family == 1:
+            return {(-(x * x + x)) % q for x in range(n)}
+        if family == 2:
+            return set(O.dickson_forbidden(q, n))
+        return {(-j * O.primorial(n)) % q for j in range(n)}
+    ks = {q: kills(q) for q in wheel}
+    brute = [r for r in range(M) if all(r % q not in ks[q] for q in wheel)]
+    assert brute == full
