@@ -1793,8 +1793,8 @@ static int classify_pass(fk_engine* e, const std::vector<u128>& vals,
 // Chain values one pass covers (the per-record failure masks are 64 bits).
 static constexpr int kPassValues = 64;
 
-// The chain classification behind fk_engine_classify: pass k looks at
-// chain values first + 64k ..
+// The chain classification behind fk_engine_classify and
+// fk_engine_classify_static: pass k looks at chain values first + 64k ..
 // first + 64k + 63, for the values whose chain ran through the whole of
 // pass k - 1.  `pass` runs one pass on the device.
 template <class Pass>
@@ -1848,6 +1848,77 @@ int fk_engine_classify(fk_engine* e, const uint64_t* values, size_t n,
                            [&](const std::vector<u128>& v, const DevWinInfo& w,
                                std::vector<int32_t>& r) {
                                return classify_pass(e, v, w, r);
+                           });
+}
+
+}  // extern "C"
+
+// ------------------------------------------- classification, no allocation
+//
+// fk_engine_classify_static runs the same passes in a fixed device array:
+// per value a record and two failure masks, plus the window record and the
+// count.  Each device holds its own copy of the array for the life of the
+// process; host code reaches it only through the address looked up on that
+// device (the array's name is not a device address on the host).
+
+static constexpr size_t kStaticValues = FK_CLASSIFY_STATIC_MAX;
+static constexpr size_t kStaticHead = 256;       // DevWinInfo, then the count
+static constexpr size_t kStaticRecs = kStaticValues * sizeof(DevRec);
+static constexpr size_t kStaticBytes =
+    kStaticHead + kStaticRecs + 2 * kStaticValues * sizeof(uint64_t);
+static_assert(kStaticRecs % 256 == 0, "the masks start 256-byte aligned");
+
+static __device__ __align__(256) uint8_t scratch[kStaticBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<std::pair<int, uint8_t*>> g_scratch;   // device, address
+
+// The scratch array's address on `device` (which must be current): looked
+// up on the first call per device, kept after.
+static int scratch_for(int device, uint8_t** out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    for (const auto& d : g_scratch)
+        if (d.first == device) {
+            *out = d.second;
+            return FK_OK;
+        }
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(FK_E_DEVICE, "scratch array address on device %d: %s",
+                    device, hipGetErrorString(e));
+    g_scratch.emplace_back(device, static_cast<uint8_t*>(ptr));
+    *out = static_cast<uint8_t*>(ptr);
+    return FK_OK;
+}
+
+extern "C" {
+
+int fk_engine_classify_static(fk_engine* e, const uint64_t* values, size_t n,
+                              const fk_classify_spec* cls, int32_t* runs)
+{
+    if (!e) return fail(FK_E_ARG, "null argument");
+    if (n > kStaticValues)
+        return fail(FK_E_ARG, "%zu values: the fixed classification array "
+                    "holds %zu per call", n, kStaticValues);
+    if (n == 0)          // argument checks only: no pass runs
+        return classify_chains(values, n, cls, runs,
+                               [](const std::vector<u128>&, const DevWinInfo&,
+                                  std::vector<int32_t>&) { return FK_OK; });
+    HIPCHK(hipSetDevice(e->device));
+    uint8_t* base = nullptr;
+    int rc = scratch_for(e->device, &base);
+    if (rc) return rc;
+    ClsBuffers B;
+    B.win = reinterpret_cast<DevWinInfo*>(base);
+    B.cnt = reinterpret_cast<uint32_t*>(base + 64);
+    B.recs = reinterpret_cast<DevRec*>(base + kStaticHead);
+    B.fail = reinterpret_cast<uint64_t*>(base + kStaticHead + kStaticRecs);
+    return classify_chains(values, n, cls, runs,
+                           [&](const std::vector<u128>& v, const DevWinInfo& w,
+                               std::vector<int32_t>& r) {
+                               return classify_in(e, v, w, r, B);
                            });
 }
 

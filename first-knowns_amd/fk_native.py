@@ -23,6 +23,7 @@ CLASSIFY_NONE, CLASSIFY_FULL, CLASSIFY_TWO_PASS = 0, 1, 2
 FK_OK, FK_E_ARG, FK_E_RANGE, FK_E_OVERFLOW = 0, 1, 2, 3
 FK_E_DEVICE, FK_E_OOM, FK_E_NODEV, FK_E_BUSY = 4, 5, 6, 7
 ABI_VERSION = 2
+CLASSIFY_STATIC_MAX = 65536       # values per fk_engine_classify_static call
 
 # The exported entry points, in the order include/fk_engine.h declares them.
 EXPORTS = ("fk_abi_version", "fk_last_error", "fk_device_count",
@@ -33,7 +34,7 @@ EXPORTS = ("fk_abi_version", "fk_last_error", "fk_device_count",
            "fk_batch_graph_stats", "fk_batch_counts", "fk_engine_batch",
            "fk_batch_reruns", "fk_batch_set_budget", "fk_batch_cursor_get",
            "fk_batch_cursor_set", "fk_batch_resume",
-           "fk_batch_step_times")
+           "fk_engine_classify_static", "fk_batch_step_times")
 
 u32, i32, u64 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
 
@@ -119,6 +120,7 @@ def lib():
     L.fk_batch_cursor_get.argtypes = [vp, sz, vp]
     L.fk_batch_cursor_set.argtypes = [vp, sz, vp]
     L.fk_batch_resume.argtypes = [vp]
+    L.fk_engine_classify_static.argtypes = [vp, vp, sz, P(ClassifySpec), vp]
     L.fk_batch_step_times.argtypes = [vp, vp, vp, sz, P(sz)]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
@@ -302,6 +304,22 @@ class Engine:
         check(lib().fk_engine_classify(self._h, arr.ctypes.data, len(vals),
                                        ctypes.byref(c), runs.ctypes.data))
         return [int(r) for r in runs.tolist()]
+
+    def classify_static(self, values, mode=CLASSIFY_FULL, cap=64,
+                        exact_from=0, first=0):
+        """classify() in the library's fixed device array: no device memory
+        is allocated (at most CLASSIFY_STATIC_MAX values; calls on one
+        device must not overlap)."""
+        vals = list(values)
+        arr = np.zeros(2 * max(len(vals), 1), dtype=np.uint64)
+        for i, v in enumerate(vals):
+            arr[2 * i], arr[2 * i + 1] = split128(v)
+        runs = np.zeros(max(len(vals), 1), dtype=np.int32)
+        c = ClassifySpec(int(mode), int(cap), int(exact_from), int(first))
+        check(lib().fk_engine_classify_static(self._h, arr.ctypes.data,
+                                              len(vals), ctypes.byref(c),
+                                              runs.ctypes.data))
+        return [int(r) for r in runs.tolist()[:len(vals)]]
 
     def extend_runs(self, values, runs, swept_cap, cap, mode=CLASSIFY_FULL,
                     exact_from=0):

@@ -196,6 +196,21 @@ int fk_engine_classify(fk_engine* eng, const uint64_t* values, size_t n,
                        const fk_classify_spec* cls, int32_t* runs);
 
 /*
+ * fk_engine_classify without device allocation: the same passes and the
+ * same runs, computed in a fixed device array (2.6 MB per device, held
+ * until the process ends) instead of buffers allocated per call.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  At most FK_CLASSIFY_STATIC_MAX values per call
+ * (more: FK_E_ARG).  Every call on one device shares that device's array:
+ * calls that overlap on one device race, and must not be made.
+ */
+#define FK_CLASSIFY_STATIC_MAX 65536
+int fk_engine_classify_static(fk_engine* eng, const uint64_t* values,
+                              size_t n, const fk_classify_spec* cls,
+                              int32_t* runs);
+
+/*
  * Batched engine: one stream, device queues sized for many windows of
  * many engines at once.  Every window's tiles go into one persistent
  * sieve launch, the rounds and the classification run once over the
