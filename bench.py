@@ -19,7 +19,7 @@ Beside it, in the same JSON line:
                 stream checked equal to the GPU's
   batched       256 mixed windows of the three families (lengths 2^20..2^30,
                 n 8..14), one persistent batched launch vs the plain engine
-                window by window (two in flight), with the two streams
+                window by window (four in flight), with the two streams
                 checked equal
   serving       Poisson arrivals of requests of the three families (spans
                 2^18..2^32, n 8..14, Dickson k to 1e22), continuous batching
@@ -69,16 +69,20 @@ def sweep_window(seed):
 
 def run_plain(steps, warmup):
     """The timed loop: `steps` windows swept in order on the plain engine,
-    two in flight (Engine.sweep_many: each window one engine call on its
-    own stream and captured graph, the next enqueued before the last is
-    read back, as the reference's launch loop does).  device_ms is one
-    window's device time alone: two more windows swept one at a time
-    (graph replays) after the timed loop.  The stage split comes from
-    three more windows swept eagerly with HIP events between the stages."""
+    four in flight (Engine.sweep_many: each window one engine call on its
+    own stream, hardware queue and captured graph, the next ones enqueued
+    before the last is read back: the reference's two-stream launch loop,
+    deeper).  The warm-up sweeps at least one window per pipeline slot, so
+    that every slot has captured its launch sequence before the timed loop
+    (a capture is host work of milliseconds).  device_ms is one window's
+    device time alone: more windows swept one at a time (graph replays)
+    after the timed loop.  The stage split comes from three more windows
+    swept eagerly with HIP events between the stages."""
     import fk_native as N
     import ladder_gpu
     eng = ladder_gpu.GpuEngine(SWEEP_N, q2=SWEEP_Q2)
-    for _ in eng._eng.sweep_many([sweep_window(1000 + s) for s in range(warmup)],
+    nwarm = max(warmup, N.PIPELINE_DEPTH)
+    for _ in eng._eng.sweep_many([sweep_window(1000 + s) for s in range(nwarm)],
                                  N.CLASSIFY_FULL, SWEEP_CAP):
         pass
     wins = [sweep_window(s) for s in range(steps)]
@@ -118,7 +122,7 @@ def run_plain(steps, warmup):
     eng._eng.stage_timing(False)
     return {"eng": eng, "wall": wall, "surv": surv, "fp": fp, "stage": stage,
             "cand": cand / nst, "deepest": deepest, "device_ms": device_ms,
-            "launches": launches, "counts": counts}
+            "launches": launches, "counts": counts, "warm_windows": nwarm}
 
 
 def cpu_baseline(eng, sample_log2, threads, one_core_log2):
@@ -220,7 +224,7 @@ def run_batched(repeats=3):
     ws = [(lo, hi) for _, _, lo, hi in wins]
     specs = [batch_spec(f, n) for f, n, _, _ in wins]
     cand = sum(hi - lo for lo, hi in ws)
-    # plain: one engine call per window, two in flight (the plain engine's
+    # plain: one engine call per window, four in flight (the plain engine's
     # own pipelining, as in the headline loop); median of three passes
     pipe = N.Pipeline()
     jobs = [(engines[eo[i]], *ws[i], specs[i]) for i in range(len(ws))]
@@ -342,16 +346,17 @@ def ap_window(seed):
     return base, base + AP_SPAN
 
 
-def run_ap(steps=6, warmup=2):
+def run_ap(steps=16, warmup=4):
     """The plain engine on config 2: n = 17, d = 17#, primes 61..2^24
     sieved (7..59 kill residue 0), windows of 30 floor(2^32 / 30)
     consecutive p near 1e16, survivors classified two-pass (base 2, every
-    base from depth 6, cap 17); windows in order, two in flight."""
+    base from depth 6, cap 17); windows in order, four in flight."""
     import fk_native as N
     import fk_families as F
     eng = N.Engine(N.FAMILY_AP, AP_N, AP_Q2, **F.tuning("ap"))
     spec = (N.CLASSIFY_TWO_PASS, AP_N, 6)
-    for _ in eng.sweep_many([ap_window(100 + s) for s in range(warmup)],
+    for _ in eng.sweep_many([ap_window(100 + s)
+                             for s in range(max(warmup, N.PIPELINE_DEPTH))],
                             *spec):
         pass
     wins = [ap_window(s) for s in range(steps)]
@@ -411,12 +416,13 @@ def round_work(info, counts, growth):
     """Algorithmic work of the Dickson sweep's compaction rounds, per
     window: round r tests every candidate left after the stage before it
     (counts[r]) against every prime of [q_med g^r, q_med g^(r+1)) (the last
-    round ends at q2).  A test is the exact residue of k mod q from its
-    16-bit limbs (3 FMAs), the reduction (FMA, floor, FMA, convert, two
-    integer fixes) and the fold: 9 VALU ops; then the compares against the
-    prime's folded kill set, four at a time up to its size: the size is
-    #{m <= n : (-m | q) = 1}.  Returns (tests, VALU lane-ops, per-round
-    [(first prime, primes, candidates)])."""
+    round ends at q2).  A test is the folded residue of k (below 2^53, one
+    exact double) mod q -- multiply by 1/q, round, one FMA, convert, then
+    min(u, q - u): 6 VALU ops -- and one compare per member of the prime's
+    folded kill set, whose size is #{m <= n : (-m | q) = 1}; a prime with
+    an empty set needs no test.  The kernels compare in groups of four, so
+    this is a lower bound on their VALU work.  Returns (tests, VALU
+    lane-ops, per-round [(first prime, primes, candidates)])."""
     import numpy as np
     import fk_families as F
     ps = np.array([q for q in F.primes_below(int(info.q2))
@@ -425,7 +431,7 @@ def round_work(info, counts, growth):
     cnt = np.zeros(ps.size, dtype=np.int64)
     for m in range(1, n + 1):
         cnt += legendre_is_one((ps - np.uint64(m)) % ps, ps)
-    ops = 9 + 4 * np.maximum(1, (cnt + 3) // 4)
+    ops = np.where(cnt > 0, 6 + cnt, 0)
     edges = [int(info.q_med)]
     while edges[-1] < int(info.q2):
         edges.append(min(edges[-1] * growth, int(info.q2)))
@@ -529,7 +535,7 @@ def main():
                                "2^32 consecutive j near k=1e15 per step, "
                                "n=10, primes < 2^22 sieved, full-bases "
                                "ladder depth of every survivor; windows "
-                               "in order, two in flight",
+                               "in order, four in flight",
                    "n": SWEEP_N, "q2": SWEEP_Q2, "window_j": SWEEP_SPAN,
                    "classify_cap": SWEEP_CAP,
                    "wheel_lines": int(info.nlines),
@@ -547,10 +553,13 @@ def main():
                                "k_round_c_launches": round_launches,
                                "k_round_compact": compact_ms,
                                "sieve": sieve_ms},
+        "warmup_windows": P["warm_windows"],
         "stages_note": "per-stage and per-launch HIP events on three windows "
                        "swept eagerly after the timed loop; the timed sweeps "
-                       "keep two windows in flight, each replaying its "
-                       "slot's captured launch sequence; "
+                       "keep four windows in flight, each replaying its "
+                       "slot's captured launch sequence (the warm-up sweeps "
+                       "warmup_windows = max(warmup, 4) windows, one at "
+                       "least per slot); "
                        "device_ms_per_window_alone is one window swept by "
                        "itself (graph replay, event-timed)",
         "roofline": {"bound": "valu", "kernel": "k_round_c",

@@ -170,17 +170,16 @@ FK_HD uint32_t resid_d1(double s, double qd, double qi, uint32_t q)
 }
 
 // The folded residue min(s mod q, q - s mod q) directly, for the rows that
-// hold folded kill sets.  The quotient is the nearest integer to s * qi;
-// under resid_d1's condition (s exact, s / q < 2^48) the estimate is within
-// 2^-4 of s / q, so it is the nearest integer to s / q or one off, and
-// t = s - qt * q (exact) has |t| < 3q/2.  For u = |t| in [0, q/2] the
-// folded residue is u, for u in (q/2, 3q/2) it is |u - q|: min(u, |u - q|)
-// gives both.
+// hold folded kill sets.  The quotient qt is the nearest integer to s * qi;
+// under resid_d1's condition (s exact, s / q < 2^48) s * qi is within 2^-4
+// of s / q, so |s / q - qt| <= 1/2 + 2^-4 and t = s - qt * q (exact, one
+// fma) has |t| < q.  t is then congruent to u = |t| or to -u, and both
+// fold to min(u, q - u).
 FK_HD uint32_t resid_fold_d(double s, double qd, double qi, uint32_t q)
 {
     const double qt = rint(s * qi);
-    const uint32_t u = (uint32_t)fabs(fma(-qt, qd, s));   // exact, < 3q/2
-    const uint32_t d = u > q ? u - q : q - u;
+    const uint32_t u = (uint32_t)fabs(fma(-qt, qd, s));   // exact, < q
+    const uint32_t d = q - u;
     return u < d ? u : d;
 }
 

@@ -576,32 +576,47 @@ static int32_t step_kind(const Step& st)
 }
 
 // Issue the recorded steps on `s`; stage events only when `events`, and
-// with `step_ev` (one more event than the plan has launch steps) an event
-// before the first launch and after every launch.
+// with `step_ev` (two events per launch step) a start and a stop event for
+// every launch step.  The sieve, round and compaction kernels carry theirs
+// into the launch, which stamps them at the kernel's own start and end; the
+// other steps (several kernels or a copy) are bracketed by recorded events.
 static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
                              bool events, hipStream_t s,
                              const std::vector<hipEvent_t>* step_ev = nullptr)
 {
     hipError_t e = hipSuccess;
     size_t k = 0;
-    if (step_ev) e = hipEventRecord((*step_ev)[k++], s);
-    if (e != hipSuccess) return e;
     for (const Step& st : plan) {
-        switch (st.kind) {
-        case Step::kMark:
+        if (st.kind == Step::kMark) {
             if (events) e = hipEventRecord(ev[st.ev], s);
-            break;
+            if (e != hipSuccess) return e;
+            continue;
+        }
+        LaunchEvents le;
+        if (step_ev) {
+            le.start = (*step_ev)[2 * k];
+            le.stop = (*step_ev)[2 * k + 1];
+            k++;
+        }
+        const LaunchEvents* lp = step_ev ? &le : nullptr;
+        const bool bracket = step_ev && (st.kind == Step::kGather ||
+                                         st.kind == Step::kClassify ||
+                                         st.kind == Step::kReadback);
+        if (bracket) e = hipEventRecord(le.start, s);
+        if (e != hipSuccess) return e;
+        switch (st.kind) {
         case Step::kSieve:
             e = st.wpt == 0 ? launch_sieve_short(st.sa, st.threads, st.grid,
-                                                 st.lds, s)
+                                                 st.lds, s, lp)
                             : launch_sieve(st.sa, st.threads, st.wpt, st.grid,
-                                           st.lds, s);
+                                           st.lds, s, lp);
             break;
         case Step::kRound:
-            e = launch_round_test(st.ra, st.lane_prime != 0, st.grid, st.gy, s);
+            e = launch_round_test(st.ra, st.lane_prime != 0, st.grid, st.gy, s,
+                                  lp);
             break;
         case Step::kCompact:
-            e = launch_round_compact(st.ca, st.grid, s);
+            e = launch_round_compact(st.ca, st.grid, s, lp);
             break;
         case Step::kGather:
             e = launch_gather(st.ga, st.grid, s);
@@ -612,9 +627,10 @@ static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
         case Step::kReadback:
             e = launch_readback(st.rb, s);
             break;
+        default:
+            break;
         }
-        if (e == hipSuccess && step_ev && st.kind != Step::kMark)
-            e = hipEventRecord((*step_ev)[k++], s);
+        if (e == hipSuccess && bracket) e = hipEventRecord(le.stop, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -624,8 +640,8 @@ struct fk_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
-    // eager submissions: an event around every launch (grow-only), the
-    // launches' kinds and, once collected, their durations
+    // eager submissions: a start and a stop event for every launch step
+    // (grow-only), the steps' kinds and, once collected, their durations
     std::vector<hipEvent_t> step_ev;
     std::vector<int32_t> step_kinds;
     std::vector<float> step_ms;
@@ -770,13 +786,31 @@ static int grow_uploads(fk_batch* b, size_t neng, size_t npieces,
     return FK_OK;
 }
 
-static int create_batch(int device, fk_batch** out)
+// A batch's stream.  HIP maps streams onto a few hardware queues (four by
+// default) and two streams on one queue run one after the other; a
+// stream created with a CU mask gets a hardware queue of its own, so the
+// slots of a pipeline (dedicated) are made that way, with every CU in the
+// mask.  Such a stream synchronizes with the null stream, which the
+// engine uses only when it grows a buffer.
+static hipError_t create_stream(int device, bool dedicated, hipStream_t* s)
+{
+    if (!dedicated) return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+    int ncu = 0;
+    hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                         device);
+    if (e != hipSuccess) return e;
+    if (ncu <= 0) return hipErrorInvalidValue;
+    std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0u);
+    for (int c = 0; c < ncu; c++) mask[(size_t)c / 32] |= 1u << (c % 32);
+    return hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data());
+}
+
+static int create_batch(int device, bool dedicated, fk_batch** out)
 {
     fk_batch* b = new fk_batch();
     b->device = device;
     hipError_t e = hipSetDevice(device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream,
-                                                      hipStreamNonBlocking);
+    if (e == hipSuccess) e = create_stream(device, dedicated, &b->stream);
     for (int i = 0; i < 6 && e == hipSuccess; i++)
         e = hipEventCreate(&b->ev[i]);
     if (e != hipSuccess) {
@@ -1385,7 +1419,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         b->step_kinds.clear();
         for (const Step& st : plan)
             if (st.kind != Step::kMark) b->step_kinds.push_back(step_kind(st));
-        while (b->step_ev.size() < b->step_kinds.size() + 1) {
+        while (b->step_ev.size() < 2 * b->step_kinds.size()) {
             hipEvent_t e = nullptr;
             HIPCHK(hipEventCreate(&e));
             b->step_ev.push_back(e);
@@ -1455,7 +1489,8 @@ static int batch_collect(fk_batch* b)
     } else {
         for (size_t k = 0; k < b->step_kinds.size(); k++) {
             float t = 0.0f;
-            HIPCHK(hipEventElapsedTime(&t, b->step_ev[k], b->step_ev[k + 1]));
+            HIPCHK(hipEventElapsedTime(&t, b->step_ev[2 * k],
+                                       b->step_ev[2 * k + 1]));
             b->step_ms.push_back(t);
         }
         HIPCHK(hipEventElapsedTime(&b->ms[0], b->ev[0], b->ev[1]));
@@ -1624,7 +1659,7 @@ int fk_engine_create(const fk_engine_desc* desc, fk_engine** out)
     fk_engine* e = new fk_engine();
     e->desc = *desc;
     int rc = build_engine(e);
-    if (rc == FK_OK) rc = create_batch(e->device, &e->own);
+    if (rc == FK_OK) rc = create_batch(e->device, false, &e->own);
     // sweeps of one window shape repeat one launch sequence: the engine's
     // own batch replays it as a graph (fk_engine_batch + fk_batch_set_graph
     // turn this off, for stage timings)
@@ -1924,11 +1959,18 @@ int fk_engine_classify_static(fk_engine* e, const uint64_t* values, size_t n,
 
 int fk_batch_create(int device, fk_batch** out)
 {
+    return fk_batch_create_ex(device, 0u, out);
+}
+
+int fk_batch_create_ex(int device, uint32_t flags, fk_batch** out)
+{
     if (!out) return fail(FK_E_ARG, "null argument");
+    if (flags & ~(uint32_t)FK_BATCH_DEDICATED_QUEUE)
+        return fail(FK_E_ARG, "unknown batch flags 0x%x", flags);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
         return fail(FK_E_NODEV, "device %d not present", device);
-    return create_batch(device, out);
+    return create_batch(device, (flags & FK_BATCH_DEDICATED_QUEUE) != 0, out);
 }
 
 int fk_batch_destroy(fk_batch* b)

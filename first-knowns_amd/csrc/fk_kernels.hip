@@ -28,6 +28,7 @@
 // huntlib/primes.py:23-80.  None of them is translated: the tile layout,
 // the shared line model and the on-device classification are this build's.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <type_traits>
@@ -726,9 +727,16 @@ __device__ __forceinline__ uint64_t row_match(uint32_t r, const RowView<KG>& v,
                                               const FK_CONST uint32_t* row)
 {
     uint64_t hit = match4(r, v.k[0]);
+    // nested: a row of at most four residues costs one scalar test
+    if (v.cnt > 4u) {
+        hit |= match4(r, v.k[1]);
+        if (KG > 2 && v.cnt > 8u) {
+            hit |= match4(r, v.k[2]);
 #pragma unroll
-    for (int g = 1; g < KG; g++)
-        if (v.cnt > 4u * g) hit |= match4(r, v.k[g]);
+            for (int g = 3; g < KG; g++)
+                if (v.cnt > 4u * g) hit |= match4(r, v.k[g]);
+        }
+    }
     if (v.cnt > 4u * KG) {
         for (uint32_t c = 4u * KG; c < v.cnt; c += 4)
             hit |= match4(r, load4(row + kRowHead + c));
@@ -780,6 +788,129 @@ __device__ __forceinline__ void round_scan(const double (&a)[KC][8],
 #pragma unroll
         for (int c = 0; c < KC; c++)
             hit[c] |= row_match<KG>(row_residue<NL, FOLD>(a[c], v0, pd + p), v0, r0);
+    }
+}
+
+// ---- values below 2^53 (every live lane of the wave): the residue comes
+// from the value itself, one exact double, with no limb sum (resid_d1 and
+// resid_fold_d hold for s / q < 2^48: values below 2^53, primes of at least
+// 32), and a row needs only q, 1/q, the count and the residues.
+//
+// The loop is bound by instruction issue (VALU and SALU in about equal
+// number per test) more than by the row loads, which hit the scalar cache
+// after the block's touch; what the loads cost is the wait each one adds
+// when the compiler sinks it into the count branch that uses it.  So the
+// first 24 words of both rows of a pass (header and twelve residues) are
+// loaded by one asm block that issues the four loads back to back and
+// waits for them there (no load is left in flight past the block, and the
+// outputs never share registers with the addresses).  A table carries
+// kRowTail (32) words past its last row, so 24 words from any row start
+// stay inside it; words past a row's own stride are never compared, since
+// only residues below the row's count in groups of four are.
+typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
+
+struct NarrowRow {
+    double qd, qi;
+    uint32_t q, cnt;
+    uint32_t k[12];
+};
+
+__device__ __forceinline__ NarrowRow narrow_row(const u32x16& a, const u32x8& b)
+{
+    NarrowRow v;
+    v.qd = __hiloint2double((int)a[1], (int)a[0]);
+    v.qi = __hiloint2double((int)a[3], (int)a[2]);
+    v.q = a[10];
+    v.cnt = a[11];
+#pragma unroll
+    for (int i = 0; i < 4; i++) v.k[i] = a[12 + i];
+#pragma unroll
+    for (int i = 0; i < 8; i++) v.k[4 + i] = b[i];
+    return v;
+}
+
+// Both rows of a pass, loaded together (and one row, for an odd count).
+__device__ __forceinline__ void load_narrow_pair(const FK_CONST uint32_t* r0,
+                                                 const FK_CONST uint32_t* r1,
+                                                 NarrowRow& v0, NarrowRow& v1)
+{
+    static_assert(kRowTail >= 24, "a pass reads 24 words from a row start");
+    u32x16 a0, a1;
+    u32x8 b0, b1;
+    asm volatile("s_load_dwordx16 %0, %4, 0x0\n\t"
+                 "s_load_dwordx8 %1, %4, 0x40\n\t"
+                 "s_load_dwordx16 %2, %5, 0x0\n\t"
+                 "s_load_dwordx8 %3, %5, 0x40\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&s"(a0), "=&s"(b0), "=&s"(a1), "=&s"(b1)
+                 : "s"(r0), "s"(r1));
+    v0 = narrow_row(a0, b0);
+    v1 = narrow_row(a1, b1);
+}
+
+__device__ __forceinline__ NarrowRow load_narrow_one(const FK_CONST uint32_t* r0)
+{
+    u32x16 a0;
+    u32x8 b0;
+    asm volatile("s_load_dwordx16 %0, %2, 0x0\n\t"
+                 "s_load_dwordx8 %1, %2, 0x40\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&s"(a0), "=&s"(b0)
+                 : "s"(r0));
+    return narrow_row(a0, b0);
+}
+
+// Lanes whose residue (folded for folded rows) is one of the row's
+// residues: groups of four below the count (uniform branches), residues
+// past twelve from memory.
+template <bool FOLD>
+__device__ __forceinline__ uint64_t narrow_test(double s, const NarrowRow& v,
+                                                const FK_CONST uint32_t* row)
+{
+    const uint32_t r = FOLD ? resid_fold_d(s, v.qd, v.qi, v.q)
+                            : resid_d1(s, v.qd, v.qi, v.q);
+    // nested count tests: a row of at most four residues costs one
+    // scalar compare and branch
+    uint64_t hit = __ballot(r == v.k[0]) | __ballot(r == v.k[1]) |
+                   __ballot(r == v.k[2]) | __ballot(r == v.k[3]);
+    if (v.cnt > 4u) {
+        hit |= __ballot(r == v.k[4]) | __ballot(r == v.k[5]) |
+               __ballot(r == v.k[6]) | __ballot(r == v.k[7]);
+        if (v.cnt > 8u) {
+            hit |= __ballot(r == v.k[8]) | __ballot(r == v.k[9]) |
+                   __ballot(r == v.k[10]) | __ballot(r == v.k[11]);
+            for (uint32_t c = 12; c < v.cnt; c += 4)
+                hit |= match4(r, load4(row + kRowHead + c));
+        }
+    }
+    return hit;
+}
+
+// KC values per lane against the rows [lo, hi), two per pass.
+template <bool FOLD, int KC>
+__device__ __forceinline__ void narrow_scan(const double (&s)[KC],
+                                            const FK_CONST uint32_t* kt,
+                                            uint32_t stride, uint32_t rf,
+                                            uint32_t lo, uint32_t hi,
+                                            uint64_t (&hit)[KC])
+{
+    // running row pointers: one 64-bit add per pass, not a product
+    const FK_CONST uint32_t* r0 = kt + (size_t)(lo - rf) * stride;
+    const size_t step = 2 * (size_t)stride;
+    uint32_t p = lo;
+    for (; p + 2 <= hi; p += 2, r0 += step) {
+        const FK_CONST uint32_t* r1 = r0 + stride;
+        NarrowRow v0, v1;
+        load_narrow_pair(r0, r1, v0, v1);
+#pragma unroll
+        for (int c = 0; c < KC; c++)
+            hit[c] |= narrow_test<FOLD>(s[c], v0, r0) | narrow_test<FOLD>(s[c], v1, r1);
+    }
+    if (p < hi) {
+        const NarrowRow v0 = load_narrow_one(r0);
+#pragma unroll
+        for (int c = 0; c < KC; c++) hit[c] |= narrow_test<FOLD>(s[c], v0, r0);
     }
 }
 
@@ -889,7 +1020,7 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
         // the flags and the records are loaded together (one latency)
         uint32_t t[KC][4], idx[KC];
         bool live[KC];
-        bool any = false, wide = false;
+        bool any = false, wide = false, big = false;
 #pragma unroll
         for (int c = 0; c < KC; c++) {
             const uint32_t li = sb.lb * (256u * KC) + 256u * c + threadIdx.x;
@@ -904,8 +1035,30 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
             live[c] = flag == 0;
             any = any || live[c];
             wide = wide || (live[c] && (t[c][2] | t[c][3]) != 0);
+            big = big || (live[c] && ((t[c][2] | t[c][3]) != 0 ||
+                                      t[c][1] >= (1u << 21)));
         }
         if (!__ballot(any)) {
+            touch ^= tv;
+            continue;
+        }
+        if (ROWS == kRowsResidue && !__ballot(big) && a.q_min >= 32u) {
+            // every live value below 2^53: the value as one double
+            double s[KC];
+#pragma unroll
+            for (int c = 0; c < KC; c++)
+                s[c] = (double)(((uint64_t)t[c][1] << 32) | t[c][0]);
+            const FK_CONST uint32_t* kt = uniform_view(E->kt);
+            uint64_t m[KC];
+#pragma unroll
+            for (int c = 0; c < KC; c++) m[c] = 0;
+            if (kill_folded(E->family))
+                narrow_scan<true, KC>(s, kt, E->kstride, E->round_first, lo, hi, m);
+            else
+                narrow_scan<false, KC>(s, kt, E->kstride, E->round_first, lo, hi, m);
+#pragma unroll
+            for (int c = 0; c < KC; c++)
+                if (live[c] && ((m[c] >> __lane_id()) & 1)) a.flags[idx[c]] = 1;
             touch ^= tv;
             continue;
         }
@@ -1462,6 +1615,18 @@ __global__ __launch_bounds__(256) void k_cls_finish(ClassifyArgs a)
 
 // --------------------------------------------------------------- launchers
 
+// A launch, stamped with the caller's start/stop events when it gives them
+// (hipExtLaunchKernelGGL records them at the kernel's own start and end, so
+// no separate event packet sits between two launches).
+#define FK_LAUNCH(EV, FN, GRID, BLOCK, LDS, STREAM, ...)                     \
+    do {                                                                     \
+        if ((EV) && (EV)->start)                                             \
+            hipExtLaunchKernelGGL(FN, GRID, BLOCK, (uint32_t)(LDS), STREAM,  \
+                                  (EV)->start, (EV)->stop, 0, __VA_ARGS__);  \
+        else                                                                 \
+            hipLaunchKernelGGL(FN, GRID, BLOCK, LDS, STREAM, __VA_ARGS__);   \
+    } while (0)
+
 // Tile shapes (threads, words per thread): a tile is 64 * T * WPT line
 // positions.  The odd word counts let a window's tiles fit its lines more
 // tightly (the planner picks the shape with the fewest padded positions).
@@ -1511,15 +1676,16 @@ static const void* sieve_short_fn(int chunk)
 bool sieve_short_chunk_ok(int chunk) { return sieve_short_fn(chunk) != nullptr; }
 
 hipError_t launch_sieve_short(const SieveArgs& a, int chunk, uint32_t grid,
-                              size_t lds_bytes, hipStream_t stream)
+                              size_t lds_bytes, hipStream_t stream,
+                              const LaunchEvents* ev)
 {
     const void* fn = sieve_short_fn(chunk);
     if (!fn) return hipErrorInvalidValue;
     allow_lds(fn, lds_bytes);
 #define FK_SHORT_CASE(CC)                                                  \
     case CC:                                                               \
-        hipLaunchKernelGGL(k_sieve_short<CC>, dim3(grid), dim3(256),        \
-                           lds_bytes, stream, a);                          \
+        FK_LAUNCH(ev, k_sieve_short<CC>, dim3(grid), dim3(256), lds_bytes,  \
+                  stream, a);                                              \
         break;
     switch (chunk) {
         FK_SHORT_CASE(1) FK_SHORT_CASE(2) FK_SHORT_CASE(4) FK_SHORT_CASE(8)
@@ -1542,13 +1708,14 @@ int sieve_short_occupancy(int chunk, size_t lds_bytes)
 }
 
 hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,
-                        uint32_t grid, size_t lds_bytes, hipStream_t stream)
+                        uint32_t grid, size_t lds_bytes, hipStream_t stream,
+                        const LaunchEvents* ev)
 {
 #define FK_SIEVE_CASE(TT, WW)                                              \
     if (threads == TT && wpt == WW) {                                      \
         auto fn = k_sieve<TT, WW>;                                         \
         allow_lds((const void*)fn, lds_bytes);                             \
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(TT), lds_bytes, stream, a); \
+        FK_LAUNCH(ev, fn, dim3(grid), dim3(TT), lds_bytes, stream, a);     \
         return hipGetLastError();                                          \
     }
     FK_SIEVE_SHAPES(FK_SIEVE_CASE)
@@ -1581,7 +1748,8 @@ bool sieve_shape_ok(int threads, int wpt)
 }
 
 hipError_t launch_round_test(const RoundArgs& a, bool lane_prime, uint32_t gx,
-                             uint32_t gy, hipStream_t stream)
+                             uint32_t gy, hipStream_t stream,
+                             const LaunchEvents* ev)
 {
     const uint64_t np = a.p_end - a.p_begin;
     if (gx < 1 || gy < 1 || gy > 65535) return hipErrorInvalidValue;
@@ -1589,11 +1757,11 @@ hipError_t launch_round_test(const RoundArgs& a, bool lane_prime, uint32_t gx,
         // every prime must have its lane: 256 primes per blockIdx.x
         if ((uint64_t)gx * 256 < np) return hipErrorInvalidValue;
         if (a.rows == kRowsResidue)
-            hipLaunchKernelGGL(k_round_p<kRowsResidue>, dim3(gx, gy), dim3(256), 0,
-                               stream, a);
+            FK_LAUNCH(ev, k_round_p<kRowsResidue>, dim3(gx, gy), dim3(256), 0,
+                      stream, a);
         else if (a.rows == kRowsIndex)
-            hipLaunchKernelGGL(k_round_p<kRowsIndex>, dim3(gx, gy), dim3(256), 0,
-                               stream, a);
+            FK_LAUNCH(ev, k_round_p<kRowsIndex>, dim3(gx, gy), dim3(256), 0,
+                      stream, a);
         else
             return hipErrorInvalidValue;
     } else {
@@ -1601,14 +1769,14 @@ hipError_t launch_round_test(const RoundArgs& a, bool lane_prime, uint32_t gx,
         if (a.pblock == 0 || (uint64_t)gy * a.pblock < np)
             return hipErrorInvalidValue;
         if (a.rows == kRowsIndex)
-            hipLaunchKernelGGL((k_round_c<3, kRowsIndex>), dim3(gx, gy), dim3(256),
-                               0, stream, a);
+            FK_LAUNCH(ev, (k_round_c<3, kRowsIndex>), dim3(gx, gy), dim3(256),
+                      0, stream, a);
         else if (a.rows == kRowsResidue && a.kgroups == 3)
-            hipLaunchKernelGGL((k_round_c<3, kRowsResidue>), dim3(gx, gy), dim3(256),
-                               0, stream, a);
+            FK_LAUNCH(ev, (k_round_c<3, kRowsResidue>), dim3(gx, gy), dim3(256),
+                      0, stream, a);
         else if (a.rows == kRowsResidue && a.kgroups == 5)
-            hipLaunchKernelGGL((k_round_c<5, kRowsResidue>), dim3(gx, gy), dim3(256),
-                               0, stream, a);
+            FK_LAUNCH(ev, (k_round_c<5, kRowsResidue>), dim3(gx, gy), dim3(256),
+                      0, stream, a);
         else
             return hipErrorInvalidValue;
     }
@@ -1631,10 +1799,10 @@ hipError_t launch_kill_table(const PrimeK* primes, uint32_t p0, uint32_t np,
 }
 
 hipError_t launch_round_compact(const CompactArgs& a, uint32_t grid,
-                                hipStream_t stream)
+                                hipStream_t stream, const LaunchEvents* ev)
 {
-    hipLaunchKernelGGL(k_round_compact, dim3(grid ? grid : 1), dim3(256), 0,
-                       stream, a);
+    FK_LAUNCH(ev, k_round_compact, dim3(grid ? grid : 1), dim3(256), 0,
+              stream, a);
     return hipGetLastError();
 }
 

@@ -112,8 +112,16 @@ struct ClassifyArgs {
                               // bases run in one pass
 };
 
+// Events the launch itself stamps at its kernel's start and end (timed
+// eager submissions; null otherwise, and never under graph capture).
+struct LaunchEvents {
+    hipEvent_t start = nullptr;
+    hipEvent_t stop = nullptr;
+};
+
 hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,
-                        uint32_t grid, size_t lds_bytes, hipStream_t stream);
+                        uint32_t grid, size_t lds_bytes, hipStream_t stream,
+                        const LaunchEvents* ev = nullptr);
 int sieve_occupancy(int threads, int wpt, size_t lds_bytes);
 // windows with short lines: one lane per chunk of `chunk` words of a line
 // (1, 2, 4, 8 or 16), 256-item tiles
@@ -124,17 +132,20 @@ inline size_t short_lds(size_t nlds, size_t bm_words)
     return 4 * ((nlds + 3) & ~(size_t)3) + 4 * bm_words;
 }
 hipError_t launch_sieve_short(const SieveArgs& a, int chunk, uint32_t grid,
-                              size_t lds_bytes, hipStream_t stream);
+                              size_t lds_bytes, hipStream_t stream,
+                              const LaunchEvents* ev = nullptr);
 int sieve_short_occupancy(int chunk, size_t lds_bytes);
 bool sieve_short_chunk_ok(int chunk);
 bool sieve_shape_ok(int threads, int wpt);
 hipError_t launch_round_test(const RoundArgs& a, bool lane_prime, uint32_t gx,
-                             uint32_t gy, hipStream_t stream);
+                             uint32_t gy, hipStream_t stream,
+                             const LaunchEvents* ev = nullptr);
 hipError_t launch_kill_table(const PrimeK* primes, uint32_t p0, uint32_t np,
                              int family, int n, const uint32_t d[4], bool index,
                              uint32_t stride, uint32_t* kt, hipStream_t stream);
 hipError_t launch_round_compact(const CompactArgs& a, uint32_t grid,
-                                hipStream_t stream);
+                                hipStream_t stream,
+                                const LaunchEvents* ev = nullptr);
 hipError_t launch_readback(const ReadbackArgs& a, hipStream_t stream);
 hipError_t launch_gather(const GatherArgs& a, uint32_t grid,
                          hipStream_t stream);

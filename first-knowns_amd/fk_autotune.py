@@ -25,8 +25,10 @@ Coordinate descent from the current tuning (the saved one, else the family
 defaults): each pass tries every value of one parameter with the others
 held, keeps the fastest, and moves on; the result replaces the starting
 point only if it is faster again when both are re-measured.
-Timing is the device time of whole sweeps (HIP events around the batch)
-over a representative window, median of `reps`.  The result is written to
+Timing is the wall time per window of a pipelined run over consecutive
+representative windows (as the engine sweeps a range, PIPELINE_DEPTH
+windows in flight), median of `reps`; --alone times one window's device
+time instead (HIP events around the batch).  The result is written to
 fk_tuning.json next to this file, which fk_families.tuning() reads.
 
 python fk_autotune.py --family dickson --n 10 --q2 4194304 \
@@ -67,9 +69,13 @@ VALID_SHAPES = {(256, 1), (256, 2), (256, 3), (256, 4), (256, 6), (256, 8),
 
 
 def measure(family, n, q2, lo, span, params, reps=3, classify=None,
-            wheel_primes=None):
-    """Median device milliseconds of one sweep, or None if the
-    configuration is not buildable (LDS, shape, wheel size)."""
+            wheel_primes=None, pipelined=True):
+    """Median milliseconds per window, or None if the configuration is not
+    buildable (LDS, shape, wheel size).  pipelined: the wall time per
+    window of 4 * PIPELINE_DEPTH consecutive windows swept as the engine
+    sweeps a range (Engine.sweep_many, that many windows in flight), after
+    one pass that captures the slots' launch sequences; else the device
+    time of one window swept alone (HIP events around the batch)."""
     if (params["threads"], params["words_per_thread"]) not in VALID_SHAPES:
         return None, None
     kw = dict(params)
@@ -84,7 +90,19 @@ def measure(family, n, q2, lo, span, params, reps=3, classify=None,
     try:
         eng.sweep(lo, lo + span, mode, cap, ef)       # warm
         times, stages = [], []
+        if pipelined:
+            wins = [(lo + k * span, lo + (k + 1) * span)
+                    for k in range(4 * N.PIPELINE_DEPTH)]
+            for _ in eng.sweep_many(wins[:N.PIPELINE_DEPTH], mode, cap, ef):
+                pass
         for _ in range(reps):
+            if pipelined:
+                t0 = time.perf_counter()
+                for _ in eng.sweep_many(wins, mode, cap, ef):
+                    pass
+                times.append(1e3 * (time.perf_counter() - t0) / len(wins))
+                stages.append(None)
+                continue
             eng.sweep(lo, lo + span, mode, cap, ef)
             ms, _, _ = eng.last_timing()
             times.append(ms[4])
@@ -98,7 +116,7 @@ def measure(family, n, q2, lo, span, params, reps=3, classify=None,
 
 
 def tune(family, n, q2, lo, span, passes=2, reps=3, classify=None,
-         wheel_primes=None, log=print):
+         wheel_primes=None, log=print, pipelined=True):
     cur = dict(F.tuning(family))
     cur.setdefault("nwheel", 0)
     cur.setdefault("round_growth", 2)
@@ -106,7 +124,7 @@ def tune(family, n, q2, lo, span, passes=2, reps=3, classify=None,
     cur.setdefault("q_bits", 0)
     cur.pop("round_lane_prime", None)
     best, st = measure(family, n, q2, lo, span, cur, reps, classify,
-                       wheel_primes)
+                       wheel_primes, pipelined)
     log(f"start {cur} -> {best:.3f} ms {st}")
     start = dict(cur)
     keys = ["q_med", "q_bits", "q_mark", "q_pat", "threads",
@@ -126,7 +144,7 @@ def tune(family, n, q2, lo, span, passes=2, reps=3, classify=None,
                     bits = cur["threads"] * cur["words_per_thread"]
                     trial["words_per_thread"] = max(1, bits // val)
                 t, st = measure(family, n, q2, lo, span, trial, reps,
-                                classify, wheel_primes)
+                                classify, wheel_primes, pipelined)
                 log(f"  {key}={val}: "
                     + ("unbuildable" if t is None else f"{t:.3f} ms {st}"))
                 if t is not None and t < best * 0.98:
@@ -138,9 +156,9 @@ def tune(family, n, q2, lo, span, passes=2, reps=3, classify=None,
         # the winner must beat the starting point again, at three times the
         # repetitions, before it replaces it
         t_new, _ = measure(family, n, q2, lo, span, cur, 3 * reps, classify,
-                           wheel_primes)
+                           wheel_primes, pipelined)
         t_old, _ = measure(family, n, q2, lo, span, start, 3 * reps, classify,
-                           wheel_primes)
+                           wheel_primes, pipelined)
         log(f"confirm: start {t_old:.3f} ms, best {t_new:.3f} ms")
         if t_old is not None and (t_new is None or t_new > 0.99 * t_old):
             cur, best = start, t_old
@@ -173,13 +191,16 @@ def main():
     ap.add_argument("--cap", type=int, default=0)
     ap.add_argument("--wheel", type=str, default="")
     ap.add_argument("--save", action="store_true")
+    ap.add_argument("--alone", action="store_true",
+                    help="time one window swept alone, not a pipelined run")
     args = ap.parse_args()
     cls = (N.CLASSIFY_FULL, args.cap, 0) if args.cap else None
     wheel = [int(x) for x in args.wheel.split(",") if x] or None
     t0 = time.time()
     params, best = tune(args.family, args.n, args.q2, args.lo, args.span,
                         args.passes, args.reps, cls, wheel,
-                        log=lambda s: print(s, flush=True))
+                        log=lambda s: print(s, flush=True),
+                        pipelined=not args.alone)
     print(json.dumps({"family": args.family, "params": params,
                       "ms": best, "tuning_s": time.time() - t0}), flush=True)
     if args.save:

@@ -24,12 +24,14 @@ FK_OK, FK_E_ARG, FK_E_RANGE, FK_E_OVERFLOW = 0, 1, 2, 3
 FK_E_DEVICE, FK_E_OOM, FK_E_NODEV, FK_E_BUSY = 4, 5, 6, 7
 ABI_VERSION = 2
 CLASSIFY_STATIC_MAX = 65536       # values per fk_engine_classify_static call
+BATCH_DEDICATED_QUEUE = 1         # fk_batch_create_ex: a hardware queue of its own
 
 # The exported entry points, in the order include/fk_engine.h declares them.
 EXPORTS = ("fk_abi_version", "fk_last_error", "fk_device_count",
            "fk_engine_create", "fk_engine_destroy", "fk_engine_get_info",
            "fk_engine_sweep", "fk_engine_classify", "fk_batch_create",
-           "fk_batch_destroy", "fk_batch_submit", "fk_batch_query",
+           "fk_batch_destroy", "fk_batch_create_ex", "fk_batch_submit",
+           "fk_batch_query",
            "fk_batch_results", "fk_batch_timing", "fk_batch_set_graph",
            "fk_batch_graph_stats", "fk_batch_counts", "fk_engine_batch",
            "fk_batch_reruns", "fk_batch_set_budget", "fk_batch_cursor_get",
@@ -107,6 +109,7 @@ def lib():
     L.fk_engine_classify.argtypes = [vp, vp, sz, P(ClassifySpec), vp]
     L.fk_batch_create.argtypes = [ctypes.c_int, P(vp)]
     L.fk_batch_destroy.argtypes = [vp]
+    L.fk_batch_create_ex.argtypes = [ctypes.c_int, u32, P(vp)]
     L.fk_batch_submit.argtypes = [vp, vp, sz, vp, vp, vp, sz]
     L.fk_batch_query.argtypes = [vp, P(ctypes.c_int)]
     L.fk_batch_results.argtypes = [vp, vp, sz, P(sz), vp]
@@ -269,8 +272,8 @@ class Engine:
 
     def sweep_many(self, windows, mode=CLASSIFY_NONE, cap=0, exact_from=0):
         """Yield the sorted records of each window [lo, hi) of `windows` in
-        order, two windows in flight on two streams (Pipeline): the same
-        records as sweep() window by window."""
+        order, PIPELINE_DEPTH windows in flight on as many streams
+        (Pipeline): the same records as sweep() window by window."""
         ws = [(int(lo), int(hi)) for lo, hi in windows]
         if getattr(self, "_pipe", None) is None:
             self._pipe = Pipeline(self.device)
@@ -402,9 +405,13 @@ def batch_timing(handle):
 class Batch:
     """An fk_batch: many windows of many engines, submitted asynchronously."""
 
-    def __init__(self, device=0, graph=False):
+    def __init__(self, device=0, graph=False, dedicated=False):
+        """dedicated: the batch's stream gets a hardware queue of its own
+        (FK_BATCH_DEDICATED_QUEUE), as the slots of a Pipeline do."""
         h = ctypes.c_void_p()
-        check(lib().fk_batch_create(int(device), ctypes.byref(h)))
+        check(lib().fk_batch_create_ex(int(device),
+                                       BATCH_DEDICATED_QUEUE if dedicated else 0,
+                                       ctypes.byref(h)))
         self._h = h
         self._nwin = 0
         if graph:
@@ -495,16 +502,24 @@ class Batch:
         return batch_counts(self._h)
 
 
-class Pipeline:
-    """Windows swept one engine call each, `depth` (2) in flight: window i
-    is enqueued on its own batch (its own stream, queues and captured
-    graph) before window i - 1 is read back, so the device starts the next
-    window's sieve while the last one's rounds and classification (few
-    candidates, few busy CUs) finish, and never waits on the host.  The
-    launch loop of dickson-ladders/ladder_gpu.py:717-747 (two CUDA streams,
-    chain i + 1 enqueued before chain i is read back)."""
+# Windows in flight: HIP gives a process four hardware queues by default,
+# so four streams run their kernels side by side; a window's rounds and
+# classification are latency-bound launches over few candidates, and the
+# other windows' launches fill the device meanwhile.
+PIPELINE_DEPTH = 4
 
-    def __init__(self, device=0, depth=2):
+
+class Pipeline:
+    """Windows swept one engine call each, `depth` (PIPELINE_DEPTH) in
+    flight: window i is enqueued on its own batch (its own stream, queues
+    and captured graph) before window i - depth + 1 is read back, so the
+    device works on the next windows while the last ones' rounds and
+    classification (few candidates, few busy CUs) finish, and never waits
+    on the host.  The launch loop of dickson-ladders/ladder_gpu.py:717-747
+    (two CUDA streams, chain i + 1 enqueued before chain i is read back),
+    deeper."""
+
+    def __init__(self, device=0, depth=PIPELINE_DEPTH):
         self.device, self.depth = int(device), int(depth)
         self._slots = []
         self._graph = None
@@ -513,7 +528,9 @@ class Pipeline:
     def _ready(self, graph):
         if self._graph != graph:
             self.close()
-            self._slots = [Batch(self.device, graph=graph)
+            # one hardware queue per slot: slots that shared one would
+            # run their windows in turn
+            self._slots = [Batch(self.device, graph=graph, dedicated=True)
                            for _ in range(self.depth)]
             self._graph = graph
 

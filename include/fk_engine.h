@@ -221,6 +221,18 @@ int fk_batch_create(int device, fk_batch** out);
 int fk_batch_destroy(fk_batch* b);
 
 /*
+ * fk_batch_create with flags.  FK_BATCH_DEDICATED_QUEUE puts the batch's
+ * stream on a hardware queue of its own (a CU-masked stream, every CU
+ * enabled): HIP maps ordinary streams onto a few shared hardware queues,
+ * and batches that share one run in turn, so the slots of a pipeline of
+ * windows in flight are created this way.  Such a stream synchronizes with
+ * the null stream.  No reference counterpart (the reference's two CUDA
+ * streams, dickson-ladders/ladder_gpu.py:717-747, always run side by side).
+ */
+#define FK_BATCH_DEDICATED_QUEUE 1u
+int fk_batch_create_ex(int device, uint32_t flags, fk_batch** out);
+
+/*
  * Enqueue windows (window i runs on engines[engine_of[i]] with cls[i]) and
  * return without waiting.  The sieve workgroup size, the longest tile and
  * the round-kernel crossover come from the engine with the most candidates

@@ -68,3 +68,15 @@ def test_drop_in_modules_refuse_bad_parameters_before_the_device():
         ladder_gpu.GpuEngine(10, q2=2)
     with pytest.raises(ValueError):
         ap_gpu.GpuEngine(16, q2=5)
+
+
+def test_batch_flags_are_checked_before_the_device():
+    """fk_batch_create_ex refuses flags it does not know (FK_E_ARG) without
+    touching a device; the dedicated-queue flag matches the header."""
+    text = HEADER.read_text()
+    m = re.search(r"#define FK_BATCH_DEDICATED_QUEUE (\d+)u", text)
+    assert int(m.group(1)) == N.BATCH_DEDICATED_QUEUE
+    h = ctypes.c_void_p()
+    rc = N.lib().fk_batch_create_ex(0, 2, ctypes.byref(h))
+    assert rc == N.FK_E_ARG
+    assert not h.value

@@ -103,7 +103,7 @@ def test_engine_sweeps_replay_equal_eager():
 
 
 def test_pipelined_sweeps_equal_plain():
-    """Engine.sweep_many keeps two windows in flight on two batches: equal
+    """Engine.sweep_many keeps four windows in flight on four batches: equal
     windows (each slot captures its launch sequence at once and replays
     it), windows of mixed lengths (eager until a sequence repeats), a
     caller that stops early, and a mixed-engine Pipeline; every window's
@@ -123,8 +123,9 @@ def test_pipelined_sweeps_equal_plain():
             assert N.record_values(recs) == N.record_values(plain), (name, lo)
             assert recs["run"].tolist() == plain["run"].tolist()
         if name == "equal":
-            # slot 0 swept windows 0, 2, 4, 6: captured at once, replayed
-            assert eng._pipe._slots[0].graph_stats() == (1, 4)
+            # slot 0 swept windows 0, d, 2d, ...: captured at once, replayed
+            d = eng._pipe.depth
+            assert eng._pipe._slots[0].graph_stats() == (1, len(range(0, len(wins), d)))
     say("pipelined early stop")
     it = eng.sweep_many(equal, *SPEC)
     first = next(it)
