@@ -663,13 +663,6 @@ __device__ __forceinline__ double load_f64(const FK_CONST uint32_t* p)
     return __hiloint2double((int)p[1], (int)p[0]);
 }
 
-// Lanes whose residue r matches one of the four words.
-__device__ __forceinline__ uint64_t match4(uint32_t r, uint4 v)
-{
-    return __ballot(r == v.x) | __ballot(r == v.y) | __ballot(r == v.z) |
-           __ballot(r == v.w);
-}
-
 // ---- lane = candidate (many candidates, few primes): the prime is
 // wave-uniform, its row is read with scalar loads into SGPRs.
 
@@ -718,30 +711,37 @@ __device__ __forceinline__ uint32_t row_residue(const double (&a)[8],
     return FOLD ? resid_fold_d(s, v.qd, v.qi, v.q) : resid_d1(s, v.qd, v.qi, v.q);
 }
 
-// Lanes whose residue matches the row's kill set: only the four-residue
-// groups the row's count reaches (uniform branches), the first 4 KG
-// residues from registers, longer rows in a loop.  (Folded rows get a
-// folded residue from row_residue.)
-template <int KG>
-__device__ __forceinline__ uint64_t row_match(uint32_t r, const RowView<KG>& v,
-                                              const FK_CONST uint32_t* row)
+// The lane's residue against the row's kill set, kept in a per-lane
+// minimum of r ^ k that falls to zero on a match (see narrow_test): only
+// the four-residue groups the row's count reaches (nested uniform
+// branches), the first 4 KG residues from registers, longer rows in a
+// loop.  (Folded rows get a folded residue from row_residue.)
+__device__ __forceinline__ uint32_t min4x(uint32_t acc, uint32_t r, uint4 k)
 {
-    uint64_t hit = match4(r, v.k[0]);
-    // nested: a row of at most four residues costs one scalar test
+    acc = min(min(acc, r ^ k.x), r ^ k.y);         // v_min3_u32
+    return min(min(acc, r ^ k.z), r ^ k.w);
+}
+
+template <int KG>
+__device__ __forceinline__ uint32_t row_min(uint32_t acc, uint32_t r,
+                                            const RowView<KG>& v,
+                                            const FK_CONST uint32_t* row)
+{
+    acc = min4x(acc, r, v.k[0]);
     if (v.cnt > 4u) {
-        hit |= match4(r, v.k[1]);
+        acc = min4x(acc, r, v.k[1]);
         if (KG > 2 && v.cnt > 8u) {
-            hit |= match4(r, v.k[2]);
+            acc = min4x(acc, r, v.k[2]);
 #pragma unroll
             for (int g = 3; g < KG; g++)
-                if (v.cnt > 4u * g) hit |= match4(r, v.k[g]);
+                if (v.cnt > 4u * g) acc = min4x(acc, r, v.k[g]);
         }
     }
     if (v.cnt > 4u * KG) {
         for (uint32_t c = 4u * KG; c < v.cnt; c += 4)
-            hit |= match4(r, load4(row + kRowHead + c));
+            acc = min4x(acc, r, load4(row + kRowHead + c));
     }
-    return hit;
+    return acc;
 }
 
 // KC candidates per lane against the primes [lo, hi), two primes per pass;
@@ -754,6 +754,9 @@ __device__ __forceinline__ void round_scan(const double (&a)[KC][8],
                                            uint32_t lo, uint32_t hi,
                                            uint64_t (&hit)[KC])
 {
+    uint32_t acc[KC];
+#pragma unroll
+    for (int c = 0; c < KC; c++) acc[c] = 0xffffffffu;
     uint32_t p = lo;
     for (; p + 1 < hi; p += 2) {
         const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * stride;
@@ -779,16 +782,17 @@ __device__ __forceinline__ void round_scan(const double (&a)[KC][8],
         }
 #pragma unroll
         for (int c = 0; c < KC; c++)
-            hit[c] |= row_match<KG>(x0[c], v0, r0) |
-                      row_match<KG>(x1[c], v1, r1);
+            acc[c] = row_min<KG>(row_min<KG>(acc[c], x0[c], v0, r0), x1[c], v1, r1);
     }
     if (p < hi) {
         const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * stride;
         const RowView<KG> v0 = load_row<KG>(r0);
 #pragma unroll
         for (int c = 0; c < KC; c++)
-            hit[c] |= row_match<KG>(row_residue<NL, FOLD>(a[c], v0, pd + p), v0, r0);
+            acc[c] = row_min<KG>(acc[c], row_residue<NL, FOLD>(a[c], v0, pd + p), v0, r0);
     }
+#pragma unroll
+    for (int c = 0; c < KC; c++) hit[c] |= __ballot(acc[c] == 0u);
 }
 
 // ---- values below 2^53 (every live lane of the wave): the residue comes
@@ -861,33 +865,39 @@ __device__ __forceinline__ NarrowRow load_narrow_one(const FK_CONST uint32_t* r0
     return narrow_row(a0, b0);
 }
 
-// Lanes whose residue (folded for folded rows) is one of the row's
-// residues: groups of four below the count (uniform branches), residues
-// past twelve from memory.
+// The lane's residue (folded for folded rows) against the row's residues,
+// kept in a per-lane minimum of r ^ k that falls to zero when r is one of
+// them (padding kKillPad never matches a residue below 2^30): exclusive
+// ors and three-way minima on the vector unit, where ballots would cost a
+// scalar OR per compare and the scalar unit was the busier of the two.
+// Groups of four below the count, nested, so a row of at most four
+// residues costs one scalar compare and branch.
+__device__ __forceinline__ uint32_t min4x(uint32_t acc, uint32_t r, uint32_t a,
+                                         uint32_t b, uint32_t c, uint32_t d)
+{
+    return min4x(acc, r, make_uint4(a, b, c, d));
+}
+
 template <bool FOLD>
-__device__ __forceinline__ uint64_t narrow_test(double s, const NarrowRow& v,
-                                                const FK_CONST uint32_t* row)
+__device__ __forceinline__ uint32_t narrow_test(double s, const NarrowRow& v,
+                                                const FK_CONST uint32_t* row,
+                                                uint32_t acc)
 {
     const uint32_t r = FOLD ? resid_fold_d(s, v.qd, v.qi, v.q)
                             : resid_d1(s, v.qd, v.qi, v.q);
-    // nested count tests: a row of at most four residues costs one
-    // scalar compare and branch
-    uint64_t hit = __ballot(r == v.k[0]) | __ballot(r == v.k[1]) |
-                   __ballot(r == v.k[2]) | __ballot(r == v.k[3]);
+    acc = min4x(acc, r, v.k[0], v.k[1], v.k[2], v.k[3]);
     if (v.cnt > 4u) {
-        hit |= __ballot(r == v.k[4]) | __ballot(r == v.k[5]) |
-               __ballot(r == v.k[6]) | __ballot(r == v.k[7]);
+        acc = min4x(acc, r, v.k[4], v.k[5], v.k[6], v.k[7]);
         if (v.cnt > 8u) {
-            hit |= __ballot(r == v.k[8]) | __ballot(r == v.k[9]) |
-                   __ballot(r == v.k[10]) | __ballot(r == v.k[11]);
-            for (uint32_t c = 12; c < v.cnt; c += 4)
-                hit |= match4(r, load4(row + kRowHead + c));
+            acc = min4x(acc, r, v.k[8], v.k[9], v.k[10], v.k[11]);
+            for (uint32_t c = 12; c < v.cnt; c++) acc = min(acc, r ^ row[kRowHead + c]);
         }
     }
-    return hit;
+    return acc;
 }
 
-// KC values per lane against the rows [lo, hi), two per pass.
+// KC values per lane against the rows [lo, hi), two per pass; hit[c]
+// gets the lanes whose c-th value some row kills.
 template <bool FOLD, int KC>
 __device__ __forceinline__ void narrow_scan(const double (&s)[KC],
                                             const FK_CONST uint32_t* kt,
@@ -895,6 +905,9 @@ __device__ __forceinline__ void narrow_scan(const double (&s)[KC],
                                             uint32_t lo, uint32_t hi,
                                             uint64_t (&hit)[KC])
 {
+    uint32_t acc[KC];
+#pragma unroll
+    for (int c = 0; c < KC; c++) acc[c] = 0xffffffffu;
     // running row pointers: one 64-bit add per pass, not a product
     const FK_CONST uint32_t* r0 = kt + (size_t)(lo - rf) * stride;
     const size_t step = 2 * (size_t)stride;
@@ -905,13 +918,15 @@ __device__ __forceinline__ void narrow_scan(const double (&s)[KC],
         load_narrow_pair(r0, r1, v0, v1);
 #pragma unroll
         for (int c = 0; c < KC; c++)
-            hit[c] |= narrow_test<FOLD>(s[c], v0, r0) | narrow_test<FOLD>(s[c], v1, r1);
+            acc[c] = narrow_test<FOLD>(s[c], v1, r1, narrow_test<FOLD>(s[c], v0, r0, acc[c]));
     }
     if (p < hi) {
         const NarrowRow v0 = load_narrow_one(r0);
 #pragma unroll
-        for (int c = 0; c < KC; c++) hit[c] |= narrow_test<FOLD>(s[c], v0, r0);
+        for (int c = 0; c < KC; c++) acc[c] = narrow_test<FOLD>(s[c], v0, r0, acc[c]);
     }
+#pragma unroll
+    for (int c = 0; c < KC; c++) hit[c] |= __ballot(acc[c] == 0u);
 }
 
 // ---- index rows (AP): the lane's t = -p / d mod q from the row's linear
@@ -938,9 +953,12 @@ __device__ __forceinline__ IdxView load_idx(const FK_CONST uint32_t* row)
 
 // Each product a_k u_k is below 2^46 (q < 2^30) and the sum of eight below
 // 2^49, all exact in a double; s / q < 2^19 keeps resid_d1 exact.
+// t - n for the lane's value: each product a_k u_k is below 2^46 (q <
+// 2^30) and the sum of eight below 2^49, all exact in a double; s / q <
+// 2^19 keeps resid_d1 exact.
 template <int NL>
-__device__ __forceinline__ bool idx_kills(const double (&a)[8], const IdxView& v,
-                                          const FK_CONST uint32_t* row)
+__device__ __forceinline__ int32_t idx_t(const double (&a)[8], const IdxView& v,
+                                         const FK_CONST uint32_t* row)
 {
     double s = fma(a[3], v.u3, fma(a[2], v.u2, fma(a[1], v.u1, a[0] * v.u0)));
     if (NL > 4) {
@@ -948,11 +966,13 @@ __device__ __forceinline__ bool idx_kills(const double (&a)[8], const IdxView& v
         for (int k = 4; k < NL; k++)
             s = fma(a[k], load_f64(row + kIndexHead + 2 * (k - 4)), s);
     }
-    return resid_d1(s, v.qd, v.qi, v.q) < v.n;
+    return (int32_t)resid_d1(s, v.qd, v.qi, v.q) - (int32_t)v.n;
 }
 
 // KC candidates per lane against the index rows [lo, hi): R rows per pass
 // (their scalar loads in flight together), four for values below 2^64.
+// The row kills the lane when t - n < 0 (t < 2^30, n small): a per-lane
+// minimum of t - n on the vector unit, one ballot at the end.
 template <int NL, int KC>
 __device__ __forceinline__ void index_scan(const double (&a)[KC][8],
                                            const FK_CONST uint32_t* kt,
@@ -960,6 +980,9 @@ __device__ __forceinline__ void index_scan(const double (&a)[KC][8],
                                            uint64_t (&hit)[KC])
 {
     constexpr uint32_t R = NL > 4 ? 2 : 4;
+    int32_t acc[KC];
+#pragma unroll
+    for (int c = 0; c < KC; c++) acc[c] = 0x7fffffff;
     uint32_t p = lo;
     for (; p + R <= hi; p += R) {
         const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * kIndexStride;
@@ -970,14 +993,16 @@ __device__ __forceinline__ void index_scan(const double (&a)[KC][8],
         for (uint32_t i = 0; i < R; i++)
 #pragma unroll
             for (int c = 0; c < KC; c++)
-                hit[c] |= __ballot(idx_kills<NL>(a[c], v[i], r0 + i * kIndexStride));
+                acc[c] = min(acc[c], idx_t<NL>(a[c], v[i], r0 + i * kIndexStride));
     }
     for (; p < hi; p++) {
         const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * kIndexStride;
         const IdxView v = load_idx(r0);
 #pragma unroll
-        for (int c = 0; c < KC; c++) hit[c] |= __ballot(idx_kills<NL>(a[c], v, r0));
+        for (int c = 0; c < KC; c++) acc[c] = min(acc[c], idx_t<NL>(a[c], v, r0));
     }
+#pragma unroll
+    for (int c = 0; c < KC; c++) hit[c] |= __ballot(acc[c] < 0);
 }
 
 // One round, lane = candidate: blockIdx.x walks the engines' segments in
