@@ -844,9 +844,11 @@ static double round_survival(const fk_engine* e, double lo, double hi)
 // one pass
 static constexpr uint32_t kSingleLimit = 8192;
 
-// short-line sieve: chunk sizes 1, 2, 4 .. kShortChunkMax words per lane,
+// short-line sieve: chunk sizes up to kShortChunkMax words per lane (3, 6
+// and 12 divide lines of 24 words, which a 2^32-j Dickson window has),
 // chosen per window by instructions per lane: about kShortStartCost to form
 // a chunk's start in a prime's coordinates, kShortWordCost per word
+static constexpr uint32_t kShortChunks[] = {1, 2, 3, 4, 6, 8, 12, 16};
 static constexpr uint64_t kShortStartCost = 28, kShortWordCost = 5;
 
 // lane-per-candidate rounds: workgroups to aim for (candidate blocks times
@@ -899,7 +901,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         wp[w].chunk = 0;
         if (npw <= e->short_words) {
             uint64_t best = ~0ull;
-            for (uint32_t c = 1; c <= kShortChunkMax; c *= 2) {
+            for (uint32_t c : kShortChunks) {
                 const uint64_t nch = (npw + c - 1) / c;
                 const uint64_t cost = nch * (kShortStartCost + c * kShortWordCost);
                 if (cost < best && e->nlines * nch < (1ull << 32)) {

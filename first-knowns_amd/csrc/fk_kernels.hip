@@ -1691,8 +1691,11 @@ static const void* sieve_short_fn(int chunk)
     switch (chunk) {
     case 1: return (const void*)k_sieve_short<1>;
     case 2: return (const void*)k_sieve_short<2>;
+    case 3: return (const void*)k_sieve_short<3>;
     case 4: return (const void*)k_sieve_short<4>;
+    case 6: return (const void*)k_sieve_short<6>;
     case 8: return (const void*)k_sieve_short<8>;
+    case 12: return (const void*)k_sieve_short<12>;
     case 16: return (const void*)k_sieve_short<16>;
     default: return nullptr;
     }
@@ -1713,8 +1716,8 @@ hipError_t launch_sieve_short(const SieveArgs& a, int chunk, uint32_t grid,
                   stream, a);                                              \
         break;
     switch (chunk) {
-        FK_SHORT_CASE(1) FK_SHORT_CASE(2) FK_SHORT_CASE(4) FK_SHORT_CASE(8)
-        FK_SHORT_CASE(16)
+        FK_SHORT_CASE(1) FK_SHORT_CASE(2) FK_SHORT_CASE(3) FK_SHORT_CASE(4)
+        FK_SHORT_CASE(6) FK_SHORT_CASE(8) FK_SHORT_CASE(12) FK_SHORT_CASE(16)
     }
 #undef FK_SHORT_CASE
     return hipGetLastError();
