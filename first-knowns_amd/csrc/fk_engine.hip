@@ -646,6 +646,7 @@ struct fk_batch {
     std::vector<int32_t> step_kinds;
     std::vector<float> step_ms;
     bool pending = false;
+    bool throughput = false;       // FK_BATCH_THROUGHPUT
     bool have_results = false;
     // device buffers (grow-only)
     // the per-submission descriptors live in one device area, uploaded in
@@ -805,10 +806,12 @@ static hipError_t create_stream(int device, bool dedicated, hipStream_t* s)
     return hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data());
 }
 
-static int create_batch(int device, bool dedicated, fk_batch** out)
+static int create_batch(int device, uint32_t flags, fk_batch** out)
 {
+    const bool dedicated = (flags & FK_BATCH_DEDICATED_QUEUE) != 0;
     fk_batch* b = new fk_batch();
     b->device = device;
+    b->throughput = (flags & FK_BATCH_THROUGHPUT) != 0;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = create_stream(device, dedicated, &b->stream);
     for (int i = 0; i < 6 && e == hipSuccess; i++)
@@ -852,8 +855,17 @@ static constexpr uint32_t kShortChunks[] = {1, 2, 3, 4, 6, 8, 12, 16};
 static constexpr uint64_t kShortStartCost = 28, kShortWordCost = 5;
 
 // lane-per-candidate rounds: workgroups to aim for (candidate blocks times
-// prime blocks)
+// prime blocks).  A workgroup's set-up (records, flags, the rows' touch,
+// the final ballot) costs about 100 VALU per wave, so fewer, longer prime
+// blocks spend more of the VALU on tests, while more, shorter ones finish
+// a lone window sooner.  A batch that runs beside others
+// (FK_BATCH_THROUGHPUT, the slots of a pipeline) takes 1024: the Dickson
+// sweep went from 0.152-0.164 to 0.139-0.152 ms per window with four in
+// flight (512: 0.16, too few workgroups to fill the chip).  Alone, 4096:
+// 1024 cost the Euler SCORE_WIDE range 7% and the one-submission mixed
+// batch 20%.
 static constexpr uint32_t kRoundWorkgroups = 4096;
+static constexpr uint32_t kRoundWorkgroupsThroughput = 1024;
 
 static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
 {
@@ -1260,7 +1272,9 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
                 gy = std::max(1u, std::min(gy, 65535u));
             } else {
                 gx = (uint32_t)std::min(1024.0, std::max(1.0, std::ceil(alive / (256.0 * kRoundKC))));
-                gy = std::min((np + 15) / 16, (kRoundWorkgroups + gx - 1) / gx);
+                const uint32_t target = b->throughput ? kRoundWorkgroupsThroughput
+                                                      : kRoundWorkgroups;
+                gy = std::min((np + 15) / 16, (target + gx - 1) / gx);
                 gy = std::max(1u, std::min(gy, 65535u));
                 ra.pblock = (np + gy - 1) / gy;
                 gy = (np + ra.pblock - 1) / ra.pblock;
@@ -1661,7 +1675,7 @@ int fk_engine_create(const fk_engine_desc* desc, fk_engine** out)
     fk_engine* e = new fk_engine();
     e->desc = *desc;
     int rc = build_engine(e);
-    if (rc == FK_OK) rc = create_batch(e->device, false, &e->own);
+    if (rc == FK_OK) rc = create_batch(e->device, 0u, &e->own);
     // sweeps of one window shape repeat one launch sequence: the engine's
     // own batch replays it as a graph (fk_engine_batch + fk_batch_set_graph
     // turn this off, for stage timings)
@@ -1967,12 +1981,12 @@ int fk_batch_create(int device, fk_batch** out)
 int fk_batch_create_ex(int device, uint32_t flags, fk_batch** out)
 {
     if (!out) return fail(FK_E_ARG, "null argument");
-    if (flags & ~(uint32_t)FK_BATCH_DEDICATED_QUEUE)
+    if (flags & ~(uint32_t)(FK_BATCH_DEDICATED_QUEUE | FK_BATCH_THROUGHPUT))
         return fail(FK_E_ARG, "unknown batch flags 0x%x", flags);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
         return fail(FK_E_NODEV, "device %d not present", device);
-    return create_batch(device, (flags & FK_BATCH_DEDICATED_QUEUE) != 0, out);
+    return create_batch(device, flags, out);
 }
 
 int fk_batch_destroy(fk_batch* b)

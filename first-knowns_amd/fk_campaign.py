@@ -134,9 +134,11 @@ class Campaign:
             self._truncate_ledger(int(state["ledger_bytes"]))
         self.state = state
         # segments of one length repeat one launch sequence: the batches
-        # run it as a captured HIP graph after the first segment
+        # run it as a captured HIP graph after the first segment; the two
+        # run side by side, each on a hardware queue of its own
         self.batches = batches if batches is not None else [
-            N.Batch(getattr(engine, "device", 0), graph=True) for _ in range(2)]
+            N.Batch(getattr(engine, "device", 0), graph=True, dedicated=True,
+                    throughput=True) for _ in range(2)]
 
     # ------------------------------------------------------------ ledger
 

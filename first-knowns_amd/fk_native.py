@@ -25,6 +25,7 @@ FK_E_DEVICE, FK_E_OOM, FK_E_NODEV, FK_E_BUSY = 4, 5, 6, 7
 ABI_VERSION = 2
 CLASSIFY_STATIC_MAX = 65536       # values per fk_engine_classify_static call
 BATCH_DEDICATED_QUEUE = 1         # fk_batch_create_ex: a hardware queue of its own
+BATCH_THROUGHPUT = 2              # fk_batch_create_ex: planned to run beside others
 
 # The exported entry points, in the order include/fk_engine.h declares them.
 EXPORTS = ("fk_abi_version", "fk_last_error", "fk_device_count",
@@ -405,13 +406,16 @@ def batch_timing(handle):
 class Batch:
     """An fk_batch: many windows of many engines, submitted asynchronously."""
 
-    def __init__(self, device=0, graph=False, dedicated=False):
+    def __init__(self, device=0, graph=False, dedicated=False,
+                 throughput=False):
         """dedicated: the batch's stream gets a hardware queue of its own
-        (FK_BATCH_DEDICATED_QUEUE), as the slots of a Pipeline do."""
+        (FK_BATCH_DEDICATED_QUEUE); throughput: its launches are planned to
+        run beside other batches' (FK_BATCH_THROUGHPUT).  The slots of a
+        Pipeline are both."""
         h = ctypes.c_void_p()
-        check(lib().fk_batch_create_ex(int(device),
-                                       BATCH_DEDICATED_QUEUE if dedicated else 0,
-                                       ctypes.byref(h)))
+        flags = ((BATCH_DEDICATED_QUEUE if dedicated else 0) |
+                 (BATCH_THROUGHPUT if throughput else 0))
+        check(lib().fk_batch_create_ex(int(device), flags, ctypes.byref(h)))
         self._h = h
         self._nwin = 0
         if graph:
@@ -530,7 +534,8 @@ class Pipeline:
             self.close()
             # one hardware queue per slot: slots that shared one would
             # run their windows in turn
-            self._slots = [Batch(self.device, graph=graph, dedicated=True)
+            self._slots = [Batch(self.device, graph=graph, dedicated=True,
+                                 throughput=True)
                            for _ in range(self.depth)]
             self._graph = graph
 

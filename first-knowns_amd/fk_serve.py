@@ -101,7 +101,10 @@ class Server:
         self.clock = clock
         self.idle_sleep = idle_sleep
         self._engine_factory = engine_factory or self._make_engine
-        self._batch_factory = batch_factory or (lambda: N.Batch(device))
+        # slots on hardware queues of their own, so that they run side by
+        # side; planned for latency (a request waits on its own launches)
+        self._batch_factory = batch_factory or (
+            lambda: N.Batch(device, dedicated=True))
         self.engines = {}
         self.free_batches = [self._batch_factory() for _ in range(self.slots)]
         self.waiting = collections.deque()

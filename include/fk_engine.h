@@ -226,10 +226,15 @@ int fk_batch_destroy(fk_batch* b);
  * enabled): HIP maps ordinary streams onto a few shared hardware queues,
  * and batches that share one run in turn, so the slots of a pipeline of
  * windows in flight are created this way.  Such a stream synchronizes with
- * the null stream.  No reference counterpart (the reference's two CUDA
- * streams, dickson-ladders/ladder_gpu.py:717-747, always run side by side).
+ * the null stream.  FK_BATCH_THROUGHPUT plans the batch for running beside
+ * others: fewer, longer workgroups in the rounds, which spend more of the
+ * device on tests and finish a lone submission later.  Results do not
+ * depend on either flag.  No reference counterpart (the reference's two
+ * CUDA streams, dickson-ladders/ladder_gpu.py:717-747, always run side by
+ * side).
  */
 #define FK_BATCH_DEDICATED_QUEUE 1u
+#define FK_BATCH_THROUGHPUT 2u
 int fk_batch_create_ex(int device, uint32_t flags, fk_batch** out);
 
 /*

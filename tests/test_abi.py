@@ -76,7 +76,9 @@ def test_batch_flags_are_checked_before_the_device():
     text = HEADER.read_text()
     m = re.search(r"#define FK_BATCH_DEDICATED_QUEUE (\d+)u", text)
     assert int(m.group(1)) == N.BATCH_DEDICATED_QUEUE
+    m = re.search(r"#define FK_BATCH_THROUGHPUT (\d+)u", text)
+    assert int(m.group(1)) == N.BATCH_THROUGHPUT
     h = ctypes.c_void_p()
-    rc = N.lib().fk_batch_create_ex(0, 2, ctypes.byref(h))
+    rc = N.lib().fk_batch_create_ex(0, 4, ctypes.byref(h))
     assert rc == N.FK_E_ARG
     assert not h.value
