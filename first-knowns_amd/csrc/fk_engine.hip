@@ -866,6 +866,19 @@ static constexpr uint64_t kShortStartCost = 28, kShortWordCost = 5;
 // batch 20%.
 static constexpr uint32_t kRoundWorkgroups = 4096;
 static constexpr uint32_t kRoundWorkgroupsThroughput = 1024;
+// ... and only for rounds that start with at least this many expected
+// candidates: a round of few is latency-bound either way (this kept the
+// Dickson sweep's rate and took the plain engine on the mixed windows of
+// 2^20..2^30 values from 1.1-1.3e12 to 1.5-1.7e12 candidates/s)
+static constexpr double kThroughputAlive = 2048.0;
+
+// A submission whose windows expect fewer LDS-stage survivors than
+// kMergeAlive in all sweeps with every other round boundary of each
+// engine's table (round growth squared): its rounds are launch- and
+// latency-bound, and half as many cost less than the tests they add.  (A
+// submission of many candidates keeps every boundary: its launches serve
+// all its engines at once.)  The records do not depend on the boundaries.
+static constexpr double kMergeAlive = 4096.0;
 
 static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
 {
@@ -1071,10 +1084,25 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     b->seg_caps = caps;
 
     // ---- rounds: round r of every engine together (each engine keeps its
-    // own q_med * growth^r boundaries, DevEngine::rb)
+    // own q_med * growth^r boundaries, DevEngine::rb; an engine of few
+    // expected candidates in this submission takes every other boundary)
+    std::vector<DevEngine> eff(b->engines.size());
     uint32_t nround_max = 0;
+    double alive_all = 0.0;
     for (size_t k = 0; k < b->engines.size(); k++)
-        if (used[k]) nround_max = std::max(nround_max, b->engines[k]->host.nround);
+        if (used[k]) alive_all += e_alive[k];
+    for (size_t k = 0; k < b->engines.size(); k++) {
+        DevEngine& h = eff[k];
+        h = b->engines[k]->host;
+        if (used[k] && alive_all < kMergeAlive && h.nround > 1) {
+            uint32_t nr = 0;
+            for (uint32_t r = 0; r < h.nround; r += 2)
+                h.rb[++nr] = h.rb[std::min(r + 2, h.nround)];
+            for (uint32_t i = nr + 1; i < 33; i++) h.rb[i] = h.rb[nr];
+            h.nround = nr;
+        }
+        if (used[k]) nround_max = std::max(nround_max, h.nround);
+    }
     // counters: per stage (the sieve, then each round) one per engine,
     // then the packed total
     const size_t nstage = 1 + nround_max;
@@ -1114,7 +1142,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     {
         uint8_t* h = b->h_up;
         DevEngine* he = reinterpret_cast<DevEngine*>(h + b->o_engines);
-        for (size_t k = 0; k < b->engines.size(); k++) he[k] = b->engines[k]->host;
+        for (size_t k = 0; k < b->engines.size(); k++) he[k] = eff[k];
         if (!pieces.empty())
             std::memcpy(h + b->o_pieces, pieces.data(), pieces.size() * sizeof(DevPiece));
         std::memcpy(h + b->o_win_engine, win_engine.data(), nwin * sizeof(uint32_t));
@@ -1222,7 +1250,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             uint32_t kg = 1;
             alive = 0.0;
             for (size_t k = 0; k < b->engines.size(); k++) {
-                const DevEngine& h = b->engines[k]->host;
+                const DevEngine& h = eff[k];
                 if (!used[k] || r >= h.nround || h.rb[r + 1] <= h.rb[r]) continue;
                 if ((h.kidx ? kRowsIndex : kRowsResidue) != kind) continue;
                 P0 = std::min(P0, h.rb[r]);
@@ -1272,8 +1300,9 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
                 gy = std::max(1u, std::min(gy, 65535u));
             } else {
                 gx = (uint32_t)std::min(1024.0, std::max(1.0, std::ceil(alive / (256.0 * kRoundKC))));
-                const uint32_t target = b->throughput ? kRoundWorkgroupsThroughput
-                                                      : kRoundWorkgroups;
+                const uint32_t target =
+                    b->throughput && alive >= kThroughputAlive
+                        ? kRoundWorkgroupsThroughput : kRoundWorkgroups;
                 gy = std::min((np + 15) / 16, (target + gx - 1) / gx);
                 gy = std::max(1u, std::min(gy, 65535u));
                 ra.pblock = (np + gy - 1) / gy;
@@ -1303,7 +1332,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         b->round_slots.push_back(cstage);
         for (size_t k = 0; k < b->engines.size(); k++) {
             const fk_engine* e = b->engines[k];
-            const DevEngine& h = e->host;
+            const DevEngine& h = eff[k];
             if (r >= h.nround) continue;
             const double lo = ctx->primes[h.rb[r]];
             const double hi = h.rb[r + 1] < ctx->primes.size()
