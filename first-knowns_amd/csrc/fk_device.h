@@ -83,11 +83,15 @@ struct DevPiece {
     uint32_t npos;            // positions per line in this piece
     uint32_t nseg;            // tiles per line
     uint32_t tile0;           // first global tile index
-    uint32_t engine;          // index into the engine array
+    uint32_t engine;          // index into the engine array: the wheel and
+                              // LDS tables the piece is sieved with
     uint32_t window;          // caller's window index
     uint32_t npw;             // short-line pieces: 64-bit words per line,
                               // tiles of 256 (line, chunk) items; 0: one
                               // tile per S positions of a line
+    uint32_t queue;           // the engine whose queue segment and counter
+                              // take the survivors: `engine` itself, or the
+                              // engine whose prefix wheel `engine` is
 };
 
 // Short-line sieve: at most this many words per lane, and the bits its

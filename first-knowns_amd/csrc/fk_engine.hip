@@ -166,11 +166,22 @@ struct fk_engine {
     std::vector<void*> allocs;
     uint64_t device_bytes = 0;
     fk_batch* own = nullptr;
+    // Prefix wheels for short windows (the wheel without its last 1, 2, ...
+    // primes, which their LDS stage then sieves): sieve-side tables only,
+    // the round table is the parent's.  A window whose lines hold few
+    // positions on the full wheel is sieved on the one with the least work.
+    std::vector<fk_engine*> alts;
+    const fk_engine* parent = nullptr;
 };
 
 static void engine_free(fk_engine* e)
 {
     if (!e) return;
+    for (fk_engine* a : e->alts) {
+        engine_free(a);
+        delete a;
+    }
+    e->alts.clear();
     (void)hipSetDevice(e->device);
     for (void* p : e->allocs) (void)hipFree(p);
     e->allocs.clear();
@@ -499,7 +510,16 @@ static int build_engine(fk_engine* e)
     }
     if ((rc = eng_upload(e, sbm, &h.sbm))) return rc;
     if ((rc = eng_upload(e, tr_end, &h.tr_end))) return rc;
-    {
+    if (e->parent) {
+        // a prefix wheel tests the same round primes with its parent's table
+        const DevEngine& ph = e->parent->host;
+        if (ph.round_first != h.round_first || ph.round_end != h.round_end ||
+            ph.nround != h.nround)
+            return fail(FK_E_ARG, "prefix wheel's rounds differ from its engine's");
+        h.kt = ph.kt;
+        h.kidx = ph.kidx;
+        h.kstride = ph.kstride;
+    } else {
         // kill sets of the round primes, built on the device
         const uint32_t np = e->round_end - e->round_first;
         const size_t nt = (size_t)np * h.kstride + kRowTail;
@@ -517,6 +537,50 @@ static int build_engine(fk_engine* e)
     const DevEngine* de = nullptr;
     if ((rc = eng_upload(e, one, &de))) return rc;
     e->d_engine = const_cast<DevEngine*>(de);
+    return FK_OK;
+}
+
+// Prefix wheels: by default up to kAltWheels, of at most kAltLines
+// residues each (their host tables are built at engine creation).
+static constexpr int kAltWheels = 4;
+static constexpr uint64_t kAltLines = 1ull << 21;
+
+// The engine's prefix wheels (fk_engine_desc.alt_wheels): the wheel without
+// its last 1, 2, ... primes, each an engine of its own sieve-side tables
+// over the parent's round table.  Only when every wheel prime is below
+// q_med, so that the primes a prefix drops fall to its LDS stage and the
+// rounds start where the parent's do.  A prefix that cannot be built (its
+// LDS tables do not fit) is skipped: the full wheel always works.
+static int build_alts(fk_engine* e)
+{
+    const int want = e->desc.alt_wheels == 0 ? kAltWheels : e->desc.alt_wheels;
+    if (want < 0 || e->wheel.size() < 2 || e->wheel.back() >= e->q_med ||
+        e->short_words == 0)
+        return FK_OK;
+    for (size_t len = e->wheel.size() - 1; len >= 1 && (int)e->alts.size() < want;
+         len--) {
+        std::vector<uint32_t> prefix(e->wheel.begin(), e->wheel.begin() + len);
+        bool ok = false;
+        const uint64_t count = wheel_count(e->family, e->n, prefix, &ok);
+        if (!ok || count == 0 || count > kAltLines) continue;
+        fk_engine* a = new fk_engine();
+        a->desc = e->desc;
+        std::memset(a->desc.wheel_primes, 0, sizeof a->desc.wheel_primes);
+        for (size_t i = 0; i < len; i++) a->desc.wheel_primes[i] = prefix[i];
+        a->desc.nwheel = (int32_t)len;
+        a->desc.alt_wheels = -1;
+        a->parent = e;
+        const std::string keep = g_err;
+        const int rc = build_engine(a);
+        if (rc || a->short_words == 0) {
+            g_err = keep;
+            engine_free(a);
+            delete a;
+            continue;
+        }
+        e->device_bytes += a->device_bytes;
+        e->alts.push_back(a);
+    }
     return FK_OK;
 }
 
@@ -698,6 +762,7 @@ struct fk_batch {
     uint64_t reruns = 0;              // batches rerun after a queue overflow
     uint32_t copied = 0;
     uint64_t tiles = 0, candidates = 0;
+    std::vector<uint64_t> win_wheel;  // per window: the wheel period sieved on
     std::vector<fk_record> out;       // sorted results
     std::vector<uint64_t> counts;
     float ms[5] = {0, 0, 0, 0, 0};
@@ -854,6 +919,52 @@ static constexpr uint32_t kSingleLimit = 8192;
 static constexpr uint32_t kShortChunks[] = {1, 2, 3, 4, 6, 8, 12, 16};
 static constexpr uint64_t kShortStartCost = 28, kShortWordCost = 5;
 
+// A window on wheel t with short lines: its base, the chunk with the least
+// work per line, its (line, chunk) items and the whole window's work (lane
+// instructions: per LDS prime a start per chunk and a bitmap window per
+// word, padded words included).  False when t's lines are too long for the
+// short-line sieve.
+struct ShortPlan {
+    u128 B;
+    uint64_t H;
+    uint32_t chunk;
+    uint64_t items;
+    double work;
+};
+
+static bool short_plan(const fk_engine* t, u128 lo, u128 hi, ShortPlan* sp)
+{
+    if (!t->short_words) return false;
+    const u128 B = lo - lo % t->D;
+    const uint64_t H = hi > B ? (uint64_t)(hi - B) : 0;
+    const uint64_t npw = (H / t->D + 1 + 63) / 64;
+    if (npw > t->short_words) return false;
+    uint64_t best = ~0ull, nch_best = 0;
+    uint32_t chunk = 0;
+    for (uint32_t c : kShortChunks) {
+        const uint64_t nch = (npw + c - 1) / c;
+        const uint64_t cost = nch * (kShortStartCost + c * kShortWordCost);
+        if (cost < best && t->nlines * nch < (1ull << 32)) {
+            best = cost;
+            chunk = c;
+            nch_best = nch;
+        }
+    }
+    if (!chunk) return false;
+    sp->B = B;
+    sp->H = H;
+    sp->chunk = chunk;
+    sp->items = t->nlines * nch_best;
+    sp->work = (double)best * (double)t->nlines *
+               (double)std::max<uint32_t>(t->host.nlds, 1u);
+    return true;
+}
+
+// A prefix wheel takes a window only if it leaves the window at least this
+// many (line, chunk) items (or as many as the full wheel): fewer lanes
+// than two tiles would run the sieve as a few long chains.
+static constexpr uint64_t kAltMinItems = 512;
+
 // lane-per-candidate rounds: workgroups to aim for (candidate blocks times
 // prime blocks).  A workgroup's set-up (records, flags, the rows' touch,
 // the final ballot) costs about 100 VALU per wave, so fewer, longer prime
@@ -910,7 +1021,9 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     // holds a line), and each group of windows with one shape is one sieve
     // launch over a contiguous range of tiles
     const uint32_t T = e0->threads;
-    struct WinPlan { u128 B; uint64_t H; uint32_t wpt, chunk; };
+    // tab: the engine whose wheel and LDS tables sieve the window (the
+    // window's own, or one of its prefix wheels)
+    struct WinPlan { u128 B; uint64_t H; uint32_t wpt, chunk; const fk_engine* tab; };
     std::vector<WinPlan> wp(nwin);
     for (size_t w = 0; w < nwin; w++) {
         const fk_engine* e = b->engines[b->engine_of[w]];
@@ -918,24 +1031,30 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         const u128 B = lo - lo % e->D;
         wp[w].B = B;
         wp[w].H = hi > B ? (uint64_t)(hi - B) : 0;
+        wp[w].tab = e;
         const uint64_t npos = wp[w].H / e->D + 1;
         // lines of a few words: one lane per chunk of a line (k_sieve_short),
         // the chunk with the least work per line: a start per prime and
-        // chunk, a bitmap window per prime and word (padded words included)
-        const uint64_t npw = (npos + 63) / 64;
+        // chunk, a bitmap window per prime and word (padded words included).
+        // Lines of a word or less on a wide wheel are mostly padding (a
+        // 2^20-j Dickson window has 0.4 positions on each of the 66,759
+        // lines of its 2,800,733-wide wheel): a prefix wheel, fewer and
+        // longer lines, may sieve the window with a small part of the work.
+        ShortPlan sp;
         wp[w].chunk = 0;
-        if (npw <= e->short_words) {
-            uint64_t best = ~0ull;
-            for (uint32_t c : kShortChunks) {
-                const uint64_t nch = (npw + c - 1) / c;
-                const uint64_t cost = nch * (kShortStartCost + c * kShortWordCost);
-                if (cost < best && e->nlines * nch < (1ull << 32)) {
-                    best = cost;
-                    wp[w].chunk = c;
+        if (short_plan(e, lo, hi, &sp)) {
+            const uint64_t min_items = std::min(kAltMinItems, sp.items);
+            for (const fk_engine* a : e->alts) {
+                ShortPlan ap;
+                if (short_plan(a, lo, hi, &ap) && ap.items >= min_items &&
+                    ap.work < sp.work) {
+                    sp = ap;
+                    wp[w].tab = a;
                 }
             }
-        }
-        if (wp[w].chunk) {
+            wp[w].B = sp.B;
+            wp[w].H = sp.H;
+            wp[w].chunk = sp.chunk;
             wp[w].wpt = 0;
             continue;
         }
@@ -965,6 +1084,19 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     struct Group { uint32_t wpt, chunk; uint64_t tile_begin, tile_end; };
     std::vector<Group> groups;
 
+    // the engine array the kernels index: the submission's engines, then
+    // the prefix wheels its windows use (sieve-side tables only; the
+    // queues, counters and rounds are the submission's engines')
+    std::vector<const fk_engine*> tabs(b->engines.begin(), b->engines.end());
+    auto tab_index = [&](const fk_engine* t, uint32_t own) {
+        if (t == b->engines[own]) return own;
+        for (size_t i = b->engines.size(); i < tabs.size(); i++)
+            if (tabs[i] == t) return (uint32_t)i;
+        tabs.push_back(t);
+        return (uint32_t)(tabs.size() - 1);
+    };
+    b->win_wheel.assign(nwin, 0);
+
     std::vector<DevPiece> pieces;
     std::vector<DevWinInfo> wins(nwin);
     std::vector<uint32_t> win_engine(nwin);
@@ -990,12 +1122,15 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         const fk_engine* e = b->engines[b->engine_of[w]];
         const u128 lo = mk128(b->windows[w].lo), hi = mk128(b->windows[w].hi);
         if (hi <= lo) continue;
+        b->win_wheel[w] = wp[w].tab->D;
         if (wp[w].wpt == 0) {
-            // short lines: one piece, tiles of 256 (line, chunk) items
+            // short lines: one piece, tiles of 256 (line, chunk) items, on
+            // the window's wheel (its engine's or a prefix wheel)
+            const fk_engine* t = wp[w].tab;
             if (groups.empty() || groups.back().wpt != 0 ||
                 groups.back().chunk != wp[w].chunk)
                 groups.push_back({0u, wp[w].chunk, tiles, tiles});
-            const uint64_t npos_all = wp[w].H / e->D + 1;
+            const uint64_t npos_all = wp[w].H / t->D + 1;
             DevPiece P{};
             for (int i = 0; i < 4; i++) P.base[i] = (uint32_t)(wp[w].B >> (32 * i));
             P.lo_rel = lo > wp[w].B ? (uint64_t)(lo - wp[w].B) : 0;
@@ -1003,17 +1138,18 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             P.npos = (uint32_t)npos_all;
             P.npw = (uint32_t)((npos_all + 63) / 64);
             const uint64_t nch = (P.npw + wp[w].chunk - 1) / wp[w].chunk;
-            P.nseg = (uint32_t)((e->nlines * nch + 255) / 256);
+            P.nseg = (uint32_t)((t->nlines * nch + 255) / 256);
             P.tile0 = (uint32_t)tiles;
-            P.engine = b->engine_of[w];
+            P.engine = tab_index(t, b->engine_of[w]);
+            P.queue = b->engine_of[w];
             P.window = (uint32_t)w;
             if (tiles + P.nseg >= (1ull << 32))
                 return fail(FK_E_ARG, "batch too large: more than 2^32 tiles");
             tiles += P.nseg;
             pieces.push_back(P);
             groups.back().tile_end = tiles;
-            s_nlds_max = std::max(s_nlds_max, e->host.nlds);
-            s_bm_max = std::max(s_bm_max, e->host.sbm_words);
+            s_nlds_max = std::max(s_nlds_max, t->host.nlds);
+            s_bm_max = std::max(s_bm_max, t->host.sbm_words);
             const double span = (double)(hi - lo);
             expect += expected_lds(e, span);
             e_alive[b->engine_of[w]] += expected_lds(e, span);
@@ -1035,6 +1171,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             P.nseg = (uint32_t)((P.npos + S - 1) / S);
             P.tile0 = (uint32_t)tiles;
             P.engine = b->engine_of[w];
+            P.queue = b->engine_of[w];
             P.window = (uint32_t)w;
             const uint64_t t = (uint64_t)P.nseg * e->nlines;
             if (tiles + t >= (1ull << 32))
@@ -1109,7 +1246,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     const size_t ncounts = nstage * ne + 1;
 
     // ---- buffers
-    if ((rc = grow_uploads(b, b->engines.size(), std::max<size_t>(pieces.size(), 1),
+    if ((rc = grow_uploads(b, tabs.size(), std::max<size_t>(pieces.size(), 1),
                            nwin, ne, ncounts)))
         return rc;
     if (queue_cap > b->cap_q || !b->d_q[0]) {
@@ -1143,6 +1280,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         uint8_t* h = b->h_up;
         DevEngine* he = reinterpret_cast<DevEngine*>(h + b->o_engines);
         for (size_t k = 0; k < b->engines.size(); k++) he[k] = eff[k];
+        for (size_t k = b->engines.size(); k < tabs.size(); k++) he[k] = tabs[k]->host;
         if (!pieces.empty())
             std::memcpy(h + b->o_pieces, pieces.data(), pieces.size() * sizeof(DevPiece));
         std::memcpy(h + b->o_win_engine, win_engine.data(), nwin * sizeof(uint32_t));
@@ -1602,6 +1740,7 @@ static int batch_start(fk_batch* b)
     if (total == 0) {
         // nothing left to sweep: an empty result, nothing on the device
         b->out.clear();
+        b->win_wheel.assign(nwin, 0);
         b->counts.assign(nwin, 0);
         b->stage_counts.clear();
         b->have_results = true;
@@ -1704,6 +1843,7 @@ int fk_engine_create(const fk_engine_desc* desc, fk_engine** out)
     fk_engine* e = new fk_engine();
     e->desc = *desc;
     int rc = build_engine(e);
+    if (rc == FK_OK) rc = build_alts(e);
     if (rc == FK_OK) rc = create_batch(e->device, 0u, &e->own);
     // sweeps of one window shape repeat one launch sequence: the engine's
     // own batch replays it as a graph (fk_engine_batch + fk_batch_set_graph
@@ -2119,6 +2259,16 @@ int fk_batch_step_times(fk_batch* b, int32_t* kinds, float* ms, size_t cap,
                ? fail(FK_E_OVERFLOW, "%zu launches, room for %zu",
                       b->step_ms.size(), cap)
                : FK_OK;
+}
+
+int fk_batch_window_wheel(fk_batch* b, size_t window, uint64_t* line_step)
+{
+    if (!b || !line_step) return fail(FK_E_ARG, "null argument");
+    if (b->pending) return fail(FK_E_BUSY, "batch has work in flight");
+    if (window >= b->windows.size())
+        return fail(FK_E_ARG, "window %zu of %zu", window, b->windows.size());
+    *line_step = window < b->win_wheel.size() ? b->win_wheel[window] : 0;
+    return FK_OK;
 }
 
 int fk_batch_reruns(fk_batch* b, uint64_t* reruns)

@@ -133,7 +133,7 @@ __device__ __forceinline__ void emit_record(const SieveArgs& a,
 {
     unsigned __int128 x = B + off + (unsigned __int128)E.D * v;
     if (E.family == kDickson) x *= E.W;
-    const uint2 seg = a.segs[P.engine];
+    const uint2 seg = a.segs[P.queue];
     if (slot < seg.y) {
         a.flags[seg.x + slot] = 0;        // live for the first round
         DevRec r;
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
             ((unsigned __int128)(((uint64_t)P.base[3] << 32) | P.base[2]) << 64)
             | (((uint64_t)P.base[1] << 32) | P.base[0]);
         if (listed) {
-            if (tid == 0 && n) sh_base = atomicAdd(a.counts + P.engine, n);
+            if (tid == 0 && n) sh_base = atomicAdd(a.counts + P.queue, n);
             __syncthreads();
             for (uint32_t i = tid; i < n; i += T)
                 emit_record(a, P, E, B, off, v0 + cur[i], sh_base + i);
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
             if (cnt) mine = atomicAdd(&sh_count, cnt);
             __syncthreads();
             if (tid == 0 && sh_count)
-                sh_base = atomicAdd(a.counts + P.engine, sh_count);
+                sh_base = atomicAdd(a.counts + P.queue, sh_count);
             __syncthreads();
             if (cnt) {
                 uint32_t slot = sh_base + mine;
@@ -586,7 +586,7 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
         const uint32_t total = __shfl(incl, 63);
         uint32_t base = 0;
         if (total) {
-            if (lane == 63) base = atomicAdd(a.counts + P.engine, total);
+            if (lane == 63) base = atomicAdd(a.counts + P.queue, total);
             base = __shfl(base, 63);
         }
         if (c) {

@@ -22,7 +22,7 @@ CLASSIFY_NONE, CLASSIFY_FULL, CLASSIFY_TWO_PASS = 0, 1, 2
 
 FK_OK, FK_E_ARG, FK_E_RANGE, FK_E_OVERFLOW = 0, 1, 2, 3
 FK_E_DEVICE, FK_E_OOM, FK_E_NODEV, FK_E_BUSY = 4, 5, 6, 7
-ABI_VERSION = 2
+ABI_VERSION = 3
 CLASSIFY_STATIC_MAX = 65536       # values per fk_engine_classify_static call
 BATCH_DEDICATED_QUEUE = 1         # fk_batch_create_ex: a hardware queue of its own
 BATCH_THROUGHPUT = 2              # fk_batch_create_ex: planned to run beside others
@@ -37,7 +37,8 @@ EXPORTS = ("fk_abi_version", "fk_last_error", "fk_device_count",
            "fk_batch_graph_stats", "fk_batch_counts", "fk_engine_batch",
            "fk_batch_reruns", "fk_batch_set_budget", "fk_batch_cursor_get",
            "fk_batch_cursor_set", "fk_batch_resume",
-           "fk_engine_classify_static", "fk_batch_step_times")
+           "fk_engine_classify_static", "fk_batch_step_times",
+           "fk_batch_window_wheel")
 
 u32, i32, u64 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
 
@@ -48,7 +49,8 @@ class EngineDesc(ctypes.Structure):
                 ("q_pat", u32), ("q_med", u32), ("threads", u32),
                 ("words_per_thread", u32), ("round_lane_prime", u32),
                 ("round_growth", u32), ("q_mark", u32), ("q_bits", u32),
-                ("short_words", u32), ("segment_cap", u32)]
+                ("short_words", u32), ("segment_cap", u32),
+                ("alt_wheels", i32)]
 
 
 class EngineInfo(ctypes.Structure):
@@ -186,7 +188,9 @@ class Engine:
     def __init__(self, family, n, q2, device=0, nwheel=0, wheel_primes=None,
                  q_pat=0, q_med=0, threads=0, words_per_thread=0,
                  round_lane_prime=0, round_growth=0, q_mark=0, q_bits=0,
-                 short_words=0, segment_cap=0):
+                 short_words=0, segment_cap=0, alt_wheels=0):
+        """alt_wheels: prefix wheels kept for short windows (0: the
+        library's default, -1: none); records never depend on it."""
         d = EngineDesc()
         d.family, d.n, d.q2, d.device = int(family), int(n), int(q2), device
         d.nwheel = int(nwheel)
@@ -200,6 +204,7 @@ class Engine:
         d.q_bits = int(q_bits)
         d.short_words = int(short_words)
         d.segment_cap = int(segment_cap)
+        d.alt_wheels = int(alt_wheels)
         h = ctypes.c_void_p()
         check(lib().fk_engine_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
@@ -368,6 +373,14 @@ class Engine:
         check(lib().fk_engine_batch(self._h, ctypes.byref(b)))
         return batch_counts(b)
 
+    def last_wheel(self):
+        """Wheel period the last sweep's window was sieved on."""
+        b = ctypes.c_void_p()
+        check(lib().fk_engine_batch(self._h, ctypes.byref(b)))
+        d = u64(0)
+        check(lib().fk_batch_window_wheel(b, 0, ctypes.byref(d)))
+        return int(d.value)
+
 
 def batch_counts(handle):
     """Queue sizes after the sieve and after each round of the last run."""
@@ -504,6 +517,13 @@ class Batch:
 
     def counts(self):
         return batch_counts(self._h)
+
+    def window_wheel(self, window):
+        """Wheel period the window was sieved on in the last submission
+        (the engine's own, or a prefix wheel's for a short window)."""
+        d = u64(0)
+        check(lib().fk_batch_window_wheel(self._h, int(window), ctypes.byref(d)))
+        return int(d.value)
 
 
 # Windows in flight: HIP gives a process four hardware queues by default,

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FK_ABI_VERSION 2
+#define FK_ABI_VERSION 3
 
 enum fk_family {
     FK_FAMILY_EULER = 1,    /* x^2 + x + p prime for x = 0..n-1 */
@@ -96,6 +96,14 @@ typedef struct fk_engine_desc {
                                 resized to the true count and the batch run
                                 again, so a small value only costs a rerun;
                                 at most 2^30 */
+    int32_t alt_wheels;      /* prefix wheels kept for short windows: the
+                                engine's wheel without its last 1, 2, ...
+                                primes (those are then sieved in LDS).  A
+                                window whose lines hold few positions on the
+                                full wheel is sieved on the prefix wheel
+                                with the least work; records do not depend
+                                on the choice.  0 = default (up to 4), -1 =
+                                none, k > 0 = at most k */
 } fk_engine_desc;
 
 typedef struct fk_engine_info {
@@ -318,6 +326,15 @@ enum fk_step {
 };
 int fk_batch_step_times(fk_batch* b, int32_t* kinds, float* ms, size_t cap,
                         size_t* n);
+
+/*
+ * The wheel period (line step D) window `window` of the last submission
+ * was sieved on: the engine's own, or a prefix wheel's for a short window
+ * (fk_engine_desc.alt_wheels).  0 for an empty window.  No reference
+ * counterpart (the reference sieves every window on one wheel); for tests
+ * and the planner's accounting.
+ */
+int fk_batch_window_wheel(fk_batch* b, size_t window, uint64_t* line_step);
 
 /* The engine's own internal batch, for timing its last sweep. */
 int fk_engine_batch(fk_engine* eng, fk_batch** out);
