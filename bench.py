@@ -59,7 +59,7 @@ def splitmix64(x):
 # ------------------------------------ headline: the Dickson ladder sweep
 
 SWEEP_N, SWEEP_Q2, SWEEP_SPAN, SWEEP_CAP = 10, 1 << 22, 1 << 32, 18
-N_STEP_SIEVE, N_STEP_ROUND_C, N_STEP_COMPACT = 1, 2, 4
+N_STEP_SIEVE, N_STEP_ROUND_C, N_STEP_ROUND_P, N_STEP_COMPACT = 1, 2, 3, 4
 
 
 def sweep_window(seed):
@@ -432,18 +432,29 @@ def round_work(info, counts, growth):
     for m in range(1, n + 1):
         cnt += legendre_is_one((ps - np.uint64(m)) % ps, ps)
     ops = np.where(cnt > 0, 6 + cnt, 0)
-    edges = [int(info.q_med)]
-    while edges[-1] < int(info.q2):
-        edges.append(min(edges[-1] * growth, int(info.q2)))
+    # the engine's boundaries: q_med g^r (g an integer, or hundredths from
+    # 101 up), each at the first prime at or past it, repeats dropped
+    g = growth / 100.0 if growth >= 100 else float(growth)
+    q2 = int(info.q2)
+    idx = [0]
+    v = int(info.q_med) * g
+    while True:
+        i = int(np.searchsorted(ps, int(min(v, q2))))
+        if i > idx[-1]:
+            idx.append(i)
+        if v >= q2:
+            break
+        v *= g
+    if idx[-1] < ps.size:
+        idx.append(int(ps.size))
     tests = vops = 0
     rounds = []
-    for r in range(len(edges) - 1):
-        lo = np.searchsorted(ps, edges[r])
-        hi = np.searchsorted(ps, edges[r + 1])
+    for r in range(len(idx) - 1):
+        lo, hi = idx[r], idx[r + 1]
         c = int(counts[r]) if r < len(counts) else 0
         tests += c * int(hi - lo)
         vops += c * int(ops[lo:hi].sum())
-        rounds.append((int(ps[lo]) if hi > lo else edges[r], int(hi - lo), c))
+        rounds.append((int(ps[lo]), int(hi - lo), c))
     return int(tests), int(vops), rounds
 
 
@@ -489,6 +500,10 @@ def main():
     ap.add_argument("--cpu-one-core-log2", type=int, default=27)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=400.0)
+    # a second arrival rate, past what one request at a time could serve,
+    # where requests that arrive while the slots are busy share submissions
+    ap.add_argument("--serve-requests-high", type=int, default=1200)
+    ap.add_argument("--serve-rate-high", type=float, default=4000.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device copy of MIB MiB (counter "
                          "calibration) and exit")
@@ -512,6 +527,7 @@ def main():
         n = [sum(1 for k, _ in w if k in kinds) for w in P["launches"]]
         return sum(per) / len(per), sum(n) / len(n)
     round_ms, round_launches = launch_ms({N_STEP_ROUND_C})
+    roundp_ms, roundp_launches = launch_ms({N_STEP_ROUND_P})
     sieve_ms, _ = launch_ms({N_STEP_SIEVE})
     compact_ms, _ = launch_ms({N_STEP_COMPACT})
     # the rounds: tests and VALU lane-ops per window from the queue sizes
@@ -551,6 +567,8 @@ def main():
                                         "classify", "whole"), stage)),
         "launch_ms_per_step": {"k_round_c": round_ms,
                                "k_round_c_launches": round_launches,
+                               "k_round_p": roundp_ms,
+                               "k_round_p_launches": roundp_launches,
                                "k_round_compact": compact_ms,
                                "sieve": sieve_ms},
         "warmup_windows": P["warm_windows"],
@@ -597,6 +615,8 @@ def main():
         out["batched"] = run_batched()
     if not args.no_serving:
         out["serving"] = run_serving(args.serve_requests, args.serve_rate)
+        out["serving_high_rate"] = run_serving(args.serve_requests_high,
+                                               args.serve_rate_high)
     if not args.no_ap:
         out["config2_ap"] = run_ap()
     if not args.no_euler:

@@ -315,7 +315,12 @@ static int build_engine(fk_engine* e)
     e->q_bits = d.q_bits ? d.q_bits : e->q_mark;
     if (e->q_bits < e->q_mark) e->q_bits = e->q_mark;
     if (e->q_bits > e->q_med) e->q_bits = e->q_med;
-    e->growth = d.round_growth >= 2 ? d.round_growth : 2;
+    // round growth: an integer factor (2..99), or hundredths from 101 up
+    // (141: each round's prime bound 1.41 times the last)
+    if (d.round_growth == 1 || d.round_growth == 100 || d.round_growth > 9999)
+        return fail(FK_E_ARG, "round_growth = %u: use 2..99, or hundredths "
+                    "101..9999", d.round_growth);
+    e->growth = d.round_growth ? d.round_growth : 2;
 
     // ---- LDS-stage primes: past the wheel, below q_med; in order the
     // pattern primes, the stride-marked primes, the bitmap primes (ORed in
@@ -407,12 +412,13 @@ static int build_engine(fk_engine* e)
     // round boundaries: prime values q_med * growth^r, as table indices
     std::vector<uint32_t> rb(1, e->round_first);
     if (e->round_end > e->round_first) {
-        for (uint64_t v = (uint64_t)e->q_med * e->growth;; v *= e->growth) {
+        const double g = e->growth >= 100 ? e->growth / 100.0 : (double)e->growth;
+        for (double v = (double)e->q_med * g;; v *= g) {
             const uint32_t idx = std::min(
                 e->round_end,
-                prime_index(ctx->primes, (uint32_t)std::min<uint64_t>(v, d.q2)));
+                prime_index(ctx->primes, (uint32_t)std::min<double>(v, (double)d.q2)));
             if (idx > rb.back()) rb.push_back(idx);
-            if (v >= d.q2) break;
+            if (v >= (double)d.q2) break;
         }
         if (rb.back() < e->round_end) rb.push_back(e->round_end);
     }

@@ -128,6 +128,7 @@ def lib():
     L.fk_batch_resume.argtypes = [vp]
     L.fk_engine_classify_static.argtypes = [vp, vp, sz, P(ClassifySpec), vp]
     L.fk_batch_step_times.argtypes = [vp, vp, vp, sz, P(sz)]
+    L.fk_batch_window_wheel.argtypes = [vp, sz, P(u64)]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
     if L.fk_abi_version() != ABI_VERSION:

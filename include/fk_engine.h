@@ -76,8 +76,9 @@ typedef struct fk_engine_desc {
                                 least 1/8 as many expected candidates, test
                                 one prime per lane; the others one candidate
                                 per lane (0 = default 8192) */
-    uint32_t round_growth;   /* prime bound grows by this factor per round
-                                (0 = default 2) */
+    uint32_t round_growth;   /* prime bound grows by this factor per round:
+                                2..99, or hundredths from 101 to 9999 (141:
+                                1.41 per round); 0 = default 2 */
     uint32_t q_mark;         /* primes in [q_pat, q_mark) are stride-marked
                                 in LDS (0 = q_pat: no marking) */
     uint32_t q_bits;         /* primes in [q_mark, q_bits) are ORed in as

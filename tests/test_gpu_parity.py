@@ -209,6 +209,7 @@ def test_streams_do_not_depend_on_cuts_or_tiles():
                {"q_med": 2048, "q_pat": 256},
                {"nwheel": 1}, {"nwheel": 2},
                {"round_growth": 4, "round_lane_prime": 1},
+               {"round_growth": 141}, {"round_growth": 168},
                {"round_lane_prime": 1}, {"round_lane_prime": 1 << 30},
                {"q_bits": 1}, {"q_bits": 300, "q_med": 1024},
                {"q_bits": 1, "q_med": 2048, "q_pat": 64},
@@ -219,6 +220,9 @@ def test_streams_do_not_depend_on_cuts_or_tiles():
                {"q_pat": 16, "q_bits": 1024, "q_med": 1024}):
         eng = ladder_gpu.GpuEngine(n, q2=q2, short_words=1, **kw)
         assert eng.survivors_j(lo, hi).tolist() == ref, kw
+    for bad in (1, 100, 10_000):
+        with pytest.raises(ValueError):
+            ladder_gpu.GpuEngine(n, q2=q2, round_growth=bad)
     eng = ladder_gpu.GpuEngine(n, q2=q2)
     for cut in (lo + 1, lo + 12_345_678, hi - 1):
         a = eng.survivors_j(lo, cut).tolist()
