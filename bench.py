@@ -43,7 +43,10 @@ sys.path.insert(0, str(ROOT / "first-knowns_amd"))
 
 METRIC = ("candidates swept/sec at bit-exact survivor stream; "
           "batched vs plain GPU engine")
-LDS_PEAK = 256 * 128 * 2.4e9          # B/s: 256 CUs x 128 B/clk x 2.4 GHz
+# LDS read bandwidth: 256 B/clk/CU for 16-byte reads on gfx950 (128 B/clk
+# was the previous generation's); fk_probe_peaks measures about 131 TB/s
+# of it on the box, which 128 B/clk could not deliver
+LDS_PEAK = 256 * 256 * 2.4e9          # B/s: 256 CUs x 256 B/clk x 2.4 GHz
 HBM_PEAK = 8.0e12                      # B/s
 MASK64 = (1 << 64) - 1
 
@@ -541,6 +544,12 @@ def main():
     s_launch = sieve_ms / 1e3
     s_achieved = P["cand"] * per_cand / s_launch if s_launch > 0 else 0.0
     s_traffic = rocprof_traffic(kernel)
+    # the denominators measured on this device (fk_probe_peaks), beside the
+    # published ones: VALU issue (the faster of the two probe loops) and LDS
+    # read bandwidth
+    import fk_native as N
+    pf64, pint, plds = N.probe_peaks(0)
+    valu_meas = max(pf64, pint)
     out = {
         "metric": METRIC, "value": value, "unit": "j-candidates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -580,9 +589,19 @@ def main():
                        "least per slot); "
                        "device_ms_per_window_alone is one window swept by "
                        "itself (graph replay, event-timed)",
+        "measured_peaks": {"valu_f64_Tlane_op_s": pf64 / 1e12,
+                           "valu_int_Tlane_op_s": pint / 1e12,
+                           "lds_GBps": plds / 1e9,
+                           "note": "fk_probe_peaks: eight dependent chains "
+                                   "per lane of double FMAs / 32-bit funnel "
+                                   "shifts over every CU, and conflict-free "
+                                   "16-byte LDS reads; HIP-event timed"},
         "roofline": {"bound": "valu", "kernel": "k_round_c",
                      "achieved": r_achieved / 1e12, "peak": VALU_PEAK / 1e12,
                      "unit": "Tlane-op/s", "frac": r_achieved / VALU_PEAK,
+                     "peak_measured": valu_meas / 1e12,
+                     "frac_measured": (r_achieved / valu_meas
+                                       if valu_meas > 0 else None),
                      "traffic": r_traffic,
                      "tests_per_window": tests,
                      "valu_lane_ops_per_window": vops,
@@ -597,6 +616,9 @@ def main():
                            "achieved": s_achieved / 1e9,
                            "peak": LDS_PEAK / 1e9, "unit": "GB/s",
                            "frac": s_achieved / LDS_PEAK,
+                           "peak_measured": plds / 1e9,
+                           "frac_measured": (s_achieved / plds
+                                             if plds > 0 else None),
                            "traffic": s_traffic,
                            "algorithmic_bytes_per_candidate": per_cand,
                            "candidates_per_launch": P["cand"],

@@ -2267,6 +2267,32 @@ int fk_batch_step_times(fk_batch* b, int32_t* kinds, float* ms, size_t cap,
                : FK_OK;
 }
 
+int fk_probe_peaks(int device, double* valu_f64_per_s, double* valu_int_per_s,
+                   double* lds_bytes_per_s)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(FK_E_NODEV, "no HIP device visible");
+    if (device < 0 || device >= ndev)
+        return fail(FK_E_NODEV, "device %d not present (%d visible)", device,
+                    ndev);
+    HIPCHK(hipSetDevice(device));
+    int ncu = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                 device));
+    hipStream_t s = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    double f = 0.0, i = 0.0, l = 0.0;
+    const hipError_t err = probe_peaks(ncu, s, &f, &i, &l);
+    (void)hipStreamDestroy(s);
+    if (err != hipSuccess)
+        return fail(FK_E_DEVICE, "peak probes: %s", hipGetErrorString(err));
+    if (valu_f64_per_s) *valu_f64_per_s = f;
+    if (valu_int_per_s) *valu_int_per_s = i;
+    if (lds_bytes_per_s) *lds_bytes_per_s = l;
+    return FK_OK;
+}
+
 int fk_batch_window_wheel(fk_batch* b, size_t window, uint64_t* line_step)
 {
     if (!b || !line_step) return fail(FK_E_ARG, "null argument");

@@ -951,14 +951,13 @@ __device__ __forceinline__ IdxView load_idx(const FK_CONST uint32_t* row)
     return v;
 }
 
-// Each product a_k u_k is below 2^46 (q < 2^30) and the sum of eight below
-// 2^49, all exact in a double; s / q < 2^19 keeps resid_d1 exact.
-// t - n for the lane's value: each product a_k u_k is below 2^46 (q <
-// 2^30) and the sum of eight below 2^49, all exact in a double; s / q <
-// 2^19 keeps resid_d1 exact.
+// The lane's u for the row: s = sum a_k u_k (each product below 2^46 for
+// q < 2^30, the sum of eight below 2^49, all exact in a double), then u =
+// s - floor(s/q - 1/8) q, exact and in [0, 2q) as in resid_d1 (s / q <
+// 2^19).  t = s mod q is min(u, u - q) in unsigned arithmetic.
 template <int NL>
-__device__ __forceinline__ int32_t idx_t(const double (&a)[8], const IdxView& v,
-                                         const FK_CONST uint32_t* row)
+__device__ __forceinline__ uint32_t idx_u(const double (&a)[8], const IdxView& v,
+                                          const FK_CONST uint32_t* row)
 {
     double s = fma(a[3], v.u3, fma(a[2], v.u2, fma(a[1], v.u1, a[0] * v.u0)));
     if (NL > 4) {
@@ -966,23 +965,26 @@ __device__ __forceinline__ int32_t idx_t(const double (&a)[8], const IdxView& v,
         for (int k = 4; k < NL; k++)
             s = fma(a[k], load_f64(row + kIndexHead + 2 * (k - 4)), s);
     }
-    return (int32_t)resid_d1(s, v.qd, v.qi, v.q) - (int32_t)v.n;
+    const double qt = floor(fma(s, v.qi, -0.125));
+    return (uint32_t)fma(-qt, v.qd, s);
 }
 
 // KC candidates per lane against the index rows [lo, hi): R rows per pass
 // (their scalar loads in flight together), four for values below 2^64.
-// The row kills the lane when t - n < 0 (t < 2^30, n small): a per-lane
-// minimum of t - n on the vector unit, one ballot at the end.
+// A row kills the lane when t < n (n, the engine's depth, is every row's
+// bound): the lane keeps min over the rows of u and u - q (one three-way
+// minimum per row, the minimum of the t's), compared with n once at the
+// end and one ballot.
 template <int NL, int KC>
 __device__ __forceinline__ void index_scan(const double (&a)[KC][8],
                                            const FK_CONST uint32_t* kt,
                                            uint32_t rf, uint32_t lo, uint32_t hi,
-                                           uint64_t (&hit)[KC])
+                                           uint32_t n, uint64_t (&hit)[KC])
 {
     constexpr uint32_t R = NL > 4 ? 2 : 4;
-    int32_t acc[KC];
+    uint32_t acc[KC];
 #pragma unroll
-    for (int c = 0; c < KC; c++) acc[c] = 0x7fffffff;
+    for (int c = 0; c < KC; c++) acc[c] = 0xffffffffu;
     uint32_t p = lo;
     for (; p + R <= hi; p += R) {
         const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * kIndexStride;
@@ -992,17 +994,22 @@ __device__ __forceinline__ void index_scan(const double (&a)[KC][8],
 #pragma unroll
         for (uint32_t i = 0; i < R; i++)
 #pragma unroll
-            for (int c = 0; c < KC; c++)
-                acc[c] = min(acc[c], idx_t<NL>(a[c], v[i], r0 + i * kIndexStride));
+            for (int c = 0; c < KC; c++) {
+                const uint32_t u = idx_u<NL>(a[c], v[i], r0 + i * kIndexStride);
+                acc[c] = min(min(acc[c], u), u - v[i].q);      // v_min3_u32
+            }
     }
     for (; p < hi; p++) {
         const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * kIndexStride;
         const IdxView v = load_idx(r0);
 #pragma unroll
-        for (int c = 0; c < KC; c++) acc[c] = min(acc[c], idx_t<NL>(a[c], v, r0));
+        for (int c = 0; c < KC; c++) {
+            const uint32_t u = idx_u<NL>(a[c], v, r0);
+            acc[c] = min(min(acc[c], u), u - v.q);
+        }
     }
 #pragma unroll
-    for (int c = 0; c < KC; c++) hit[c] |= __ballot(acc[c] < 0);
+    for (int c = 0; c < KC; c++) hit[c] |= __ballot(acc[c] < n);
 }
 
 // One round, lane = candidate: blockIdx.x walks the engines' segments in
@@ -1102,8 +1109,9 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
 #pragma unroll
         for (int c = 0; c < KC; c++) m[c] = 0;
         if (ROWS == kRowsIndex) {
-            if (wwide) index_scan<8, KC>(l, kt, rf, lo, hi, m);
-            else index_scan<4, KC>(l, kt, rf, lo, hi, m);
+            const uint32_t n = (uint32_t)E->n;
+            if (wwide) index_scan<8, KC>(l, kt, rf, lo, hi, n, m);
+            else index_scan<4, KC>(l, kt, rf, lo, hi, n, m);
         } else if (kill_folded(E->family)) {
             if (wwide) round_scan<8, KG, true, KC>(l, pd, kt, st, rf, lo, hi, m);
             else round_scan<4, KG, true, KC>(l, pd, kt, st, rf, lo, hi, m);
@@ -1924,5 +1932,108 @@ hipError_t launch_gather(const GatherArgs& a, uint32_t grid,
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ peak probes
+// The rooflines' denominators, measured: the vector ALU's issue rate on
+// two dependent-chain loops (eight independent chains per lane, one VALU
+// instruction per step: a double FMA, as the round kernels' residues, and
+// a 32-bit funnel shift, as the sieve's words) and the LDS read bandwidth
+// (16-byte reads, consecutive lanes on consecutive addresses, no bank
+// conflicts).  The results stay in registers; a store no input can reach
+// keeps the loops.
+
+__global__ __launch_bounds__(256) void k_probe_valu_f64(uint32_t iters, double seed,
+                                                        double* out)
+{
+    double x[8];
+    const double m = 1.0 - 0x1p-40;
+#pragma unroll
+    for (int i = 0; i < 8; i++) x[i] = (double)(threadIdx.x + i) + seed;
+    for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) x[i] = fma(x[i], m, seed);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) s += x[i];
+    if (s == -1.0) out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_probe_valu_int(uint32_t iters, uint32_t seed,
+                                                        uint32_t* out)
+{
+    uint32_t x[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) x[i] = threadIdx.x * 2654435761u + i + seed;
+    for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) x[i] = __builtin_amdgcn_alignbit(x[i], seed, 7);
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) s ^= x[i];
+    if (s == 0x9e3779b9u && seed == 0u) out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+constexpr uint32_t kProbeLdsEntries = 2048;      // 16-byte entries: 32 KB
+
+__global__ __launch_bounds__(256) void k_probe_lds(uint32_t iters, uint32_t* out)
+{
+    __shared__ uint4 buf[kProbeLdsEntries];
+    for (uint32_t i = threadIdx.x; i < kProbeLdsEntries; i += 256)
+        buf[i] = make_uint4(i, 3 * i, 5 * i, 7 * i);
+    __syncthreads();
+    uint4 acc = make_uint4(0, 0, 0, 0);
+    for (uint32_t it = 0; it < iters; it++) {
+        // eight reads at 2 KB strides from a base that moves each pass
+        const uint4* p = buf + ((threadIdx.x + it * 64u) & 1023u);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint4 v = p[k * 128];
+            acc.x ^= v.x;
+            acc.y ^= v.y;
+            acc.z ^= v.z;
+            acc.w ^= v.w;
+        }
+    }
+    if ((acc.x ^ acc.y ^ acc.z ^ acc.w) == 0x9e3779b9u && iters == 0u)
+        out[blockIdx.x * blockDim.x + threadIdx.x] = acc.x;
+}
+
+hipError_t probe_peaks(int ncu, hipStream_t s, double* valu_f64, double* valu_int,
+                       double* lds)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    const uint32_t grid = (uint32_t)ncu * 8u;
+    const uint32_t it_valu = 8192, it_lds = 4096;
+    auto timed = [&](auto launch, double work, double* rate) {
+        for (int rep = 0; rep < 2 && err == hipSuccess; rep++) {
+            err = hipEventRecord(e0, s);
+            if (err == hipSuccess) {
+                launch();
+                err = hipGetLastError();
+            }
+            if (err == hipSuccess) err = hipEventRecord(e1, s);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+            if (err == hipSuccess && ms > 0.0f) *rate = work / (ms * 1e-3);
+        }
+    };
+    const double lanes = (double)grid * 256.0;
+    timed([&] { hipLaunchKernelGGL(k_probe_valu_f64, dim3(grid), dim3(256), 0, s,
+                                   it_valu, 0.5, (double*)nullptr); },
+          lanes * 8.0 * it_valu, valu_f64);
+    timed([&] { hipLaunchKernelGGL(k_probe_valu_int, dim3(grid), dim3(256), 0, s,
+                                   it_valu, 12345u, (uint32_t*)nullptr); },
+          lanes * 8.0 * it_valu, valu_int);
+    timed([&] { hipLaunchKernelGGL(k_probe_lds, dim3(grid), dim3(256), 0, s,
+                                   it_lds, (uint32_t*)nullptr); },
+          lanes * 8.0 * 16.0 * it_lds, lds);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return err;
+}
 
 }  // namespace fk

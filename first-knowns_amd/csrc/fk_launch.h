@@ -152,6 +152,11 @@ hipError_t launch_gather(const GatherArgs& a, uint32_t grid,
 hipError_t launch_classify(const ClassifyArgs& a, uint32_t grid,
                            hipStream_t stream);
 hipError_t launch_zero_u64(uint64_t* p, size_t n, hipStream_t stream);
+// Measured peaks (one probe launch each, timed twice with HIP events on
+// `s`): VALU lane-operations per second for double FMAs and 32-bit funnel
+// shifts, LDS bytes read per second.
+hipError_t probe_peaks(int ncu, hipStream_t s, double* valu_f64, double* valu_int,
+                       double* lds);
 hipError_t launch_wheel_expand(const uint64_t* base, uint64_t nbase,
                                const uint32_t* lifts, uint32_t nj, uint32_t L,
                                uint64_t Mb, uint64_t* out, hipStream_t stream);

@@ -38,7 +38,7 @@ EXPORTS = ("fk_abi_version", "fk_last_error", "fk_device_count",
            "fk_batch_reruns", "fk_batch_set_budget", "fk_batch_cursor_get",
            "fk_batch_cursor_set", "fk_batch_resume",
            "fk_engine_classify_static", "fk_batch_step_times",
-           "fk_batch_window_wheel")
+           "fk_batch_window_wheel", "fk_probe_peaks")
 
 u32, i32, u64 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
 
@@ -129,6 +129,8 @@ def lib():
     L.fk_engine_classify_static.argtypes = [vp, vp, sz, P(ClassifySpec), vp]
     L.fk_batch_step_times.argtypes = [vp, vp, vp, sz, P(sz)]
     L.fk_batch_window_wheel.argtypes = [vp, sz, P(u64)]
+    L.fk_probe_peaks.argtypes = [ctypes.c_int, P(ctypes.c_double),
+                                 P(ctypes.c_double), P(ctypes.c_double)]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
     if L.fk_abi_version() != ABI_VERSION:
@@ -381,6 +383,15 @@ class Engine:
         d = u64(0)
         check(lib().fk_batch_window_wheel(b, 0, ctypes.byref(d)))
         return int(d.value)
+
+
+def probe_peaks(device=0):
+    """Measured peaks: (VALU lane-ops/s for double FMAs, VALU lane-ops/s
+    for 32-bit funnel shifts, LDS bytes/s)."""
+    f, i, l = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    check(lib().fk_probe_peaks(int(device), ctypes.byref(f), ctypes.byref(i),
+                               ctypes.byref(l)))
+    return f.value, i.value, l.value
 
 
 def batch_counts(handle):

@@ -370,6 +370,20 @@ int fk_batch_cursor_get(fk_batch* b, size_t window, uint64_t next[2]);
 int fk_batch_cursor_set(fk_batch* b, size_t window, const uint64_t next[2]);
 int fk_batch_resume(fk_batch* b);
 
+/*
+ * The device's measured peaks, the denominators of the rooflines beside
+ * the published ones: the vector ALU's issue rate in lane-operations per
+ * second (eight independent dependent chains per lane over every CU, of
+ * double-precision FMAs and of 32-bit funnel shifts: the two kinds of
+ * instruction the round kernels and the sieve spend their time in) and
+ * the LDS read bandwidth in bytes per second (16-byte reads, no bank
+ * conflicts).  Any output pointer may be NULL.  Takes a few
+ * milliseconds.  No reference counterpart (the reference prices its
+ * kernels against spec-sheet peaks).
+ */
+int fk_probe_peaks(int device, double* valu_f64_per_s, double* valu_int_per_s,
+                   double* lds_bytes_per_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,5 +1,10 @@
 """Parity at the benchmark configurations' own parameters.
 
+* Euler-form runs, n = 8, the 13# wheel (180 of 30,030 residues), every
+  prime below 2^16 sieved, 64 seeded windows of 2^24 consecutive p in
+  [1e12, 1e13]: each window's survivors and its hits (run >= 8 over x =
+  0..10, cap 11, all seven bases) against the oracle, window by window
+  through the drop-in and all 64 in one batch submission.
 * Dickson ladders, n = 10, every prime below 2^22 sieved, a full seeded
   window of 2^32 consecutive j near k = 1e15: the survivor stream against
   the oracle's sieve, and every survivor's ladder depth (all seven bases,
@@ -52,6 +57,48 @@ def threads():
 
 LANE_PRIME = {"round_lane_prime": 1}
 LANE_CAND = {"round_lane_prime": 1 << 30}
+
+# ----------------------------------------------------------------- Euler
+
+C0_N, C0_CAP, C0_SPAN, C0_WHEEL = 8, 11, 1 << 24, [2, 3, 5, 7, 11, 13]
+
+
+def c0_window(seed):
+    lo = 10**12 + splitmix64(seed) % (9 * 10**12)
+    return lo, lo + C0_SPAN
+
+
+def test_config0_windows_match_oracle():
+    import euler_gpu
+    eng = euler_gpu.GpuEngine(C0_N, wheel_primes=C0_WHEEL)
+    assert (eng.n_offs, eng.M) == (180, 30030)
+    wins = [c0_window(s) for s in range(64)]
+    say("config 0", len(wins), "windows")
+    want_s, want_h = [], []
+    for lo, hi in wins:
+        surv = O.euler_survivors(C0_N, lo, hi, wheel_primes=C0_WHEEL,
+                                 q2=1 << 16, nthreads=threads())
+        runs = [O.euler_run(p, cap=C0_CAP) for p in surv]
+        want_s.append(surv)
+        want_h.append([(p, r) for p, r in zip(surv, runs) if r >= C0_N])
+    for (lo, hi), ws, wh in zip(wins, want_s, want_h):
+        assert eng.survivors_pre_mr(lo, hi) == ws, lo
+        assert eng.hunt(lo, hi, cap=C0_CAP) == wh, lo
+    # all 64 windows in one batch: the same streams and runs
+    b = N.Batch()
+    b.submit([eng._eng], [0] * len(wins), wins,
+             [(N.CLASSIFY_FULL, C0_CAP, 0)] * len(wins))
+    recs, counts = b.results()
+    pos = 0
+    for i, ws in enumerate(want_s):
+        mine = recs[pos:pos + int(counts[i])]
+        pos += int(counts[i])
+        assert N.record_values(mine) == ws
+        assert mine["run"].tolist() == [O.euler_run(p, cap=C0_CAP)
+                                        for p in ws]
+    assert sum(len(w) for w in want_s) >= 40
+    assert sum(len(w) for w in want_h) >= 1
+
 
 # --------------------------------------------------------------- Dickson
 
