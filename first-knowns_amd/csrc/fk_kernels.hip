@@ -1185,13 +1185,13 @@ __device__ __forceinline__ void load_lane_row_wide(LaneRow& L, uint32_t kmax)
         L.k[i] = ((uint32_t)i < kmax) ? res[i] : kKillPad;
 }
 
-// Does the lane's prime kill a candidate of residue-limbs a[] (wave
-// uniform, read from LDS)?  ncmp (wave uniform, a multiple of 4) residues
-// are compared from registers; residues past 32 (kill sets larger than 32)
-// from memory.
+// The lane's prime against a candidate of residue-limbs a[] (wave
+// uniform, read from LDS): zero when the prime kills it.  ncmp (wave
+// uniform, a multiple of 4) residues are compared from registers; residues
+// past 32 (kill sets larger than 32) from memory.
 template <int NL, bool FOLD>
-__device__ __forceinline__ bool lane_kills(const LaneRow& L, const double* a,
-                                           uint32_t ncmp)
+__device__ __forceinline__ uint32_t lane_kills(const LaneRow& L, const double* a,
+                                               uint32_t ncmp)
 {
     // NL == 1: a[0] is the value itself (below 2^53, primes at least 32)
     double s = a[0];
@@ -1215,7 +1215,7 @@ __device__ __forceinline__ bool lane_kills(const LaneRow& L, const double* a,
         for (uint32_t c = kRegResidues; c < L.cnt; c++)
             d = min(d, r ^ L.row[kRowHead + c]);
     }
-    return d == 0;
+    return d;
 }
 
 // One round, lane = prime: blockIdx.x picks 256 primes (one per lane),
@@ -1224,49 +1224,47 @@ __device__ __forceinline__ bool lane_kills(const LaneRow& L, const double* a,
 // candidate's 16-bit limbs (as doubles) in LDS once; each wave then walks
 // them with broadcast LDS reads, every lane testing its own prime: the
 // exact double-precision residue (resid_d1) and one compare per killing
-// residue.  A candidate that some lane's prime kills is flagged by the
-// lowest such lane.  There is no global memory access per (candidate,
-// prime).
+// residue.  A candidate that some lane's prime kills is flagged by every
+// such lane (each storing 1).  There is no global memory access per
+// (candidate, prime).
 constexpr uint32_t kRoundSlice = 256;
 
 // One slice's candidates against the lane's prime; candidate c's input is
 // a + step * c, and kills(x) says whether the lane's prime kills the
-// candidate of input x.
+// candidate of input x.  Every lane computes (lanes past the round's
+// primes hold a row that kills nothing), with no per-lane branch around
+// the test, and the flags are written only when one of two candidates in
+// a row is killed, which is rare: one ballot per two candidates.
 template <typename Kills>
-__device__ __forceinline__ void lane_prime_walk(Kills kills, bool mine,
-                                                const double* a, uint32_t step,
+__device__ __forceinline__ void lane_prime_walk(Kills kills, const double* a,
+                                                uint32_t step,
                                                 const uint32_t* live_idx,
                                                 uint32_t nlive, uint32_t* flags)
 {
-    const uint32_t lane = __lane_id();
-    // four candidates per step: four independent residue chains in flight
-    // per lane instead of one
-    constexpr uint32_t U = 4;
+    // two candidates per step: two independent residue chains in flight
+    // per lane (four hold 120 VGPRs in the index form, half the waves)
+    constexpr uint32_t U = 2;
     uint32_t c = 0;
     for (; c + U <= nlive; c += U) {
         bool kill[U];
 #pragma unroll
-        for (uint32_t u = 0; u < U; u++)
-            kill[u] = mine && kills(a + step * (c + u));
+        for (uint32_t u = 0; u < U; u++) kill[u] = kills(a + step * (c + u));
+        if (__ballot(kill[0] | kill[1])) {
 #pragma unroll
-        for (uint32_t u = 0; u < U; u++) {
-            const uint64_t m = __ballot(kill[u]);
-            if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1))
-                flags[live_idx[c + u]] = 1;
+            for (uint32_t u = 0; u < U; u++)
+                if (kill[u]) flags[live_idx[c + u]] = 1;
         }
     }
-    for (; c < nlive; c++) {
-        const bool kill = mine && kills(a + step * c);
-        const uint64_t m = __ballot(kill);
-        if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1))
-            flags[live_idx[c]] = 1;
-    }
+    for (; c < nlive; c++)
+        if (kills(a + step * c)) flags[live_idx[c]] = 1;
 }
 
 // Residue rows: 16-bit limbs (8 past 2^64, else 4), or with every value
 // below 2^53 the value itself (NL == 1).
+// miss: 0 for a lane whose prime is in the round, 1 (the test never
+// reaches zero) for the others.
 template <bool FOLD>
-__device__ __forceinline__ void lane_prime_slice(const LaneRow& L, bool mine,
+__device__ __forceinline__ void lane_prime_slice(const LaneRow& L, uint32_t miss,
                                                  uint32_t ncmp, bool wide,
                                                  bool narrow,
                                                  const double* limbs,
@@ -1275,14 +1273,17 @@ __device__ __forceinline__ void lane_prime_slice(const LaneRow& L, bool mine,
                                                  uint32_t nlive, uint32_t* flags)
 {
     if (wide)
-        lane_prime_walk([&](const double* x) { return lane_kills<8, FOLD>(L, x, ncmp); },
-                        mine, limbs, 8, live_idx, nlive, flags);
+        lane_prime_walk([&](const double* x) {
+                            return (lane_kills<8, FOLD>(L, x, ncmp) | miss) == 0u; },
+                        limbs, 8, live_idx, nlive, flags);
     else if (narrow)
-        lane_prime_walk([&](const double* x) { return lane_kills<1, FOLD>(L, x, ncmp); },
-                        mine, vals, 1, live_idx, nlive, flags);
+        lane_prime_walk([&](const double* x) {
+                            return (lane_kills<1, FOLD>(L, x, ncmp) | miss) == 0u; },
+                        vals, 1, live_idx, nlive, flags);
     else
-        lane_prime_walk([&](const double* x) { return lane_kills<4, FOLD>(L, x, ncmp); },
-                        mine, limbs, 8, live_idx, nlive, flags);
+        lane_prime_walk([&](const double* x) {
+                            return (lane_kills<4, FOLD>(L, x, ncmp) | miss) == 0u; },
+                        limbs, 8, live_idx, nlive, flags);
 }
 
 // An index row (AP) in a lane's registers.
@@ -1317,13 +1318,16 @@ __device__ __forceinline__ void load_lane_idx(LaneIdx& I, const uint32_t* row)
     I.n = h.n;
 }
 
+// t < nlim: nlim is the row's n for a lane whose prime is in the round, 0
+// (nothing killed) for the others.
 template <int NL>
-__device__ __forceinline__ bool lane_idx_kills(const LaneIdx& I, const double* a)
+__device__ __forceinline__ bool lane_idx_kills(const LaneIdx& I, const double* a,
+                                               uint32_t nlim)
 {
     double s = a[0] * I.u[0];
 #pragma unroll
     for (int k = 1; k < NL; k++) s = fma(a[k], I.u[k], s);
-    return resid_d1(s, I.qd, I.qi, I.q) < I.n;
+    return resid_d1(s, I.qd, I.qi, I.q) < nlim;
 }
 
 template <uint32_t ROWS>
@@ -1383,33 +1387,40 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
             // compares per test: the family's largest kill set, in fours
             const uint32_t ncmp = (kill_max(engs[e].family, engs[e].n) + 3u) & ~3u;
             const uint32_t stride = engs[e].kstride;
-            if (ROWS == kRowsResidue && mine) {
-                load_lane_row(L, engs[e].kt + (size_t)(p - engs[e].round_first) * stride,
-                              a.primed + p, stride);
+            // lanes past the round's primes hold the row of its last prime
+            // (in the table) and are masked in the test: no lane branches
+            // around it (the workgroup holds some of the round's primes)
+            const uint32_t pc = min(max(p, rlo), rhi - 1u);
+            if (ROWS == kRowsResidue && __ballot(mine)) {
+                load_lane_row(L, engs[e].kt + (size_t)(pc - engs[e].round_first) * stride,
+                              a.primed + pc, stride);
                 if (ncmp > 20) load_lane_row_wide(L, stride - kRowHead);
             }
             // the one-value form needs every value below 2^53 and every
             // prime of the round at least 32
             const bool narrow = below53 && a.q_min >= 32u;
             if (ROWS == kRowsIndex) {
-                LaneIdx I;
-                if (mine)
-                    load_lane_idx(I, engs[e].kt + (size_t)(p - engs[e].round_first) *
-                                                      kIndexStride);
                 if (__ballot(mine)) {
+                    LaneIdx I;
+                    load_lane_idx(I, engs[e].kt + (size_t)(pc - engs[e].round_first) *
+                                                      kIndexStride);
+                    const uint32_t nlim = mine ? I.n : 0u;
                     if (wide)
-                        lane_prime_walk([&](const double* x) { return lane_idx_kills<8>(I, x); },
-                                        mine, sh_limbs, 8, sh_idx, nlive, a.flags);
+                        lane_prime_walk([&](const double* x) {
+                                            return lane_idx_kills<8>(I, x, nlim); },
+                                        sh_limbs, 8, sh_idx, nlive, a.flags);
                     else
-                        lane_prime_walk([&](const double* x) { return lane_idx_kills<4>(I, x); },
-                                        mine, sh_limbs, 8, sh_idx, nlive, a.flags);
+                        lane_prime_walk([&](const double* x) {
+                                            return lane_idx_kills<4>(I, x, nlim); },
+                                        sh_limbs, 8, sh_idx, nlive, a.flags);
                 }
             } else if (__ballot(mine)) {
+                const uint32_t miss = mine ? 0u : 1u;
                 if (kill_folded(engs[e].family))
-                    lane_prime_slice<true>(L, mine, ncmp, wide, narrow, sh_limbs,
+                    lane_prime_slice<true>(L, miss, ncmp, wide, narrow, sh_limbs,
                                            sh_val, sh_idx, nlive, a.flags);
                 else
-                    lane_prime_slice<false>(L, mine, ncmp, wide, narrow, sh_limbs,
+                    lane_prime_slice<false>(L, miss, ncmp, wide, narrow, sh_limbs,
                                             sh_val, sh_idx, nlive, a.flags);
             }
         }
