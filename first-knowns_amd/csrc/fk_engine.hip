@@ -604,6 +604,8 @@ struct Step {
                           kGather, kClassify };
     int32_t kind;
     int32_t ev;                      // kMark: event index
+    int32_t side;                    // kRound: launched on the side stream,
+                                     // beside the round's other launch
     uint32_t grid, gy;
     int32_t threads, wpt;
     size_t lds;
@@ -645,18 +647,36 @@ static int32_t step_kind(const Step& st)
     }
 }
 
+// The side stream of a batch whose rounds test both kinds of row: a round's
+// two launches touch disjoint queue segments, so one runs on the side
+// stream (forked from the batch's stream before the other is issued) and
+// the next step waits for it.  In a captured graph they are two parallel
+// nodes.
+struct SideStream {
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
 // Issue the recorded steps on `s`; stage events only when `events`, and
 // with `step_ev` (two events per launch step) a start and a stop event for
 // every launch step.  The sieve, round and compaction kernels carry theirs
 // into the launch, which stamps them at the kernel's own start and end; the
 // other steps (several kernels or a copy) are bracketed by recorded events.
 static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
-                             bool events, hipStream_t s,
+                             bool events, hipStream_t s, const SideStream& side,
                              const std::vector<hipEvent_t>* step_ev = nullptr)
 {
     hipError_t e = hipSuccess;
     size_t k = 0;
+    bool joined = true;
     for (const Step& st : plan) {
+        const bool on_side = st.kind == Step::kRound && st.side && side.stream;
+        if (!joined && !(st.kind == Step::kRound && !st.side)) {
+            // the side launch is done before anything after the round
+            e = hipStreamWaitEvent(s, side.join, 0);
+            if (e != hipSuccess) return e;
+            joined = true;
+        }
         if (st.kind == Step::kMark) {
             if (events) e = hipEventRecord(ev[st.ev], s);
             if (e != hipSuccess) return e;
@@ -669,6 +689,17 @@ static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
             k++;
         }
         const LaunchEvents* lp = step_ev ? &le : nullptr;
+        if (on_side) {
+            e = hipEventRecord(side.fork, s);
+            if (e == hipSuccess) e = hipStreamWaitEvent(side.stream, side.fork, 0);
+            if (e == hipSuccess)
+                e = launch_round_test(st.ra, st.lane_prime != 0, st.grid, st.gy,
+                                      side.stream, lp);
+            if (e == hipSuccess) e = hipEventRecord(side.join, side.stream);
+            if (e != hipSuccess) return e;
+            joined = false;
+            continue;
+        }
         const bool bracket = step_ev && (st.kind == Step::kGather ||
                                          st.kind == Step::kClassify ||
                                          st.kind == Step::kReadback);
@@ -703,12 +734,14 @@ static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
         if (e == hipSuccess && bracket) e = hipEventRecord(le.stop, s);
         if (e != hipSuccess) return e;
     }
-    return hipSuccess;
+    if (!joined) e = hipStreamWaitEvent(s, side.join, 0);
+    return e;
 }
 
 struct fk_batch {
     int device = 0;
     hipStream_t stream = nullptr;
+    SideStream side;                  // made when a plan first needs it
     hipEvent_t ev[6] = {};
     // eager submissions: a start and a stop event for every launch step
     // (grow-only), the steps' kinds and, once collected, their durations
@@ -791,6 +824,10 @@ static int batch_free(fk_batch* b)
     for (auto& e : b->step_ev) if (e) (void)hipEventDestroy(e);
     b->step_ev.clear();
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->side.stream) (void)hipStreamDestroy(b->side.stream);
+    if (b->side.fork) (void)hipEventDestroy(b->side.fork);
+    if (b->side.join) (void)hipEventDestroy(b->side.join);
+    b->side = SideStream{};
     return FK_OK;
 }
 
@@ -1458,6 +1495,12 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             plan.push_back(rst);
         }
         if (!any_round) continue;
+        // a round of both row kinds is two launches on disjoint segments:
+        // the first goes to the side stream, beside the second
+        if (plan.size() >= 2 && plan[plan.size() - 2].kind == Step::kRound &&
+            plan[plan.size() - 2].ra.round == r &&
+            plan.back().kind == Step::kRound && plan.back().ra.round == r)
+            plan[plan.size() - 2].side = 1;
         alive = alive_r;
         Step cst = new_step(Step::kCompact);
         CompactArgs& ca = cst.ca;
@@ -1568,6 +1611,13 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     // launch sequence repeats (captured on its second submission in a row,
     // replayed while it stays the same), and eagerly without stage events
     // while it has not repeated, so one-off sequences pay no capture
+    bool need_side = false;
+    for (const Step& st : plan) need_side = need_side || (st.kind == Step::kRound && st.side);
+    if (need_side && !b->side.stream) {
+        HIPCHK(hipStreamCreateWithFlags(&b->side.stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&b->side.fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&b->side.join, hipEventDisableTiming));
+    }
     b->graph_ran = false;
     b->staged = false;
     if (b->use_graph) {
@@ -1581,7 +1631,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             b->graph = nullptr;
             b->graph_plan.clear();
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            const hipError_t ce = emit_steps(plan, b->ev, false, s);
+            const hipError_t ce = emit_steps(plan, b->ev, false, s, b->side);
             hipGraph_t g = nullptr;
             const hipError_t ee = hipStreamEndCapture(s, &g);
             if (ce != hipSuccess || ee != hipSuccess) {
@@ -1600,7 +1650,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             b->graph_launches++;
             b->graph_ran = true;
         } else {
-            HIPCHK(emit_steps(plan, b->ev, false, s));
+            HIPCHK(emit_steps(plan, b->ev, false, s, b->side));
         }
         HIPCHK(hipEventRecord(b->ev[5], s));
         b->prev_plan = plan;
@@ -1613,7 +1663,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             HIPCHK(hipEventCreate(&e));
             b->step_ev.push_back(e);
         }
-        HIPCHK(emit_steps(plan, b->ev, true, s, &b->step_ev));
+        HIPCHK(emit_steps(plan, b->ev, true, s, b->side, &b->step_ev));
         b->staged = true;
     }
     b->final_slot = total_slot;
