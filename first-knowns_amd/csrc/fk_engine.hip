@@ -668,11 +668,15 @@ static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
 {
     hipError_t e = hipSuccess;
     size_t k = 0;
+    // a parallel section: side launches of one kind (forked once, in a row
+    // on the side stream), then the batch stream's launches of that kind;
+    // the first step of another kind waits for the side stream
     bool joined = true;
+    int32_t section = 0;
     for (const Step& st : plan) {
-        const bool on_side = st.kind == Step::kRound && st.side && side.stream;
-        if (!joined && !(st.kind == Step::kRound && !st.side)) {
-            // the side launch is done before anything after the round
+        const bool on_side = (st.kind == Step::kRound || st.kind == Step::kSieve) &&
+                             st.side && side.stream;
+        if (!joined && !(st.kind == section && (on_side || !st.side))) {
             e = hipStreamWaitEvent(s, side.join, 0);
             if (e != hipSuccess) return e;
             joined = true;
@@ -690,14 +694,22 @@ static hipError_t emit_steps(const std::vector<Step>& plan, hipEvent_t* ev,
         }
         const LaunchEvents* lp = step_ev ? &le : nullptr;
         if (on_side) {
-            e = hipEventRecord(side.fork, s);
-            if (e == hipSuccess) e = hipStreamWaitEvent(side.stream, side.fork, 0);
+            if (joined) {
+                e = hipEventRecord(side.fork, s);
+                if (e == hipSuccess) e = hipStreamWaitEvent(side.stream, side.fork, 0);
+            }
             if (e == hipSuccess)
-                e = launch_round_test(st.ra, st.lane_prime != 0, st.grid, st.gy,
-                                      side.stream, lp);
+                e = st.kind == Step::kRound
+                        ? launch_round_test(st.ra, st.lane_prime != 0, st.grid,
+                                            st.gy, side.stream, lp)
+                        : (st.wpt == 0 ? launch_sieve_short(st.sa, st.threads, st.grid,
+                                                            st.lds, side.stream, lp)
+                                       : launch_sieve(st.sa, st.threads, st.wpt, st.grid,
+                                                      st.lds, side.stream, lp));
             if (e == hipSuccess) e = hipEventRecord(side.join, side.stream);
             if (e != hipSuccess) return e;
             joined = false;
+            section = st.kind;
             continue;
         }
         const bool bracket = step_ev && (st.kind == Step::kGather ||
@@ -1413,6 +1425,21 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
         st.lds = lds;
         plan.push_back(st);
     }
+    // the sieve launches write disjoint records through atomic slots, so
+    // they need no order: the one of most tiles runs on the batch's stream,
+    // the others (issued first) beside it on the side stream
+    {
+        size_t first = plan.size();
+        while (first > 0 && plan[first - 1].kind == Step::kSieve) first--;
+        if (plan.size() - first >= 2) {
+            size_t big = first;
+            auto tiles = [](const Step& st) { return st.sa.ntiles - st.sa.tile_begin; };
+            for (size_t i = first; i < plan.size(); i++)
+                if (tiles(plan[i]) > tiles(plan[big])) big = i;
+            std::rotate(plan.begin() + big, plan.begin() + big + 1, plan.end());
+            for (size_t i = first; i + 1 < plan.size(); i++) plan[i].side = 1;
+        }
+    }
     mark(2);
 
     // ---- rounds
@@ -1612,7 +1639,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     // replayed while it stays the same), and eagerly without stage events
     // while it has not repeated, so one-off sequences pay no capture
     bool need_side = false;
-    for (const Step& st : plan) need_side = need_side || (st.kind == Step::kRound && st.side);
+    for (const Step& st : plan) need_side = need_side || st.side;
     if (need_side && !b->side.stream) {
         HIPCHK(hipStreamCreateWithFlags(&b->side.stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&b->side.fork, hipEventDisableTiming));
