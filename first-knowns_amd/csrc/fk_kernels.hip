@@ -1012,6 +1012,82 @@ __device__ __forceinline__ void index_scan(const double (&a)[KC][8],
     for (int c = 0; c < KC; c++) hit[c] |= __ballot(acc[c] < n);
 }
 
+// The same test in 22-bit limbs (launches whose primes are all below 2^24;
+// values below 2^88): NL limbs a_k < 2^22 against the row's u 2^(22k) mod q,
+// three for values below 2^66, four below 2^88.
+struct IdxView22 {
+    double qd, qi, v0, v1, v2, v3;
+    uint32_t q;
+};
+
+__device__ __forceinline__ IdxView22 load_idx22(const FK_CONST uint32_t* row)
+{
+    IdxView22 v;
+    v.qd = load_f64(row + 0);
+    v.qi = load_f64(row + 2);
+    v.v0 = load_f64(row + kIndexV);
+    v.v1 = load_f64(row + kIndexV + 2);
+    v.v2 = load_f64(row + kIndexV + 4);
+    v.v3 = load_f64(row + kIndexV + 6);
+    v.q = row[12];
+    return v;
+}
+
+template <int NL>
+__device__ __forceinline__ uint32_t idx_u22(const double (&a)[4], const IdxView22& v)
+{
+    double s = fma(a[2], v.v2, fma(a[1], v.v1, a[0] * v.v0));
+    if (NL > 3) s = fma(a[3], v.v3, s);
+    const double qt = floor(fma(s, v.qi, -0.125));
+    return (uint32_t)fma(-qt, v.qd, s);
+}
+
+template <int NL, int KC>
+__device__ __forceinline__ void index_scan22(const double (&a)[KC][4],
+                                             const FK_CONST uint32_t* kt,
+                                             uint32_t rf, uint32_t lo, uint32_t hi,
+                                             uint32_t n, uint64_t (&hit)[KC])
+{
+    constexpr uint32_t R = 4;
+    uint32_t acc[KC];
+#pragma unroll
+    for (int c = 0; c < KC; c++) acc[c] = 0xffffffffu;
+    uint32_t p = lo;
+    for (; p + R <= hi; p += R) {
+        const FK_CONST uint32_t* r0 = kt + (size_t)(p - rf) * kIndexStride;
+        IdxView22 v[R];
+#pragma unroll
+        for (uint32_t i = 0; i < R; i++) v[i] = load_idx22(r0 + i * kIndexStride);
+#pragma unroll
+        for (uint32_t i = 0; i < R; i++)
+#pragma unroll
+            for (int c = 0; c < KC; c++) {
+                const uint32_t u = idx_u22<NL>(a[c], v[i]);
+                acc[c] = min(min(acc[c], u), u - v[i].q);
+            }
+    }
+    for (; p < hi; p++) {
+        const IdxView22 v = load_idx22(kt + (size_t)(p - rf) * kIndexStride);
+#pragma unroll
+        for (int c = 0; c < KC; c++) {
+            const uint32_t u = idx_u22<NL>(a[c], v);
+            acc[c] = min(min(acc[c], u), u - v.q);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < KC; c++) hit[c] |= __ballot(acc[c] < n);
+}
+
+// A value's 22-bit limbs (below 2^88: t[3] == 0, t[2] < 2^24).
+__device__ __forceinline__ void limbs22(const uint32_t (&t)[4], double (&a)[4])
+{
+    constexpr uint32_t M = 0x3fffffu;
+    a[0] = (double)(t[0] & M);
+    a[1] = (double)(((t[0] >> 22) | (t[1] << 10)) & M);
+    a[2] = (double)(((t[1] >> 12) | (t[2] << 20)) & M);
+    a[3] = (double)((t[2] >> 2) & M);
+}
+
 // One round, lane = candidate: blockIdx.x walks the engines' segments in
 // blocks of 256 KC candidates (grid-stride; lane l takes entries l,
 // l + 256, ...), blockIdx.y takes a block of pblock primes.  The engine is
@@ -1093,6 +1169,33 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
                 if (live[c] && ((m[c] >> __lane_id()) & 1)) a.flags[idx[c]] = 1;
             touch ^= tv;
             continue;
+        }
+        if (ROWS == kRowsIndex && a.idx22) {
+            // every prime of the launch below 2^24: 22-bit limbs, three or
+            // four, unless some live value reaches 2^88
+            bool huge = false, w66 = false;
+#pragma unroll
+            for (int c = 0; c < KC; c++) {
+                huge = huge || (live[c] && (t[c][3] != 0 || t[c][2] >= (1u << 24)));
+                w66 = w66 || (live[c] && t[c][2] >= 4u);
+            }
+            if (!__ballot(huge)) {
+                double l22[KC][4];
+#pragma unroll
+                for (int c = 0; c < KC; c++) limbs22(t[c], l22[c]);
+                const FK_CONST uint32_t* kt = uniform_view(E->kt);
+                const uint32_t n = (uint32_t)E->n;
+                uint64_t m[KC];
+#pragma unroll
+                for (int c = 0; c < KC; c++) m[c] = 0;
+                if (__ballot(w66)) index_scan22<4, KC>(l22, kt, E->round_first, lo, hi, n, m);
+                else index_scan22<3, KC>(l22, kt, E->round_first, lo, hi, n, m);
+#pragma unroll
+                for (int c = 0; c < KC; c++)
+                    if (live[c] && ((m[c] >> __lane_id()) & 1)) a.flags[idx[c]] = 1;
+                touch ^= tv;
+                continue;
+            }
         }
         double l[KC][8];
 #pragma unroll
@@ -1290,6 +1393,7 @@ __device__ __forceinline__ void lane_prime_slice(const LaneRow& L, uint32_t miss
 struct LaneIdx {
     double qd, qi;
     double u[8];                  // u 2^(16k) mod q
+    double v[4];                  // u 2^(22k) mod q
     uint32_t q, n;
 };
 
@@ -1313,6 +1417,7 @@ __device__ __forceinline__ void load_lane_idx(LaneIdx& I, const uint32_t* row)
     for (int k = 0; k < 4; k++) {
         I.u[k] = h.u[k];
         I.u[4 + k] = h.uw[k];
+        I.v[k] = h.v[k];
     }
     I.q = h.q;
     I.n = h.n;
@@ -1330,13 +1435,25 @@ __device__ __forceinline__ bool lane_idx_kills(const LaneIdx& I, const double* a
     return resid_d1(s, I.qd, I.qi, I.q) < nlim;
 }
 
+// The same in 22-bit limbs (every prime of the launch below 2^24).
+template <int NL>
+__device__ __forceinline__ bool lane_idx_kills22(const LaneIdx& I, const double* a,
+                                                 uint32_t nlim)
+{
+    double s = a[0] * I.v[0];
+#pragma unroll
+    for (int k = 1; k < NL; k++) s = fma(a[k], I.v[k], s);
+    return resid_d1(s, I.qd, I.qi, I.q) < nlim;
+}
+
 template <uint32_t ROWS>
 __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
 {
     __shared__ double sh_limbs[kRoundSlice * 8];
     __shared__ double sh_val[kRoundSlice];      // the value, when below 2^53
+    __shared__ double sh_l22[ROWS == kRowsIndex ? kRoundSlice * 4 : 1];
     __shared__ uint32_t sh_idx[kRoundSlice];
-    __shared__ uint32_t sh_n, sh_wide, sh_wide53;
+    __shared__ uint32_t sh_n, sh_wide, sh_wide53, sh_w66, sh_huge;
     const uint32_t p = a.p_begin + blockIdx.x * 256 + threadIdx.x;
     const uint32_t blk_p0 = a.p_begin + blockIdx.x * 256;
     if (blk_p0 >= a.p_end) return;                // whole workgroup past it
@@ -1357,7 +1474,13 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
         if (blk_p0 + 256 <= rlo || blk_p0 >= rhi) continue;
         const bool mine = p >= rlo && p < rhi && p < a.p_end;
         // stage the slice's live candidates: limbs and queue index
-        if (threadIdx.x == 0) { sh_n = 0; sh_wide = 0; sh_wide53 = 0; }
+        if (threadIdx.x == 0) {
+            sh_n = 0;
+            sh_wide = 0;
+            sh_wide53 = 0;
+            sh_w66 = 0;
+            sh_huge = 0;
+        }
         __syncthreads();
         {
             const uint32_t li = sb.lb * kRoundSlice + threadIdx.x;
@@ -1376,6 +1499,15 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
                     if ((rec.t[2] | rec.t[3]) != 0) sh_wide = 1;
                     if ((rec.t[2] | rec.t[3]) != 0 || rec.t[1] >= (1u << 21))
                         sh_wide53 = 1;
+                    if (ROWS == kRowsIndex) {
+                        const uint32_t tt[4] = {rec.t[0], rec.t[1], rec.t[2], rec.t[3]};
+                        double l22[4];
+                        limbs22(tt, l22);
+#pragma unroll
+                        for (int k = 0; k < 4; k++) sh_l22[4 * slot + k] = l22[k];
+                        if (rec.t[2] >= 4u || rec.t[3] != 0) sh_w66 = 1;
+                        if (rec.t[3] != 0 || rec.t[2] >= (1u << 24)) sh_huge = 1;
+                    }
                 }
             }
         }
@@ -1405,7 +1537,16 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
                     load_lane_idx(I, engs[e].kt + (size_t)(pc - engs[e].round_first) *
                                                       kIndexStride);
                     const uint32_t nlim = mine ? I.n : 0u;
-                    if (wide)
+                    if (a.idx22 && sh_huge == 0) {
+                        if (sh_w66)
+                            lane_prime_walk([&](const double* x) {
+                                                return lane_idx_kills22<4>(I, x, nlim); },
+                                            sh_l22, 4, sh_idx, nlive, a.flags);
+                        else
+                            lane_prime_walk([&](const double* x) {
+                                                return lane_idx_kills22<3>(I, x, nlim); },
+                                            sh_l22, 4, sh_idx, nlive, a.flags);
+                    } else if (wide)
                         lane_prime_walk([&](const double* x) {
                                             return lane_idx_kills<8>(I, x, nlim); },
                                         sh_limbs, 8, sh_idx, nlive, a.flags);

@@ -16,6 +16,8 @@
 // value min(r, q - r) per pair, and a residue x is tested as min(x, q - x).
 #pragma once
 
+#include <stddef.h>
+
 #include "fk_arith.h"
 
 namespace fk {
@@ -172,7 +174,10 @@ FK_HD void round_row(int family, int n, const uint32_t d[4], const PrimeK& p,
 // limbs: t = sum_k a_k (u 2^(16k) mod q) mod q with u = -1/d mod q, the
 // same fused-multiply-add sum and exact reduction as a residue, so one
 // compare replaces n.  (A prime dividing d kills p iff q | p; an engine
-// with such a round prime keeps the residue rows.)
+// with such a round prime keeps the residue rows.)  For q < 2^24 the same
+// form in 22-bit limbs is exact too (each product below 2^46, four of them
+// below 2^48) and takes p to 2^88 in four limbs (three below 2^66), where
+// 16-bit limbs take eight past 2^64.
 struct IndexRow {
     double qd;        // q
     double qi;        // nearest double to 1/q
@@ -181,10 +186,13 @@ struct IndexRow {
     uint32_t n;       // t < n kills
     uint32_t pad[2];
     double uw[4];     // u 2^(16k) mod q, k = 4..7 (values past 2^64)
+    double v[4];      // u 2^(22k) mod q, k = 0..3 (22-bit limbs, q < 2^24)
 };
-constexpr uint32_t kIndexStride = 24;       // words of IndexRow
+constexpr uint32_t kIndexStride = 32;       // words of IndexRow
 constexpr uint32_t kIndexHead = 16;         // words read for values < 2^64
+constexpr uint32_t kIndexV = 24;            // first word of v
 static_assert(sizeof(IndexRow) == 4 * kIndexStride, "IndexRow layout");
+static_assert(offsetof(IndexRow, v) == 4 * kIndexV, "IndexRow layout");
 
 // Families whose rows may be index rows.
 FK_HD bool kill_indexable(int family) { return family == 3; }
@@ -197,9 +205,15 @@ FK_HD void index_row(int n, const uint32_t d[4], const PrimeK& p, uint32_t* row)
     const uint32_t di = pow_mod(dq, p.q - 2, p);          // 1/d, q prime
     uint32_t c = di ? p.q - di : 0u;                      // u = -1/d
     const uint32_t two16 = (uint32_t)((1ull << 16) % p.q);
+    const uint32_t two22 = (uint32_t)((1ull << 22) % p.q);
     IndexRow h;
     h.qd = pd.qd;
     h.qi = pd.qi;
+    uint32_t c22 = c;
+    for (int k = 0; k < 4; k++) {
+        h.v[k] = (double)c22;
+        c22 = mulmod32(c22, two22, p.q, p.m64);
+    }
     for (int k = 0; k < 8; k++) {
         if (k < 4) h.u[k] = (double)c;
         else h.uw[k - 4] = (double)c;

@@ -63,6 +63,8 @@ struct RoundArgs {
     uint32_t rows;            // the launch tests the engines whose round
                               // rows are of this kind (kRowsResidue or
                               // kRowsIndex) and skips the others
+    uint32_t idx22;           // index rows, every prime below 2^24: the
+                              // 22-bit limb form applies
 };
 constexpr uint32_t kRowsResidue = 1, kRowsIndex = 2;
 

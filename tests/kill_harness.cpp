@@ -73,7 +73,19 @@ int main()
                 const double a = (double)((t[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
                 s = fma(a, k < 4 ? h.u[k] : h.uw[k - 4], s);
             }
-            printf("%u %u %u", fk::resid_d1(s, h.qd, h.qi, h.q), h.q, h.n);
+            // the 22-bit limb form (q < 2^24, values below 2^88)
+            uint32_t t22 = 0xffffffffu;
+            if (q < (1u << 24) && t[3] == 0 && t[2] < (1u << 24)) {
+                const uint32_t M = 0x3fffffu;
+                const double a22[4] = {(double)(t[0] & M),
+                                       (double)(((t[0] >> 22) | (t[1] << 10)) & M),
+                                       (double)(((t[1] >> 12) | (t[2] << 20)) & M),
+                                       (double)((t[2] >> 2) & M)};
+                double s22 = a22[0] * h.v[0];
+                for (int k = 1; k < 4; k++) s22 = fma(a22[k], h.v[k], s22);
+                t22 = fk::resid_d1(s22, h.qd, h.qi, h.q);
+            }
+            printf("%u %u %u %u", fk::resid_d1(s, h.qd, h.qi, h.q), h.q, h.n, t22);
             for (uint32_t i = 0; i < fk::kIndexStride; i++) printf(" %u", row[i]);
             printf("\n");
         } else if (line[0] == 'R') {

@@ -212,7 +212,8 @@ def test_ap_index_rows(harness):
             vals = list(range(q)) if q < 400 else (
                 [k + q * rng.getrandbits(60) for k in ks] +
                 [k + 1 + q * rng.getrandbits(100) for k in ks] +
-                [rng.getrandbits(b) for b in (40, 64, 90, 128) for _ in range(8)])
+                [rng.getrandbits(b) for b in (40, 64, 66, 87, 88, 90, 128)
+                 for _ in range(8)])
             for v in vals:
                 if v >= 2**128:
                     continue
@@ -223,3 +224,10 @@ def test_ap_index_rows(harness):
     got = [[int(x) for x in line.split()] for line in call(harness, cases)]
     assert [g[0] for g in got] == want
     assert [g[0] < g[2] for g in got] == kill
+    # the 22-bit limb form, wherever it applies (q < 2^24, values < 2^88)
+    checked22 = 0
+    for g, w in zip(got, want):
+        if g[3] != 0xFFFFFFFF:
+            assert g[3] == w
+            checked22 += 1
+    assert checked22 > 1000
