@@ -125,6 +125,7 @@ struct PrimeD {
     double qd;             // q
     double qi;             // nearest double to 1/q
     double c[7];           // 2^16, 2^32, ..., 2^112 mod q
+    double c22[3];         // 2^22, 2^44, 2^66 mod q (22-bit limbs, q < 2^24)
 };
 
 FK_HD PrimeD make_primed(uint32_t q)
@@ -136,6 +137,11 @@ FK_HD PrimeD make_primed(uint32_t q)
     for (int i = 0; i < 7; i++) {
         c = (c << 16) % q;
         p.c[i] = (double)c;
+    }
+    c = 1;
+    for (int i = 0; i < 3; i++) {
+        c = (c << 22) % q;
+        p.c22[i] = (double)c;
     }
     p.q = q;
     p.pad = 0;

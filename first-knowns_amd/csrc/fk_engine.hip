@@ -1485,7 +1485,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             ra.p_begin = P0;
             ra.p_end = P1;
             ra.q_min = ctx->primes[P0];
-            ra.idx22 = (kind == kRowsIndex && ctx->primes[P1 - 1] < (1u << 24)) ? 1u : 0u;
+            ra.l22 = ctx->primes[P1 - 1] < (1u << 24) ? 1u : 0u;
             ra.kgroups = kg <= 3 ? 3u : 5u;
             ra.rows = kind;
             const uint32_t np = P1 - P0;

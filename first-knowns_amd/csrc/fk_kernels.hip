@@ -697,13 +697,20 @@ __device__ __forceinline__ RowView<KG> load_row(const FK_CONST uint32_t* row)
 // kernel waits on its scalar row loads, not on VALU: feeding a value below
 // 2^53 to the residue directly, as the lane = prime kernel does, measured
 // no faster here.)
+// NL == 22: a[0..3] are 22-bit limbs (values below 2^88, primes below
+// 2^24), against 2^22, 2^44 and 2^66 mod q from the prime table: three
+// FMAs where 16-bit limbs past 2^64 take seven.
 template <int NL, bool FOLD, int KG>
 __device__ __forceinline__ uint32_t row_residue(const double (&a)[8],
                                                 const RowView<KG>& v,
                                                 const FK_CONST PrimeD* P)
 {
-    double s = fma(a[3], v.c48, fma(a[2], v.c32, fma(a[1], v.c16, a[0])));
-    if (NL > 4) {
+    double s;
+    if (NL == 22)
+        s = fma(a[3], P->c22[2], fma(a[2], P->c22[1], fma(a[1], P->c22[0], a[0])));
+    else
+        s = fma(a[3], v.c48, fma(a[2], v.c32, fma(a[1], v.c16, a[0])));
+    if (NL > 4 && NL != 22) {
 #pragma unroll
         for (int k = 4; k < NL; k++) s = fma(a[k], P->c[k - 1], s);
     }
@@ -1170,7 +1177,7 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
             touch ^= tv;
             continue;
         }
-        if (ROWS == kRowsIndex && a.idx22) {
+        if (ROWS == kRowsIndex && a.l22) {
             // every prime of the launch below 2^24: 22-bit limbs, three or
             // four, unless some live value reaches 2^88
             bool huge = false, w66 = false;
@@ -1204,6 +1211,22 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
             for (int k = 0; k < 8; k++)
                 l[c][k] = (double)((t[c][k >> 1] >> ((k & 1) * 16)) & 0xffffu);
         const bool wwide = __ballot(wide) != 0;
+        // values past 2^64 in 22-bit limbs where every prime of the launch
+        // is below 2^24 and every live value below 2^88
+        bool huge88 = false;
+#pragma unroll
+        for (int c = 0; c < KC; c++)
+            huge88 = huge88 || (live[c] && (t[c][3] != 0 || t[c][2] >= (1u << 24)));
+        const bool w22 = ROWS == kRowsResidue && a.l22 && wwide && !__ballot(huge88);
+        if (w22) {
+#pragma unroll
+            for (int c = 0; c < KC; c++) {
+                double l22[4];
+                limbs22(t[c], l22);
+#pragma unroll
+                for (int k = 0; k < 4; k++) l[c][k] = l22[k];
+            }
+        }
         const FK_CONST PrimeD* pd = uniform_view(a.primed);
         const FK_CONST uint32_t* kt = uniform_view(E->kt);
         const uint32_t rf = E->round_first;
@@ -1216,10 +1239,12 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
             if (wwide) index_scan<8, KC>(l, kt, rf, lo, hi, n, m);
             else index_scan<4, KC>(l, kt, rf, lo, hi, n, m);
         } else if (kill_folded(E->family)) {
-            if (wwide) round_scan<8, KG, true, KC>(l, pd, kt, st, rf, lo, hi, m);
+            if (w22) round_scan<22, KG, true, KC>(l, pd, kt, st, rf, lo, hi, m);
+            else if (wwide) round_scan<8, KG, true, KC>(l, pd, kt, st, rf, lo, hi, m);
             else round_scan<4, KG, true, KC>(l, pd, kt, st, rf, lo, hi, m);
         } else {
-            if (wwide) round_scan<8, KG, false, KC>(l, pd, kt, st, rf, lo, hi, m);
+            if (w22) round_scan<22, KG, false, KC>(l, pd, kt, st, rf, lo, hi, m);
+            else if (wwide) round_scan<8, KG, false, KC>(l, pd, kt, st, rf, lo, hi, m);
             else round_scan<4, KG, false, KC>(l, pd, kt, st, rf, lo, hi, m);
         }
 #pragma unroll
@@ -1241,6 +1266,7 @@ constexpr int kRegResidues = 32;
 struct LaneRow {
     double qd, qi, c16, c32, c48;
     double cw[4];                 // 2^64 .. 2^112 mod q (values past 2^64)
+    double c22[3];                // 2^22, 2^44, 2^66 mod q (22-bit limbs)
     uint32_t q, cnt;
     uint32_t k[kRegResidues];
     const uint32_t* row;          // the row in memory (residues past 32)
@@ -1276,6 +1302,8 @@ __device__ __forceinline__ void load_lane_row(LaneRow& L, const uint32_t* row,
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) L.cw[i] = pd->c[3 + i];
+#pragma unroll
+    for (int i = 0; i < 3; i++) L.c22[i] = pd->c22[i];
 }
 
 // Round prime rows hold at least 20 residues (32 words); rows of engines
@@ -1296,10 +1324,14 @@ template <int NL, bool FOLD>
 __device__ __forceinline__ uint32_t lane_kills(const LaneRow& L, const double* a,
                                                uint32_t ncmp)
 {
-    // NL == 1: a[0] is the value itself (below 2^53, primes at least 32)
+    // NL == 1: a[0] is the value itself (below 2^53, primes at least 32);
+    // NL == 22: a[0..3] are 22-bit limbs (below 2^88, primes below 2^24)
     double s = a[0];
-    if (NL > 1) s = fma(a[3], L.c48, fma(a[2], L.c32, fma(a[1], L.c16, a[0])));
-    if (NL > 4) {
+    if (NL == 22)
+        s = fma(a[3], L.c22[2], fma(a[2], L.c22[1], fma(a[1], L.c22[0], a[0])));
+    else if (NL > 1)
+        s = fma(a[3], L.c48, fma(a[2], L.c32, fma(a[1], L.c16, a[0])));
+    if (NL > 4 && NL != 22) {
 #pragma unroll
         for (int k = 0; k < 4; k++) s = fma(a[4 + k], L.cw[k], s);
     }
@@ -1369,13 +1401,18 @@ __device__ __forceinline__ void lane_prime_walk(Kills kills, const double* a,
 template <bool FOLD>
 __device__ __forceinline__ void lane_prime_slice(const LaneRow& L, uint32_t miss,
                                                  uint32_t ncmp, bool wide,
-                                                 bool narrow,
+                                                 bool narrow, bool w22,
                                                  const double* limbs,
                                                  const double* vals,
+                                                 const double* l22,
                                                  const uint32_t* live_idx,
                                                  uint32_t nlive, uint32_t* flags)
 {
-    if (wide)
+    if (w22)
+        lane_prime_walk([&](const double* x) {
+                            return (lane_kills<22, FOLD>(L, x, ncmp) | miss) == 0u; },
+                        l22, 4, live_idx, nlive, flags);
+    else if (wide)
         lane_prime_walk([&](const double* x) {
                             return (lane_kills<8, FOLD>(L, x, ncmp) | miss) == 0u; },
                         limbs, 8, live_idx, nlive, flags);
@@ -1451,7 +1488,7 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
 {
     __shared__ double sh_limbs[kRoundSlice * 8];
     __shared__ double sh_val[kRoundSlice];      // the value, when below 2^53
-    __shared__ double sh_l22[ROWS == kRowsIndex ? kRoundSlice * 4 : 1];
+    __shared__ double sh_l22[kRoundSlice * 4];
     __shared__ uint32_t sh_idx[kRoundSlice];
     __shared__ uint32_t sh_n, sh_wide, sh_wide53, sh_w66, sh_huge;
     const uint32_t p = a.p_begin + blockIdx.x * 256 + threadIdx.x;
@@ -1499,7 +1536,7 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
                     if ((rec.t[2] | rec.t[3]) != 0) sh_wide = 1;
                     if ((rec.t[2] | rec.t[3]) != 0 || rec.t[1] >= (1u << 21))
                         sh_wide53 = 1;
-                    if (ROWS == kRowsIndex) {
+                    if (a.l22) {
                         const uint32_t tt[4] = {rec.t[0], rec.t[1], rec.t[2], rec.t[3]};
                         double l22[4];
                         limbs22(tt, l22);
@@ -1537,7 +1574,7 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
                     load_lane_idx(I, engs[e].kt + (size_t)(pc - engs[e].round_first) *
                                                       kIndexStride);
                     const uint32_t nlim = mine ? I.n : 0u;
-                    if (a.idx22 && sh_huge == 0) {
+                    if (a.l22 && sh_huge == 0) {
                         if (sh_w66)
                             lane_prime_walk([&](const double* x) {
                                                 return lane_idx_kills22<4>(I, x, nlim); },
@@ -1557,12 +1594,15 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
                 }
             } else if (__ballot(mine)) {
                 const uint32_t miss = mine ? 0u : 1u;
+                // values past 2^64 in 22-bit limbs (primes below 2^24,
+                // values below 2^88)
+                const bool w22 = a.l22 && wide && sh_huge == 0;
                 if (kill_folded(engs[e].family))
-                    lane_prime_slice<true>(L, miss, ncmp, wide, narrow, sh_limbs,
-                                           sh_val, sh_idx, nlive, a.flags);
+                    lane_prime_slice<true>(L, miss, ncmp, wide, narrow, w22, sh_limbs,
+                                           sh_val, sh_l22, sh_idx, nlive, a.flags);
                 else
-                    lane_prime_slice<false>(L, miss, ncmp, wide, narrow, sh_limbs,
-                                            sh_val, sh_idx, nlive, a.flags);
+                    lane_prime_slice<false>(L, miss, ncmp, wide, narrow, w22, sh_limbs,
+                                            sh_val, sh_l22, sh_idx, nlive, a.flags);
             }
         }
         __syncthreads();                           // before the next staging

@@ -63,8 +63,8 @@ struct RoundArgs {
     uint32_t rows;            // the launch tests the engines whose round
                               // rows are of this kind (kRowsResidue or
                               // kRowsIndex) and skips the others
-    uint32_t idx22;           // index rows, every prime below 2^24: the
-                              // 22-bit limb form applies
+    uint32_t l22;             // every prime of the launch below 2^24: the
+                              // 22-bit limb forms apply (values below 2^88)
 };
 constexpr uint32_t kRowsResidue = 1, kRowsIndex = 2;
 

@@ -100,9 +100,22 @@ int main()
                 a[k] = (double)((t[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
             double s = a[0];
             for (int k = 1; k < nl; k++) s = fma(a[k], P.c[k - 1], s);
-            printf("%u %u %u\n", fk::resid_d(s, P.qd, P.qi, P.q),
+            // the 22-bit limb form (q < 2^24, values below 2^88)
+            uint32_t t22 = 0xffffffffu, f22 = 0xffffffffu;
+            if (q < (1u << 24) && t[3] == 0 && t[2] < (1u << 24)) {
+                const uint32_t M = 0x3fffffu;
+                const double a22[4] = {(double)(t[0] & M),
+                                       (double)(((t[0] >> 22) | (t[1] << 10)) & M),
+                                       (double)(((t[1] >> 12) | (t[2] << 20)) & M),
+                                       (double)((t[2] >> 2) & M)};
+                const double s22 = fma(a22[3], P.c22[2],
+                                       fma(a22[2], P.c22[1], fma(a22[1], P.c22[0], a22[0])));
+                t22 = fk::resid_d1(s22, P.qd, P.qi, P.q);
+                f22 = fk::resid_fold_d(s22, P.qd, P.qi, P.q);
+            }
+            printf("%u %u %u %u %u\n", fk::resid_d(s, P.qd, P.qi, P.q),
                    fk::resid_d1(s, P.qd, P.qi, P.q),
-                   fk::resid_fold_d(s, P.qd, P.qi, P.q));
+                   fk::resid_fold_d(s, P.qd, P.qi, P.q), t22, f22);
         }
     }
 }

@@ -46,10 +46,10 @@ def limbs(v):
 
 def test_residue_matches_python(harness):
     rng = random.Random(21)
-    qs = [3, 5, 65521, 65537, 2**22 - 3, 2**29 - 3, 2**30 - 35]
+    qs = [3, 5, 65521, 65537, 2**22 - 3, 2**24 - 3, 2**29 - 3, 2**30 - 35]
     qs += [random_prime(rng, 3, 2**30) for _ in range(40)]
     vals = [0, 1, 2**16 - 1, 2**16, 2**32 - 1, 2**63, 2**64 - 1, 2**64,
-            2**96 + 12345, 2**128 - 1]
+            2**66, 2**88 - 1, 2**96 + 12345, 2**128 - 1]
     vals += [rng.getrandbits(b) for b in (20, 40, 63, 64, 80, 100, 127, 128)
              for _ in range(6)]
     cases, want, wq = [], [], []
@@ -72,6 +72,14 @@ def test_residue_matches_python(harness):
     assert [g[1] for g in got] == want          # resid_d1 (one fix)
     fold = [min(w, q - w) for w, q in zip(want, wq)]
     assert [g[2] for g in got] == fold          # resid_fold_d
+    # the 22-bit limb form with 2^22, 2^44, 2^66 mod q, where it applies
+    # (q < 2^24, values below 2^88)
+    n22 = 0
+    for g, w, q in zip(got, want, wq):
+        if g[3] != 0xFFFFFFFF:
+            assert (g[3], g[4]) == (w, min(w, q - w))
+            n22 += 1
+    assert n22 > 200
 
 
 def test_residue_of_a_53_bit_value_in_one_double(harness):
