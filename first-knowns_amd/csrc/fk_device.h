@@ -63,9 +63,11 @@ struct DevEngine {
     uint32_t sbm_words;       // kill bitmaps of every LDS prime with a
     const uint32_t* sbm_off;  // tail of kShortTailBits: [nlds] first words,
     const uint32_t* sbm;      // [sbm_words] bitmaps (short-line sieve)
-    const uint32_t* spk;      // [4 nlds] the short-line sieve's per-prime
-                              // constants q, m32, sbm_off, D^-1, read
-                              // with scalar loads
+    const uint32_t* spd;      // [kShortPrimeWords nlds] the short-line
+                              // sieve's per-prime constants, read with
+                              // scalar loads
+    const double* spe;        // [nlds] (2^32 D^-1) mod q, the same sieve's
+                              // constant for wheel periods past 2^32
     uint32_t ntr;             // in-tile rounds over [nmark, nlds)
     const uint32_t* tr_end;   // [ntr] LDS-prime index where round r ends
     const uint32_t* kt;       // [round_end - round_first][kstride] round
@@ -99,6 +101,10 @@ struct DevPiece {
 // x + 64, ... with no reduction mod q.
 constexpr uint32_t kShortChunkMax = 16;
 constexpr uint32_t kShortTailBits = 64 * kShortChunkMax;
+// Words per LDS prime in the short-line sieve's constant table: q, 1/q and
+// D^-1 mod q as doubles, q and the prime's first bitmap word (two 16-byte
+// scalar loads).
+constexpr uint32_t kShortPrimeWords = 8;
 
 // A candidate in flight: the value the kill tests and the strong test need
 // (Euler p, Dickson k = W j, AP p) and its window.

@@ -505,14 +505,19 @@ static int build_engine(fk_engine* e)
     if ((rc = eng_upload(e, bm, &h.bm))) return rc;
     if ((rc = eng_upload(e, sbm_off, &h.sbm_off))) return rc;
     {
-        std::vector<uint32_t> spk(4 * lq.size());
+        std::vector<uint32_t> spd(kShortPrimeWords * lq.size(), 0);
+        std::vector<double> spe(lq.size());
         for (size_t p = 0; p < lq.size(); p++) {
-            spk[4 * p] = lq[p];
-            spk[4 * p + 1] = lm32[p];
-            spk[4 * p + 2] = sbm_off[p];
-            spk[4 * p + 3] = ldinv[p];
+            const uint32_t q = lq[p];
+            const double c[3] = {(double)q, 1.0 / (double)q, (double)ldinv[p]};
+            uint32_t* k = &spd[kShortPrimeWords * p];
+            memcpy(k, c, sizeof c);
+            k[6] = q;
+            k[7] = sbm_off[p];
+            spe[p] = (double)(((1ull << 32) % q) * ldinv[p] % q);
         }
-        if ((rc = eng_upload(e, spk, &h.spk))) return rc;
+        if ((rc = eng_upload(e, spd, &h.spd))) return rc;
+        if ((rc = eng_upload(e, spe, &h.spe))) return rc;
     }
     if ((rc = eng_upload(e, sbm, &h.sbm))) return rc;
     if ((rc = eng_upload(e, tr_end, &h.tr_end))) return rc;
@@ -970,9 +975,12 @@ static constexpr uint32_t kSingleLimit = 8192;
 // short-line sieve: chunk sizes up to kShortChunkMax words per lane (3, 6
 // and 12 divide lines of 24 words, which a 2^32-j Dickson window has),
 // chosen per window by instructions per lane: about kShortStartCost to form
-// a chunk's start in a prime's coordinates, kShortWordCost per word
+// a chunk's start in a prime's coordinates (one fma and a double-precision
+// residue), kShortWordCost per word.  Measured on the 256-window mixed
+// batch: start costs 8 and 12 give the same time, 16 is 4% slower and the
+// 28 of the 32-bit Barrett start 10% slower.
 static constexpr uint32_t kShortChunks[] = {1, 2, 3, 4, 6, 8, 12, 16};
-static constexpr uint64_t kShortStartCost = 28, kShortWordCost = 5;
+static constexpr uint64_t kShortStartCost = 12, kShortWordCost = 5;
 
 // A window on wheel t with short lines: its base, the chunk with the least
 // work per line, its (line, chunk) items and the whole window's work (lane

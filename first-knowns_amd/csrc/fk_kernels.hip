@@ -457,35 +457,47 @@ __global__ __launch_bounds__(T) void k_sieve(SieveArgs a)
 // piece changes.
 
 // The chunk's C words ORed with every LDS prime's bitmap windows.  x is
-// (v0 + (B + off) D^-1) mod q: with D < 2^32, off mod q comes from a 32-bit
-// reduction, and every product below stays under 2^32 because the LDS
-// primes are below 2^16.  The prime's constants come from the engine's
-// table with scalar loads (the index is wave-uniform), B mod q from LDS
-// (one broadcast word): LDS bandwidth goes to the bitmap windows.
-template <int C, bool D32>
+// (v0 + (B + off) D^-1) mod q, formed in double precision: with
+// bqd = (B D^-1) mod q from LDS (one broadcast word per prime),
+// s = off D^-1 + bqd + v0 is an integer congruent to x, and x = s mod q by
+// resid_d1.  For D <= 2^32, off D^-1 < 2^32 2^16 (LDS primes are below
+// 2^16), so s < 2^49 and one fma forms it exactly.  WIDE periods split off
+// into 32-bit halves: s = off_hi e32 + off_lo D^-1 + bqd + v0 with
+// e32 = (2^32 D^-1) mod q, below 2^47 + 2^48 + 2^33 < 2^49, two exact fmas.
+// Either way s < 2^49 meets resid_d1's condition for every q >= 2: five or
+// six double operations and a conversion per prime, where 32-bit Barrett
+// reductions took three reductions and a multiply.  The prime's constants
+// come from the engine's table with scalar loads (the index is
+// wave-uniform): LDS bandwidth goes to the bitmap windows.
+template <int C, bool WIDE>
 __device__ __forceinline__ void short_words(const FK_CONST uint32_t* sp,
-                                            const FK_CONST uint64_t* m64,
-                                            const uint32_t* bqs,
+                                            const FK_CONST double* se,
+                                            const double* bqd,
                                             const uint32_t* bm, uint32_t nlds,
-                                            uint64_t off, uint32_t v0,
-                                            uint64_t (&acc)[C])
+                                            double offlo, double offhi,
+                                            double v0d, uint64_t (&acc)[C])
 {
     static_assert(C <= (int)kShortChunkMax, "chunk past the bitmap tail");
     // A wave walks the primes as one chain of dependent latencies (scalar
-    // load, reductions, LDS reads); unrolled, several primes' chains are in
+    // load, residue, LDS reads); unrolled, several primes' chains are in
     // flight at once.  The workgroups' LDS already caps a SIMD at a few
-    // waves, so the registers this takes are free.
-#pragma unroll 4
+    // waves, so the registers this takes are free (the wide form, for
+    // wheels past 2^32, is not unrolled: unrolled it would take the
+    // kernel past 128 registers, three waves per SIMD instead of four).
+    constexpr int kUnroll = WIDE ? 1 : 4;
+#pragma unroll kUnroll
     for (uint32_t p = 0; p < nlds; p++) {
-        const uint4 pq = load4(sp + 4 * p);      // {q, m32, bitmap, D^-1}
-        const uint32_t q = pq.x, m32 = pq.y;
-        const uint32_t r = D32 ? red32((uint32_t)off, q, m32)
-                               : red64(off, q, m64[p]);
-        uint32_t t = bqs[p] + r;
-        if (t >= q) t -= q;
-        t = red32(t * pq.w, q, m32);
-        const uint32_t x = red32(v0 + t, q, m32);
-        const uint32_t* b = bm + pq.z + (x >> 5);
+        // {q, 1/q, D^-1} as doubles, q, bitmap word
+        const FK_CONST uint32_t* k = sp + kShortPrimeWords * p;
+        const uint4 c0 = load4(k);
+        const uint4 c1 = load4(k + 4);
+        const double qd = __hiloint2double((int)c0.y, (int)c0.x);
+        const double qi = __hiloint2double((int)c0.w, (int)c0.z);
+        const double di = __hiloint2double((int)c1.y, (int)c1.x);
+        double t = bqd[p] + v0d;
+        if (WIDE) t = fma(offhi, se[p], t);
+        const uint32_t x = resid_d1(fma(offlo, di, t), qd, qi, c1.z);
+        const uint32_t* b = bm + c1.w + (x >> 5);
         const uint32_t o = x & 31u;
         uint32_t w[2 * C + 1];
 #pragma unroll
@@ -502,8 +514,9 @@ template <int C>
 __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
 {
     extern __shared__ uint64_t lds64[];
-    uint32_t* bqs = reinterpret_cast<uint32_t*>(lds64);   // [nlds_max] B mod q
-    uint32_t* bm = bqs + ((a.nlds_max + 3) & ~3u);         // [bm_max] bitmaps
+    // [nlds_max] (B D^-1) mod q per LDS prime as a double, then the bitmaps
+    double* bqd = reinterpret_cast<double*>(lds64);
+    uint32_t* bm = reinterpret_cast<uint32_t*>(lds64 + ((a.nlds_max + 1) & ~1u));
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     uint32_t cur_engine = 0xffffffffu, cur_piece = 0xffffffffu;
@@ -531,8 +544,11 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
                 for (uint32_t w = 4 * n4 + tid; w < E.sbm_words; w += 256)
                     bm[w] = E.sbm[w];
             }
-            for (uint32_t p = tid; p < E.nlds; p += 256)
-                bqs[p] = base_mod(P, E.lds_q[p], E.lds_m64[p]);
+            for (uint32_t p = tid; p < E.nlds; p += 256) {
+                const uint32_t q = E.lds_q[p];
+                const uint32_t bq = base_mod(P, q, E.lds_m64[p]);
+                bqd[p] = (double)((bq * E.lds_dinv[p]) % q);   // < 2^32
+            }
             cur_engine = P.engine;
             cur_piece = pi;
             __syncthreads();
@@ -563,12 +579,15 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
             }
         }
         if (here) {
-            const FK_CONST uint32_t* sp = uniform_view(E.spk);
-            const FK_CONST uint64_t* m64 = uniform_view(E.lds_m64);
+            const FK_CONST uint32_t* sp = uniform_view(E.spd);
+            const FK_CONST double* se = uniform_view(E.spe);
             if ((E.D >> 32) == 0)
-                short_words<C, true>(sp, m64, bqs, bm, E.nlds, off, v0, acc);
+                short_words<C, false>(sp, se, bqd, bm, E.nlds, (double)off,
+                                      0.0, (double)v0, acc);
             else
-                short_words<C, false>(sp, m64, bqs, bm, E.nlds, off, v0, acc);
+                short_words<C, true>(sp, se, bqd, bm, E.nlds,
+                                     (double)(uint32_t)off,
+                                     (double)(off >> 32), (double)v0, acc);
         }
         // records: a wave-wide prefix of the per-lane counts, one atomic
         uint32_t c = 0;

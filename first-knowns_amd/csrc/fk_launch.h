@@ -127,11 +127,11 @@ hipError_t launch_sieve(const SieveArgs& a, int threads, int wpt,
 int sieve_occupancy(int threads, int wpt, size_t lds_bytes);
 // windows with short lines: one lane per chunk of `chunk` words of a line
 // (1, 2, 4, 8 or 16), 256-item tiles
-// LDS bytes of k_sieve_short: B mod q per LDS prime (whole 16-byte groups),
-// then the kill bitmaps.
+// LDS bytes of k_sieve_short: one 8-byte word per LDS prime for the
+// piece's base term (whole 16-byte groups), then the kill bitmaps.
 inline size_t short_lds(size_t nlds, size_t bm_words)
 {
-    return 4 * ((nlds + 3) & ~(size_t)3) + 4 * bm_words;
+    return 8 * ((nlds + 1) & ~(size_t)1) + 4 * bm_words;
 }
 hipError_t launch_sieve_short(const SieveArgs& a, int chunk, uint32_t grid,
                               size_t lds_bytes, hipStream_t stream,
