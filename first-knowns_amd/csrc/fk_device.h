@@ -74,6 +74,11 @@ struct DevEngine {
                               // table: per round prime a RoundRow header
                               // (fk_kill.h), then the killing residues of
                               // the value, padded; or an IndexRow
+    const uint32_t* kperm;    // residue rows: [round_end - round_first] the
+                              // rows of each round ordered by their groups
+                              // of four residues (a permutation within
+                              // every round), for the lane = prime kernel;
+                              // null for index rows
 };
 
 // One piece of one window: a contiguous range of line positions of every

@@ -530,6 +530,8 @@ static int build_engine(fk_engine* e)
         h.kt = ph.kt;
         h.kidx = ph.kidx;
         h.kstride = ph.kstride;
+        // the row order is per round: shared when the rounds are the same
+        h.kperm = std::equal(ph.rb, ph.rb + ph.nround + 1, h.rb) ? ph.kperm : nullptr;
     } else {
         // kill sets of the round primes, built on the device
         const uint32_t np = e->round_end - e->round_first;
@@ -543,6 +545,29 @@ static int build_engine(fk_engine* e)
                                  d.n, h.d, h.kidx != 0, h.kstride, kt, nullptr));
         HIPCHK(hipDeviceSynchronize());
         h.kt = kt;
+        h.kperm = nullptr;
+        if (!h.kidx && np > 0) {
+            // each round's rows by groups of four residues (1, 2, 3, more),
+            // stable: a wave of the lane = prime kernel then mostly holds
+            // rows of one group count and compares no more groups than its
+            // rows need
+            std::vector<uint32_t> cnt(np), perm(np);
+            HIPCHK(hipMemcpy2D(cnt.data(), sizeof(uint32_t),
+                               kt + offsetof(RoundRow, cnt) / sizeof(uint32_t),
+                               h.kstride * sizeof(uint32_t), sizeof(uint32_t), np,
+                               hipMemcpyDeviceToHost));
+            for (uint32_t r = 0; r < h.nround; r++) {
+                const uint32_t lo = h.rb[r] - e->round_first;
+                const uint32_t hi = h.rb[r + 1] - e->round_first;
+                uint32_t at = lo;
+                for (uint32_t g = 1; g <= 4; g++)
+                    for (uint32_t i = lo; i < hi; i++) {
+                        const uint32_t gi = std::min(4u, std::max(1u, (cnt[i] + 3u) / 4u));
+                        if (gi == g) perm[at++] = i;
+                    }
+            }
+            if ((rc = eng_upload(e, perm, &h.kperm))) return rc;
+        }
     }
     std::vector<DevEngine> one(1, h);
     const DevEngine* de = nullptr;
@@ -1512,7 +1537,7 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             uint32_t gx, gy;
             if (lane_prime) {
                 gx = (np + 255) / 256;
-                gy = (uint32_t)std::min(std::ceil(alive / 256.0),
+                gy = (uint32_t)std::min(std::ceil(alive / kRoundSlice),
                                         std::max(1.0, std::floor((1u << 22) / (double)gx)));
                 gy = std::max(1u, std::min(gy, 65535u));
             } else {

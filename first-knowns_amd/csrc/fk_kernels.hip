@@ -1291,8 +1291,14 @@ struct LaneRow {
     const uint32_t* row;          // the row in memory (residues past 32)
 };
 
+// The row's header and the residue groups a wave compares from registers:
+// three groups (twelve residues) when the family's kill sets fit them
+// (ncmp <= 12), five otherwise; the constants for values past 2^53 only
+// when the slice holds such values (wide: 16-bit limbs past 2^64, w22:
+// 22-bit limbs), each a load of its own per lane.
 __device__ __forceinline__ void load_lane_row(LaneRow& L, const uint32_t* row,
-                                              const PrimeD* pd, uint32_t stride)
+                                              const PrimeD* pd, uint32_t stride,
+                                              uint32_t ncmp, bool wide, bool w22)
 {
     RoundRow h;
     const uint4* r4 = reinterpret_cast<const uint4*>(row);
@@ -1308,10 +1314,11 @@ __device__ __forceinline__ void load_lane_row(LaneRow& L, const uint32_t* row,
     L.q = h.q;
     L.cnt = h.cnt;
     L.row = row;
+    const int ng = ncmp <= 12u ? 3 : 5;
 #pragma unroll
     for (int i = 0; i < kRegResidues / 4; i++) {
         // residue groups inside the row only (rows are 16..32 words here)
-        const uint4 v = (4 * i + 4 <= 20 && kRowHead + 4u * i + 4u <= stride)
+        const uint4 v = (i < ng && 4 * i + 4 <= 20 && kRowHead + 4u * i + 4u <= stride)
                             ? r4[3 + i]
                             : make_uint4(kKillPad, kKillPad, kKillPad, kKillPad);
         L.k[4 * i] = v.x;
@@ -1319,10 +1326,14 @@ __device__ __forceinline__ void load_lane_row(LaneRow& L, const uint32_t* row,
         L.k[4 * i + 2] = v.z;
         L.k[4 * i + 3] = v.w;
     }
+    if (wide && !w22) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) L.cw[i] = pd->c[3 + i];
+        for (int i = 0; i < 4; i++) L.cw[i] = pd->c[3 + i];
+    }
+    if (w22) {
 #pragma unroll
-    for (int i = 0; i < 3; i++) L.c22[i] = pd->c22[i];
+        for (int i = 0; i < 3; i++) L.c22[i] = pd->c22[i];
+    }
 }
 
 // Round prime rows hold at least 20 residues (32 words); rows of engines
@@ -1336,10 +1347,12 @@ __device__ __forceinline__ void load_lane_row_wide(LaneRow& L, uint32_t kmax)
 }
 
 // The lane's prime against a candidate of residue-limbs a[] (wave
-// uniform, read from LDS): zero when the prime kills it.  ncmp (wave
-// uniform, a multiple of 4) residues are compared from registers; residues
-// past 32 (kill sets larger than 32) from memory.
-template <int NL, bool FOLD>
+// uniform, read from LDS): zero when the prime kills it.  NG > 0: the
+// wave's rows hold at most 4 NG residues, NG groups of four compared from
+// registers, unrolled.  NG == 0: ncmp (wave uniform, a multiple of 4)
+// residues from registers, residues past 32 (kill sets larger than 32)
+// from memory.
+template <int NL, bool FOLD, int NG>
 __device__ __forceinline__ uint32_t lane_kills(const LaneRow& L, const double* a,
                                                uint32_t ncmp)
 {
@@ -1359,6 +1372,13 @@ __device__ __forceinline__ uint32_t lane_kills(const LaneRow& L, const double* a
     // zero iff r is one of the residues: a minimum of r ^ k in vector
     // registers (padding 0xffffffff never matches a residue below 2^30)
     uint32_t d = 0xffffffffu;
+    if (NG > 0) {
+#pragma unroll
+        for (int i = 0; i < 4 * NG; i += 4)
+            d = min(d, min(min(r ^ L.k[i], r ^ L.k[i + 1]),
+                           min(r ^ L.k[i + 2], r ^ L.k[i + 3])));
+        return d;
+    }
 #pragma unroll
     for (int i = 0; i < kRegResidues; i += 4) {
         if ((uint32_t)i < ncmp)
@@ -1381,7 +1401,6 @@ __device__ __forceinline__ uint32_t lane_kills(const LaneRow& L, const double* a
 // residue.  A candidate that some lane's prime kills is flagged by every
 // such lane (each storing 1).  There is no global memory access per
 // (candidate, prime).
-constexpr uint32_t kRoundSlice = 256;
 
 // One slice's candidates against the lane's prime; candidate c's input is
 // a + step * c, and kills(x) says whether the lane's prime kills the
@@ -1416,8 +1435,8 @@ __device__ __forceinline__ void lane_prime_walk(Kills kills, const double* a,
 // Residue rows: 16-bit limbs (8 past 2^64, else 4), or with every value
 // below 2^53 the value itself (NL == 1).
 // miss: 0 for a lane whose prime is in the round, 1 (the test never
-// reaches zero) for the others.
-template <bool FOLD>
+// reaches zero) for the others.  NG as in lane_kills.
+template <bool FOLD, int NG>
 __device__ __forceinline__ void lane_prime_slice(const LaneRow& L, uint32_t miss,
                                                  uint32_t ncmp, bool wide,
                                                  bool narrow, bool w22,
@@ -1429,20 +1448,45 @@ __device__ __forceinline__ void lane_prime_slice(const LaneRow& L, uint32_t miss
 {
     if (w22)
         lane_prime_walk([&](const double* x) {
-                            return (lane_kills<22, FOLD>(L, x, ncmp) | miss) == 0u; },
+                            return (lane_kills<22, FOLD, NG>(L, x, ncmp) | miss) == 0u; },
                         l22, 4, live_idx, nlive, flags);
     else if (wide)
         lane_prime_walk([&](const double* x) {
-                            return (lane_kills<8, FOLD>(L, x, ncmp) | miss) == 0u; },
+                            return (lane_kills<8, FOLD, NG>(L, x, ncmp) | miss) == 0u; },
                         limbs, 8, live_idx, nlive, flags);
     else if (narrow)
         lane_prime_walk([&](const double* x) {
-                            return (lane_kills<1, FOLD>(L, x, ncmp) | miss) == 0u; },
+                            return (lane_kills<1, FOLD, NG>(L, x, ncmp) | miss) == 0u; },
                         vals, 1, live_idx, nlive, flags);
     else
         lane_prime_walk([&](const double* x) {
-                            return (lane_kills<4, FOLD>(L, x, ncmp) | miss) == 0u; },
+                            return (lane_kills<4, FOLD, NG>(L, x, ncmp) | miss) == 0u; },
                         limbs, 8, live_idx, nlive, flags);
+}
+
+// lane_prime_slice with the wave's group count ng as a template argument.
+template <bool FOLD>
+__device__ __forceinline__ void lane_prime_groups(uint32_t ng, const LaneRow& L,
+                                                  uint32_t miss, uint32_t ncmp,
+                                                  bool wide, bool narrow, bool w22,
+                                                  const double* limbs,
+                                                  const double* vals,
+                                                  const double* l22,
+                                                  const uint32_t* live_idx,
+                                                  uint32_t nlive, uint32_t* flags)
+{
+    if (ng == 1)
+        lane_prime_slice<FOLD, 1>(L, miss, ncmp, wide, narrow, w22, limbs, vals, l22,
+                                  live_idx, nlive, flags);
+    else if (ng == 2)
+        lane_prime_slice<FOLD, 2>(L, miss, ncmp, wide, narrow, w22, limbs, vals, l22,
+                                  live_idx, nlive, flags);
+    else if (ng == 3)
+        lane_prime_slice<FOLD, 3>(L, miss, ncmp, wide, narrow, w22, limbs, vals, l22,
+                                  live_idx, nlive, flags);
+    else
+        lane_prime_slice<FOLD, 0>(L, miss, ncmp, wide, narrow, w22, limbs, vals, l22,
+                                  live_idx, nlive, flags);
 }
 
 // An index row (AP) in a lane's registers.
@@ -1578,15 +1622,23 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
             // lanes past the round's primes hold the row of its last prime
             // (in the table) and are masked in the test: no lane branches
             // around it (the workgroup holds some of the round's primes)
-            const uint32_t pc = min(max(p, rlo), rhi - 1u);
-            if (ROWS == kRowsResidue && __ballot(mine)) {
-                load_lane_row(L, engs[e].kt + (size_t)(pc - engs[e].round_first) * stride,
-                              a.primed + pc, stride);
-                if (ncmp > 20) load_lane_row_wide(L, stride - kRowHead);
-            }
+            uint32_t pc = min(max(p, rlo), rhi - 1u);
+            // residue rows in the round's count order (a permutation of
+            // the round): the waves' rows then mostly need as many groups
+            // of four compares as each other
+            if (ROWS == kRowsResidue && engs[e].kperm)
+                pc = engs[e].round_first + engs[e].kperm[pc - engs[e].round_first];
             // the one-value form needs every value below 2^53 and every
             // prime of the round at least 32
             const bool narrow = below53 && a.q_min >= 32u;
+            // values past 2^64 in 22-bit limbs (primes below 2^24, values
+            // below 2^88)
+            const bool w22 = a.l22 && wide && sh_huge == 0;
+            if (ROWS == kRowsResidue && __ballot(mine)) {
+                load_lane_row(L, engs[e].kt + (size_t)(pc - engs[e].round_first) * stride,
+                              a.primed + pc, stride, ncmp, wide, w22);
+                if (ncmp > 20) load_lane_row_wide(L, stride - kRowHead);
+            }
             if (ROWS == kRowsIndex) {
                 if (__ballot(mine)) {
                     LaneIdx I;
@@ -1613,15 +1665,20 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
                 }
             } else if (__ballot(mine)) {
                 const uint32_t miss = mine ? 0u : 1u;
-                // values past 2^64 in 22-bit limbs (primes below 2^24,
-                // values below 2^88)
-                const bool w22 = a.l22 && wide && sh_huge == 0;
+                // groups of four the wave's largest row needs (every lane
+                // compares them all: the residues past a row's count are
+                // padding); 0: more than twelve, the general loop
+                const uint32_t ng = __ballot(L.cnt > 12u) ? 0u
+                                    : __ballot(L.cnt > 8u) ? 3u
+                                    : __ballot(L.cnt > 4u) ? 2u : 1u;
                 if (kill_folded(engs[e].family))
-                    lane_prime_slice<true>(L, miss, ncmp, wide, narrow, w22, sh_limbs,
-                                           sh_val, sh_l22, sh_idx, nlive, a.flags);
+                    lane_prime_groups<true>(ng, L, miss, ncmp, wide, narrow, w22,
+                                            sh_limbs, sh_val, sh_l22, sh_idx, nlive,
+                                            a.flags);
                 else
-                    lane_prime_slice<false>(L, miss, ncmp, wide, narrow, w22, sh_limbs,
-                                            sh_val, sh_l22, sh_idx, nlive, a.flags);
+                    lane_prime_groups<false>(ng, L, miss, ncmp, wide, narrow, w22,
+                                             sh_limbs, sh_val, sh_l22, sh_idx, nlive,
+                                             a.flags);
             }
         }
         __syncthreads();                           // before the next staging

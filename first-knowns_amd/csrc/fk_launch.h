@@ -67,6 +67,8 @@ struct RoundArgs {
                               // 22-bit limb forms apply (values below 2^88)
 };
 constexpr uint32_t kRowsResidue = 1, kRowsIndex = 2;
+// lane = prime rounds: the most candidates a workgroup stages per slice
+constexpr uint32_t kRoundSlice = 256;
 
 struct CompactArgs {
     uint32_t nengines;
