@@ -1071,6 +1071,52 @@ static constexpr uint32_t kRoundWorkgroupsThroughput = 1024;
 // 2^20..2^30 values from 1.1-1.3e12 to 1.5-1.7e12 candidates/s)
 static constexpr double kThroughputAlive = 2048.0;
 
+// Late rounds of few candidates are launch-bound: a round and its
+// compaction cost two launches (about 8 us alone) whatever they test.
+// (Submissions of one engine only; see nused in batch_enqueue.)
+// group_rounds joins a round to the one before it while the candidates the
+// earlier rounds of the group kill, tested again against the joined
+// round's primes, stay within kGroupExtraTests tests (about a microsecond
+// of the chip's VALU).  Config 2's AP windows (candidates entering the
+// rounds 372k, 65k, 13k, 3k, 795, 227, 62, 20, 9, 3, 1, 0, ...) keep 6 of
+// their 15 rounds; the Dickson sweep's rounds kill more per round (every
+// join would add 6-19M tests) and keep all 12.
+static constexpr double kGroupExtraTests = 4.0e6;
+
+// Lane = prime for AP index rows in rounds of few candidates and many
+// primes (a joined tail: 3 candidates against a million primes leaves
+// lane = candidate 3 lanes of 64); residue rows stay with lane = candidate,
+// which tests their count-nested residues for fewer VALU.
+static constexpr double kSparseAlive = 320.0;
+static constexpr uint32_t kSparsePrimes = 8192;
+
+// h's rounds grouped for a submission that expects `alive` LDS-stage
+// survivors of engine e (see kGroupExtraTests).
+static void group_rounds(const fk_engine* e, const DeviceCtx* ctx, double alive,
+                         DevEngine& h)
+{
+    auto prime_at = [&](uint32_t i) {
+        return i < ctx->primes.size() ? (double)ctx->primes[i] : (double)e->q2;
+    };
+    uint32_t rb[33];
+    rb[0] = h.rb[0];
+    uint32_t nr = 0;
+    double a = alive, a0 = alive;          // entering round r, the group
+    for (uint32_t r = 0; r < h.nround; r++) {
+        const double np = (double)(h.rb[r + 1] - h.rb[r]);
+        if (nr > 0 && (a0 - a) * np <= kGroupExtraTests) {
+            rb[nr] = h.rb[r + 1];           // joins the group
+        } else {
+            rb[++nr] = h.rb[r + 1];         // starts one
+            a0 = a;
+        }
+        a *= round_survival(e, prime_at(h.rb[r]), prime_at(h.rb[r + 1]));
+    }
+    for (uint32_t i = 0; i <= nr; i++) h.rb[i] = rb[i];
+    for (uint32_t i = nr + 1; i < 33; i++) h.rb[i] = rb[nr];
+    h.nround = nr;
+}
+
 // A submission whose windows expect fewer LDS-stage survivors than
 // kMergeAlive in all sweeps with every other round boundary of each
 // engine's table (round growth squared): its rounds are launch- and
@@ -1313,6 +1359,11 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
     // expected candidates in this submission takes every other boundary)
     std::vector<DevEngine> eff(b->engines.size());
     uint32_t nround_max = 0;
+    // the late-round grouping and the sparse lane = prime rounds serve
+    // submissions of one engine (plain sweeps, one engine's requests):
+    // with several, a round launch serves them all and its grid and kernel
+    // follow the sum of their expected candidates
+    const size_t nused = (size_t)std::count(used.begin(), used.end(), 1);
     double alive_all = 0.0;
     for (size_t k = 0; k < b->engines.size(); k++)
         if (used[k]) alive_all += e_alive[k];
@@ -1326,6 +1377,8 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             for (uint32_t i = nr + 1; i < 33; i++) h.rb[i] = h.rb[nr];
             h.nround = nr;
         }
+        if (used[k] && nused == 1 && h.nround > 1)
+            group_rounds(b->engines[k], ctx, e_alive[k], h);
         if (used[k]) nround_max = std::max(nround_max, h.nround);
     }
     // counters: per stage (the sieve, then each round) one per engine,
@@ -1532,8 +1585,10 @@ static int batch_enqueue(fk_batch* b, const std::vector<uint32_t>* caps_in)
             // per pair).  Both read the true count on the device; grids
             // sized from an estimate that is off still cover every
             // candidate.
-            const bool lane_prime = np >= e0->round_lane_prime &&
-                                    alive >= e0->round_lane_prime / 8.0;
+            const bool lane_prime =
+                (nused == 1 && kind == kRowsIndex && np >= kSparsePrimes &&
+                 alive <= kSparseAlive) ||
+                (np >= e0->round_lane_prime && alive >= e0->round_lane_prime / 8.0);
             uint32_t gx, gy;
             if (lane_prime) {
                 gx = (np + 255) / 256;

@@ -260,10 +260,10 @@ def test_config2_full_window(case):
             say("config 2 full window", label, kw)
             assert N.record_values(eng.sweep(base, hi)) == want, kw
             counts = eng.last_counts()
-            # every round of the table, or every other boundary when the
-            # window expects few LDS-stage survivors
-            assert len(counts) in (1 + eng.info.n_rounds,
-                                   1 + (eng.info.n_rounds + 1) // 2)
+            # the table's rounds, the late ones of few candidates joined
+            # (fewer stages), and the queue shrinking stage by stage
+            assert 2 <= len(counts) <= 1 + eng.info.n_rounds
+            assert all(x >= y for x, y in zip(counts, counts[1:]))
             if ref_counts is None:
                 ref_counts = counts
                 assert counts[0] > 1000 and counts[-1] == len(want)
