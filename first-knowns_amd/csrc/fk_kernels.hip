@@ -557,27 +557,30 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
         const uint32_t item = (tile - P.tile0) * 256u + tid;
         const uint32_t line = item / nch;
         const uint32_t v0 = 64u * C * (item - line * nch);
-        uint64_t acc[C];
-#pragma unroll
-        for (int k = 0; k < C; k++) acc[k] = ~0ull;
-        uint64_t off = 0;
-        bool here = false;
+        // the item's valid positions [vlo, vend); none (every word dead)
+        // for an item past its piece or outside the window.  The words
+        // start clear and take their edge masks after the sieve: an
+        // all-dead start for the items with no positions, merged with the
+        // edge masks before it, kept a second copy of the C words in
+        // registers.
+        uint64_t off = 0, vlo = 0, vend = 0;
         if (line < E.nlines && v0 < P.npos) {
             off = E.offs[line];
             const uint64_t D = E.D;
-            const uint64_t vlo =
+            const uint64_t lo =
                 (off >= P.lo_rel) ? 0 : ceil_div_u64(P.lo_rel - off, D);
             const uint64_t vhi =
                 (P.hi_rel > off) ? ceil_div_u64(P.hi_rel - off, D) : 0;
-            const uint64_t vend = (vhi < (uint64_t)P.npos) ? vhi : (uint64_t)P.npos;
-            if (vlo < vend && v0 < vend && v0 + 64 * C > vlo) {
-                here = true;
-                // words at or past vend come out all dead
-#pragma unroll
-                for (int k = 0; k < C; k++)
-                    acc[k] = edge_mask(v0 + 64u * k, vlo, vend);
+            const uint64_t end = (vhi < (uint64_t)P.npos) ? vhi : (uint64_t)P.npos;
+            if (lo < end && v0 < end && v0 + 64 * C > lo) {
+                vlo = lo;
+                vend = end;
             }
         }
+        const bool here = vlo < vend;
+        uint64_t acc[C];
+#pragma unroll
+        for (int k = 0; k < C; k++) acc[k] = 0;
         if (here) {
             const FK_CONST uint32_t* sp = uniform_view(E.spd);
             const FK_CONST double* se = uniform_view(E.spe);
@@ -593,7 +596,7 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
         uint32_t c = 0;
 #pragma unroll
         for (int k = 0; k < C; k++) {
-            acc[k] = ~acc[k];                           // now: live bits
+            acc[k] = ~(acc[k] | edge_mask(v0 + 64u * k, vlo, vend));  // live
             c += __popcll(acc[k]);
         }
         uint32_t incl = c;
