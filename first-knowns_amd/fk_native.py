@@ -240,9 +240,16 @@ class Engine:
             pass
 
     # expected sieve survivors one device sweep takes at most; wider
-    # windows are swept in consecutive pieces (the library refuses a window
-    # whose survivors would not fit one queue segment, 2^30 records)
-    PIECE_SURVIVORS = 1 << 28
+    # windows are swept in consecutive pieces.  The library refuses a
+    # window whose survivors would not fit one queue segment (2^30 records,
+    # refused from 0.9 of that); a sweep of 9e8 holds about 75 GB of queues
+    # and flags, a quarter of the device's memory, and one sweep does less
+    # work per position than several (the Euler SCORE_WIDE range, 8.8e8
+    # survivors: 0.58 s in one sweep against 0.68 s in four)
+    PIECE_SURVIVORS = 900_000_000
+    # pieces hold whole 16-word chunks of every line (1024 D - D values:
+    # 1024 positions per line, whatever the first value's residue)
+    PIECE_POSITIONS = 1024
 
     def max_span(self):
         """Widest window one device sweep takes (line values)."""
@@ -253,6 +260,9 @@ class Engine:
         """Sorted records (numpy RECORD_DTYPE) of the survivors in [lo, hi)."""
         lo, hi = int(lo), int(hi)
         step = self.max_span()
+        unit = self.PIECE_POSITIONS * self.line_step
+        if hi - lo > step and step >= 2 * unit:
+            step = (step // unit) * unit - self.line_step
         if hi - lo > step:
             parts = [self._sweep_one(a, min(a + step, hi), mode, cap,
                                      exact_from)
