@@ -482,9 +482,10 @@ __device__ __forceinline__ void short_words(const FK_CONST uint32_t* sp,
     // load, residue, LDS reads); unrolled, several primes' chains are in
     // flight at once.  The workgroups' LDS already caps a SIMD at a few
     // waves, so the registers this takes are free (the wide form, for
-    // wheels past 2^32, is not unrolled: unrolled it would take the
-    // kernel past 128 registers, three waves per SIMD instead of four).
-    constexpr int kUnroll = WIDE ? 1 : 4;
+    // wheels past 2^32, is not unrolled, and sixteen-word chunks unroll by
+    // three: more would take the kernel past 128 registers, three waves
+    // per SIMD instead of four).
+    constexpr int kUnroll = WIDE ? 1 : (C >= 16 ? 3 : 4);
 #pragma unroll kUnroll
     for (uint32_t p = 0; p < nlds; p++) {
         // {q, 1/q, D^-1} as doubles, q, bitmap word
