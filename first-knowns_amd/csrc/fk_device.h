@@ -74,6 +74,10 @@ struct DevEngine {
                               // table: per round prime a RoundRow header
                               // (fk_kill.h), then the killing residues of
                               // the value, padded; or an IndexRow
+    uint32_t pronic_first;    // Euler: prime-table index of the first round
+                              // prime above (n - 1) n, whose rows the lane =
+                              // candidate kernel tests without their
+                              // residues; round_end for the other families
     const uint32_t* kperm;    // residue rows: [round_end - round_first] the
                               // rows of each round ordered by their groups
                               // of four residues (a permutation within

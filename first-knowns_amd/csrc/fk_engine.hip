@@ -456,6 +456,12 @@ static int build_engine(fk_engine* e)
         h.kidx = (e->round_first >= e->round_end ||
                   e->dval % ctx->primes[e->round_first] != 0) ? 1u : 0u;
     h.kstride = h.kidx ? kIndexStride : round_row_stride(d.family, d.n);
+    h.pronic_first = e->round_end;
+    if (d.family == FK_FAMILY_EULER)
+        h.pronic_first = std::min(
+            e->round_end,
+            std::max(e->round_first,
+                     prime_index(ctx->primes, (uint32_t)(d.n - 1) * (uint32_t)d.n + 1u)));
     h.nround = e->nrounds;
     for (size_t i = 0; i < 33; i++)
         h.rb[i] = i < rb.size() ? rb[i] : e->round_end;

@@ -825,6 +825,49 @@ __device__ __forceinline__ void round_scan(const double (&a)[KC][8],
     for (int c = 0; c < KC; c++) hit[c] |= __ballot(acc[c] == 0u);
 }
 
+// ---- Euler rows of primes above (n - 1) n.  q kills p when p + x(x + 1)
+// is 0 mod q for some x < n, i.e. when u = (-p) mod q equals some
+// x(x + 1) <= (n - 1) n < q (a residue below q equals such a product only
+// as an integer).  The products x(x + 1) are exactly the u with 4u + 1 an
+// odd square, so these rows are tested without their residue lists: u is
+// clamped to (n - 1) n + 1 (odd, never such a product) and 4u + 1 < 4 n^2
+// is tested for a square with the hardware square root (within an ulp:
+// rounded to the nearest integer it is the exact root of a square).  About
+// ten VALU per test where the row's n residues take 1.5 per residue.  The
+// result is zero when q kills.
+__device__ __forceinline__ uint32_t pronic_d(uint32_t r, uint32_t q, uint32_t pmax)
+{
+    const uint32_t u = min(q - r, pmax + 1u);      // r = 0: u = q, clamped
+    const uint32_t w = 4u * u + 1u;
+    const uint32_t t = (uint32_t)(__builtin_amdgcn_sqrtf((float)w) + 0.5f);
+    const uint32_t d = (t * t) ^ w;
+    return r == 0u ? 0u : d;                       // x = 0: q divides p
+}
+
+// KC limb sets per lane against the rows [lo, hi) by pronic_d.
+template <int NL, int KC>
+__device__ __forceinline__ void pronic_scan(const double (&a)[KC][8],
+                                            const FK_CONST PrimeD* pd,
+                                            const FK_CONST uint32_t* kt,
+                                            uint32_t stride, uint32_t rf,
+                                            uint32_t lo, uint32_t hi,
+                                            uint32_t pmax, uint64_t (&hit)[KC])
+{
+    uint32_t acc[KC];
+#pragma unroll
+    for (int c = 0; c < KC; c++) acc[c] = 0xffffffffu;
+#pragma unroll 2
+    for (uint32_t p = lo; p < hi; p++) {
+        const RowView<1> v = load_row<1>(kt + (size_t)(p - rf) * stride);
+#pragma unroll
+        for (int c = 0; c < KC; c++)
+            acc[c] = min(acc[c], pronic_d(row_residue<NL, false>(a[c], v, pd + p),
+                                          v.q, pmax));
+    }
+#pragma unroll
+    for (int c = 0; c < KC; c++) hit[c] |= __ballot(acc[c] == 0u);
+}
+
 // ---- values below 2^53 (every live lane of the wave): the residue comes
 // from the value itself, one exact double, with no limb sum (resid_d1 and
 // resid_fold_d hold for s / q < 2^48: values below 2^53, primes of at least
@@ -954,6 +997,30 @@ __device__ __forceinline__ void narrow_scan(const double (&s)[KC],
         const NarrowRow v0 = load_narrow_one(r0);
 #pragma unroll
         for (int c = 0; c < KC; c++) acc[c] = narrow_test<FOLD>(s[c], v0, r0, acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < KC; c++) hit[c] |= __ballot(acc[c] == 0u);
+}
+
+// The same values against Euler rows of primes above (n - 1) n (pronic_d).
+template <int KC>
+__device__ __forceinline__ void narrow_pronic(const double (&s)[KC],
+                                              const FK_CONST uint32_t* kt,
+                                              uint32_t stride, uint32_t rf,
+                                              uint32_t lo, uint32_t hi,
+                                              uint32_t pmax, uint64_t (&hit)[KC])
+{
+    uint32_t acc[KC];
+#pragma unroll
+    for (int c = 0; c < KC; c++) acc[c] = 0xffffffffu;
+    const FK_CONST uint32_t* r0 = kt + (size_t)(lo - rf) * stride;
+#pragma unroll 2
+    for (uint32_t p = lo; p < hi; p++, r0 += stride) {
+        const double qd = load_f64(r0), qi = load_f64(r0 + 2);
+        const uint32_t q = r0[10];
+#pragma unroll
+        for (int c = 0; c < KC; c++)
+            acc[c] = min(acc[c], pronic_d(resid_d1(s[c], qd, qi, q), q, pmax));
     }
 #pragma unroll
     for (int c = 0; c < KC; c++) hit[c] |= __ballot(acc[c] == 0u);
@@ -1190,9 +1257,16 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
             uint64_t m[KC];
 #pragma unroll
             for (int c = 0; c < KC; c++) m[c] = 0;
+            // Euler rows past (n - 1) n without their residues
+            const uint32_t pf = min(max(lo, E->pronic_first), hi);
             if (kill_folded(E->family))
                 narrow_scan<true, KC>(s, kt, E->kstride, E->round_first, lo, hi, m);
-            else
+            else if (pf < hi) {
+                if (lo < pf)
+                    narrow_scan<false, KC>(s, kt, E->kstride, E->round_first, lo, pf, m);
+                narrow_pronic<KC>(s, kt, E->kstride, E->round_first, pf, hi,
+                                  (uint32_t)(E->n - 1) * (uint32_t)E->n, m);
+            } else
                 narrow_scan<false, KC>(s, kt, E->kstride, E->round_first, lo, hi, m);
 #pragma unroll
             for (int c = 0; c < KC; c++)
@@ -1266,9 +1340,19 @@ __global__ __launch_bounds__(256) void k_round_c(RoundArgs a)
             else if (wwide) round_scan<8, KG, true, KC>(l, pd, kt, st, rf, lo, hi, m);
             else round_scan<4, KG, true, KC>(l, pd, kt, st, rf, lo, hi, m);
         } else {
-            if (w22) round_scan<22, KG, false, KC>(l, pd, kt, st, rf, lo, hi, m);
-            else if (wwide) round_scan<8, KG, false, KC>(l, pd, kt, st, rf, lo, hi, m);
-            else round_scan<4, KG, false, KC>(l, pd, kt, st, rf, lo, hi, m);
+            // Euler rows past (n - 1) n without their residues
+            const uint32_t pf = min(max(lo, E->pronic_first), hi);
+            if (lo < pf) {
+                if (w22) round_scan<22, KG, false, KC>(l, pd, kt, st, rf, lo, pf, m);
+                else if (wwide) round_scan<8, KG, false, KC>(l, pd, kt, st, rf, lo, pf, m);
+                else round_scan<4, KG, false, KC>(l, pd, kt, st, rf, lo, pf, m);
+            }
+            if (pf < hi) {
+                const uint32_t pmax = (uint32_t)(E->n - 1) * (uint32_t)E->n;
+                if (w22) pronic_scan<22, KC>(l, pd, kt, st, rf, pf, hi, pmax, m);
+                else if (wwide) pronic_scan<8, KC>(l, pd, kt, st, rf, pf, hi, pmax, m);
+                else pronic_scan<4, KC>(l, pd, kt, st, rf, pf, hi, pmax, m);
+            }
         }
 #pragma unroll
         for (int c = 0; c < KC; c++)
