@@ -337,6 +337,79 @@ def run_euler_wide(golden_xor=None):
     return out
 
 
+# ------------------------------------ the reference's frozen benchmark shapes
+
+# The reference's own published rates at these shapes, all on one RTX 4090
+# (host survivor tests excluded): dickson-ladders/BENCHMARKS.md:38-40,
+# primorial-ap/BENCHMARKS.md:35-38 (best of four captures),
+# euler-prime-runs/BENCHMARKS.md:107 and :420-424 (best capture).
+REF_SHAPE_RATES = {
+    ("dickson", "SCORE"): 2.06e12, ("dickson", "SCORE12"): 2.49e12,
+    ("dickson", "SCORE13"): 5.43e12, ("ap", "SCORE"): 4.45e10,
+    ("euler", "SCORE"): 6.23e15, ("euler", "SCORE128"): 6.34e15,
+    ("euler", "SCORE_WIDE"): 3.73e16,
+}
+
+
+def run_reference_shapes():
+    """Each frozen benchmark shape of the reference (dickson-ladders/
+    score.py:54-59, primorial-ap/score.py:42-60, euler-prime-runs/
+    score.py:57-77), swept by the drop-in engines at its own parameters:
+    line values per second of one sweep after a warm-up sweep of the same
+    shape (survivors found, not classified, as the reference's score), the
+    stream's count and XOR against the reference's recorded fingerprint,
+    and the reference's published rate."""
+    import ap_gpu
+    import euler_gpu
+    import ladder_gpu
+    g = json.loads((ROOT / "tests" / "golden" / "reference_vectors.json")
+                   .read_text())
+    out = []
+
+    def big(v):
+        return int(v)
+
+    def timed(fam, label, make, sweep, span, want):
+        eng = make()
+        sweep(eng)                                 # warm-up: tables, graphs
+        t0 = time.perf_counter()
+        got = sweep(eng)
+        wall = time.perf_counter() - t0
+        x = 0
+        for v in got:
+            x ^= int(v)
+        ref = REF_SHAPE_RATES[(fam, label)]
+        out.append({"family": fam, "shape": label, "span": span,
+                    "seconds": wall, "rate": span / wall,
+                    "fingerprint_equal": (len(got), x) == want,
+                    "reference_rate_rtx4090": ref,
+                    "ratio": span / wall / ref})
+
+    for s in g["dickson_fingerprints"]:
+        j0, span = big(s["j0"]), big(s["span"])
+        timed("dickson", s["label"],
+              lambda s=s: ladder_gpu.GpuEngine(s["n"], q2=s["q2"]),
+              lambda e, j0=j0, span=span: e.survivors_j(j0, j0 + span).tolist(),
+              span, (s["count"], big(s["xor"])))
+    for s in g["ap_fingerprints"]:
+        base, span = big(s["base"]), big(s["span"])
+
+        def ap_sweep(e, base=base, span=span):
+            vals = []
+            for c in e.survivors(base, span):
+                vals.extend(c.tolist())
+            return vals
+        timed("ap", s["label"],
+              lambda s=s: ap_gpu.GpuEngine(s["n"], s["q2"], launch_u=s["launch_u"]),
+              ap_sweep, span, (s["count"], big(s["xor"])))
+    for s in g["euler_fingerprints"]:
+        lo, span = big(s["lo"]), big(s["span"])
+        timed("euler", s["label"], lambda s=s: euler_gpu.GpuEngine(s["n"]),
+              lambda e, lo=lo, span=span: e.survivors_pre_mr(lo, lo + span),
+              span, (s["count"], big(s["xor"])))
+    return out
+
+
 # ------------------------------------ config 2: the AP sweep
 
 AP_N, AP_Q2 = 17, 1 << 24
@@ -643,6 +716,7 @@ def main():
         out["config2_ap"] = run_ap()
     if not args.no_euler:
         out["euler_score_wide"] = run_euler_wide()
+        out["reference_shapes"] = run_reference_shapes()
     print(json.dumps(out, default=_plain), flush=True)
     return 0
 

@@ -395,7 +395,8 @@ static int build_engine(fk_engine* e)
                         "lower q_med, q_pat or the tile", need);
     }
     // the short-line sieve's LDS: per-prime arrays and every kill bitmap
-    if (short_lds(lq.size(), sbm.size()) > 160 * 1024) e->short_words = 0;
+    // (the kernel's own survivor buffer takes 4 KB more)
+    if (short_lds(lq.size(), sbm.size()) > 152 * 1024) e->short_words = 0;
     e->lds_q = lq;
     e->pat_words = (uint32_t)pat.size();
     e->surv_lds = surv;

@@ -511,16 +511,53 @@ __device__ __forceinline__ void short_words(const FK_CONST uint32_t* sp,
     }
 }
 
+// A workgroup gathers its tiles' survivors, as (line, position) pairs, in
+// LDS and reserves their queue slots with one atomic per kShortBuf of them
+// (when the buffer would overflow, when the piece changes, and at the
+// end), instead of one atomic per wave per tile: on a wide wheel with short
+// lines (the Euler SCORE range: 2.3M tiles, nine million atomics on one
+// counter) those serialised on the counter's L2 line and took most of the
+// launch.  A tile of more survivors than the buffer holds reserves its own
+// slots, one atomic per wave.
+constexpr uint32_t kShortBuf = 512;
+
 template <int C>
 __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
 {
     extern __shared__ uint64_t lds64[];
+    __shared__ uint2 sbuf[kShortBuf];
+    __shared__ uint32_t sh_wtot[4];
+    __shared__ uint32_t sh_n, sh_base;
     // [nlds_max] (B D^-1) mod q per LDS prime as a double, then the bitmaps
     double* bqd = reinterpret_cast<double*>(lds64);
     uint32_t* bm = reinterpret_cast<uint32_t*>(lds64 + ((a.nlds_max + 1) & ~1u));
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
+    const uint32_t wave = tid >> 6;
     uint32_t cur_engine = 0xffffffffu, cur_piece = 0xffffffffu;
+    if (tid == 0) sh_n = 0;
+    // the buffered survivors of piece pb out to the queue (every thread of
+    // the workgroup calls this at the same point; entries are complete)
+    auto flush = [&](uint32_t pb) {
+        __syncthreads();
+        const uint32_t n = sh_n;
+        if (n == 0) return;
+        const FK_CONST DevPiece& Q = uniform_view(a.pieces)[pb];
+        const FK_CONST DevEngine& F = uniform_view(a.engines)[Q.engine];
+        if (tid == 0) sh_base = atomicAdd(a.counts + Q.queue, n);
+        __syncthreads();
+        const uint32_t base = sh_base;
+        const unsigned __int128 B =
+            ((unsigned __int128)(((uint64_t)Q.base[3] << 32) | Q.base[2]) << 64)
+            | (((uint64_t)Q.base[1] << 32) | Q.base[0]);
+        for (uint32_t i = tid; i < n; i += 256) {
+            const uint2 e = sbuf[i];
+            emit_record(a, Q, F, B, F.offs[e.x], e.y, base + i);
+        }
+        __syncthreads();
+        if (tid == 0) sh_n = 0;
+        __syncthreads();
+    };
     const uint64_t tiles_per =
         ((uint64_t)(a.ntiles - a.tile_begin) + gridDim.x - 1) / gridDim.x;
     const uint64_t tile_lo = a.tile_begin + (uint64_t)blockIdx.x * tiles_per;
@@ -530,8 +567,9 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
         const FK_CONST DevPiece& P = uniform_view(a.pieces)[pi];
         const FK_CONST DevEngine& E = uniform_view(a.engines)[P.engine];
         if (pi != cur_piece) {
-            // the engine's tables when the engine changes, the piece's
-            // B mod q always
+            // the last piece's survivors out; the engine's tables when the
+            // engine changes, the piece's B mod q always
+            if (cur_piece != 0xffffffffu) flush(cur_piece);
             __syncthreads();
             if (P.engine != cur_engine) {
                 // the bitmaps in 16-byte pieces, eight loads in flight per
@@ -593,7 +631,9 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
                                      (double)(uint32_t)off,
                                      (double)(off >> 32), (double)v0, acc);
         }
-        // records: a wave-wide prefix of the per-lane counts, one atomic
+        // records: a wave-wide prefix of the per-lane counts, then the
+        // workgroup's: into the LDS buffer, or straight out for a tile that
+        // overflows it
         uint32_t c = 0;
 #pragma unroll
         for (int k = 0; k < C; k++) {
@@ -607,28 +647,55 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
             if (lane >= (uint32_t)d) incl += o;
         }
         const uint32_t total = __shfl(incl, 63);
-        uint32_t base = 0;
-        if (total) {
-            if (lane == 63) base = atomicAdd(a.counts + P.queue, total);
-            base = __shfl(base, 63);
+        if (lane == 63) sh_wtot[wave] = total;
+        __syncthreads();
+        uint32_t woff = 0, ttot = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < 4; w++) {
+            const uint32_t t = sh_wtot[w];
+            woff += w < wave ? t : 0u;
+            ttot += t;
         }
-        if (c) {
-            const unsigned __int128 B =
-                ((unsigned __int128)(((uint64_t)P.base[3] << 32) | P.base[2]) << 64)
-                | (((uint64_t)P.base[1] << 32) | P.base[0]);
-            uint32_t slot = base + incl - c;
+        if (sh_n + ttot > kShortBuf) flush(pi);
+        if (ttot <= kShortBuf) {
+            uint32_t at = sh_n + woff + incl - c;
 #pragma unroll
             for (int k = 0; k < C; k++) {
                 uint64_t m = acc[k];
                 while (m) {
                     const int bit = __builtin_ctzll(m);
                     m &= m - 1;
-                    emit_record(a, P, E, B, off, (uint64_t)v0 + 64u * k + bit,
-                                slot++);
+                    sbuf[at++] = make_uint2(line, v0 + 64u * k + (uint32_t)bit);
                 }
             }
+            __syncthreads();                // every thread has read sh_n
+            if (tid == 0) sh_n += ttot;
+        } else {
+            uint32_t base = 0;
+            if (total) {
+                if (lane == 63) base = atomicAdd(a.counts + P.queue, total);
+                base = __shfl(base, 63);
+            }
+            if (c) {
+                const unsigned __int128 B =
+                    ((unsigned __int128)(((uint64_t)P.base[3] << 32) | P.base[2]) << 64)
+                    | (((uint64_t)P.base[1] << 32) | P.base[0]);
+                uint32_t slot = base + incl - c;
+#pragma unroll
+                for (int k = 0; k < C; k++) {
+                    uint64_t m = acc[k];
+                    while (m) {
+                        const int bit = __builtin_ctzll(m);
+                        m &= m - 1;
+                        emit_record(a, P, E, B, off, (uint64_t)v0 + 64u * k + bit,
+                                    slot++);
+                    }
+                }
+            }
+            __syncthreads();                // sh_wtot is rewritten next tile
         }
     }
+    if (cur_piece != 0xffffffffu) flush(cur_piece);
 }
 
 // ---------------------------------------------------------------- rounds
