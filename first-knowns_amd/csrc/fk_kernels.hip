@@ -512,12 +512,12 @@ __device__ __forceinline__ void short_words(const FK_CONST uint32_t* sp,
 }
 
 // A workgroup gathers its tiles' survivors, as (line, position) pairs, in
-// LDS and reserves their queue slots with one atomic per kShortBuf of them
-// (when the buffer would overflow, when the piece changes, and at the
-// end), instead of one atomic per wave per tile: on a wide wheel with short
-// lines (the Euler SCORE range: 2.3M tiles, nine million atomics on one
-// counter) those serialised on the counter's L2 line and took most of the
-// launch.  A tile of more survivors than the buffer holds reserves its own
+// LDS and reserves their queue slots with one atomic per flush of the
+// buffer (once it is half full, when the piece changes, and at the end),
+// instead of one atomic per wave per tile: on a wide wheel with short lines
+// (the Euler SCORE range: 2.3M tiles, nine million atomics on one counter)
+// those serialised on the counter's L2 line and took most of the launch.
+// A tile of more survivors than the buffer has room for reserves its own
 // slots, one atomic per wave.
 constexpr uint32_t kShortBuf = 512;
 
@@ -526,8 +526,8 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
 {
     extern __shared__ uint64_t lds64[];
     __shared__ uint2 sbuf[kShortBuf];
-    __shared__ uint32_t sh_wtot[4];
-    __shared__ uint32_t sh_n, sh_base;
+    __shared__ uint32_t sh_wtot[2][4];      // per wave, tiles alternating
+    __shared__ uint32_t sh_base;
     // [nlds_max] (B D^-1) mod q per LDS prime as a double, then the bitmaps
     double* bqd = reinterpret_cast<double*>(lds64);
     uint32_t* bm = reinterpret_cast<uint32_t*>(lds64 + ((a.nlds_max + 1) & ~1u));
@@ -535,13 +535,15 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
     const uint32_t lane = __lane_id();
     const uint32_t wave = tid >> 6;
     uint32_t cur_engine = 0xffffffffu, cur_piece = 0xffffffffu;
-    if (tid == 0) sh_n = 0;
+    // entries in the buffer: every thread keeps the same count
+    uint32_t nbuf = 0;
     // the buffered survivors of piece pb out to the queue (every thread of
-    // the workgroup calls this at the same point; entries are complete)
+    // the workgroup calls this at the same point)
     auto flush = [&](uint32_t pb) {
-        __syncthreads();
-        const uint32_t n = sh_n;
+        const uint32_t n = nbuf;
         if (n == 0) return;
+        nbuf = 0;
+        __syncthreads();                            // entries complete
         const FK_CONST DevPiece& Q = uniform_view(a.pieces)[pb];
         const FK_CONST DevEngine& F = uniform_view(a.engines)[Q.engine];
         if (tid == 0) sh_base = atomicAdd(a.counts + Q.queue, n);
@@ -554,9 +556,7 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
             const uint2 e = sbuf[i];
             emit_record(a, Q, F, B, F.offs[e.x], e.y, base + i);
         }
-        __syncthreads();
-        if (tid == 0) sh_n = 0;
-        __syncthreads();
+        __syncthreads();                            // entries consumed
     };
     const uint64_t tiles_per =
         ((uint64_t)(a.ntiles - a.tile_begin) + gridDim.x - 1) / gridDim.x;
@@ -566,6 +566,10 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
         const uint32_t pi = find_piece(uniform_view(a.pieces), a.npieces, tile);
         const FK_CONST DevPiece& P = uniform_view(a.pieces)[pi];
         const FK_CONST DevEngine& E = uniform_view(a.engines)[P.engine];
+        // the buffer is emptied between tiles, while no words are live (a
+        // flush beside them took the sixteen-word kernel past 128
+        // registers): when half full, and when the piece changes
+        if (pi == cur_piece && nbuf >= kShortBuf / 2) flush(pi);
         if (pi != cur_piece) {
             // the last piece's survivors out; the engine's tables when the
             // engine changes, the piece's B mod q always
@@ -647,18 +651,21 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
             if (lane >= (uint32_t)d) incl += o;
         }
         const uint32_t total = __shfl(incl, 63);
-        if (lane == 63) sh_wtot[wave] = total;
+        // (a wave writes the other tile parity's slots next, after this
+        // tile's barrier, which every wave passes only once it has read
+        // these)
+        const uint32_t par = tile & 1u;
+        if (lane == 63) sh_wtot[par][wave] = total;
         __syncthreads();
         uint32_t woff = 0, ttot = 0;
 #pragma unroll
         for (uint32_t w = 0; w < 4; w++) {
-            const uint32_t t = sh_wtot[w];
+            const uint32_t t = sh_wtot[par][w];
             woff += w < wave ? t : 0u;
             ttot += t;
         }
-        if (sh_n + ttot > kShortBuf) flush(pi);
-        if (ttot <= kShortBuf) {
-            uint32_t at = sh_n + woff + incl - c;
+        if (nbuf + ttot <= kShortBuf) {
+            uint32_t at = nbuf + woff + incl - c;
 #pragma unroll
             for (int k = 0; k < C; k++) {
                 uint64_t m = acc[k];
@@ -668,8 +675,7 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
                     sbuf[at++] = make_uint2(line, v0 + 64u * k + (uint32_t)bit);
                 }
             }
-            __syncthreads();                // every thread has read sh_n
-            if (tid == 0) sh_n += ttot;
+            nbuf += ttot;
         } else {
             uint32_t base = 0;
             if (total) {
@@ -692,7 +698,6 @@ __global__ __launch_bounds__(256) void k_sieve_short(SieveArgs a)
                     }
                 }
             }
-            __syncthreads();                // sh_wtot is rewritten next tile
         }
     }
     if (cur_piece != 0xffffffffu) flush(cur_piece);
