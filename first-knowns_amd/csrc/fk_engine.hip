@@ -584,9 +584,13 @@ static int build_engine(fk_engine* e)
 }
 
 // Prefix wheels: by default up to kAltWheels, of at most kAltLines
-// residues each (their host tables are built at engine creation).
+// residues each (their host tables are built at engine creation).  2^25
+// admits the Euler 31# wheel (29,937,600 residues at n = 17, 240 MB) under
+// the 37# one: a 5e14 window (the reference's SCORE shapes) holds 67
+// positions on each 37# line and 2,493 on each 31# line, and sweeps in 13
+// against 27 ms on it.
 static constexpr int kAltWheels = 4;
-static constexpr uint64_t kAltLines = 1ull << 21;
+static constexpr uint64_t kAltLines = 1ull << 25;
 
 // The engine's prefix wheels (fk_engine_desc.alt_wheels): the wheel without
 // its last 1, 2, ... primes, each an engine of its own sieve-side tables
