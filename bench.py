@@ -355,10 +355,12 @@ def run_reference_shapes():
     """Each frozen benchmark shape of the reference (dickson-ladders/
     score.py:54-59, primorial-ap/score.py:42-60, euler-prime-runs/
     score.py:57-77), swept by the drop-in engines at its own parameters:
-    line values per second of one sweep after a warm-up sweep of the same
-    shape (survivors found, not classified, as the reference's score), the
-    stream's count and XOR against the reference's recorded fingerprint,
-    and the reference's published rate."""
+    line values per second, the median of five sweeps after two warm-up
+    sweeps of the same shape (the first sweeps build the engine's batch
+    buffers and capture its launch sequence; survivors found, not
+    classified, as the reference's score), the stream's count and XOR
+    against the reference's recorded fingerprint, and the reference's
+    published rate."""
     import ap_gpu
     import euler_gpu
     import ladder_gpu
@@ -371,10 +373,14 @@ def run_reference_shapes():
 
     def timed(fam, label, make, sweep, span, want):
         eng = make()
-        sweep(eng)                                 # warm-up: tables, graphs
-        t0 = time.perf_counter()
-        got = sweep(eng)
-        wall = time.perf_counter() - t0
+        for _ in range(2):                         # warm-up: buffers, graphs
+            sweep(eng)
+        walls = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            got = sweep(eng)
+            walls.append(time.perf_counter() - t0)
+        wall = statistics.median(walls)
         x = 0
         for v in got:
             x ^= int(v)
