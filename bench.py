@@ -593,9 +593,9 @@ def main():
     if args.copy_probe:
         return copy_probe(args.copy_probe)
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path "
-              f"(windows shard across GPUs by range; see DESIGN.md). "
-              f"--gpus {args.gpus} is not supported.", file=sys.stderr)
+        print(f"bench.py runs on one GPU: the engine has no multi-GPU path "
+              f"(windows would shard across GPUs by range of the search "
+              f"line). --gpus {args.gpus} is not supported.", file=sys.stderr)
         return 2
 
     P = run_plain(args.steps, args.warmup)
