@@ -26,7 +26,8 @@
 // (sieve1a, sieve1b, sieve1b_warp), primorial-ap/ap_gpu.py:155-392
 // (build_tab, mark_shared, mark, compact); the strong test restates
 // huntlib/primes.py:23-80.  None of them is translated: the tile layout,
-// the shared line model and the on-device classification are this build's.
+// the shared line model and the on-device classification have no
+// counterpart there.
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
