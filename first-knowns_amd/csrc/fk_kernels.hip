@@ -1112,10 +1112,10 @@ __device__ __forceinline__ IdxView load_idx(const FK_CONST uint32_t* row)
     IdxView v;
     v.qd = load_f64(row + 0);
     v.qi = load_f64(row + 2);
-    v.u0 = load_f64(row + 4);
-    v.u1 = load_f64(row + 6);
-    v.u2 = load_f64(row + 8);
-    v.u3 = load_f64(row + 10);
+    v.u0 = load_f64(row + kIndexU);
+    v.u1 = load_f64(row + kIndexU + 2);
+    v.u2 = load_f64(row + kIndexU + 4);
+    v.u3 = load_f64(row + kIndexU + 6);
     v.q = row[12];
     v.n = row[13];
     return v;
@@ -1133,7 +1133,7 @@ __device__ __forceinline__ uint32_t idx_u(const double (&a)[8], const IdxView& v
     if (NL > 4) {
 #pragma unroll
         for (int k = 4; k < NL; k++)
-            s = fma(a[k], load_f64(row + kIndexHead + 2 * (k - 4)), s);
+            s = fma(a[k], load_f64(row + kIndexUW + 2 * (k - 4)), s);
     }
     const double qt = floor(fma(s, v.qi, -0.125));
     return (uint32_t)fma(-qt, v.qd, s);
@@ -1658,14 +1658,18 @@ struct LaneIdx {
     uint32_t q, n;
 };
 
-__device__ __forceinline__ void load_lane_idx(LaneIdx& I, const uint32_t* row)
+// half: the slice's values take the 22-bit form, which reads only the
+// row's first 64 bytes (the 16-bit constants are left zero)
+__device__ __forceinline__ void load_lane_idx(LaneIdx& I, const uint32_t* row,
+                                              bool half)
 {
     IndexRow h;
     const uint4* r4 = reinterpret_cast<const uint4*>(row);
     uint32_t hw[kIndexStride];
 #pragma unroll
     for (int i = 0; i < (int)kIndexStride / 4; i++) {
-        const uint4 v = r4[i];
+        const uint4 v = (!half || i < (int)kIndexHalf / 4) ? r4[i]
+                                                           : make_uint4(0, 0, 0, 0);
         hw[4 * i] = v.x;
         hw[4 * i + 1] = v.y;
         hw[4 * i + 2] = v.z;
@@ -1804,7 +1808,8 @@ __global__ __launch_bounds__(256) void k_round_p(RoundArgs a)
                 if (__ballot(mine)) {
                     LaneIdx I;
                     load_lane_idx(I, engs[e].kt + (size_t)(pc - engs[e].round_first) *
-                                                      kIndexStride);
+                                                      kIndexStride,
+                                  a.l22 && sh_huge == 0);
                     const uint32_t nlim = mine ? I.n : 0u;
                     if (a.l22 && sh_huge == 0) {
                         if (sh_w66)

@@ -178,21 +178,30 @@ FK_HD void round_row(int family, int n, const uint32_t d[4], const PrimeK& p,
 // form in 22-bit limbs is exact too (each product below 2^46, four of them
 // below 2^48) and takes p to 2^88 in four limbs (three below 2^66), where
 // 16-bit limbs take eight past 2^64.
+// The first 64 bytes hold all that the 22-bit form reads, so a round of
+// 22-bit tests fetches half of each row (a million-prime round of a few
+// candidates is bound by those fetches).
 struct IndexRow {
     double qd;        // q
     double qi;        // nearest double to 1/q
-    double u[4];      // u 2^(16k) mod q, k = 0..3
+    double v[4];      // u 2^(22k) mod q, k = 0..3 (22-bit limbs, q < 2^24)
     uint32_t q;
     uint32_t n;       // t < n kills
     uint32_t pad[2];
+    double u[4];      // u 2^(16k) mod q, k = 0..3
     double uw[4];     // u 2^(16k) mod q, k = 4..7 (values past 2^64)
-    double v[4];      // u 2^(22k) mod q, k = 0..3 (22-bit limbs, q < 2^24)
 };
 constexpr uint32_t kIndexStride = 32;       // words of IndexRow
-constexpr uint32_t kIndexHead = 16;         // words read for values < 2^64
-constexpr uint32_t kIndexV = 24;            // first word of v
+constexpr uint32_t kIndexV = 4;             // first word of v
+constexpr uint32_t kIndexU = 16;            // first word of u
+constexpr uint32_t kIndexUW = 24;           // first word of uw
+constexpr uint32_t kIndexHalf = 16;         // words the 22-bit form reads
 static_assert(sizeof(IndexRow) == 4 * kIndexStride, "IndexRow layout");
 static_assert(offsetof(IndexRow, v) == 4 * kIndexV, "IndexRow layout");
+static_assert(offsetof(IndexRow, q) == 48 && offsetof(IndexRow, n) == 52,
+              "IndexRow layout");
+static_assert(offsetof(IndexRow, u) == 4 * kIndexU, "IndexRow layout");
+static_assert(offsetof(IndexRow, uw) == 4 * kIndexUW, "IndexRow layout");
 
 // Families whose rows may be index rows.
 FK_HD bool kill_indexable(int family) { return family == 3; }
